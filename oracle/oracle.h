@@ -1,0 +1,144 @@
+/* oracle.h -- CPU oracle for the xtax L1 congruence annealer.
+ *
+ * TEST INFRASTRUCTURE ONLY. Nothing in the product (xtax_amd/, include/) links,
+ * loads or calls this library. It is the checker the parity tests, smoke() and
+ * bench.py's cpu_baseline leg compare the HIP engine against.
+ *
+ * Contents, each restating the reference (flinty-mathematics/xtax) in code of
+ * our own:
+ *   - exact int64 move primitives of the L1 objective    (src/main.cpp:46-242)
+ *   - Matrix helpers: score, nonzero count, congruence,  (src/mat_io.hpp:41-48,
+ *     Gram, X^T L                                         219-235, 340-372, 545-580)
+ *   - Philox4x32-10 (Salmon et al., SC'11, "Parallel random numbers: as easy as
+ *     1, 2, 3"), pinned by the Random123 known-answer vectors in tests/
+ *   - a deterministic exp() (Cody-Waite reduction + Taylor, all fma) that the
+ *     engine's Metropolis test is specified with
+ *   - an exact verifier: X^T A0 X == A checked modulo enough 61-bit primes to
+ *     cover the integer bound (CRT), det X mod the same primes
+ *   - move-log replay through the restated primitives
+ *   - a trajectory model of the engine's epoch-synchronous algorithm (the
+ *     reference's anneal_worker loop, congruence_anneal.hpp:177-416, with the
+ *     reference's mt19937 replaced by counter-based Philox and its 5 ms monitor
+ *     exchange replaced by an exchange every `exchange_interval` moves)
+ *
+ * Parity status of each function is stated in the oracle section of DESIGN.md
+ * and pinned by tests/test_oracle_*.py.
+ */
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_MAGNITUDE_LIMIT (((int64_t)1) << 48) /* mat_io.hpp:35 */
+#define ORC_SHEAR_CAP 64                          /* mat_io.hpp:38 */
+
+/* ---- primitives (restated from the reference) ---- */
+int64_t orc_rounded_div(int64_t a, int64_t b);
+int64_t orc_score(const int64_t* A, int32_t n);
+int64_t orc_offdiag_nonzero(const int64_t* A, int32_t n);
+int32_t orc_try_add(const int64_t* A, const int64_t* Xt, int32_t n, int32_t i, int32_t j,
+                    int64_t s, int64_t* new_a_row, int64_t* new_xt_row /* nullable */,
+                    int64_t* delta, int64_t* d_nonzero);
+int32_t orc_add_transform_row(const int64_t* Xt, int32_t n, int32_t i, int32_t j, int64_t s,
+                              int64_t* new_xt_row);
+void orc_commit_add(int64_t* A, int64_t* Xt, int32_t n, int32_t j, const int64_t* new_a_row,
+                    const int64_t* new_xt_row, int64_t* row_mass);
+void orc_compute_row_mass(const int64_t* A, int32_t n, int64_t* row_mass);
+int64_t orc_score_delta_l1(const int64_t* A, int32_t n, int32_t i, int32_t j, int64_t s);
+int64_t orc_best_shear_l1(const int64_t* A, int32_t n, int32_t i, int32_t j);
+/* congruence_of: out = X^T A X (int64, unchecked like the reference) */
+void orc_congruence_of(const int64_t* A, const int64_t* X, int32_t n, int64_t* out);
+void orc_gram_of(const int64_t* L, int32_t m, int32_t d, int64_t* out);
+void orc_xt_times(const int64_t* X, const int64_t* L, int32_t m, int32_t d, int64_t* out);
+
+/* ---- engine building blocks ---- */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+double orc_xexp(double x);
+
+/* ---- exact verification ----
+ * status bits: 1 = X^T A0 X != A, 2 = A not diagonal, 4 = |det X| != 1,
+ *              8 = |det X| = 1 not decidable within the prime budget.
+ */
+typedef struct {
+    int32_t status;
+    int32_t primes_used;
+    int32_t det_sign; /* +1 / -1 when |det X| = 1 */
+    int32_t reserved;
+} orc_verify_report;
+int32_t orc_verify(const int64_t* A0, const int64_t* X, const int64_t* A, int32_t n,
+                   orc_verify_report* rep);
+
+/* ---- move-log replay ----
+ * Entry layout (16 bytes, shared with the engine's log format):
+ *   int64 move; int16 i; int16 j; int16 s; uint16 flags (bit0 = sweep commit)
+ * Applies the moves in order to (A, Xt) with the restated primitives; returns
+ * the index of the first entry that is infeasible (or count on success).
+ */
+typedef struct {
+    int64_t move;
+    int16_t i, j, s;
+    uint16_t flags;
+} orc_log_entry;
+int64_t orc_replay(int64_t* A, int64_t* Xt, int32_t n, const orc_log_entry* log, int64_t count);
+
+/* ---- trajectory model of the engine ---- */
+typedef struct {
+    double t_init;          /* <= 0: auto (suggest_t_init of the start state) */
+    double t_min;
+    double cooling;
+    double reheat;
+    double reseed_factor;
+    double greedy_fraction;
+    double target_fraction;
+    double target_accept;
+    double worker_diversity;
+    double sweep_fraction;
+    int32_t moves_per_cool;
+    int32_t stuck_threshold;
+    int32_t target_samples;
+    int32_t tempering;
+    int32_t adaptive_cooling;
+    int32_t exchange_interval;
+    int32_t replicas;
+    int32_t reserved;
+    uint64_t seed;
+    uint64_t max_moves;     /* per replica; 0 = unbounded */
+} orc_params;
+
+typedef struct {
+    int32_t solved;          /* 1 if some replica reached a diagonal */
+    int32_t solve_replica;
+    int64_t solve_move;
+    int64_t best_score;      /* instance best (argmin of score, move, replica) */
+    int32_t best_replica;
+    int32_t epochs;
+    int64_t total_moves;     /* proposals executed by all replicas */
+    int64_t accepts;
+    int64_t exchanges;
+} orc_model_result;
+
+/* Runs the model for at most max_epochs epochs (stops at the first epoch in
+ * which the instance is solved). Per-replica outputs (each may be null):
+ *   A_out, Xt_out   R*n*n  final state of each replica
+ *   best_A, best_Xt R*n*n  best state of each replica
+ *   scal_out        R*8    {score, offnz, moves, best_score, best_move, rung,
+ *                           accepts, local_best}
+ * log_out/log_count: per-replica accepted-move logs, log_cap entries each. */
+int32_t orc_model_run(const int64_t* A_start, const int64_t* Xt_start, int32_t n,
+                      const int32_t* active, int32_t n_active, const orc_params* p,
+                      int32_t max_epochs, int64_t* A_out, int64_t* Xt_out,
+                      int64_t* best_A, int64_t* best_Xt, int64_t* scal_out,
+                      orc_log_entry* log_out, int64_t* log_count, int64_t log_cap,
+                      orc_model_result* res);
+
+/* Ladder and per-replica fractions exactly as the model derives them (exposed
+ * so tests can check the engine's host setup). */
+void orc_ladder(double top, double t_min, int32_t R, double* temps);
+void orc_fractions(const orc_params* p, int32_t r, double* greedy, double* target);
+double orc_suggest_t_init(const int64_t* A, int32_t n);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,309 @@
+"""Generate golden vectors from the reference implementation.
+
+Run in the build container only (it needs the reference sources):
+
+    make -C oracle ref            # builds oracle/_ref/{xtax_ref,libref_harness.so}
+    python tests/golden/gen_golden.py
+
+G1 (per primitive): seeded matrices and (i, j, s) cases are pushed through the
+reference's own try_add / add_transform_row / commit_add / compute_row_mass /
+score_delta_l1 / best_shear_l1 / rounded_div / score / count_offdiag_nonzero /
+congruence_of / gram_of / xt_times (src/main.cpp:46-242, src/mat_io.hpp) via
+oracle/_ref/libref_harness.so, and inputs plus outputs are written to
+g1_*.npz.
+
+G2 (end to end): the reference binary xtax_ref is run on the reference's sample
+input (10x10.csv, stored here as data) and on small planted forms, single
+threaded with fixed seeds. Runs that stop on a diagonal are deterministic and
+record moves and the final best_A/best_X; every run's (A0, X, A) triple is also
+kept as a verifier fixture.
+"""
+import ctypes as C
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+OUT = os.path.dirname(os.path.abspath(__file__))
+REF_DIR = os.path.join(ROOT, "oracle", "_ref")
+I64P = C.POINTER(C.c_int64)
+LIM = 1 << 48
+
+
+def load_harness():
+    L = C.CDLL(os.path.join(REF_DIR, "libref_harness.so"))
+    L.ref_rounded_div.restype = C.c_int64
+    L.ref_rounded_div.argtypes = [C.c_int64, C.c_int64]
+    L.ref_score.restype = C.c_int64
+    L.ref_score.argtypes = [I64P, C.c_int]
+    L.ref_offdiag_nonzero.restype = C.c_int64
+    L.ref_offdiag_nonzero.argtypes = [I64P, C.c_int]
+    L.ref_try_add.restype = C.c_int
+    L.ref_try_add.argtypes = [I64P, I64P, C.c_int, C.c_int, C.c_int, C.c_int64, I64P, I64P,
+                              I64P, I64P]
+    L.ref_add_transform_row.restype = C.c_int
+    L.ref_add_transform_row.argtypes = [I64P, C.c_int, C.c_int, C.c_int, C.c_int64, I64P]
+    L.ref_commit_add.restype = None
+    L.ref_commit_add.argtypes = [I64P, I64P, C.c_int, C.c_int, I64P, I64P, I64P]
+    L.ref_compute_row_mass.restype = None
+    L.ref_compute_row_mass.argtypes = [I64P, C.c_int, I64P]
+    L.ref_score_delta_l1.restype = C.c_int64
+    L.ref_score_delta_l1.argtypes = [I64P, C.c_int, C.c_int, C.c_int, C.c_int64]
+    L.ref_best_shear_l1.restype = C.c_int64
+    L.ref_best_shear_l1.argtypes = [I64P, C.c_int, C.c_int, C.c_int]
+    L.ref_congruence_of.restype = None
+    L.ref_congruence_of.argtypes = [I64P, I64P, C.c_int, I64P]
+    L.ref_gram_of.restype = None
+    L.ref_gram_of.argtypes = [I64P, C.c_int, C.c_int, I64P]
+    L.ref_xt_times.restype = None
+    L.ref_xt_times.argtypes = [I64P, I64P, C.c_int, C.c_int, I64P]
+    return L
+
+
+def P(a):
+    return a.ctypes.data_as(I64P)
+
+
+def sym(M):
+    U = np.triu(M)
+    return U + np.triu(U, 1).T
+
+
+def gen_matrix(rng, n, kind):
+    """Symmetric int64 test matrices of several shapes."""
+    if kind == "small":
+        A = rng.integers(-20, 21, (n, n))
+    elif kind == "sparse":
+        A = rng.integers(-5, 6, (n, n)) * (rng.random((n, n)) < 0.3)
+    elif kind == "big":
+        A = rng.integers(-(1 << 47), 1 << 47, (n, n))
+    elif kind == "edge":
+        A = rng.integers(-(1 << 20), 1 << 20, (n, n))
+        k = rng.integers(0, n, max(1, n // 3))
+        A[k, :] = rng.choice([LIM - 1, LIM, -LIM, -(LIM - 1), LIM >> 1, 3], size=(len(k), n))
+    elif kind == "median":
+        # pivot rows of +/-1, +/-2, +/-3 and target rows with large multiples:
+        # breakpoints spread far beyond the +/-64 shear cap, ties common.
+        A = rng.integers(-3, 4, (n, n)) * rng.integers(1, 400, (n, n))
+        A[rng.random((n, n)) < 0.2] = 0
+    elif kind == "diag":
+        A = np.diag(rng.integers(-50, 51, n))
+    elif kind == "clamp":
+        # Gram of rows that are large multiples of one another: breakpoints far
+        # outside +/-64, so the best shear clamps to the cap.
+        L = rng.integers(-2, 3, (n, n))
+        mult = rng.integers(70, 300, n) * rng.choice([-1, 1], n)
+        mult[0] = 1
+        L = L + mult[:, None] * L[0][None, :] * (rng.random(n) < 0.7)[:, None]
+        return np.asarray(L @ L.T, dtype=np.int64)
+    else:
+        raise ValueError(kind)
+    return sym(np.asarray(A, dtype=np.int64))
+
+
+def gen_xt(rng, n, kind):
+    if kind == "eye":
+        return np.eye(n, dtype=np.int64)
+    if kind == "small":
+        return rng.integers(-3, 4, (n, n)).astype(np.int64)
+    X = rng.integers(-(1 << 46), 1 << 46, (n, n)).astype(np.int64)
+    X[rng.integers(0, n), :] = LIM - 2
+    return X
+
+
+def checksum(M):
+    """Weighted sum mod 2^61-1 of all entries, for large-n commit outputs."""
+    p = (1 << 61) - 1
+    acc = 0
+    for t, v in enumerate(np.asarray(M).ravel().tolist()):
+        acc = (acc + (t + 1) * (int(v) % p)) % p
+    return acc
+
+
+def g1(L):
+    rng = np.random.default_rng(20261203)
+    sizes = list(range(2, 13)) + [24, 48, 64, 80]
+    kinds = ["clamp", "median", "small", "sparse", "big", "edge", "diag"]
+    out = {}
+    for n in sizes:
+        nmat = 12 if n <= 12 else (6 if n <= 48 else 3)
+        ncase = 6 if n <= 12 else 4
+        As, Xts, cases = [], [], []
+        res = {k: [] for k in ["ok", "delta", "dnz", "row", "okx", "deltax", "dnzx", "rowx",
+                               "xrow", "okt", "trow", "sdl", "best", "mass0", "masspost",
+                               "Apost", "Xtpost", "cks"]}
+        for mi in range(nmat):
+            kind = kinds[mi % len(kinds)]
+            A = gen_matrix(rng, n, kind)
+            Xt = gen_xt(rng, n, ["eye", "small", "big"][mi % 3])
+            As.append(A)
+            Xts.append(Xt)
+            mass0 = np.zeros(n, np.int64)
+            L.ref_compute_row_mass(P(A), n, P(mass0))
+            for ci in range(ncase):
+                i = int(rng.integers(0, n))
+                j = int(rng.integers(0, n - 1))
+                j += j >= i
+                best = L.ref_best_shear_l1(P(A), n, i, j)
+                choice = ci % 4
+                if choice == 0:
+                    s = best
+                elif choice == 1:
+                    s = int(rng.choice([-1, 1]))
+                elif choice == 2:
+                    s = int(rng.integers(-64, 65))
+                else:
+                    s = int(rng.choice([64, -64, 2, -2, 0]))
+                cases.append((mi, i, j, s))
+                row = np.zeros(n, np.int64)
+                d = C.c_int64(0)
+                dz = C.c_int64(0)
+                ok = L.ref_try_add(P(A), P(Xt), n, i, j, s, P(row), None, C.byref(d), C.byref(dz))
+                rowx = np.zeros(n, np.int64)
+                xrow = np.zeros(n, np.int64)
+                dx = C.c_int64(0)
+                dzx = C.c_int64(0)
+                okx = L.ref_try_add(P(A), P(Xt), n, i, j, s, P(rowx), P(xrow), C.byref(dx),
+                                    C.byref(dzx))
+                trow = np.zeros(n, np.int64)
+                okt = L.ref_add_transform_row(P(Xt), n, i, j, s, P(trow))
+                sdl = L.ref_score_delta_l1(P(A), n, i, j, s)
+                Apost = A.copy()
+                Xtpost = Xt.copy()
+                mpost = mass0.copy()
+                if okx:
+                    L.ref_commit_add(P(Apost), P(Xtpost), n, j, P(rowx), P(xrow), P(mpost))
+                for k, v in [("ok", ok), ("delta", d.value), ("dnz", dz.value), ("row", row),
+                             ("okx", okx), ("deltax", dx.value), ("dnzx", dzx.value),
+                             ("rowx", rowx), ("xrow", xrow), ("okt", okt), ("trow", trow),
+                             ("sdl", sdl), ("best", best), ("mass0", mass0),
+                             ("masspost", mpost)]:
+                    res[k].append(v)
+                if n <= 24:
+                    res["Apost"].append(Apost)
+                    res["Xtpost"].append(Xtpost)
+                res["cks"].append([checksum(Apost), checksum(Xtpost)])
+            # exhaustive best_shear over all (i, j) pairs for small n
+        out[f"n{n}_A"] = np.array(As)
+        out[f"n{n}_Xt"] = np.array(Xts)
+        out[f"n{n}_cases"] = np.array(cases, np.int64)
+        for k, v in res.items():
+            if v:
+                out[f"n{n}_{k}"] = np.array(v, dtype=np.int64)
+        # best_shear for every ordered pair of the first two matrices
+        allb = []
+        for mi in range(min(2, nmat)):
+            A = As[mi]
+            allb.append([[L.ref_best_shear_l1(P(A), n, i, j) if i != j else 0
+                          for j in range(n)] for i in range(n)])
+        out[f"n{n}_best_all"] = np.array(allb, np.int64)
+        # score and nonzero count of every matrix
+        out[f"n{n}_score"] = np.array([L.ref_score(P(A), n) for A in As], np.int64)
+        out[f"n{n}_nnz"] = np.array([L.ref_offdiag_nonzero(P(A), n) for A in As], np.int64)
+    np.savez_compressed(os.path.join(OUT, "g1_primitives.npz"), **out)
+
+    # rounded_div: signs, exact ties, remainders either side of a half.
+    vals = [0, 1, 2, 3, 4, 5, 6, 7, 9, 10, 11, 13, 1000001, (1 << 48) - 1, 1 << 48]
+    rd = []
+    for a in vals:
+        for b in vals[1:]:
+            for sa in (1, -1):
+                for sb in (1, -1):
+                    rd.append((sa * a, sb * b, L.ref_rounded_div(sa * a, sb * b)))
+    np.savez_compressed(os.path.join(OUT, "g1_rounded_div.npz"), cases=np.array(rd, np.int64))
+
+    # lattice helpers and congruence_of
+    lat = {}
+    for t, (m, d) in enumerate([(3, 3), (4, 6), (6, 4), (24, 24)]):
+        Lm = rng.integers(-9, 10, (m, d)).astype(np.int64)
+        G = np.zeros((m, m), np.int64)
+        L.ref_gram_of(P(Lm), m, d, P(G))
+        X = rng.integers(-2, 3, (m, m)).astype(np.int64)
+        R = np.zeros((m, d), np.int64)
+        L.ref_xt_times(P(X), P(Lm), m, d, P(R))
+        A = gen_matrix(rng, m, "small")
+        XAX = np.zeros((m, m), np.int64)
+        L.ref_congruence_of(P(A), P(X), m, P(XAX))
+        lat.update({f"c{t}_L": Lm, f"c{t}_G": G, f"c{t}_X": X, f"c{t}_R": R, f"c{t}_A": A,
+                    f"c{t}_XAX": XAX})
+    np.savez_compressed(os.path.join(OUT, "g1_lattice.npz"), **lat)
+
+
+def planted(seed, n, dmax, nshear):
+    """Small planted form U^T D U (Python ints), as the fixtures use."""
+    rng = np.random.default_rng(seed)
+    d = [int(v) for v in rng.integers(1, dmax + 1, n)]
+    d = [v if rng.integers(0, 2) else -v for v in d]
+    U = [[int(r == c) for c in range(n)] for r in range(n)]
+    for _ in range(nshear):
+        i = int(rng.integers(0, n))
+        j = int(rng.integers(0, n - 1))
+        j += j >= i
+        s = 1 if rng.integers(0, 2) else -1
+        for r in range(n):
+            U[r][j] += s * U[r][i]
+    A = [[sum(U[k][r] * d[k] * U[k][c] for k in range(n)) for c in range(n)] for r in range(n)]
+    return np.array(A, np.int64)
+
+
+def read_csv(path):
+    rows = [r for r in open(path).read().splitlines() if r.strip()]
+    return np.array([[int(x) for x in r.split(",")] for r in rows], np.int64)
+
+
+def run_ref(A, args, timeout=60):
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "A.csv")
+        with open(path, "w") as f:
+            for row in A:
+                f.write(",".join(str(int(v)) for v in row) + "\n")
+        cp = subprocess.run([os.path.join(REF_DIR, "xtax_ref"), "-A", path] + args, cwd=td,
+                            capture_output=True, text=True, timeout=timeout)
+        moves = None
+        solved = "matrix is diagonal" in cp.stdout
+        for line in cp.stdout.splitlines():
+            if line.startswith("done moves="):
+                moves = int(line.split()[1].split("=")[1])
+        return (solved, moves, read_csv(os.path.join(td, "best_A.csv")),
+                read_csv(os.path.join(td, "best_X.csv")), cp.stdout)
+
+
+def g2():
+    A10 = read_csv("/root/reference/10x10.csv")
+    runs = []
+    cases = [("10x10", A10, ["-t", "1", "--seed", str(s), "--max-seconds", "20"])
+             for s in (1, 2, 3, 4)]
+    for s in (11, 12, 13):
+        for n in (4, 5, 6):
+            cases.append((f"planted{n}_{s}", planted(s, n, 9, 2 * n),
+                          ["-t", "1", "--seed", str(s), "--max-seconds", "5"]))
+    cases.append(("10x10_t4", A10, ["-t", "4", "--seed", "5", "--max-seconds", "10"]))
+    cases.append(("planted8_t4", planted(7, 8, 100, 24),
+                  ["-t", "4", "--seed", "5", "--max-seconds", "5", "--t-init", "200",
+                   "--t-min", "5"]))
+    out = {}
+    meta = []
+    for t, (name, A0, args) in enumerate(cases):
+        solved, moves, bA, bX, _ = run_ref(A0, args)
+        det = solved and "-t" in args and args[args.index("-t") + 1] == "1"
+        meta.append((t, int(solved), int(moves or -1), int(det)))
+        out[f"r{t}_A0"] = A0
+        out[f"r{t}_A"] = bA
+        out[f"r{t}_X"] = bX
+        print(f"G2 {name}: solved={solved} moves={moves} deterministic={det}", file=sys.stderr)
+    out["meta"] = np.array(meta, np.int64)
+    out["names"] = np.array([c[0] for c in cases])
+    out["args"] = np.array([" ".join(c[2]) for c in cases])
+    np.savez_compressed(os.path.join(OUT, "g2_end_to_end.npz"), **out)
+
+
+if __name__ == "__main__":
+    L = load_harness()
+    g1(L)
+    g2()
+    for f in sorted(os.listdir(OUT)):
+        if f.endswith(".npz"):
+            print(f, os.path.getsize(os.path.join(OUT, f)), file=sys.stderr)

@@ -1,0 +1,344 @@
+"""The oracle against the reference's golden vectors (G1/G2, generated from the
+reference's own code by tests/golden/gen_golden.py) and against independent
+closed forms: P^T A P with Python integers, brute force over every shear, exact
+rational medians, Random123's Philox known answers."""
+import math
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+LIM = 1 << 48
+
+
+def load(name):
+    return np.load(os.path.join(GOLD, name), allow_pickle=False)
+
+
+def sizes(g):
+    return sorted({int(k.split("_")[0][1:]) for k in g.files if k.startswith("n")})
+
+
+# ---------------------------------------------------------------- G1 vectors
+def test_primitives_match_reference(orc):
+    g = load("g1_primitives.npz")
+    ncases = 0
+    for n in sizes(g):
+        As, Xts, cases = g[f"n{n}_A"], g[f"n{n}_Xt"], g[f"n{n}_cases"]
+        for t, (mi, i, j, s) in enumerate(cases.tolist()):
+            A, Xt = As[mi], Xts[mi]
+            ok, d, dnz, row, _ = orc.try_add(A, Xt, i, j, s)
+            assert ok == bool(g[f"n{n}_ok"][t]), (n, t)
+            if ok:
+                assert d == g[f"n{n}_delta"][t] and dnz == g[f"n{n}_dnz"][t], (n, t)
+                assert np.array_equal(row, g[f"n{n}_row"][t]), (n, t)
+            okx, dx, dnzx, rowx, xrow = orc.try_add(A, Xt, i, j, s, with_xt=True)
+            assert okx == bool(g[f"n{n}_okx"][t]), (n, t)
+            if okx:
+                assert dx == g[f"n{n}_deltax"][t] and dnzx == g[f"n{n}_dnzx"][t]
+                assert np.array_equal(rowx, g[f"n{n}_rowx"][t])
+                assert np.array_equal(xrow, g[f"n{n}_xrow"][t])
+            okt, trow = orc.add_transform_row(Xt, i, j, s)
+            assert okt == bool(g[f"n{n}_okt"][t])
+            if okt:
+                assert np.array_equal(trow, g[f"n{n}_trow"][t])
+            assert orc.score_delta_l1(A, i, j, s) == g[f"n{n}_sdl"][t], (n, t)
+            assert orc.best_shear_l1(A, i, j) == g[f"n{n}_best"][t], (n, t)
+            mass0 = orc.compute_row_mass(A)
+            assert np.array_equal(mass0, g[f"n{n}_mass0"][t])
+            if okx:
+                Ap, Xp, mp = orc.commit_add(A, Xt, j, rowx, xrow, mass0)
+                assert np.array_equal(mp, g[f"n{n}_masspost"][t])
+                if f"n{n}_Apost" in g.files:
+                    assert np.array_equal(Ap, g[f"n{n}_Apost"][t])
+                    assert np.array_equal(Xp, g[f"n{n}_Xtpost"][t])
+                from tests.golden.gen_golden import checksum
+                assert [checksum(Ap), checksum(Xp)] == g[f"n{n}_cks"][t].tolist()
+            ncases += 1
+        for mi, tab in enumerate(g[f"n{n}_best_all"]):
+            for i in range(n):
+                for j in range(n):
+                    if i != j:
+                        assert orc.best_shear_l1(As[mi], i, j) == tab[i][j], (n, mi, i, j)
+        for mi, A in enumerate(As):
+            assert orc.score(A) == g[f"n{n}_score"][mi]
+            assert orc.offdiag_nonzero(A) == g[f"n{n}_nnz"][mi]
+    assert ncases > 200
+
+
+def test_golden_covers_bounds():
+    """The fixtures contain both outcomes of the strict 2^48 bound check."""
+    g = load("g1_primitives.npz")
+    oks = np.concatenate([g[f"n{n}_ok"] for n in sizes(g)])
+    okt = np.concatenate([g[f"n{n}_okt"] for n in sizes(g)])
+    assert oks.min() == 0 and oks.max() == 1
+    assert okt.min() == 0 and okt.max() == 1
+    nz = np.concatenate([g[f"n{n}_best"] for n in sizes(g)])
+    assert (nz == 64).any() or (nz == -64).any()
+
+
+def test_rounded_div_matches_reference(orc):
+    for a, b, q in load("g1_rounded_div.npz")["cases"].tolist():
+        assert orc.rounded_div(a, b) == q, (a, b)
+
+
+def test_rounded_div_closed_form(orc):
+    # nearest integer, ties away from zero, checked with exact rationals
+    for a in range(-40, 41):
+        for b in [v for v in range(-9, 10) if v]:
+            x = Fraction(a, b)
+            lo = math.floor(x)
+            frac = x - lo
+            want = lo + 1 if frac > Fraction(1, 2) or (frac == Fraction(1, 2) and x > 0) else lo
+            assert orc.rounded_div(a, b) == want, (a, b)
+
+
+def test_lattice_helpers_match_reference(orc):
+    g = load("g1_lattice.npz")
+    for t in range(4):
+        L, X, A = g[f"c{t}_L"], g[f"c{t}_X"], g[f"c{t}_A"]
+        assert np.array_equal(orc.gram_of(L), g[f"c{t}_G"])
+        assert np.array_equal(orc.xt_times(X, L), g[f"c{t}_R"])
+        assert np.array_equal(orc.congruence_of(A, X), g[f"c{t}_XAX"])
+        # closed forms
+        assert np.array_equal(g[f"c{t}_G"], L @ L.T)
+        assert np.array_equal(g[f"c{t}_R"], X.T @ L)
+        assert np.array_equal(g[f"c{t}_XAX"], X.T @ A @ X)
+
+
+# ------------------------------------------------------------- closed forms
+def py_score(M):
+    n = len(M)
+    return 2 * sum(abs(v) for r in M for v in r) - sum(abs(M[i][i]) for i in range(n))
+
+
+def py_nnz(M):
+    n = len(M)
+    return sum(1 for i in range(n) for j in range(i + 1, n) if M[i][j])
+
+
+def py_shear(M, i, j, s):
+    """P^T M P for P = I + s e_i e_j^T, Python integers."""
+    n = len(M)
+    C = [row[:] for row in M]
+    for r in range(n):
+        C[r][j] += s * C[r][i]
+    R = [row[:] for row in C]
+    for c in range(n):
+        R[j][c] += s * R[i][c]
+    return R
+
+
+def test_try_add_equals_congruence(orc):
+    rng = np.random.default_rng(5)
+    for trial in range(300):
+        n = int(rng.integers(2, 9))
+        A = rng.integers(-30, 31, (n, n))
+        A = np.triu(A) + np.triu(A, 1).T
+        i = int(rng.integers(0, n))
+        j = int(rng.integers(0, n - 1))
+        j += j >= i
+        s = int(rng.integers(-64, 65))
+        ok, d, dnz, row, _ = orc.try_add(A, np.eye(n, dtype=np.int64), i, j, s)
+        M = A.tolist()
+        R = py_shear(M, i, j, s)
+        feasible = all(abs(v) <= LIM for r in R for v in r)
+        assert ok == feasible
+        if ok:
+            assert row.tolist() == R[j]
+            assert d == py_score(R) - py_score(M)
+            assert dnz == py_nnz(R) - py_nnz(M)
+            assert orc.score_delta_l1(A, i, j, s) == d
+
+
+def test_bound_is_strict(orc):
+    # an entry landing on exactly 2^48 is allowed, 2^48 + 1 is rejected
+    n = 3
+    for target, ok_want in [(LIM, True), (LIM + 1, False), (-LIM, True), (-LIM - 1, False)]:
+        A = np.zeros((n, n), np.int64)
+        A[0, 0] = 1
+        A[0, 2] = A[2, 0] = 1
+        A[1, 2] = A[2, 1] = target - 1
+        ok, *_ = orc.try_add(A, np.eye(n, dtype=np.int64), 0, 1, 1)
+        assert ok == ok_want, target
+
+
+def exact_best_shear(M, i, j):
+    """best_shear_l1 with exact rationals instead of doubles."""
+    n = len(M)
+    keys = [(Fraction(-M[j][k], M[i][k]), abs(M[i][k])) for k in range(n)
+            if k != j and M[i][k] != 0]
+    total = sum(w for _, w in keys)
+    med = Fraction(0)
+    if keys:
+        keys.sort(key=lambda kv: kv[0])
+        acc = 0
+        med = keys[-1][0]
+        for b, w in keys:
+            acc += w
+            if 2 * acc >= total:
+                med = b
+                break
+    cands = []
+
+    def push(v):
+        v = max(-64, min(64, v))
+        if v not in cands:
+            cands.append(v)
+    push(math.floor(med))
+    push(math.ceil(med))
+    if M[i][i] != 0:
+        a, b = M[j][i], M[i][i]
+        q0 = Fraction(a, b)
+        fl = math.floor(q0)
+        frac = q0 - fl
+        rq = fl + 1 if frac > Fraction(1, 2) or (frac == Fraction(1, 2) and q0 > 0) else fl
+        q = -rq
+        for v in (q, q + 1, q - 1):
+            push(v)
+    best_s, best_d = 0, 0
+    for s in cands:
+        if s == 0:
+            continue
+        R = py_shear(M, i, j, s)
+        if any(abs(R[j][k]) > LIM for k in range(n)):
+            continue
+        d = py_score(R) - py_score(M)
+        if d < best_d:
+            best_s, best_d = s, d
+    return best_s
+
+
+def test_best_shear_exact_rational(orc):
+    rng = np.random.default_rng(9)
+    for trial in range(400):
+        n = int(rng.integers(2, 10))
+        scale = int(rng.choice([3, 50, 5000]))
+        A = rng.integers(-scale, scale + 1, (n, n)) * (rng.random((n, n)) < 0.8)
+        A = np.triu(A) + np.triu(A, 1).T
+        M = A.tolist()
+        for _ in range(3):
+            i = int(rng.integers(0, n))
+            j = int(rng.integers(0, n - 1))
+            j += j >= i
+            assert orc.best_shear_l1(A, i, j) == exact_best_shear(M, i, j), (M, i, j)
+
+
+def test_best_shear_never_worse_than_brute_force_sign(orc):
+    """A nonzero best shear strictly lowers the score; zero means no candidate
+    helps (the brute-force minimum over all s may still be lower: the method
+    only tries the median and quotient neighbourhoods)."""
+    rng = np.random.default_rng(11)
+    for trial in range(200):
+        n = int(rng.integers(2, 8))
+        A = rng.integers(-40, 41, (n, n))
+        A = np.triu(A) + np.triu(A, 1).T
+        i = int(rng.integers(0, n))
+        j = int(rng.integers(0, n - 1))
+        j += j >= i
+        s = orc.best_shear_l1(A, i, j)
+        if s:
+            assert orc.score_delta_l1(A, i, j, s) < 0
+
+
+# ----------------------------------------------------------------- Philox/exp
+def test_philox_known_answers(orc):
+    # Random123 kat_vectors, philox4x32 with 10 rounds.
+    kat = [
+        ([0, 0, 0, 0], [0, 0], [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]),
+        ([0xffffffff] * 4, [0xffffffff] * 2, [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]),
+        ([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0],
+         [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]),
+    ]
+    for ctr, key, want in kat:
+        assert orc.philox(ctr, key) == want
+
+
+def test_xexp_accuracy(orc):
+    for x in np.concatenate([-np.logspace(-12, np.log10(39.9), 400), [0.0, -1.0, -0.5]]):
+        assert abs(orc.xexp(x) - math.exp(x)) <= 4e-16 * math.exp(x) + 1e-300, x
+    assert orc.xexp(0.0) == 1.0
+    assert orc.xexp(-40.0) == 0.0 and orc.xexp(-1e9) == 0.0
+
+
+# ------------------------------------------------------------------- verify
+def test_verify_reference_end_to_end_outputs(orc):
+    g = load("g2_end_to_end.npz")
+    for t, solved, moves, det in g["meta"].tolist():
+        A0, A, X = g[f"r{t}_A0"], g[f"r{t}_A"], g[f"r{t}_X"]
+        status, sign, _ = orc.verify(A0, X, A)
+        want = 0 if solved else 2
+        assert status == want, (t, status)
+        assert sign in (1, -1)
+
+
+def test_verify_known_answer_10x10(orc):
+    """The reference's sample (E8 + <1> + <-1>) is odd indefinite unimodular of
+    signature (9,1): every diagonal form reached is nine +1 and one -1."""
+    g = load("g2_end_to_end.npz")
+    names = g["names"].tolist()
+    for t, name in enumerate(names):
+        if name.startswith("10x10") and g["meta"][t][1]:
+            d = np.diag(g[f"r{t}_A"])
+            assert sorted(d.tolist()) == [-1] + [1] * 9
+
+
+def test_verify_catches_planted_mistakes(orc):
+    g = load("g2_end_to_end.npz")
+    A0, A, X = g["r0_A0"], g["r0_A"], g["r0_X"]
+    Xb = X.copy()
+    Xb[0, 0] += 1
+    assert orc.verify(A0, Xb, A)[0] & 1
+    Ab = A.copy()
+    Ab[0, 1] = Ab[1, 0] = 1
+    assert orc.verify(A0, X, Ab)[0] & 3 == 3
+    # a congruence by a non-unimodular X: X = diag(2,1,...) keeps A = X^T A0 X
+    n = A0.shape[0]
+    X2 = np.eye(n, dtype=np.int64)
+    X2[0, 0] = 2
+    st, _, _ = orc.verify(A0, X2, X2.T @ A0 @ X2)
+    assert st & 4 and not st & 1
+
+
+def test_verify_large_entries_needs_several_primes(orc):
+    rng = np.random.default_rng(3)
+    n = 6
+    X = np.eye(n, dtype=np.int64)
+    for _ in range(40):
+        i, j = rng.choice(n, 2, replace=False)
+        X[:, j] += int(rng.integers(-64, 65)) * X[:, i]
+        if np.abs(X).max() > (1 << 40):
+            break
+    A0 = np.diag(rng.integers(1, 1000, n)).astype(np.int64)
+    Xi = [[int(v) for v in r] for r in X.tolist()]
+    A = [[sum(Xi[k][r] * int(A0[k][k]) * Xi[k][c] for k in range(n)) for c in range(n)]
+         for r in range(n)]
+    if max(abs(v) for r in A for v in r) < (1 << 62):
+        st, sign, primes = orc.verify(A0, X, np.array(A, np.int64))
+        assert st & 1 == 0 and st & 4 == 0 and sign == 1
+        assert primes >= 2
+
+
+def test_replay_equals_congruence(orc):
+    rng = np.random.default_rng(21)
+    n = 7
+    A = rng.integers(-9, 10, (n, n))
+    A0 = np.triu(A) + np.triu(A, 1).T
+    M = A0.tolist()
+    log = np.zeros(50, orc.LOG_DTYPE)
+    X = [[int(r == c) for c in range(n)] for r in range(n)]
+    for t in range(50):
+        i = int(rng.integers(0, n))
+        j = int(rng.integers(0, n - 1))
+        j += j >= i
+        s = int(rng.choice([-2, -1, 1, 2]))
+        log[t] = (t + 1, i, j, s, 0)
+        M = py_shear(M, i, j, s)
+        for r in range(n):
+            X[r][j] += s * X[r][i]
+    k, A, Xt = orc.replay(A0, np.eye(n, dtype=np.int64), log)
+    assert k == 50
+    assert A.tolist() == M
+    assert Xt.T.tolist() == X
