@@ -1,0 +1,243 @@
+/* xtx.h -- C ABI of the MI355X (gfx950) engine for the xtax L1 congruence
+ * annealer.
+ *
+ * The engine replaces the reference's CPU annealer for the L1 objective,
+ * `canneal::run_annealer<L1Objective>(...)` (src/congruence_anneal.hpp:421-423)
+ * at its xtax call sites (src/main.cpp:716-718 A mode, src/main.cpp:844-846
+ * L mode). Where the reference passes a policy object by value and returns the
+ * best one by value, this boundary takes caller-owned host buffers (row-major
+ * n x n int64) and writes the best A and X back into caller-owned buffers.
+ *
+ * Conventions: every function returns XTX_OK (0) or a negative XTX_E* code;
+ * no exception crosses the boundary; the message of the last failure on a
+ * context is available from xtx_last_error(). The library writes no files.
+ * Calls on one context are not thread-safe except xtx_request_stop(), which
+ * may be called from a signal handler.
+ */
+#ifndef XTX_H
+#define XTX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XTX_OK 0
+#define XTX_E_ARG -1        /* invalid argument (shape, parameter range) */
+#define XTX_E_BOUND -2      /* an input entry exceeds the 2^48 magnitude bound */
+#define XTX_E_DEVICE -3     /* HIP runtime failure */
+#define XTX_E_NOMEM -4      /* device or host allocation failed */
+#define XTX_E_STATE -5      /* call not valid in the current state */
+#define XTX_E_FULL -6       /* serve queue full */
+#define XTX_E_VERIFY -7     /* a result failed exact host verification */
+
+#define XTX_MAGNITUDE_LIMIT (((int64_t)1) << 48) /* mat_io.hpp:35 */
+#define XTX_SHEAR_CAP 64                          /* mat_io.hpp:38 */
+#define XTX_MAX_N 128                             /* largest order per replica */
+#define XTX_MAX_TARGET_SAMPLES 16
+
+typedef struct xtx_ctx xtx_ctx;
+
+/* Engine parameters. The first block mirrors canneal::EngineParams
+ * (src/congruence_anneal.hpp:122-153) field for field with the same defaults
+ * (see xtx_params_default); the second block is engine-specific. */
+typedef struct {
+    double t_init;            /* <= 0: auto, max(1, 2*score/n^2) (main.cpp:383-387) */
+    double t_min;             /* 1e-3: cooling floor and ladder bottom */
+    double cooling;           /* 0.999 */
+    double reheat;            /* 1.0 */
+    double reseed_factor;     /* 1.25 */
+    double greedy_fraction;   /* 0.5 */
+    double target_fraction;   /* 0.5 */
+    double target_accept;     /* 0.44 */
+    double worker_diversity;  /* 0.4 */
+    double sweep_fraction;    /* 0.0 */
+    double max_seconds;       /* 0: run until solved (or max_moves / max_epochs) */
+    double save_interval;     /* 2.0 (used by the CLI for best_*.csv throttling) */
+    int32_t moves_per_cool;   /* 200 */
+    int32_t stuck_threshold;  /* 20000 */
+    int32_t target_samples;   /* 8, at most XTX_MAX_TARGET_SAMPLES */
+    int32_t tempering;        /* 1 (needs replicas >= 2, as threads >= 2) */
+    int32_t adaptive_cooling; /* 0 */
+    int32_t exchange_interval;/* 2000: moves per replica between exchange sweeps */
+    uint64_t seed;            /* Philox key; 0 is a valid key (no random_device) */
+    /* engine-specific */
+    int32_t replicas;         /* R: tempering rungs / independent workers (-t) */
+    int32_t move_log;         /* 1: record every accepted move of every replica */
+    int64_t move_log_cap;     /* entries kept per replica when move_log = 1 */
+    uint64_t max_moves;       /* per-replica proposal cap; 0 = none */
+    int32_t max_epochs;       /* 0 = none */
+    int32_t verify;           /* 1: exact host verification of every result */
+    int32_t waves_per_block;  /* 0 = autotuned */
+    int32_t reserved0;
+} xtx_params;
+
+typedef struct {
+    int32_t compute_units;
+    int32_t lds_bytes_per_cu;
+    int32_t lds_bytes_per_block_max;
+    int32_t wave_size;
+    int32_t max_n;            /* XTX_MAX_N */
+    int32_t max_n_lds;        /* largest n whose replica state fits one block */
+    int64_t hbm_bytes;
+    char arch[64];
+} xtx_limits;
+
+typedef struct {
+    int64_t moves;            /* proposals executed, all replicas */
+    int64_t accepts;          /* committed moves (incl. sweep commits) */
+    int64_t greedy_calls;     /* best-shear computations */
+    int64_t infeasible;       /* proposals rejected by the magnitude bound */
+    int64_t sweeps;           /* stall reduction sweeps run */
+    int64_t exchanges_tried;
+    int64_t exchanges_accepted;
+    int32_t solved;           /* a replica reached a diagonal */
+    int32_t solve_replica;
+    int64_t solve_move;       /* that replica's move index at the diagonal */
+    int64_t best_score;       /* score of the returned A */
+    int32_t best_replica;
+    int32_t epochs;
+    int32_t stop_reason;      /* 0 solved, 1 max_seconds, 2 stop request,
+                                 3 max_moves/max_epochs, 4 trivial (n < 2) */
+    int32_t verify_status;    /* 0 ok; bits as in xtx_verify_report.status; -1 not run */
+    double seconds;           /* wall time of the call */
+    double kernel_ms;         /* summed anneal-kernel time (HIP events) */
+} xtx_stats;
+
+/* One matrix for the batch and serve forms. A0 is the input matrix; X0 (null =
+ * identity) is a start transform, so the engine starts from X0^T A0 X0 exactly
+ * as `xtax -A A.csv -X X.csv` does (main.cpp:709-712). Results are written to
+ * A_out / X_out (n x n, caller-owned; either may be null). */
+typedef struct {
+    const int64_t* A0;
+    const int64_t* X0;
+    int32_t n;
+    int32_t replicas;         /* 0 = params.replicas */
+    uint64_t seed;            /* Philox key of this instance */
+    const int32_t* active;    /* null = all indices */
+    int32_t n_active;
+    int32_t reserved;
+    int64_t* A_out;
+    int64_t* X_out;
+} xtx_instance;
+
+typedef struct {
+    int32_t solved;
+    int32_t verify_status;    /* 0 ok, -1 not run, else xtx_verify_report bits */
+    int64_t solve_move;
+    int64_t best_score;
+    int64_t moves;            /* proposals executed for this instance */
+    double submit_s;          /* times relative to the batch start / serve open */
+    double admit_s;
+    double retire_s;
+    uint64_t tag;
+} xtx_result;
+
+typedef struct {
+    int64_t moves;
+    int64_t accepts;
+    int32_t instances;
+    int32_t solved;
+    int32_t verified;
+    int32_t epochs;
+    double seconds;
+    double kernel_ms;
+    int32_t buckets;
+    int32_t graph_epochs;     /* epochs per captured graph launch */
+} xtx_batch_stats;
+
+typedef struct {
+    int32_t status;           /* 1: X^T A0 X != A, 2: A not diagonal,
+                                 4: |det X| != 1, 8: undecided within budget */
+    int32_t primes_used;
+    int32_t det_sign;
+    int32_t exact_int128;     /* 1 if the int128 direct product was used */
+} xtx_verify_report;
+
+/* One accepted move (16 bytes): the proposal index of the replica, pivot i,
+ * target j, shear s; flags bit 0 marks a stall reduction-sweep commit. */
+typedef struct {
+    int64_t move;
+    int16_t i, j, s;
+    uint16_t flags;
+} xtx_log_entry;
+
+/* ---- context ---- */
+int xtx_open(int device, xtx_ctx** out);
+int xtx_close(xtx_ctx* ctx);
+const char* xtx_last_error(const xtx_ctx* ctx); /* ctx may be null: open errors */
+int xtx_query_limits(xtx_ctx* ctx, xtx_limits* out);
+void xtx_params_default(xtx_params* p);
+/* Async-signal-safe: asks running solves to stop at the next poll point
+ * (the reference's g_stop, stop_signal.hpp:40-64). */
+void xtx_request_stop(xtx_ctx* ctx);
+void xtx_clear_stop(xtx_ctx* ctx);
+
+/* ---- plain form: one instance per launch ----
+ * Replaces run_annealer(L1Objective(X0^T A0 X0, X0), params, active)
+ * (congruence_anneal.hpp:421-569 via main.cpp:712-718). Writes the best (or
+ * the solving) state to A_out / X_out. active (nullable) restricts pivots and
+ * targets to those indices (the reference's `active`, :437-443). */
+int xtx_solve_plain(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t n,
+                    const xtx_params* params, const int32_t* active, int32_t n_active,
+                    int64_t* A_out, int64_t* X_out, xtx_stats* stats);
+
+/* Per-replica state of the last plain solve (for replay and trajectory checks):
+ * A, Xt (R*n*n each, nullable) and 8 scalars per replica
+ * {score, offnz, moves, best_score, best_move, rung, accepts, local_best}. */
+int xtx_last_replicas(xtx_ctx* ctx, int64_t* A, int64_t* Xt, int64_t* scalars);
+
+/* Move log of replica r of the last plain solve; the pointer stays valid until
+ * xtx_movelog_free or the next solve. *truncated is set when the cap was hit. */
+int xtx_movelog_get(xtx_ctx* ctx, int32_t replica, const xtx_log_entry** entries,
+                    int64_t* count, int32_t* truncated);
+int xtx_movelog_free(xtx_ctx* ctx);
+
+/* ---- batched form: many instances of uneven n in one captured launch ---- */
+int xtx_batch_solve(xtx_ctx* ctx, const xtx_instance* instances, int32_t count,
+                    const xtx_params* params, xtx_result* results, xtx_batch_stats* stats);
+
+/* ---- served form: continuous batching ---- */
+typedef struct {
+    int64_t lds_budget_bytes;   /* 0 = whole device */
+    int64_t hbm_budget_bytes;   /* 0 = 8 GiB */
+    int32_t max_resident;       /* 0 = unlimited (budget-bound) */
+    int32_t queue_capacity;     /* 0 = 65536 */
+} xtx_serve_config;
+
+int xtx_serve_open(xtx_ctx* ctx, const xtx_params* params, const xtx_serve_config* cfg);
+/* Queues one request; it is admitted at the next epoch boundary at which the
+ * LDS/HBM budget allows. The instance's buffers must stay valid until its
+ * result has been polled. */
+int xtx_serve_submit(xtx_ctx* ctx, const xtx_instance* inst, uint64_t tag);
+/* Runs one epoch over the resident requests, retiring solved ones and
+ * admitting queued ones. *resident / *queued report the state after it. */
+int xtx_serve_step(xtx_ctx* ctx, int32_t* resident, int32_t* queued);
+/* Retired results since the last poll (at most max). */
+int xtx_serve_poll(xtx_ctx* ctx, xtx_result* out, int32_t max, int32_t* count);
+/* Force-retires a resident or queued request (its best so far is written). */
+int xtx_serve_retire(xtx_ctx* ctx, uint64_t tag);
+int xtx_serve_close(xtx_ctx* ctx);
+
+/* ---- exact host verification ----
+ * X^T A0 X == A (int128 when the bound allows, else multi-modular CRT),
+ * A diagonal, |det X| == 1 (det mod primes within the Hadamard bound). */
+int xtx_verify(const int64_t* A0, const int64_t* X, const int64_t* A, int32_t n,
+               xtx_verify_report* report);
+
+/* ---- device primitives, for parity tests of the kernels' building blocks ----
+ * Each query q is (matrix index, i, j, s); the device evaluates it with the
+ * same device functions the annealer uses. out per query:
+ *   {feasible, delta, d_nonzero, best_shear(i,j), xt_feasible} */
+int xtx_probe_moves(xtx_ctx* ctx, const int64_t* A, const int64_t* Xt, int32_t n,
+                    int32_t n_mats, const int32_t* queries, int32_t n_queries,
+                    int64_t* out);
+/* Philox words and exp values as the device computes them. */
+int xtx_probe_rng(xtx_ctx* ctx, const uint32_t* ctr_key, int32_t count, uint32_t* out,
+                  const double* x, int32_t nx, double* exp_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* XTX_H */

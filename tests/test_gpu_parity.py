@@ -1,0 +1,255 @@
+"""Parity of the gfx950 engine (through the C ABI) with the oracle.
+
+1. Building blocks: Philox words and exp bits; try_add / score delta / nonzero
+   delta / best shear / transform bound on every golden matrix and on random
+   (i, j, s) queries, for every lane-group layout (n <= 16, 32, 64, 128).
+2. Trajectories: a plain solve for a fixed number of epochs ends in exactly the
+   state the oracle's trajectory model reaches (every replica's A, X^T, score,
+   nonzero count, move count, best, rung, accepts) with identical accepted-move
+   logs; each log replays through the restated reference primitives to that
+   replica's final state.
+3. Results: every reported A, X passes the exact verifier; the batched and
+   served forms return the same result for an instance as the plain form
+   (alone vs in the mix).
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from xtax_amd import EngineParams, workloads
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def eng():
+    from xtax_amd import Engine
+    e = Engine(0)
+    yield e
+    e.close()
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+# ------------------------------------------------------------- building blocks
+def test_rng_and_exp_bits(eng, orc):
+    rng = np.random.default_rng(1)
+    ck = rng.integers(0, 1 << 32, (257, 6), dtype=np.uint64).astype(np.uint32)
+    ck[0] = 0
+    xs = np.concatenate([-rng.exponential(5.0, 2000), [0.0, -1e-300, -39.9999, -40.0, -41.0, -1e9]])
+    words, ex = eng.probe_rng(ck, xs)
+    for t in range(len(ck)):
+        want = orc.philox(ck[t, :4].tolist(), ck[t, 4:].tolist())
+        assert words[t].tolist() == want, t
+    for t, x in enumerate(xs):
+        assert ex[t] == orc.xexp(x), (x, ex[t], orc.xexp(x))  # bit-exact
+
+
+def _check_probe(eng, orc, mats, xts, queries):
+    out = eng.probe_moves(np.array(mats), np.array(xts), queries)
+    for q, (mi, i, j, s) in enumerate(queries):
+        A, Xt = mats[mi], xts[mi]
+        ok, d, dnz, _, _ = orc.try_add(A, Xt, i, j, s)
+        okt, _ = orc.add_transform_row(Xt, i, j, s)
+        got = out[q].tolist()
+        assert got[0] == int(ok), (mi, i, j, s)
+        if ok:
+            assert got[1] == d and got[2] == dnz, (mi, i, j, s, got, d, dnz)
+        assert got[3] == orc.best_shear_l1(A, i, j), (mi, i, j, got[3])
+        assert got[4] == int(okt)
+
+
+def test_move_primitives_on_golden_matrices(eng, orc):
+    g = np.load(os.path.join(GOLD, "g1_primitives.npz"))
+    ns = sorted({int(k.split("_")[0][1:]) for k in g.files if k.startswith("n")})
+    for n in ns:
+        mats, xts = list(g[f"n{n}_A"]), list(g[f"n{n}_Xt"])
+        queries = [tuple(c) for c in g[f"n{n}_cases"].tolist()]
+        # add every ordered pair of the first matrices with their best-shear s
+        for mi in range(min(2, len(mats))):
+            for i in range(n):
+                for j in range(n):
+                    if i != j:
+                        queries.append((mi, i, j, int(g[f"n{n}_best_all"][mi][i][j]) or 1))
+        say(f"probe golden n={n} queries={len(queries)}")
+        _check_probe(eng, orc, mats, xts, queries)
+
+
+@pytest.mark.parametrize("n", [2, 3, 7, 16, 17, 31, 32, 33, 47, 64, 65, 80, 96])
+def test_move_primitives_random(eng, orc, n):
+    rng = np.random.default_rng(n)
+    mats, xts = [], []
+    for kind in range(6):
+        scale = [3, 40, 3000, 1 << 30, 1 << 46, 1 << 47][kind]
+        A = rng.integers(-scale, scale + 1, (n, n)) * (rng.random((n, n)) < 0.85)
+        A = np.triu(A) + np.triu(A, 1).T
+        mats.append(A.astype(np.int64))
+        X = rng.integers(-scale, scale + 1, (n, n)).astype(np.int64)
+        xts.append(X)
+    queries = []
+    for _ in range(400):
+        mi = int(rng.integers(0, len(mats)))
+        i = int(rng.integers(0, n))
+        j = int(rng.integers(0, n - 1))
+        j += j >= i
+        s = int(rng.choice([-64, -2, -1, 1, 2, 64, int(rng.integers(-64, 65))]))
+        queries.append((mi, i, j, s))
+    say(f"probe random n={n}")
+    _check_probe(eng, orc, mats, xts, queries)
+
+
+# ----------------------------------------------------------------- trajectories
+def _both(**kw):
+    """EngineParams and oracle params from one set of values."""
+    from oracle import pyoracle
+    ep = EngineParams(**kw)
+    op = pyoracle.make_params(**{k: v for k, v in kw.items()
+                                 if k in pyoracle.DEFAULTS})
+    return ep, op
+
+
+CASES = [
+    # name, matrix, params
+    ("10x10-temper", "ten", dict(replicas=8, seed=3, exchange_interval=300, t_init=50, t_min=0.5)),
+    ("10x10-cool", "ten", dict(replicas=4, seed=5, tempering=False, exchange_interval=200,
+                               stuck_threshold=150, sweep_fraction=0.7, adaptive_cooling=True,
+                               moves_per_cool=50)),
+    ("rand20-temper", "rand20", dict(replicas=6, seed=9, exchange_interval=250, t_init=40,
+                                     t_min=0.5, target_samples=3, stuck_threshold=100,
+                                     sweep_fraction=0.5)),
+    ("lattice24", "lat24", dict(replicas=5, seed=11, exchange_interval=200, greedy_fraction=0.5,
+                                sweep_fraction=0.05, stuck_threshold=120, target_samples=16)),
+    ("planted40", "p40", dict(replicas=4, seed=13, exchange_interval=150, t_init=200, t_min=5,
+                              greedy_fraction=1.0, worker_diversity=0.0)),
+    ("planted72", "p72", dict(replicas=3, seed=17, exchange_interval=120, t_init=300, t_min=5,
+                              greedy_fraction=0.7)),
+    ("rand12-cool-reseed", "rand12", dict(replicas=3, seed=19, tempering=False,
+                                           exchange_interval=100, stuck_threshold=60,
+                                           reseed_factor=1.0, t_init=30)),
+]
+
+
+def _matrix(name):
+    if name == "ten":
+        return np.load(os.path.join(GOLD, "g2_end_to_end.npz"))["r0_A0"]
+    if name == "rand20":
+        return workloads.random_symmetric(20, -20, 20, seed=4)
+    if name == "rand12":
+        return workloads.random_symmetric(12, -20, 20, seed=6)
+    if name == "lat24":
+        return workloads.lattice(24, 24, seed=2)[1]
+    if name == "p40":
+        return workloads.planted(40, 100, 120, seed=5)[0]
+    if name == "p72":
+        return workloads.planted(72, 1000, 288, seed=6)[0]
+    raise KeyError(name)
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_trajectory_matches_model(eng, orc, case):
+    name, mat, kw = case
+    A0 = _matrix(mat)
+    n = A0.shape[0]
+    epochs = 3
+    cap = 200000
+    ep, op = _both(max_epochs=epochs, move_log=True, move_log_cap=cap, **kw)
+    say(f"trajectory {name} n={n} R={ep.replicas}")
+    res = eng.run_annealer(A0, None, ep)
+    model = orc.model_run(A0, np.eye(n, dtype=np.int64), op, epochs, log_cap=cap)
+    st = res.stats
+    R = ep.replicas
+    gA, gXt, gsc = eng.last_replicas(R, n)
+    assert st["solved"] == model["solved"]
+    assert st["verify_status"] == 0
+    eye = np.eye(n, dtype=np.int64)
+    if model["solved"]:
+        r = model["solve_replica"]
+        assert st["solve_replica"] == r and st["solve_move"] == model["solve_move"]
+        assert np.array_equal(gA[r], model["A"][r]) and np.array_equal(gXt[r], model["Xt"][r])
+        assert np.array_equal(res.A, model["A"][r]) and np.array_equal(res.X, model["Xt"][r].T)
+        reps = [r]
+    else:
+        assert st["epochs"] == model["epochs"] == epochs
+        assert np.array_equal(gA, model["A"]), "replica A differ"
+        assert np.array_equal(gXt, model["Xt"]), "replica Xt differ"
+        assert np.array_equal(gsc, model["scal"]), (gsc, model["scal"])
+        b = model["best_replica"]
+        assert st["best_replica"] == b and st["best_score"] == model["best_score"]
+        assert np.array_equal(res.A, model["best_A"][b])
+        assert np.array_equal(res.X, model["best_Xt"][b].T)
+        assert st["exchanges_accepted"] == model["exchanges"]
+        reps = range(R)
+    for r in reps:
+        glog, trunc = eng.movelog(r)
+        assert not trunc
+        mlog = model["logs"][r]
+        assert len(glog) == len(mlog), (r, len(glog), len(mlog))
+        assert np.array_equal(glog, mlog), r
+        k, A, Xt = orc.replay(A0, eye, glog)
+        assert k == len(glog)
+        assert np.array_equal(A, gA[r]) and np.array_equal(Xt, gXt[r])
+    st_o, sign, _ = orc.verify(A0, res.X, res.A)
+    assert st_o & ~2 == 0 and sign in (1, -1)
+
+
+# --------------------------------------------------------------- result forms
+def test_plain_solves_reference_sample(eng, orc):
+    A0 = _matrix("ten")
+    ep = EngineParams(replicas=32, seed=1, t_init=50, t_min=0.5, exchange_interval=500,
+                      max_epochs=400)
+    res = eng.run_annealer(A0, None, ep)
+    assert res.stats["solved"] == 1 and res.stats["verify_status"] == 0
+    assert sorted(np.diag(res.A).tolist()) == [-1] + [1] * 9
+    assert orc.verify(A0, res.X, res.A)[0] == 0
+
+
+def test_alone_vs_batch_vs_serve(eng):
+    mats = [workloads.planted(n, 9, 2 * n, seed=100 + n)[0] for n in (5, 6, 8, 9, 12, 20, 33)]
+    mats.append(_matrix("ten"))
+    ep = EngineParams(replicas=8, t_init=50, t_min=0.5, exchange_interval=200, max_epochs=50)
+    seeds = [1000 + q for q in range(len(mats))]
+    alone = []
+    for q, A0 in enumerate(mats):
+        say(f"alone {q} n={A0.shape[0]}")
+        p = EngineParams(**{**ep.__dict__, "seed": seeds[q]})
+        r = eng.run_annealer(A0, None, p)
+        alone.append((r.A, r.X, r.stats["solved"], r.stats["solve_move"]))
+    say("batch")
+    batch, bst = eng.batch_solve(mats, ep, seeds=seeds)
+    for q in range(len(mats)):
+        A, X, d = batch[q]
+        assert d["verify_status"] == 0
+        assert d["solved"] == alone[q][2]
+        if d["solved"]:
+            assert d["solve_move"] == alone[q][3]
+        assert np.array_equal(A, alone[q][0]) and np.array_equal(X, alone[q][1]), q
+    say("serve")
+    eng.serve_open(ep)
+    try:
+        for q, A0 in enumerate(mats):
+            eng.serve_submit(A0, q, seeds[q])
+            eng.serve_step()
+        got = {}
+        for _ in range(2000):
+            r, qd = eng.serve_step()
+            for A, X, d in eng.serve_poll():
+                got[d["tag"]] = (A, X, d)
+            if r == 0 and qd == 0:
+                break
+        for A, X, d in eng.serve_poll():
+            got[d["tag"]] = (A, X, d)
+        assert len(got) == len(mats)
+        for q in range(len(mats)):
+            A, X, d = got[q]
+            assert d["solved"] == alone[q][2] and d["verify_status"] == 0
+            if alone[q][2]:
+                assert d["solve_move"] == alone[q][3]
+            assert np.array_equal(A, alone[q][0]) and np.array_equal(X, alone[q][1]), q
+    finally:
+        eng.serve_close()

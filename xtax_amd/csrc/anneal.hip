@@ -1,0 +1,1014 @@
+// anneal.hip -- gfx950 kernels of the xtax L1 congruence annealer.
+//
+// Mapping: one replica per lane group of G lanes (G = 16, 32 or 64 by n), lane g
+// owning columns k = g + c*G (c < C). A replica's A and X^T sit in LDS with an
+// odd row stride P; its row masses (main.cpp:136-144) in registers of the lanes
+// owning those rows. Every sum over columns is a group reduction; every choice
+// (pair, shear, accept) is made identically by all G lanes from broadcast
+// values, so the group runs the reference's sequential worker loop
+// (congruence_anneal.hpp:276-413) with O(n/G) work per lane per step.
+// Arithmetic is exact int64 throughout; the only floating point is the
+// Metropolis test and the ladder, specified bit-exactly (IEEE ops + fma).
+//
+// Kernels:
+//   k_anneal<G,C>   one epoch (exchange_interval moves) of many replicas
+//   k_epoch_end     per instance: solve retirement, replica exchange sweep,
+//                   instance-best bookkeeping
+//   k_probe<G,C>    the move primitives on given matrices (parity tests)
+//   k_probe_rng     Philox words / exp values (parity tests)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "engine_types.h"
+
+namespace xtx {
+
+// ---------------------------------------------------------------------------
+// building blocks
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int64_t iabs64(int64_t v) { return v < 0 ? -v : v; }
+__device__ __forceinline__ bool over(int64_t v) { return v > kLimit || v < -kLimit; }
+
+// Philox4x32-10 (Salmon et al., SC'11).
+__device__ __forceinline__ uint4 philox10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                          uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int round = 0; round < 10; ++round) {
+        if (round) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        const uint32_t lo0 = 0xD2511F53u * c0;
+        const uint32_t hi0 = __umulhi(0xD2511F53u, c0);
+        const uint32_t lo1 = 0xCD9E8D57u * c2;
+        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2);
+        const uint32_t n0 = hi1 ^ c1 ^ k0;
+        const uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+    }
+    return make_uint4(c0, c1, c2, c3);
+}
+
+__device__ __forceinline__ uint32_t mulhi32(uint32_t w, uint32_t n) { return __umulhi(w, n); }
+__device__ __forceinline__ double u01(uint32_t w) { return (double)w * 0x1p-32; }
+__device__ __forceinline__ double u53(uint32_t hi, uint32_t lo) {
+    return (double)(((uint64_t)(hi >> 5) << 26) | (lo >> 6)) * 0x1p-53;
+}
+
+// exp(x), x <= 0, with the same IEEE operation sequence on every device:
+// 0 below -40; k = rint(x/ln2); r = x - k ln2 (two fma); degree-13 Taylor in
+// Horner form (fma); times 2^k.
+__device__ __forceinline__ double xexp(double x) {
+    if (!(x > -40.0)) return 0.0;
+    if (x > 0.0) x = 0.0;
+    const double kd = __builtin_rint(x * 1.4426950408889634);
+    double r = __builtin_fma(-kd, 0x1.62e42fefa39efp-1, x);
+    r = __builtin_fma(-kd, 0x1.abc9e3b39803fp-56, r);
+    double p = 1.0 / 6227020800.0;
+    p = __builtin_fma(p, r, 1.0 / 479001600.0);
+    p = __builtin_fma(p, r, 1.0 / 39916800.0);
+    p = __builtin_fma(p, r, 1.0 / 3628800.0);
+    p = __builtin_fma(p, r, 1.0 / 362880.0);
+    p = __builtin_fma(p, r, 1.0 / 40320.0);
+    p = __builtin_fma(p, r, 1.0 / 5040.0);
+    p = __builtin_fma(p, r, 1.0 / 720.0);
+    p = __builtin_fma(p, r, 1.0 / 120.0);
+    p = __builtin_fma(p, r, 1.0 / 24.0);
+    p = __builtin_fma(p, r, 1.0 / 6.0);
+    p = __builtin_fma(p, r, 1.0 / 2.0);
+    p = __builtin_fma(p, r, 1.0);
+    p = __builtin_fma(p, r, 1.0);
+    return __builtin_ldexp(p, (int)kd);
+}
+
+// Nearest integer a/b, ties away from zero (mat_io.hpp:41-48).
+__device__ __forceinline__ int64_t rounded_div(int64_t a, int64_t b) {
+    int64_t q = a / b;
+    const int64_t r = a - q * b;
+    if (2 * iabs64(r) >= iabs64(b)) q += ((a < 0) == (b < 0)) ? 1 : -1;
+    return q;
+}
+
+__device__ __forceinline__ int64_t clamp_cap(int64_t v) {
+    return v > kShearCap ? kShearCap : (v < -kShearCap ? -kShearCap : v);
+}
+
+// Reductions and broadcasts inside a lane group of G lanes.
+template <int G>
+struct Grp {
+    static __device__ __forceinline__ int64_t sum(int64_t v) {
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
+        return v;
+    }
+    static __device__ __forceinline__ int32_t sum32(int32_t v) {
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
+        return v;
+    }
+    static __device__ __forceinline__ int64_t max(int64_t v) {
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1) {
+            const int64_t u = __shfl_xor(v, o, G);
+            v = u > v ? u : v;
+        }
+        return v;
+    }
+    static __device__ __forceinline__ int32_t min32(int32_t v) {
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1) {
+            const int32_t u = __shfl_xor(v, o, G);
+            v = u < v ? u : v;
+        }
+        return v;
+    }
+    static __device__ __forceinline__ bool any(bool p) {
+        const unsigned long long b = __ballot(p);
+        if (G == 64) return b != 0ull;
+        const int base = (int)(threadIdx.x & 63u) & ~(G - 1);
+        return ((b >> base) & ((1ull << G) - 1ull)) != 0ull;
+    }
+    template <class T>
+    static __device__ __forceinline__ T bcast(T v, int src) {
+        return __shfl(v, src, G);
+    }
+};
+
+// Result of evaluating Add(i, j, s) (main.cpp:46-91) over one group.
+template <int C>
+struct MoveEval {
+    int64_t nv[C];    // proposed A[j][k] for the lane's columns (k == j: new A_jj)
+    int64_t off;      // sum_{k != j} |new| - |old|
+    int64_t delta;
+    int32_t dnz;
+    bool bad;         // some entry would exceed the bound
+};
+
+// try_add without the transform row (main.cpp:46-91).
+template <int G, int C>
+__device__ __forceinline__ void evaluate(const int64_t (&ai)[C], const int64_t (&aj)[C], int g,
+                                         int n, int j, int64_t s, int64_t a_ii, int64_t a_ji,
+                                         int64_t a_jj, MoveEval<C>& e) {
+    int64_t off = 0;
+    int32_t dnz = 0;
+    bool bad = false;
+    const int64_t new_jj = a_jj + 2 * s * a_ji + s * s * a_ii;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const int k = g + c * G;
+        e.nv[c] = new_jj;
+        if (k < n && k != j) {
+            const int64_t old = aj[c];
+            const int64_t nv = old + s * ai[c];
+            bad |= over(nv);
+            off += iabs64(nv) - iabs64(old);
+            dnz += (old == 0) ? (nv != 0) : -(int32_t)(nv == 0);
+            e.nv[c] = nv;
+        }
+    }
+    off = Grp<G>::sum(off);
+    dnz = Grp<G>::sum32(dnz);
+    bad = Grp<G>::any(bad) || over(new_jj);
+    e.off = off;
+    e.dnz = dnz;
+    e.bad = bad;
+    e.delta = 4 * off + (iabs64(new_jj) - iabs64(a_jj));
+}
+
+// Exact best integer shear (main.cpp:181-242). The |a_ik|-weighted lower median
+// m of the breakpoints b_k = -a_jk/a_ik is not formed; its floor and ceiling are
+// found exactly with integer comparisons: ceil(m) = min{t : 2 W(b <= t) >= W}
+// (binary search over t in [-64, 64], clamped) and floor(m) = ceil(m) - 1 iff
+// 2 W(b < ceil m) >= W. Since |a| <= 2^48 these equal the reference's double
+// floor/ceil of the median (every breakpoint is at least 1/|a_ik| from an
+// integer while double rounding moves it by at most 2^-5/|a_ik|).
+// Candidates, clamping, dedup order and the strict-improvement rule follow the
+// reference; returns 0 when no candidate lowers the score.
+template <int G, int C>
+__device__ int64_t best_shear(const int64_t (&ai)[C], const int64_t (&aj)[C], int g, int n, int j,
+                              int64_t a_ii, int64_t a_ji, int64_t a_jj) {
+    int64_t w[C];
+    int64_t wl = 0;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const int k = g + c * G;
+        w[c] = (k < n && k != j && ai[c] != 0) ? iabs64(ai[c]) : 0;
+        wl += w[c];
+    }
+    const int64_t tot = Grp<G>::sum(wl);
+    int64_t cf = 0, cc = 0;
+    if (tot != 0) {
+        int lo = -kShearCap, hi = kShearCap + 1;  // P(hi) holds by convention
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            int64_t acc = 0;
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const int64_t t = (int64_t)mid * ai[c];
+                const bool le = ai[c] > 0 ? (-aj[c] <= t) : (-aj[c] >= t);
+                acc += le ? w[c] : 0;
+            }
+            acc = Grp<G>::sum(acc);
+            if (2 * acc >= tot)
+                hi = mid;
+            else
+                lo = mid + 1;
+        }
+        if (lo <= -kShearCap) {
+            cf = cc = -kShearCap;
+        } else if (lo > kShearCap) {
+            cf = cc = kShearCap;
+        } else {
+            cc = lo;
+            int64_t acc = 0;
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const int64_t t = (int64_t)lo * ai[c];
+                const bool lt = ai[c] > 0 ? (-aj[c] < t) : (-aj[c] > t);
+                acc += lt ? w[c] : 0;
+            }
+            acc = Grp<G>::sum(acc);
+            cf = (2 * acc >= tot) ? lo - 1 : lo;
+        }
+    }
+    // Candidate list in the reference's order, deduplicated against every
+    // earlier entry (statically indexed: no scratch).
+    const bool has_q = a_ii != 0;
+    const int64_t q = has_q ? -rounded_div(a_ji, a_ii) : 0;
+    int64_t cand[5];
+    bool valid[5];
+    cand[0] = clamp_cap(cf);
+    cand[1] = clamp_cap(cc);
+    cand[2] = clamp_cap(q);
+    cand[3] = clamp_cap(q + 1);
+    cand[4] = clamp_cap(q - 1);
+#pragma unroll
+    for (int t = 0; t < 5; ++t) {
+        bool v = t < 2 || has_q;
+#pragma unroll
+        for (int u = 0; u < t; ++u) v &= cand[u] != cand[t];
+        valid[t] = v;
+    }
+    // Score every candidate in one pass (score_delta_l1, main.cpp:150-168).
+    int64_t part[5];
+    bool bad[5];
+#pragma unroll
+    for (int t = 0; t < 5; ++t) {
+        part[t] = 0;
+        bad[t] = false;
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const int k = g + c * G;
+        if (k < n && k != j) {
+            const int64_t old = aj[c];
+            const int64_t ao = iabs64(old);
+#pragma unroll
+            for (int t = 0; t < 5; ++t) {
+                const int64_t nv = old + cand[t] * ai[c];
+                bad[t] |= over(nv);
+                part[t] += iabs64(nv) - ao;
+            }
+        }
+    }
+    int64_t best_s = 0, best_d = 0;
+#pragma unroll
+    for (int t = 0; t < 5; ++t) {
+        if (valid[t]) {
+            const int64_t s = cand[t];
+            const int64_t off = Grp<G>::sum(part[t]);
+            const bool b = Grp<G>::any(bad[t]);
+            if (s != 0) {
+                const int64_t new_jj = a_jj + 2 * s * a_ji + s * s * a_ii;
+                const int64_t d = (b || over(new_jj))
+                                      ? ((int64_t)1 << 62)
+                                      : 4 * off + (iabs64(new_jj) - iabs64(a_jj));
+                if (d < best_d) {
+                    best_d = d;
+                    best_s = s;
+                }
+            }
+        }
+    }
+    return best_s;
+}
+
+// ---------------------------------------------------------------------------
+// the annealer
+// ---------------------------------------------------------------------------
+
+template <int G, int C>
+struct Replica {
+    // LDS views
+    int64_t* sA;
+    int64_t* sX;
+    const uint8_t* act;
+    int P, n, N, g;
+    int64_t rm[C];  // row masses of the lane's rows
+
+    __device__ __forceinline__ void load_rows(int i, int j, int64_t (&ai)[C],
+                                              int64_t (&aj)[C]) const {
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const int k = g + c * G;
+            ai[c] = k < n ? sA[i * P + k] : 0;
+            aj[c] = k < n ? sA[j * P + k] : 0;
+        }
+    }
+
+    __device__ __forceinline__ void compute_row_mass() {
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const int k = g + c * G;
+            int64_t m = 0;
+            if (k < n)
+                for (int t = 0; t < n; ++t)
+                    if (t != k) m += iabs64(sA[k * P + t]);
+            rm[c] = m;
+        }
+    }
+
+    // row_mass[row] as seen by any lane of the group
+    __device__ __forceinline__ int64_t mass_of(int row) const {
+        int64_t v = Grp<G>::bcast(rm[0], row & (G - 1));
+        if (C > 1) {
+            const int64_t v1 = Grp<G>::bcast(rm[C > 1 ? 1 : 0], row & (G - 1));
+            v = (row >= G) ? v1 : v;
+        }
+        return v;
+    }
+
+    // Copy the LDS state to n x n row-major HBM arrays.
+    __device__ __forceinline__ void store(int64_t* gA, int64_t* gX) const {
+        for (int row = 0; row < n; ++row) {
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const int k = g + c * G;
+                if (k < n) {
+                    gA[row * n + k] = sA[row * P + k];
+                    gX[row * n + k] = sX[row * P + k];
+                }
+            }
+        }
+    }
+
+    __device__ __forceinline__ void load(const int64_t* gA, const int64_t* gX) {
+        for (int row = 0; row < n; ++row) {
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const int k = g + c * G;
+                if (k < n) {
+                    sA[row * P + k] = gA[row * n + k];
+                    sX[row * P + k] = gX[row * n + k];
+                }
+            }
+        }
+    }
+
+    // Commit an evaluated Add(i, j, s) after the transform-row bound check
+    // (L1Objective::commit, main.cpp:393-399 -> add_transform_row :97-108,
+    // commit_add :112-133). Returns false (state untouched) when a transform
+    // entry would exceed the bound.
+    __device__ __forceinline__ bool xt_row(int i, int j, int64_t s, int64_t (&nx)[C]) const {
+        bool bad = false;
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const int k = g + c * G;
+            nx[c] = 0;
+            if (k < n) {
+                nx[c] = sX[j * P + k] + s * sX[i * P + k];
+                bad |= over(nx[c]);
+            }
+        }
+        return !Grp<G>::any(bad);
+    }
+
+    __device__ __forceinline__ void write(int j, const int64_t (&aj)[C], const MoveEval<C>& e,
+                                          const int64_t (&nx)[C]) {
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const int k = g + c * G;
+            if (k < n) {
+                if (k != j) {
+                    sA[j * P + k] = e.nv[c];
+                    sA[k * P + j] = e.nv[c];
+                    rm[c] += iabs64(e.nv[c]) - iabs64(aj[c]);
+                } else {
+                    sA[j * P + j] = e.nv[c];
+                    rm[c] += e.off;
+                }
+                sX[j * P + k] = nx[c];
+            }
+        }
+    }
+};
+
+template <int G, int C>
+__global__ void __launch_bounds__(256)
+    k_anneal(const RepRef* __restrict__ table, int count, Inst* __restrict__ insts, int P,
+             int nmax, int rep_bytes, const volatile int* __restrict__ stop) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    using Gp = Grp<G>;
+    const int g = (int)threadIdx.x & (G - 1);
+    const int slot = (int)threadIdx.x / G;
+    const int gid = (int)blockIdx.x * ((int)blockDim.x / G) + slot;
+    if (gid >= count) return;
+    const RepRef ref = table[gid];
+    Inst* I = insts + ref.inst;
+    if (I->status != ST_RUNNING) return;
+    const int r = ref.rep;
+    RepState* S = I->rs + r;
+    if (S->flags & RF_STOPPED) return;
+
+    Replica<G, C> rp;
+    rp.sA = (int64_t*)(smem + (size_t)slot * rep_bytes);
+    rp.sX = rp.sA + (size_t)nmax * P;
+    uint8_t* sact = (uint8_t*)(rp.sX + (size_t)nmax * P);
+    rp.act = sact;
+    rp.P = P;
+    rp.n = I->n;
+    rp.N = I->N;
+    rp.g = g;
+    const int n = rp.n, N = rp.N;
+    const int64_t nn = (int64_t)n * n;
+    rp.load(I->A + (int64_t)r * nn, I->Xt + (int64_t)r * nn);
+    for (int q = g; q < N; q += G) sact[q] = I->act[q];
+    rp.compute_row_mass();
+
+    const InstParams Pp = I->P;
+    const uint32_t key0 = I->key0, key1 = I->key1;
+    const uint64_t epoch_end = (uint64_t)(I->epoch + 1u) * (uint64_t)Pp.epoch_moves;
+    const int64_t gb_epoch = I->gb_score;
+    const bool tempering = Pp.tempering != 0;
+    const int TS = Pp.target_samples;
+    int64_t* bA = I->bA + (int64_t)r * nn;
+    int64_t* bX = I->bXt + (int64_t)r * nn;
+    LogEnt* logp = I->log ? I->log + (int64_t)r * I->log_cap : nullptr;
+    const int64_t log_cap = I->log_cap;
+
+    uint64_t moves = S->moves;
+    int64_t score = S->score, offnz = S->offnz, local_best = S->local_best, msi = S->msi;
+    double T = tempering ? I->temps[S->rung] : S->T;
+    double cooling = S->cooling;
+    int32_t cool_counter = S->cool_counter, win_p = S->win_p, win_a = S->win_a;
+    int64_t best_score = S->best_score, best_move = S->best_move;
+    int32_t flags = S->flags;
+    const double gf = S->gf, tf = S->tf;
+    int64_t accepts = S->accepts, greedy_n = S->greedy, infeasible = S->infeasible;
+    int64_t sweeps = S->sweeps, log_count = S->log_count, solve_move = S->solve_move;
+
+    // Lazy best flush: the pre-change state goes to bA/bXt only when it is the
+    // replica's best and could still be the instance's best.
+    auto leave_best = [&]() {
+        if (flags & RF_CUR_BEST) {
+            if (best_score <= gb_epoch) {
+                rp.store(bA, bX);
+                flags &= ~RF_BEST_STALE;
+            } else {
+                flags |= RF_BEST_STALE;
+            }
+            flags &= ~RF_CUR_BEST;
+        }
+    };
+    auto log_move = [&](uint64_t m, int i, int j, int64_t s, uint16_t f) {
+        if (logp) {
+            if (log_count < log_cap && g == 0) {
+                LogEnt e;
+                e.move = (int64_t)m;
+                e.i = (int16_t)i;
+                e.j = (int16_t)j;
+                e.s = (int16_t)s;
+                e.flags = f;
+                logp[log_count] = e;
+            }
+            ++log_count;
+        }
+    };
+    // Greedy reduction sweep from a pivot (congruence_anneal.hpp:246-260).
+    auto sweep = [&](int pv, uint64_t m) -> bool {
+        const int64_t before = score;
+        ++sweeps;
+        for (int q = 0; q < N; ++q) {
+            const int jj = sact[q];
+            if (jj == pv) continue;
+            int64_t ai[C], aj[C];
+            rp.load_rows(pv, jj, ai, aj);
+            const int64_t a_ii = rp.sA[pv * P + pv];
+            const int64_t a_ji = rp.sA[jj * P + pv];
+            const int64_t a_jj = rp.sA[jj * P + jj];
+            const int64_t s = best_shear<G, C>(ai, aj, g, n, jj, a_ii, a_ji, a_jj);
+            ++greedy_n;
+            if (s == 0) continue;
+            MoveEval<C> e;
+            evaluate<G, C>(ai, aj, g, n, jj, s, a_ii, a_ji, a_jj, e);
+            if (e.bad || e.delta > 0) continue;
+            int64_t nx[C];
+            if (!rp.xt_row(pv, jj, s, nx)) continue;
+            leave_best();
+            rp.write(jj, aj, e, nx);
+            score += e.delta;
+            offnz += e.dnz;
+            ++accepts;
+            log_move(m, pv, jj, s, 1);
+        }
+        return score < before;
+    };
+    auto track_best = [&](uint64_t m) {
+        if (score < best_score) {
+            best_score = score;
+            best_move = (int64_t)m;
+            flags |= RF_CUR_BEST;
+            flags &= ~RF_BEST_STALE;
+        }
+    };
+
+    bool halt = false;
+    for (uint32_t it = 0; moves < epoch_end; ++it) {
+        if ((it & 15u) == 0u) {
+            const unsigned long long key =
+                __hip_atomic_load(&I->solve_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (key != kNoKey && moves >= (key >> kRepBits)) break;
+            if (*stop) {
+                halt = true;
+                break;
+            }
+        }
+        const uint64_t m = ++moves;
+        const uint4 w = philox10((uint32_t)m, (uint32_t)(m >> 32), (uint32_t)r, (uint32_t)g,
+                                 key0, key1);
+        const uint32_t w0z = Gp::bcast(w.z, 0), w0w = Gp::bcast(w.w, 0);
+        const uint32_t w1z = Gp::bcast(w.z, 1), w1w = Gp::bcast(w.w, 1);
+        const uint32_t w2z = Gp::bcast(w.z, 2), w2w = Gp::bcast(w.w, 2);
+        const uint32_t w3z = Gp::bcast(w.z, 3), w3w = Gp::bcast(w.w, 3);
+
+        // ---- pair (congruence_anneal.hpp:283-288, :211-228) ----
+        int i, j;
+        if (u01(w0z) < tf) {
+            const bool in = g < TS;
+            const uint32_t p = in ? mulhi32(w.x, (uint32_t)N) : 0u;
+            const int row = sact[p];
+            const int64_t wt = rp.mass_of(row);
+            const int64_t key_wt = in ? wt : -1;
+            const int64_t mx = Gp::max(key_wt);
+            const int cb = Gp::min32((in && key_wt == mx) ? g : G);
+            const uint32_t tpos = Gp::bcast(p, cb);
+            const int target = sact[tpos];
+            const uint32_t pp = in ? mulhi32(w.y, (uint32_t)N) : 0u;
+            const int cand = sact[pp];
+            const int64_t v = (in && cand != target) ? iabs64(rp.sA[target * P + cand]) : -1;
+            const int64_t vmax = Gp::max(v);
+            int ibest;
+            if (vmax < 0) {
+                ibest = sact[(tpos + 1u + mulhi32(w3w, (uint32_t)(N - 1))) % (uint32_t)N];
+            } else {
+                const int cv = Gp::min32((v == vmax && in && cand != target) ? g : G);
+                ibest = Gp::bcast(cand, cv);
+            }
+            i = ibest;
+            j = target;
+        } else {
+            const uint32_t ip = mulhi32(w2w, (uint32_t)N);
+            const uint32_t jp = (ip + 1u + mulhi32(w3z, (uint32_t)(N - 1))) % (uint32_t)N;
+            i = sact[ip];
+            j = sact[jp];
+        }
+
+        int64_t ai[C], aj[C];
+        rp.load_rows(i, j, ai, aj);
+        const int64_t a_ii = rp.sA[i * P + i];
+        const int64_t a_ji = rp.sA[j * P + i];
+        const int64_t a_jj = rp.sA[j * P + j];
+
+        // ---- shear (congruence_anneal.hpp:290-296) ----
+        const int64_t pm = (w1z & 1u) ? 1 : -1;
+        int64_t s = pm;
+        if (u01(w0w) < gf) {
+            s = best_shear<G, C>(ai, aj, g, n, j, a_ii, a_ji, a_jj);
+            ++greedy_n;
+            if (s == 0) s = pm;
+        }
+
+        // ---- attempt_add (congruence_anneal.hpp:232-240) ----
+        MoveEval<C> e;
+        evaluate<G, C>(ai, aj, g, n, j, s, a_ii, a_ji, a_jj, e);
+        int res = -1;
+        if (e.bad) {
+            ++infeasible;
+        } else if (e.delta <= 0 || u53(w1w, w2z) < xexp(-(double)e.delta / T)) {
+            int64_t nx[C];
+            if (rp.xt_row(i, j, s, nx)) {
+                const int64_t ns = score + e.delta;
+                if (ns < best_score) {
+                    best_score = ns;
+                    best_move = (int64_t)m;
+                    flags |= RF_CUR_BEST;
+                    flags &= ~RF_BEST_STALE;
+                } else {
+                    leave_best();
+                }
+                rp.write(j, aj, e, nx);
+                score = ns;
+                offnz += e.dnz;
+                ++accepts;
+                log_move(m, i, j, s, 0);
+                res = e.delta < 0 ? 1 : 0;
+            }
+        }
+        ++win_p;
+        if (res >= 0) ++win_a;
+        if (score < local_best) {
+            local_best = score;
+            msi = 0;
+        } else {
+            ++msi;
+        }
+        if (offnz == 0) {
+            flags |= RF_SOLVED | RF_STOPPED;
+            solve_move = (int64_t)m;
+            break;
+        }
+
+        // ---- stall handling (congruence_anneal.hpp:363-412) ----
+        if (tempering) {
+            if (msi > Pp.stuck_threshold) {
+                if (Pp.sweep_fraction > 0.0 && u01(Gp::bcast(w.z, 4)) < Pp.sweep_fraction) {
+                    sweep(sact[mulhi32(Gp::bcast(w.w, 4), (uint32_t)N)], m);
+                    track_best(m);
+                }
+                local_best = score;
+                msi = 0;
+            }
+        } else {
+            if (msi > Pp.stuck_threshold) {
+                if (Pp.sweep_fraction > 0.0 && u01(Gp::bcast(w.z, 4)) < Pp.sweep_fraction) {
+                    if (sweep(sact[mulhi32(Gp::bcast(w.w, 4), (uint32_t)N)], m)) {
+                        local_best = score;
+                        msi = 0;
+                    }
+                    track_best(m);
+                }
+                T = Pp.reheat * Pp.t_init;
+                if ((double)score > (double)gb_epoch * Pp.reseed_factor) {
+                    leave_best();
+                    rp.load(I->gA, I->gXt);
+                    rp.compute_row_mass();
+                    score = gb_epoch;
+                    offnz = I->gb_offnz;
+                    track_best(m);
+                }
+                local_best = score;
+                msi = 0;
+                cool_counter = 0;
+            } else if (++cool_counter >= Pp.moves_per_cool) {
+                cool_counter = 0;
+                if (Pp.adaptive && win_p > 0) {
+                    const double acc = (double)win_a / (double)win_p;
+                    if (acc > Pp.target_accept)
+                        cooling *= 0.999;
+                    else
+                        cooling /= 0.999;
+                    cooling = cooling < 0.90 ? 0.90 : (cooling > 0.99999 ? 0.99999 : cooling);
+                }
+                win_p = 0;
+                win_a = 0;
+                T *= cooling;
+                if (T < Pp.t_min) T = Pp.t_min;
+            }
+        }
+        if (offnz == 0) {
+            flags |= RF_SOLVED | RF_STOPPED;
+            solve_move = (int64_t)m;
+            break;
+        }
+        if (Pp.max_moves && moves >= Pp.max_moves) {
+            flags |= RF_STOPPED | RF_CAPPED;
+            break;
+        }
+    }
+    (void)halt;
+
+    rp.store(I->A + (int64_t)r * nn, I->Xt + (int64_t)r * nn);
+    if (g == 0) {
+        S->moves = moves;
+        S->score = score;
+        S->offnz = offnz;
+        S->local_best = local_best;
+        S->msi = msi;
+        S->T = T;
+        S->cooling = cooling;
+        S->cool_counter = cool_counter;
+        S->win_p = win_p;
+        S->win_a = win_a;
+        S->best_score = best_score;
+        S->best_move = best_move;
+        S->flags = flags;
+        S->accepts = accepts;
+        S->greedy = greedy_n;
+        S->infeasible = infeasible;
+        S->sweeps = sweeps;
+        S->log_count = log_count;
+        S->solve_move = solve_move;
+    }
+    if ((flags & RF_SOLVED) && g == 0)
+        atomicMin(&I->solve_key, ((unsigned long long)moves << kRepBits) | (unsigned)r);
+}
+
+// ---------------------------------------------------------------------------
+// epoch end: retirement, replica exchange, instance best
+// ---------------------------------------------------------------------------
+
+// Replica exchange between adjacent rungs (congruence_anneal.hpp:520-551):
+// owner[q] is the replica on rung q; rungs q and q+1 swap when
+// arg = (1/T_q - 1/T_{q+1}) (S_a - S_b) >= 0 or u < exp(arg), sweeping q
+// upward so a configuration can travel several rungs in one sweep.
+__global__ void __launch_bounds__(256) k_epoch_end(Inst* __restrict__ insts) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    Inst* I = insts + blockIdx.x;
+    if (I->status != ST_RUNNING) return;
+    const int t = (int)threadIdx.x;
+    const int R = I->R;
+    const int n = I->n;
+    __shared__ int s_running;
+    __shared__ int64_t s_bs[256];
+    __shared__ int64_t s_bm[256];
+    __shared__ int s_br[256];
+    if (t == 0) s_running = 0;
+    __syncthreads();
+    const unsigned long long key = I->solve_key;
+    if (key != kNoKey) {
+        if (t == 0) {
+            I->status = ST_SOLVED;
+            I->solve_rep = (int32_t)(key & ((1ull << kRepBits) - 1ull));
+            I->solve_move = (int64_t)(key >> kRepBits);
+        }
+        return;
+    }
+    int running = 0;
+    for (int r = t; r < R; r += blockDim.x) running |= !(I->rs[r].flags & RF_STOPPED);
+    if (running) atomicOr(&s_running, 1);
+
+    // instance best: argmin (best_score, best_move, replica)
+    int64_t bs = INT64_MAX, bm = INT64_MAX;
+    int br = -1;
+    for (int r = t; r < R; r += blockDim.x) {
+        const RepState& x = I->rs[r];
+        if (x.best_score < bs || (x.best_score == bs && x.best_move < bm)) {
+            bs = x.best_score;
+            bm = x.best_move;
+            br = r;
+        }
+    }
+    s_bs[t] = bs;
+    s_bm[t] = bm;
+    s_br[t] = br;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if (t < o) {
+            const int64_t b2 = s_bs[t + o], m2 = s_bm[t + o];
+            const int r2 = s_br[t + o];
+            const bool better = r2 >= 0 && (b2 < s_bs[t] || (b2 == s_bs[t] && m2 < s_bm[t]) ||
+                                            (b2 == s_bs[t] && m2 == s_bm[t] && r2 < s_br[t]));
+            if (better) {
+                s_bs[t] = b2;
+                s_bm[t] = m2;
+                s_br[t] = r2;
+            }
+        }
+        __syncthreads();
+    }
+    const int bestr = s_br[0];
+    const bool improve = s_bs[0] < I->gb_score;
+    if (improve && !I->P.tempering) {
+        const RepState& x = I->rs[bestr];
+        const bool cur = (x.flags & RF_CUR_BEST) != 0;
+        const int64_t nn = (int64_t)n * n;
+        const int64_t* srcA = (cur ? I->A : I->bA) + (int64_t)bestr * nn;
+        const int64_t* srcX = (cur ? I->Xt : I->bXt) + (int64_t)bestr * nn;
+        for (int64_t q = t; q < nn; q += blockDim.x) {
+            I->gA[q] = srcA[q];
+            I->gXt[q] = srcX[q];
+        }
+    }
+    __syncthreads();
+    if (t == 0 && improve) {
+        I->gb_score = s_bs[0];
+        I->gb_rep = bestr;
+        // offnz of the snapshot: count it from the copied matrix
+    }
+    if (improve && !I->P.tempering) {
+        __syncthreads();
+        int64_t c = 0;
+        for (int64_t q = t; q < (int64_t)n * n; q += blockDim.x) {
+            const int row = (int)(q / n), col = (int)(q % n);
+            c += (col > row && I->gA[q] != 0);
+        }
+        s_bs[t] = c;
+        __syncthreads();
+        for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+            if (t < o) s_bs[t] += s_bs[t + o];
+            __syncthreads();
+        }
+        if (t == 0) I->gb_offnz = s_bs[0];
+    }
+
+    // exchange sweep
+    if (I->P.tempering && R >= 2) {
+        int* owner = (int*)smem;
+        double* u = (double*)(smem + ((size_t)R * sizeof(int) + 7) / 8 * 8);
+        for (int r = t; r < R; r += blockDim.x) owner[I->rs[r].rung] = r;
+        const uint32_t e = I->epoch;
+        for (int q = t; q < R - 1; q += blockDim.x) {
+            const uint4 w = philox10(e, 0u, (uint32_t)q, 0x40000000u, I->key0, I->key1);
+            u[q] = u53(w.x, w.y);
+        }
+        __syncthreads();
+        if (t == 0) {
+            int64_t acc = 0;
+            for (int q = 0; q + 1 < R; ++q) {
+                const int wa = owner[q], wb = owner[q + 1];
+                const double Ta = I->temps[q], Tb = I->temps[q + 1];
+                const double Sa = (double)I->rs[wa].score, Sb = (double)I->rs[wb].score;
+                const double arg = (1.0 / Ta - 1.0 / Tb) * (Sa - Sb);
+                if (arg >= 0.0 || u[q] < xexp(arg)) {
+                    I->rs[wa].rung = q + 1;
+                    I->rs[wb].rung = q;
+                    owner[q] = wb;
+                    owner[q + 1] = wa;
+                    ++acc;
+                }
+            }
+            I->exch_tried += R - 1;
+            I->exch_acc += acc;
+        }
+    }
+    __syncthreads();
+    if (t == 0) {
+        I->epoch = I->epoch + 1u;
+        if (!s_running || (I->P.max_epochs > 0 && I->epoch >= (uint32_t)I->P.max_epochs))
+            I->status = ST_EXHAUSTED;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// probes (parity tests of the building blocks)
+// ---------------------------------------------------------------------------
+
+// One query per group: q = {mat, i, j, s}; out = {feasible, delta, dnz,
+// best_shear(i, j), xt_feasible}.
+template <int G, int C>
+__global__ void __launch_bounds__(64)
+    k_probe(const int64_t* __restrict__ A, const int64_t* __restrict__ Xt, int n,
+            const int32_t* __restrict__ queries, int nq, int64_t* __restrict__ out) {
+    const int g = (int)threadIdx.x & (G - 1);
+    const int q = (int)blockIdx.x * (64 / G) + (int)threadIdx.x / G;
+    if (q >= nq) return;
+    const int mat = queries[4 * q + 0], i = queries[4 * q + 1], j = queries[4 * q + 2];
+    const int64_t s = queries[4 * q + 3];
+    const int64_t* M = A + (int64_t)mat * n * n;
+    const int64_t* X = Xt + (int64_t)mat * n * n;
+    int64_t ai[C], aj[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const int k = g + c * G;
+        ai[c] = k < n ? M[i * n + k] : 0;
+        aj[c] = k < n ? M[j * n + k] : 0;
+    }
+    const int64_t a_ii = M[i * n + i], a_ji = M[j * n + i], a_jj = M[j * n + j];
+    MoveEval<C> e;
+    evaluate<G, C>(ai, aj, g, n, j, s, a_ii, a_ji, a_jj, e);
+    const int64_t b = best_shear<G, C>(ai, aj, g, n, j, a_ii, a_ji, a_jj);
+    bool bx = false;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const int k = g + c * G;
+        if (k < n) bx |= over(X[j * n + k] + s * X[i * n + k]);
+    }
+    bx = Grp<G>::any(bx);
+    if (g == 0) {
+        out[5 * q + 0] = !e.bad;
+        out[5 * q + 1] = e.delta;
+        out[5 * q + 2] = e.dnz;
+        out[5 * q + 3] = b;
+        out[5 * q + 4] = !bx;
+    }
+}
+
+__global__ void k_probe_rng(const uint32_t* __restrict__ ck, int count, uint32_t* __restrict__ out,
+                            const double* __restrict__ x, int nx, double* __restrict__ ex) {
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t < count) {
+        const uint32_t* c = ck + 6 * t;
+        const uint4 w = philox10(c[0], c[1], c[2], c[3], c[4], c[5]);
+        out[4 * t + 0] = w.x;
+        out[4 * t + 1] = w.y;
+        out[4 * t + 2] = w.z;
+        out[4 * t + 3] = w.w;
+    }
+    if (t < nx) ex[t] = xexp(x[t]);
+}
+
+// ---------------------------------------------------------------------------
+// host-side launch helpers
+// ---------------------------------------------------------------------------
+
+typedef void (*AnnealFn)(const RepRef*, int, Inst*, int, int, int, const volatile int*);
+
+template <int G, int C>
+static hipError_t launch_anneal_t(hipStream_t st, int groups_per_block, const RepRef* table,
+                                  int count, Inst* insts, int P, int nmax, int rep_bytes,
+                                  const volatile int* stop) {
+    const int threads = groups_per_block * G;
+    const int blocks = (count + groups_per_block - 1) / groups_per_block;
+    const size_t lds = (size_t)groups_per_block * rep_bytes;
+    hipLaunchKernelGGL((k_anneal<G, C>), dim3(blocks), dim3(threads), lds, st, table, count, insts,
+                       P, nmax, rep_bytes, stop);
+    return hipGetLastError();
+}
+
+// Dynamic LDS limits are raised once, outside any stream capture.
+hipError_t init_kernel_attrs() {
+    const int lds = 160 * 1024;
+    const void* fns[] = {(const void*)k_anneal<16, 1>, (const void*)k_anneal<32, 1>,
+                         (const void*)k_anneal<64, 1>, (const void*)k_anneal<64, 2>,
+                         (const void*)k_epoch_end};
+    for (const void* f : fns) {
+        const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_anneal(hipStream_t st, int G, int C, int groups_per_block, const RepRef* table,
+                         int count, Inst* insts, int P, int nmax, int rep_bytes,
+                         const volatile int* stop) {
+    if (count <= 0) return hipSuccess;
+    if (G == 16 && C == 1)
+        return launch_anneal_t<16, 1>(st, groups_per_block, table, count, insts, P, nmax,
+                                      rep_bytes, stop);
+    if (G == 32 && C == 1)
+        return launch_anneal_t<32, 1>(st, groups_per_block, table, count, insts, P, nmax,
+                                      rep_bytes, stop);
+    if (G == 64 && C == 1)
+        return launch_anneal_t<64, 1>(st, groups_per_block, table, count, insts, P, nmax,
+                                      rep_bytes, stop);
+    if (G == 64 && C == 2)
+        return launch_anneal_t<64, 2>(st, groups_per_block, table, count, insts, P, nmax,
+                                      rep_bytes, stop);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_epoch_end(hipStream_t st, Inst* insts, int count, int max_r) {
+    if (count <= 0) return hipSuccess;
+    const size_t lds = ((size_t)max_r * sizeof(int) + 7) / 8 * 8 + (size_t)max_r * sizeof(double);
+    hipLaunchKernelGGL(k_epoch_end, dim3(count), dim3(256), lds, st, insts);
+    return hipGetLastError();
+}
+
+__global__ void k_replicate(int64_t* __restrict__ dst, const int64_t* __restrict__ src, int64_t nn,
+                            int R) {
+    const int64_t total = nn * R;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+         q += (int64_t)gridDim.x * blockDim.x)
+        dst[q] = src[q % nn];
+}
+
+hipError_t launch_replicate(hipStream_t st, int64_t* dst, const int64_t* src, int64_t nn, int R) {
+    const int64_t total = nn * R;
+    if (total <= 0) return hipSuccess;
+    const int blocks = (int)std::min<int64_t>((total + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_replicate, dim3(blocks), dim3(256), 0, st, dst, src, nn, R);
+    return hipGetLastError();
+}
+
+hipError_t launch_probe(hipStream_t st, int G, int C, const int64_t* A, const int64_t* Xt, int n,
+                        const int32_t* q, int nq, int64_t* out) {
+    const int per = 64 / G;
+    const dim3 grid((nq + per - 1) / per), block(64);
+    if (G == 16 && C == 1)
+        hipLaunchKernelGGL((k_probe<16, 1>), grid, block, 0, st, A, Xt, n, q, nq, out);
+    else if (G == 32 && C == 1)
+        hipLaunchKernelGGL((k_probe<32, 1>), grid, block, 0, st, A, Xt, n, q, nq, out);
+    else if (G == 64 && C == 1)
+        hipLaunchKernelGGL((k_probe<64, 1>), grid, block, 0, st, A, Xt, n, q, nq, out);
+    else if (G == 64 && C == 2)
+        hipLaunchKernelGGL((k_probe<64, 2>), grid, block, 0, st, A, Xt, n, q, nq, out);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_probe_rng(hipStream_t st, const uint32_t* ck, int count, uint32_t* out,
+                            const double* x, int nx, double* ex) {
+    const int tot = count > nx ? count : nx;
+    if (tot <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_probe_rng, dim3((tot + 255) / 256), dim3(256), 0, st, ck, count, out, x,
+                       nx, ex);
+    return hipGetLastError();
+}
+
+}  // namespace xtx

@@ -1,0 +1,1383 @@
+// engine.cpp -- C++20 host side of the gfx950 xtax annealer (the C ABI of
+// include/xtx.h).
+//
+// The reference runs one std::thread per replica plus a monitor that swaps
+// tempering rungs every 5 ms (congruence_anneal.hpp:483-558). Here every
+// replica of every instance is a lane group on the GPU and the host only
+// sequences epochs: an epoch is `exchange_interval` moves of every replica
+// (k_anneal, one launch per LDS layout bucket) followed by k_epoch_end (solve
+// retirement, exchange sweep, instance best). The host polls instance status
+// between epochs (plain form), between captured multi-epoch graphs (batched
+// form), or per epoch while admitting new work (served form).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "engine_types.h"
+#include "xtx.h"
+
+namespace xtx {
+hipError_t launch_anneal(hipStream_t st, int G, int C, int groups_per_block, const RepRef* table,
+                         int count, Inst* insts, int P, int nmax, int rep_bytes,
+                         const volatile int* stop);
+hipError_t launch_epoch_end(hipStream_t st, Inst* insts, int count, int max_r);
+hipError_t launch_probe(hipStream_t st, int G, int C, const int64_t* A, const int64_t* Xt, int n,
+                        const int32_t* q, int nq, int64_t* out);
+hipError_t launch_probe_rng(hipStream_t st, const uint32_t* ck, int count, uint32_t* out,
+                            const double* x, int nx, double* ex);
+hipError_t launch_replicate(hipStream_t st, int64_t* dst, const int64_t* src, int64_t nn, int R);
+hipError_t init_kernel_attrs();
+}  // namespace xtx
+
+using namespace xtx;
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+struct Err {
+    int code;
+    std::string msg;
+};
+
+#define HIPCHK(expr)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            throw Err{XTX_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)}; \
+    } while (0)
+
+inline int64_t iabs(int64_t v) { return v < 0 ? -v : v; }
+
+// ---------------------------------------------------------------- layout
+// LDS bucket of an order n: lane-group width G, columns per lane C, padded
+// order nmax (multiple of 8) and odd row stride P = nmax + 1.
+struct Layout {
+    int G, C, nmax, P, rep_bytes;
+    bool operator<(const Layout& o) const { return nmax < o.nmax; }
+};
+
+Layout layout_for(int n) {
+    Layout L{};
+    L.nmax = std::max(8, (n + 7) / 8 * 8);
+    if (L.nmax <= 16) {
+        L.G = 16;
+        L.C = 1;
+    } else if (L.nmax <= 32) {
+        L.G = 32;
+        L.C = 1;
+    } else if (L.nmax <= 64) {
+        L.G = 64;
+        L.C = 1;
+    } else {
+        L.G = 64;
+        L.C = 2;
+    }
+    L.P = L.nmax + 1;
+    const int64_t bytes = 2 * (int64_t)L.nmax * L.P * 8 + L.nmax;
+    L.rep_bytes = (int)((bytes + 15) / 16 * 16);
+    return L;
+}
+
+constexpr int kLdsPerCu = 160 * 1024;
+
+// ---------------------------------------------------------------- arena
+// First-fit free list over one device allocation (256-byte granules).
+class Arena {
+  public:
+    ~Arena() { release(); }
+    void reserve(size_t bytes) {
+        release();
+        bytes = (bytes + 255) / 256 * 256;
+        HIPCHK(hipMalloc(&base_, bytes));
+        size_ = bytes;
+        free_.clear();
+        free_[0] = bytes;
+    }
+    void release() {
+        if (base_) (void)hipFree(base_);
+        base_ = nullptr;
+        size_ = 0;
+        free_.clear();
+    }
+    char* alloc(size_t bytes) {
+        bytes = (bytes + 255) / 256 * 256;
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= bytes) {
+                const size_t off = it->first, len = it->second;
+                free_.erase(it);
+                if (len > bytes) free_[off + bytes] = len - bytes;
+                return base_ + off;
+            }
+        }
+        return nullptr;
+    }
+    void free(char* p, size_t bytes) {
+        bytes = (bytes + 255) / 256 * 256;
+        size_t off = (size_t)(p - base_);
+        auto next = free_.lower_bound(off);
+        if (next != free_.begin()) {
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == off) {
+                off = prev->first;
+                bytes += prev->second;
+                free_.erase(prev);
+            }
+        }
+        next = free_.lower_bound(off);
+        if (next != free_.end() && off + bytes == next->first) {
+            bytes += next->second;
+            free_.erase(next);
+        }
+        free_[off] = bytes;
+    }
+    size_t size() const { return size_; }
+    bool empty() const { return base_ == nullptr; }
+
+  private:
+    char* base_ = nullptr;
+    size_t size_ = 0;
+    std::map<size_t, size_t> free_;
+};
+
+// ---------------------------------------------------------------- instance
+// Host-side description of one instance: validated inputs, start state,
+// ladder, and where its device buffers live.
+struct HostInst {
+    int n = 0, N = 0, R = 0;
+    uint64_t seed = 0;
+    std::vector<int64_t> A0, Astart, Xt0;  // original input, X0^T A0 X0, X0^T
+    std::vector<int32_t> act;
+    int64_t score0 = 0, offnz0 = 0;
+    double t_init = 0;
+    std::vector<double> temps;
+    InstParams P{};
+    bool log_on = false;
+    int64_t log_cap = 0;
+    Layout lay{};
+    size_t bytes = 0;
+    char* dmem = nullptr;
+    Inst rec{};
+    int64_t* A_out = nullptr;
+    int64_t* X_out = nullptr;
+    uint64_t tag = 0;
+    double submit_s = 0, admit_s = 0;
+};
+
+int64_t score_of(const int64_t* A, int n) {
+    int64_t full = 0, diag = 0;
+    for (int64_t q = 0; q < (int64_t)n * n; ++q) full += iabs(A[q]);
+    for (int i = 0; i < n; ++i) diag += iabs(A[(int64_t)i * n + i]);
+    return 2 * full - diag;
+}
+
+int64_t offnz_of(const int64_t* A, int n) {
+    int64_t c = 0;
+    for (int i = 0; i < n; ++i)
+        for (int j = i + 1; j < n; ++j) c += A[(int64_t)i * n + j] != 0;
+    return c;
+}
+
+void check_params(const xtx_params& p) {
+    auto bad = [](const char* m) { throw Err{XTX_E_ARG, m}; };
+    if (p.replicas < 1) bad("replicas must be >= 1");
+    if (p.replicas > (1 << kRepBits) - 1) bad("replicas too large");
+    if (p.target_samples < 1 || p.target_samples > kMaxSamples)
+        bad("target_samples must be in [1, 16]");
+    if (p.exchange_interval < 1) bad("exchange_interval must be >= 1");
+    if (p.moves_per_cool < 1) bad("moves_per_cool must be >= 1");
+    if (!(p.cooling > 0.0)) bad("cooling must be > 0");
+    if (p.move_log && p.move_log_cap < 1) bad("move_log_cap must be >= 1 when move_log is set");
+    if (p.tempering && p.replicas > 8000) bad("tempering supports at most 8000 replicas");
+}
+
+// Builds the start state exactly: A_start = X0^T A0 X0 in __int128, rejecting
+// any input or start entry beyond the 2^48 bound (the reference does not check
+// its inputs, mat_io.hpp:432-458; here such inputs are an explicit error).
+void prepare(HostInst& h, const int64_t* A0, const int64_t* X0, int n, int R, uint64_t seed,
+             const int32_t* active, int n_active, const xtx_params& p) {
+    if (!A0 || n < 1) throw Err{XTX_E_ARG, "A0 must be non-null and n >= 1"};
+    if (n > kMaxN) throw Err{XTX_E_ARG, "n exceeds the maximum order"};
+    h.n = n;
+    h.R = R;
+    h.seed = seed;
+    const size_t nn = (size_t)n * n;
+    h.A0.assign(A0, A0 + nn);
+    for (size_t q = 0; q < nn; ++q)
+        if (iabs(A0[q]) > kLimit) throw Err{XTX_E_BOUND, "an entry of A exceeds 2^48"};
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < i; ++j)
+            if (A0[(size_t)i * n + j] != A0[(size_t)j * n + i])
+                throw Err{XTX_E_ARG, "A must be symmetric"};
+    h.Xt0.assign(nn, 0);
+    if (X0) {
+        for (size_t q = 0; q < nn; ++q)
+            if (iabs(X0[q]) > kLimit) throw Err{XTX_E_BOUND, "an entry of X exceeds 2^48"};
+        for (int r = 0; r < n; ++r)
+            for (int c = 0; c < n; ++c) h.Xt0[(size_t)c * n + r] = X0[(size_t)r * n + c];
+        std::vector<__int128> AX(nn, 0);
+        for (int r = 0; r < n; ++r)
+            for (int k = 0; k < n; ++k) {
+                const __int128 a = A0[(size_t)r * n + k];
+                if (!a) continue;
+                for (int c = 0; c < n; ++c) AX[(size_t)r * n + c] += a * X0[(size_t)k * n + c];
+            }
+        h.Astart.assign(nn, 0);
+        for (int r = 0; r < n; ++r)
+            for (int c = 0; c < n; ++c) {
+                __int128 s = 0;
+                for (int k = 0; k < n; ++k) {
+                    s += (__int128)X0[(size_t)k * n + r] * AX[(size_t)k * n + c];
+                    if (s > ((__int128)1 << 100) || s < -((__int128)1 << 100))
+                        throw Err{XTX_E_BOUND, "X^T A X exceeds the 2^48 bound"};
+                }
+                if (s > kLimit || s < -kLimit)
+                    throw Err{XTX_E_BOUND, "X^T A X exceeds the 2^48 bound"};
+                h.Astart[(size_t)r * n + c] = (int64_t)s;
+            }
+    } else {
+        for (int i = 0; i < n; ++i) h.Xt0[(size_t)i * n + i] = 1;
+        h.Astart = h.A0;
+    }
+    if (active && n_active > 0) {
+        std::vector<char> seen((size_t)n, 0);
+        for (int q = 0; q < n_active; ++q) {
+            if (active[q] < 0 || active[q] >= n) throw Err{XTX_E_ARG, "active index out of range"};
+            if (seen[(size_t)active[q]]) throw Err{XTX_E_ARG, "duplicate active index"};
+            seen[(size_t)active[q]] = 1;
+            h.act.push_back(active[q]);
+        }
+    } else {
+        for (int q = 0; q < n; ++q) h.act.push_back(q);
+    }
+    h.N = (int)h.act.size();
+    h.score0 = score_of(h.Astart.data(), n);
+    h.offnz0 = offnz_of(h.Astart.data(), n);
+    // suggest_t_init (main.cpp:383-387)
+    const double avg = (double)h.score0 / (double)((int64_t)n * n);
+    h.t_init = p.t_init > 0.0 ? p.t_init : std::max(1.0, 2.0 * avg);
+    const bool tempering = p.tempering && R >= 2;
+    if (tempering) {
+        // geometric ladder from min(t_min, top) to top (congruence_anneal.hpp:454-468)
+        h.temps.resize((size_t)R);
+        const double top = h.t_init;
+        const double bot = std::min(p.t_min, top);
+        const double ratio = std::pow(top / std::max(bot, 1e-12), 1.0 / (double)(R - 1));
+        double t = bot;
+        for (int k = 0; k < R; ++k) {
+            h.temps[(size_t)k] = t;
+            t *= ratio;
+        }
+    }
+    h.P.t_init = h.t_init;
+    h.P.t_min = p.t_min;
+    h.P.cooling = p.cooling;
+    h.P.reheat = p.reheat;
+    h.P.reseed_factor = p.reseed_factor;
+    h.P.target_accept = p.target_accept;
+    h.P.sweep_fraction = p.sweep_fraction;
+    h.P.moves_per_cool = p.moves_per_cool;
+    h.P.stuck_threshold = p.stuck_threshold;
+    h.P.target_samples = p.target_samples;
+    h.P.tempering = tempering ? 1 : 0;
+    h.P.adaptive = p.adaptive_cooling;
+    h.P.epoch_moves = p.exchange_interval;
+    h.P.max_moves = p.max_moves;
+    h.P.max_epochs = p.max_epochs;
+    h.log_on = p.move_log != 0;
+    h.log_cap = h.log_on ? p.move_log_cap : 0;
+    h.lay = layout_for(n);
+    if (h.lay.rep_bytes > kLdsPerCu)
+        throw Err{XTX_E_ARG, "n too large for an LDS-resident replica (A and X)"};
+}
+
+size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+size_t device_bytes(const HostInst& h) {
+    const size_t nn = (size_t)h.n * h.n;
+    size_t b = 0;
+    b += 4 * align256((size_t)h.R * nn * 8);            // A, Xt, bA, bXt
+    b += 2 * align256(nn * 8);                          // gA, gXt
+    b += align256((size_t)h.R * sizeof(RepState));      // rs
+    b += align256((size_t)h.R * 8);                     // temps
+    b += align256((size_t)h.n);                         // act
+    b += align256((size_t)h.R * h.log_cap * sizeof(LogEnt));
+    return b;
+}
+
+// Lays out the instance's device buffers in dmem and uploads its start state
+// (staging vectors live in h until the stream is synchronized).
+struct Staging {
+    std::vector<RepState> rs;
+    std::vector<uint8_t> act;
+};
+
+void upload(HostInst& h, char* dmem, hipStream_t st, Staging& stg) {
+    h.dmem = dmem;
+    const size_t nn = (size_t)h.n * h.n;
+    char* p = dmem;
+    auto take = [&](size_t bytes) {
+        char* q = p;
+        p += align256(bytes);
+        return q;
+    };
+    Inst& I = h.rec;
+    I = Inst{};
+    I.n = h.n;
+    I.N = h.N;
+    I.R = h.R;
+    I.status = ST_RUNNING;
+    I.key0 = (uint32_t)h.seed;
+    I.key1 = (uint32_t)(h.seed >> 32);
+    I.epoch = 0;
+    I.log_on = h.log_on;
+    I.A = (int64_t*)take((size_t)h.R * nn * 8);
+    I.Xt = (int64_t*)take((size_t)h.R * nn * 8);
+    I.bA = (int64_t*)take((size_t)h.R * nn * 8);
+    I.bXt = (int64_t*)take((size_t)h.R * nn * 8);
+    I.gA = (int64_t*)take(nn * 8);
+    I.gXt = (int64_t*)take(nn * 8);
+    I.rs = (RepState*)take((size_t)h.R * sizeof(RepState));
+    I.temps = (double*)take((size_t)h.R * 8);
+    I.act = (uint8_t*)take((size_t)h.n);
+    I.log = h.log_on ? (LogEnt*)take((size_t)h.R * h.log_cap * sizeof(LogEnt)) : nullptr;
+    I.log_cap = h.log_cap;
+    I.P = h.P;
+    I.solve_key = kNoKey;
+    I.gb_score = h.score0;
+    I.gb_offnz = h.offnz0;
+    I.gb_rep = 0;
+    I.solve_rep = -1;
+    I.solve_move = -1;
+
+    stg.rs.assign((size_t)h.R, RepState{});
+    const bool div = h.R > 1;
+    for (int r = 0; r < h.R; ++r) {
+        RepState& s = stg.rs[(size_t)r];
+        s.score = h.score0;
+        s.offnz = h.offnz0;
+        s.local_best = h.score0;
+        s.T = h.t_init;
+        s.cooling = h.P.cooling;
+        s.rung = r;
+        s.best_score = h.score0;
+        s.best_move = 0;
+        s.flags = RF_CUR_BEST;
+        s.gf = 0;
+        s.tf = 0;
+        (void)div;
+    }
+    stg.act.assign(h.act.begin(), h.act.end());
+    HIPCHK(hipMemcpyAsync(I.gA, h.Astart.data(), nn * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(I.gXt, h.Xt0.data(), nn * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(launch_replicate(st, I.A, I.gA, (int64_t)nn, h.R));
+    HIPCHK(launch_replicate(st, I.Xt, I.gXt, (int64_t)nn, h.R));
+    HIPCHK(hipMemcpyAsync(I.rs, stg.rs.data(), (size_t)h.R * sizeof(RepState),
+                          hipMemcpyHostToDevice, st));
+    if (!h.temps.empty())
+        HIPCHK(hipMemcpyAsync(I.temps, h.temps.data(), (size_t)h.R * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(I.act, stg.act.data(), stg.act.size(), hipMemcpyHostToDevice, st));
+}
+
+void set_fractions(HostInst& h, Staging& stg, const xtx_params& p) {
+    // per-worker diversity (congruence_anneal.hpp:200-207), replica index in
+    // place of the thread id
+    for (int r = 0; r < h.R; ++r) {
+        double g = p.greedy_fraction, t = p.target_fraction;
+        if (p.worker_diversity > 0.0 && h.R > 1) {
+            const double off = p.worker_diversity * ((double)r / (double)(h.R - 1) - 0.5);
+            g = std::clamp(g + off, 0.0, 1.0);
+            t = std::clamp(t + off, 0.0, 1.0);
+        }
+        stg.rs[(size_t)r].gf = g;
+        stg.rs[(size_t)r].tf = t;
+    }
+}
+
+// Result of one instance, read back from the device.
+struct Outcome {
+    bool solved = false;
+    int rep = 0;
+    int64_t solve_move = -1;
+    int64_t best_score = 0;
+    std::vector<int64_t> A, Xt;
+    int64_t moves = 0, accepts = 0, greedy = 0, infeasible = 0, sweeps = 0;
+    std::vector<RepState> rs;
+};
+
+void fetch_outcome(const HostInst& h, const Inst& rec, hipStream_t st, Outcome& out) {
+    const size_t nn = (size_t)h.n * h.n;
+    out.rs.resize((size_t)h.R);
+    HIPCHK(hipMemcpyAsync(out.rs.data(), rec.rs, (size_t)h.R * sizeof(RepState),
+                          hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    out.moves = out.accepts = out.greedy = out.infeasible = out.sweeps = 0;
+    for (const auto& s : out.rs) {
+        out.moves += (int64_t)s.moves;
+        out.accepts += s.accepts;
+        out.greedy += s.greedy;
+        out.infeasible += s.infeasible;
+        out.sweeps += s.sweeps;
+    }
+    out.A.resize(nn);
+    out.Xt.resize(nn);
+    const int64_t* srcA;
+    const int64_t* srcX;
+    if (rec.status == ST_SOLVED) {
+        out.solved = true;
+        out.rep = rec.solve_rep;
+        out.solve_move = rec.solve_move;
+        srcA = rec.A + (size_t)out.rep * nn;
+        srcX = rec.Xt + (size_t)out.rep * nn;
+    } else {
+        // instance best: argmin (best_score, best_move, replica)
+        int b = 0;
+        for (int r = 1; r < h.R; ++r) {
+            const auto& x = out.rs[(size_t)r];
+            const auto& y = out.rs[(size_t)b];
+            if (x.best_score < y.best_score ||
+                (x.best_score == y.best_score && x.best_move < y.best_move))
+                b = r;
+        }
+        const auto& x = out.rs[(size_t)b];
+        if (!(x.flags & RF_CUR_BEST) && (x.flags & RF_BEST_STALE))
+            throw Err{XTX_E_STATE, "internal: best replica has a stale snapshot"};
+        out.rep = b;
+        const bool cur = (x.flags & RF_CUR_BEST) != 0;
+        srcA = (cur ? rec.A : rec.bA) + (size_t)b * nn;
+        srcX = (cur ? rec.Xt : rec.bXt) + (size_t)b * nn;
+    }
+    HIPCHK(hipMemcpyAsync(out.A.data(), srcA, nn * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(out.Xt.data(), srcX, nn * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    out.best_score = score_of(out.A.data(), h.n);
+}
+
+void write_outputs(const HostInst& h, const Outcome& o, int64_t* A_out, int64_t* X_out) {
+    const int n = h.n;
+    if (A_out) std::memcpy(A_out, o.A.data(), o.A.size() * 8);
+    if (X_out)
+        for (int r = 0; r < n; ++r)
+            for (int c = 0; c < n; ++c) X_out[(size_t)r * n + c] = o.Xt[(size_t)c * n + r];
+}
+
+int verify_outcome(const HostInst& h, const Outcome& o) {
+    std::vector<int64_t> X((size_t)h.n * h.n);
+    write_outputs(h, o, nullptr, X.data());
+    xtx_verify_report rep{};
+    xtx_verify(h.A0.data(), X.data(), o.A.data(), h.n, &rep);
+    // a diagonal is required only of solved results
+    return o.solved ? rep.status : (rep.status & ~2);
+}
+
+// Per-bucket replica tables for a set of resident instances.
+struct Bucket {
+    Layout lay;
+    std::vector<RepRef> refs;
+    RepRef* d_refs = nullptr;
+    int cap = 0;
+    int groups_per_block = 1;
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------- context
+struct Server;
+
+struct xtx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int* stop_host = nullptr;  // pinned, mapped
+    int* stop_dev = nullptr;
+    std::string err;
+    hipDeviceProp_t prop{};
+    // last plain solve
+    Arena plain_arena;
+    HostInst plain;
+    Inst* d_plain = nullptr;
+    std::vector<RepState> plain_rs;
+    std::vector<LogEnt> plain_log;
+    std::vector<int64_t> plain_log_count;
+    bool plain_log_valid = false;
+    std::unique_ptr<Server> server;
+};
+
+static std::string g_open_error;
+
+namespace {
+
+int fail(xtx_ctx* ctx, const Err& e) {
+    if (ctx) ctx->err = e.msg;
+    return e.code;
+}
+
+// Watchdog: raises the stop flag at a wall-clock deadline (the reference's
+// monitor-thread max_seconds check, congruence_anneal.hpp:512-519).
+class Deadline {
+  public:
+    Deadline(int* flag, double seconds) {
+        if (!(seconds > 0.0)) return;
+        th_ = std::thread([this, flag, seconds] {
+            std::unique_lock<std::mutex> lk(m_);
+            if (!cv_.wait_for(lk, std::chrono::duration<double>(seconds), [this] { return done_; })) {
+                fired_ = true;
+                *(volatile int*)flag = 1;
+            }
+        });
+    }
+    ~Deadline() {
+        if (th_.joinable()) {
+            {
+                std::lock_guard<std::mutex> lk(m_);
+                done_ = true;
+            }
+            cv_.notify_all();
+            th_.join();
+        }
+    }
+    bool fired() const { return fired_; }
+
+  private:
+    std::thread th_;
+    std::mutex m_;
+    std::condition_variable cv_;
+    bool done_ = false;
+    std::atomic<bool> fired_{false};
+};
+
+int groups_per_block_for(const Layout& L, int waves) {
+    const int per_wave = 64 / L.G;
+    int w = waves > 0 ? waves : 1;
+    // keep a block within the LDS of one CU
+    while (w > 1 && (int64_t)w * per_wave * L.rep_bytes > kLdsPerCu) --w;
+    return w * per_wave;
+}
+
+// Launches one epoch (all buckets, then k_epoch_end) on a stream.
+void launch_epoch(hipStream_t st, std::vector<Bucket>& buckets, Inst* d_insts, int n_insts,
+                  int max_r, const int* stop_dev) {
+    for (auto& b : buckets) {
+        if (b.refs.empty()) continue;
+        HIPCHK(launch_anneal(st, b.lay.G, b.lay.C, b.groups_per_block, b.d_refs,
+                             (int)b.refs.size(), d_insts, b.lay.P, b.lay.nmax, b.lay.rep_bytes,
+                             stop_dev));
+    }
+    HIPCHK(launch_epoch_end(st, d_insts, n_insts, max_r));
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- basics
+extern "C" {
+
+void xtx_params_default(xtx_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->t_init = 0.0;
+    p->t_min = 1e-3;
+    p->cooling = 0.999;
+    p->reheat = 1.0;
+    p->reseed_factor = 1.25;
+    p->greedy_fraction = 0.5;
+    p->target_fraction = 0.5;
+    p->target_accept = 0.44;
+    p->worker_diversity = 0.4;
+    p->sweep_fraction = 0.0;
+    p->max_seconds = 0.0;
+    p->save_interval = 2.0;
+    p->moves_per_cool = 200;
+    p->stuck_threshold = 20000;
+    p->target_samples = 8;
+    p->tempering = 1;
+    p->adaptive_cooling = 0;
+    p->exchange_interval = 2000;
+    p->seed = 0;
+    p->replicas = 64;
+    p->move_log = 0;
+    p->move_log_cap = 0;
+    p->max_moves = 0;
+    p->max_epochs = 0;
+    p->verify = 1;
+    p->waves_per_block = 0;
+}
+
+int xtx_open(int device, xtx_ctx** out) {
+    if (!out) return XTX_E_ARG;
+    *out = nullptr;
+    auto ctx = std::make_unique<xtx_ctx>();
+    try {
+        int count = 0;
+        HIPCHK(hipGetDeviceCount(&count));
+        if (device < 0 || device >= count) throw Err{XTX_E_ARG, "no such device"};
+        ctx->device = device;
+        HIPCHK(hipSetDevice(device));
+        HIPCHK(hipGetDeviceProperties(&ctx->prop, device));
+        if (std::strncmp(ctx->prop.gcnArchName, "gfx950", 6) != 0)
+            throw Err{XTX_E_DEVICE, std::string("built for gfx950, device is ") +
+                                        ctx->prop.gcnArchName};
+        HIPCHK(init_kernel_attrs());
+        HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        HIPCHK(hipEventCreate(&ctx->ev0));
+        HIPCHK(hipEventCreate(&ctx->ev1));
+        HIPCHK(hipHostMalloc((void**)&ctx->stop_host, sizeof(int), hipHostMallocMapped));
+        *ctx->stop_host = 0;
+        HIPCHK(hipHostGetDevicePointer((void**)&ctx->stop_dev, ctx->stop_host, 0));
+    } catch (const Err& e) {
+        g_open_error = e.msg;
+        return e.code;
+    }
+    *out = ctx.release();
+    return XTX_OK;
+}
+
+int xtx_serve_close(xtx_ctx* ctx);
+
+int xtx_close(xtx_ctx* ctx) {
+    if (!ctx) return XTX_E_ARG;
+    (void)hipSetDevice(ctx->device);
+    xtx_serve_close(ctx);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->d_plain) (void)hipFree(ctx->d_plain);
+    ctx->plain_arena.release();
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->stop_host) (void)hipHostFree(ctx->stop_host);
+    delete ctx;
+    return XTX_OK;
+}
+
+const char* xtx_last_error(const xtx_ctx* ctx) {
+    return ctx ? ctx->err.c_str() : g_open_error.c_str();
+}
+
+void xtx_request_stop(xtx_ctx* ctx) {
+    if (ctx && ctx->stop_host) *(volatile int*)ctx->stop_host = 1;
+}
+
+void xtx_clear_stop(xtx_ctx* ctx) {
+    if (ctx && ctx->stop_host) *(volatile int*)ctx->stop_host = 0;
+}
+
+int xtx_query_limits(xtx_ctx* ctx, xtx_limits* out) {
+    if (!ctx || !out) return XTX_E_ARG;
+    std::memset(out, 0, sizeof(*out));
+    out->compute_units = ctx->prop.multiProcessorCount;
+    out->lds_bytes_per_cu = kLdsPerCu;
+    out->lds_bytes_per_block_max = (int)ctx->prop.sharedMemPerBlock;
+    out->wave_size = ctx->prop.warpSize;
+    out->max_n = kMaxN;
+    int mx = 0;
+    for (int n = 2; n <= kMaxN; ++n)
+        if (layout_for(n).rep_bytes <= kLdsPerCu) mx = n;
+    out->max_n_lds = mx;
+    out->hbm_bytes = (int64_t)ctx->prop.totalGlobalMem;
+    std::snprintf(out->arch, sizeof(out->arch), "%s", ctx->prop.gcnArchName);
+    return XTX_OK;
+}
+
+// ---------------------------------------------------------------- plain form
+int xtx_solve_plain(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t n,
+                    const xtx_params* params, const int32_t* active, int32_t n_active,
+                    int64_t* A_out, int64_t* X_out, xtx_stats* stats) {
+    if (!ctx || !params) return XTX_E_ARG;
+    const auto t0 = clk::now();
+    try {
+        HIPCHK(hipSetDevice(ctx->device));
+        const xtx_params& p = *params;
+        check_params(p);
+        ctx->plain_log_valid = false;
+        HostInst& h = ctx->plain;
+        h = HostInst{};
+        prepare(h, A0, X0, n, p.replicas, p.seed, active, n_active, p);
+        xtx_stats st{};
+        st.verify_status = -1;
+        if (n < 2 || h.N < 2 || h.offnz0 == 0) {
+            // nothing to anneal (congruence_anneal.hpp:435, :444); an already
+            // diagonal start is reported solved
+            Outcome o;
+            o.A = h.Astart;
+            o.Xt = h.Xt0;
+            o.solved = h.offnz0 == 0;
+            o.best_score = h.score0;
+            write_outputs(h, o, A_out, X_out);
+            st.solved = o.solved;
+            st.solve_move = 0;
+            st.best_score = h.score0;
+            st.stop_reason = 4;
+            st.verify_status = p.verify ? verify_outcome(h, o) : -1;
+            st.seconds = std::chrono::duration<double>(clk::now() - t0).count();
+            if (stats) *stats = st;
+            ctx->plain_rs.clear();
+            return XTX_OK;
+        }
+        ctx->plain_arena.reserve(device_bytes(h));
+        Staging stg;
+        char* dm = ctx->plain_arena.alloc(device_bytes(h));
+        if (!dm) throw Err{XTX_E_NOMEM, "arena"};
+        upload(h, dm, ctx->stream, stg);
+        set_fractions(h, stg, p);
+        HIPCHK(hipMemcpyAsync(h.rec.rs, stg.rs.data(), (size_t)h.R * sizeof(RepState),
+                              hipMemcpyHostToDevice, ctx->stream));
+        if (!ctx->d_plain) HIPCHK(hipMalloc(&ctx->d_plain, sizeof(Inst)));
+        HIPCHK(hipMemcpyAsync(ctx->d_plain, &h.rec, sizeof(Inst), hipMemcpyHostToDevice,
+                              ctx->stream));
+        std::vector<Bucket> buckets(1);
+        buckets[0].lay = h.lay;
+        buckets[0].groups_per_block = groups_per_block_for(h.lay, p.waves_per_block);
+        for (int r = 0; r < h.R; ++r) buckets[0].refs.push_back(RepRef{0, r});
+        HIPCHK(hipMalloc(&buckets[0].d_refs, buckets[0].refs.size() * sizeof(RepRef)));
+        HIPCHK(hipMemcpyAsync(buckets[0].d_refs, buckets[0].refs.data(),
+                              buckets[0].refs.size() * sizeof(RepRef), hipMemcpyHostToDevice,
+                              ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+
+        Inst* pin = nullptr;
+        HIPCHK(hipHostMalloc((void**)&pin, sizeof(Inst), hipHostMallocDefault));
+        const int stop_before = *(volatile int*)ctx->stop_host;
+        (void)stop_before;
+        double kernel_ms = 0;
+        int epochs = 0;
+        int reason = 3;
+        {
+            Deadline dl(ctx->stop_host, p.max_seconds);
+            for (;;) {
+                HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+                for (auto& b : buckets)
+                    HIPCHK(launch_anneal(ctx->stream, b.lay.G, b.lay.C, b.groups_per_block,
+                                         b.d_refs, (int)b.refs.size(), ctx->d_plain, b.lay.P,
+                                         b.lay.nmax, b.lay.rep_bytes, ctx->stop_dev));
+                HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+                HIPCHK(launch_epoch_end(ctx->stream, ctx->d_plain, 1, h.R));
+                HIPCHK(hipMemcpyAsync(pin, ctx->d_plain, sizeof(Inst), hipMemcpyDeviceToHost,
+                                      ctx->stream));
+                HIPCHK(hipStreamSynchronize(ctx->stream));
+                float ms = 0;
+                HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+                kernel_ms += ms;
+                ++epochs;
+                if (pin->status == ST_SOLVED) {
+                    reason = 0;
+                    break;
+                }
+                if (pin->status == ST_EXHAUSTED) {
+                    reason = 3;
+                    break;
+                }
+                if (*(volatile int*)ctx->stop_host) {
+                    reason = dl.fired() ? 1 : 2;
+                    break;
+                }
+                if (p.max_epochs > 0 && epochs >= p.max_epochs) {
+                    reason = 3;
+                    break;
+                }
+            }
+        }
+        if (reason == 1) *(volatile int*)ctx->stop_host = 0;  // deadline flag is ours
+        h.rec = *pin;
+        (void)hipHostFree(pin);
+        (void)hipFree(buckets[0].d_refs);
+        Outcome o;
+        fetch_outcome(h, h.rec, ctx->stream, o);
+        ctx->plain_rs = o.rs;
+        write_outputs(h, o, A_out, X_out);
+        st.moves = o.moves;
+        st.accepts = o.accepts;
+        st.greedy_calls = o.greedy;
+        st.infeasible = o.infeasible;
+        st.sweeps = o.sweeps;
+        st.exchanges_tried = h.rec.exch_tried;
+        st.exchanges_accepted = h.rec.exch_acc;
+        st.solved = o.solved;
+        st.solve_replica = o.solved ? o.rep : -1;
+        st.solve_move = o.solve_move;
+        st.best_score = o.best_score;
+        st.best_replica = o.rep;
+        st.epochs = epochs;
+        st.stop_reason = reason;
+        st.kernel_ms = kernel_ms;
+        st.verify_status = p.verify ? verify_outcome(h, o) : -1;
+        st.seconds = std::chrono::duration<double>(clk::now() - t0).count();
+        if (stats) *stats = st;
+        if (h.log_on) {
+            ctx->plain_log.resize((size_t)h.R * h.log_cap);
+            HIPCHK(hipMemcpy(ctx->plain_log.data(), h.rec.log, ctx->plain_log.size() * sizeof(LogEnt),
+                             hipMemcpyDeviceToHost));
+            ctx->plain_log_count.resize((size_t)h.R);
+            for (int r = 0; r < h.R; ++r) ctx->plain_log_count[(size_t)r] = o.rs[(size_t)r].log_count;
+            ctx->plain_log_valid = true;
+        }
+        if (p.verify && st.verify_status != 0) {
+            ctx->err = "result failed exact verification";
+            return XTX_E_VERIFY;
+        }
+        return XTX_OK;
+    } catch (const Err& e) {
+        return fail(ctx, e);
+    }
+}
+
+int xtx_last_replicas(xtx_ctx* ctx, int64_t* A, int64_t* Xt, int64_t* scal) {
+    if (!ctx) return XTX_E_ARG;
+    try {
+        const HostInst& h = ctx->plain;
+        if (ctx->plain_rs.empty() || h.dmem == nullptr) throw Err{XTX_E_STATE, "no plain solve"};
+        const size_t nn = (size_t)h.n * h.n;
+        if (A)
+            HIPCHK(hipMemcpy(A, h.rec.A, (size_t)h.R * nn * 8, hipMemcpyDeviceToHost));
+        if (Xt)
+            HIPCHK(hipMemcpy(Xt, h.rec.Xt, (size_t)h.R * nn * 8, hipMemcpyDeviceToHost));
+        if (scal)
+            for (int r = 0; r < h.R; ++r) {
+                const RepState& s = ctx->plain_rs[(size_t)r];
+                int64_t* o = scal + (size_t)r * 8;
+                o[0] = s.score;
+                o[1] = s.offnz;
+                o[2] = (int64_t)s.moves;
+                o[3] = s.best_score;
+                o[4] = s.best_move;
+                o[5] = s.rung;
+                o[6] = s.accepts;
+                o[7] = s.local_best;
+            }
+        return XTX_OK;
+    } catch (const Err& e) {
+        return fail(ctx, e);
+    }
+}
+
+int xtx_movelog_get(xtx_ctx* ctx, int32_t replica, const xtx_log_entry** entries, int64_t* count,
+                    int32_t* truncated) {
+    if (!ctx || !entries || !count) return XTX_E_ARG;
+    if (!ctx->plain_log_valid) {
+        ctx->err = "no move log (set move_log before the solve)";
+        return XTX_E_STATE;
+    }
+    const HostInst& h = ctx->plain;
+    if (replica < 0 || replica >= h.R) return XTX_E_ARG;
+    const int64_t c = ctx->plain_log_count[(size_t)replica];
+    *entries = reinterpret_cast<const xtx_log_entry*>(ctx->plain_log.data() +
+                                                      (size_t)replica * h.log_cap);
+    *count = std::min(c, h.log_cap);
+    if (truncated) *truncated = c > h.log_cap;
+    return XTX_OK;
+}
+
+int xtx_movelog_free(xtx_ctx* ctx) {
+    if (!ctx) return XTX_E_ARG;
+    ctx->plain_log.clear();
+    ctx->plain_log.shrink_to_fit();
+    ctx->plain_log_count.clear();
+    ctx->plain_log_valid = false;
+    return XTX_OK;
+}
+
+// ---------------------------------------------------------------- probes
+int xtx_probe_moves(xtx_ctx* ctx, const int64_t* A, const int64_t* Xt, int32_t n, int32_t n_mats,
+                    const int32_t* queries, int32_t n_queries, int64_t* out) {
+    if (!ctx || !A || !Xt || !queries || !out || n < 2 || n > kMaxN || n_mats < 1) return XTX_E_ARG;
+    try {
+        for (int q = 0; q < n_queries; ++q) {
+            const int32_t* x = queries + 4 * q;
+            if (x[0] < 0 || x[0] >= n_mats || x[1] < 0 || x[1] >= n || x[2] < 0 || x[2] >= n ||
+                x[1] == x[2] || x[3] < -kShearCap || x[3] > kShearCap)
+                throw Err{XTX_E_ARG, "bad probe query"};
+        }
+        HIPCHK(hipSetDevice(ctx->device));
+        const size_t nn = (size_t)n * n * n_mats;
+        int64_t *dA = nullptr, *dX = nullptr, *dout = nullptr;
+        int32_t* dq = nullptr;
+        HIPCHK(hipMalloc(&dA, nn * 8));
+        HIPCHK(hipMalloc(&dX, nn * 8));
+        HIPCHK(hipMalloc(&dq, (size_t)n_queries * 16));
+        HIPCHK(hipMalloc(&dout, (size_t)n_queries * 40));
+        HIPCHK(hipMemcpy(dA, A, nn * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dX, Xt, nn * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dq, queries, (size_t)n_queries * 16, hipMemcpyHostToDevice));
+        const Layout L = layout_for(n);
+        HIPCHK(launch_probe(ctx->stream, L.G, L.C, dA, dX, n, dq, n_queries, dout));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        HIPCHK(hipMemcpy(out, dout, (size_t)n_queries * 40, hipMemcpyDeviceToHost));
+        (void)hipFree(dA);
+        (void)hipFree(dX);
+        (void)hipFree(dq);
+        (void)hipFree(dout);
+        return XTX_OK;
+    } catch (const Err& e) {
+        return fail(ctx, e);
+    }
+}
+
+int xtx_probe_rng(xtx_ctx* ctx, const uint32_t* ctr_key, int32_t count, uint32_t* out,
+                  const double* x, int32_t nx, double* exp_out) {
+    if (!ctx || count < 0 || nx < 0) return XTX_E_ARG;
+    try {
+        HIPCHK(hipSetDevice(ctx->device));
+        uint32_t *dck = nullptr, *dout = nullptr;
+        double *dx = nullptr, *dex = nullptr;
+        HIPCHK(hipMalloc(&dck, (size_t)std::max(count, 1) * 24));
+        HIPCHK(hipMalloc(&dout, (size_t)std::max(count, 1) * 16));
+        HIPCHK(hipMalloc(&dx, (size_t)std::max(nx, 1) * 8));
+        HIPCHK(hipMalloc(&dex, (size_t)std::max(nx, 1) * 8));
+        if (count) HIPCHK(hipMemcpy(dck, ctr_key, (size_t)count * 24, hipMemcpyHostToDevice));
+        if (nx) HIPCHK(hipMemcpy(dx, x, (size_t)nx * 8, hipMemcpyHostToDevice));
+        HIPCHK(launch_probe_rng(ctx->stream, dck, count, dout, dx, nx, dex));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        if (count) HIPCHK(hipMemcpy(out, dout, (size_t)count * 16, hipMemcpyDeviceToHost));
+        if (nx) HIPCHK(hipMemcpy(exp_out, dex, (size_t)nx * 8, hipMemcpyDeviceToHost));
+        (void)hipFree(dck);
+        (void)hipFree(dout);
+        (void)hipFree(dx);
+        (void)hipFree(dex);
+        return XTX_OK;
+    } catch (const Err& e) {
+        return fail(ctx, e);
+    }
+}
+
+}  // extern "C"
+
+// =====================================================================
+// Batched and served forms share one scheduler: a set of resident
+// instances, grouped into LDS layout buckets, advanced epoch by epoch.
+// =====================================================================
+struct Server {
+    xtx_ctx* ctx = nullptr;
+    xtx_params p{};
+    xtx_serve_config cfg{};
+    Arena arena;
+    std::vector<std::unique_ptr<HostInst>> slots;  // resident, index = slot id
+    std::vector<int> free_slots;
+    std::deque<std::unique_ptr<HostInst>> queue;
+    std::vector<xtx_result> done;
+    std::vector<Inst> h_insts;
+    Inst* d_insts = nullptr;
+    int cap_insts = 0;
+    std::map<int, Bucket> buckets;  // key: nmax
+    bool dirty = true;
+    int64_t lds_in_use = 0;
+    int max_r = 1;
+    clk::time_point t0 = clk::now();
+    double kernel_ms = 0;
+    int epochs = 0;
+    int64_t moves_retired = 0, accepts_retired = 0;
+    // graph of several epochs, rebuilt when residency changes
+    hipGraphExec_t gexec = nullptr;
+    int graph_epochs = 0;
+
+    ~Server() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        for (auto& kv : buckets)
+            if (kv.second.d_refs) (void)hipFree(kv.second.d_refs);
+        if (d_insts) (void)hipFree(d_insts);
+    }
+
+    double now() const { return std::chrono::duration<double>(clk::now() - t0).count(); }
+
+    int64_t lds_cost(const HostInst& h) const { return (int64_t)h.R * h.lay.rep_bytes; }
+
+    int64_t lds_budget() const {
+        if (cfg.lds_budget_bytes > 0) return cfg.lds_budget_bytes;
+        // every resident replica gets LDS once per pass of the device; allow
+        // a few passes of oversubscription so the dispatcher always has work
+        return (int64_t)ctx->prop.multiProcessorCount * kLdsPerCu * 4;
+    }
+
+    bool try_admit(std::unique_ptr<HostInst>& h) {
+        if (cfg.max_resident > 0 && (int)(slots.size() - free_slots.size()) >= cfg.max_resident)
+            return false;
+        if (lds_in_use + lds_cost(*h) > lds_budget() && lds_in_use > 0) return false;
+        char* dm = arena.alloc(h->bytes);
+        if (!dm) return false;
+        Staging stg;
+        upload(*h, dm, ctx->stream, stg);
+        set_fractions(*h, stg, p);
+        HIPCHK(hipMemcpyAsync(h->rec.rs, stg.rs.data(), (size_t)h->R * sizeof(RepState),
+                              hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        h->admit_s = now();
+        lds_in_use += lds_cost(*h);
+        int slot;
+        if (!free_slots.empty()) {
+            slot = free_slots.back();
+            free_slots.pop_back();
+            slots[(size_t)slot] = std::move(h);
+        } else {
+            slot = (int)slots.size();
+            slots.push_back(std::move(h));
+        }
+        (void)slot;
+        dirty = true;
+        return true;
+    }
+
+    void admit_queued() {
+        while (!queue.empty()) {
+            if (!try_admit(queue.front())) break;
+            queue.pop_front();
+        }
+    }
+
+    // Rebuild the device instance array and bucket tables after admissions or
+    // retirements.
+    void rebuild() {
+        if (!dirty) return;
+        const int ns = (int)slots.size();
+        if (ns > cap_insts) {
+            if (d_insts) (void)hipFree(d_insts);
+            cap_insts = std::max(ns, 2 * cap_insts);
+            HIPCHK(hipMalloc(&d_insts, (size_t)cap_insts * sizeof(Inst)));
+        }
+        h_insts.assign((size_t)ns, Inst{});
+        for (auto& kv : buckets) kv.second.refs.clear();
+        max_r = 1;
+        for (int s = 0; s < ns; ++s) {
+            if (!slots[(size_t)s]) {
+                h_insts[(size_t)s].status = ST_IDLE;
+                continue;
+            }
+            HostInst& h = *slots[(size_t)s];
+            h_insts[(size_t)s] = h.rec;
+            max_r = std::max(max_r, h.R);
+            Bucket& b = buckets[h.lay.nmax];
+            b.lay = h.lay;
+            b.groups_per_block = groups_per_block_for(h.lay, p.waves_per_block);
+            for (int r = 0; r < h.R; ++r) b.refs.push_back(RepRef{s, r});
+        }
+        for (auto& kv : buckets) {
+            Bucket& b = kv.second;
+            if ((int)b.refs.size() > b.cap) {
+                if (b.d_refs) (void)hipFree(b.d_refs);
+                b.cap = std::max((int)b.refs.size(), 2 * b.cap);
+                HIPCHK(hipMalloc(&b.d_refs, (size_t)b.cap * sizeof(RepRef)));
+            }
+            if (!b.refs.empty())
+                HIPCHK(hipMemcpyAsync(b.d_refs, b.refs.data(), b.refs.size() * sizeof(RepRef),
+                                      hipMemcpyHostToDevice, ctx->stream));
+        }
+        if (ns)
+            HIPCHK(hipMemcpyAsync(d_insts, h_insts.data(), (size_t)ns * sizeof(Inst),
+                                  hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        if (gexec) {
+            (void)hipGraphExecDestroy(gexec);
+            gexec = nullptr;
+        }
+        dirty = false;
+    }
+
+    std::vector<Bucket> bucket_list() {
+        std::vector<Bucket> v;
+        for (auto& kv : buckets)
+            if (!kv.second.refs.empty()) v.push_back(kv.second);
+        return v;
+    }
+
+    // Runs k epochs: captured once into a graph per residency, replayed.
+    void run_epochs(int k) {
+        const int ns = (int)slots.size();
+        if (ns == 0) return;
+        std::vector<Bucket> bl = bucket_list();
+        if (k > 1) {
+            if (!gexec || graph_epochs != k) {
+                if (gexec) (void)hipGraphExecDestroy(gexec);
+                hipGraph_t graph;
+                HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+                for (int e = 0; e < k; ++e)
+                    launch_epoch(ctx->stream, bl, d_insts, ns, max_r, ctx->stop_dev);
+                HIPCHK(hipStreamEndCapture(ctx->stream, &graph));
+                HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+                (void)hipGraphDestroy(graph);
+                graph_epochs = k;
+            }
+            HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+            HIPCHK(hipGraphLaunch(gexec, ctx->stream));
+            HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+        } else {
+            HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+            launch_epoch(ctx->stream, bl, d_insts, ns, max_r, ctx->stop_dev);
+            HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+        }
+        HIPCHK(hipMemcpyAsync(h_insts.data(), d_insts, (size_t)ns * sizeof(Inst),
+                              hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        for (int s = 0; s < ns; ++s)
+            if (slots[(size_t)s]) slots[(size_t)s]->rec = h_insts[(size_t)s];
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        kernel_ms += ms;
+        epochs += k;
+    }
+
+    // Retires every resident instance whose status left RUNNING (or all, when
+    // force); writes outputs and queues results.
+    void retire(bool force, uint64_t only_tag = 0, bool by_tag = false) {
+        for (size_t s = 0; s < slots.size(); ++s) {
+            if (!slots[s]) continue;
+            HostInst& h = *slots[s];
+            Inst& rec = h_insts[s];
+            if (by_tag && h.tag != only_tag) continue;
+            const bool fin = rec.status != ST_RUNNING;
+            if (!fin && !force && !by_tag) continue;
+            h.rec = rec;
+            Outcome o;
+            fetch_outcome(h, rec, ctx->stream, o);
+            write_outputs(h, o, h.A_out, h.X_out);
+            xtx_result res{};
+            res.solved = o.solved;
+            res.verify_status = p.verify ? verify_outcome(h, o) : -1;
+            res.solve_move = o.solve_move;
+            res.best_score = o.best_score;
+            res.moves = o.moves;
+            res.submit_s = h.submit_s;
+            res.admit_s = h.admit_s;
+            res.retire_s = now();
+            res.tag = h.tag;
+            done.push_back(res);
+            moves_retired += o.moves;
+            accepts_retired += o.accepts;
+            lds_in_use -= lds_cost(h);
+            arena.free(h.dmem, h.bytes);
+            slots[s].reset();
+            free_slots.push_back((int)s);
+            dirty = true;
+        }
+        // trim trailing free slots so the instance array stays compact
+        while (!slots.empty() && !slots.back()) {
+            slots.pop_back();
+            free_slots.erase(std::remove(free_slots.begin(), free_slots.end(), (int)slots.size()),
+                             free_slots.end());
+            dirty = true;
+        }
+    }
+
+    int resident() const { return (int)(slots.size() - free_slots.size()); }
+};
+
+namespace {
+
+std::unique_ptr<HostInst> make_inst(const xtx_instance& in, const xtx_params& p) {
+    auto h = std::make_unique<HostInst>();
+    const int R = in.replicas > 0 ? in.replicas : p.replicas;
+    xtx_params q = p;
+    q.replicas = R;
+    check_params(q);
+    prepare(*h, in.A0, in.X0, in.n, R, in.seed, in.active, in.n_active, q);
+    if (h->n < 2 || h->N < 2)
+        throw Err{XTX_E_ARG, "instance needs n >= 2 and at least two active indices"};
+    h->bytes = device_bytes(*h);
+    h->A_out = in.A_out;
+    h->X_out = in.X_out;
+    return h;
+}
+
+}  // namespace
+
+extern "C" {
+
+int xtx_batch_solve(xtx_ctx* ctx, const xtx_instance* instances, int32_t count,
+                    const xtx_params* params, xtx_result* results, xtx_batch_stats* stats) {
+    if (!ctx || !instances || count < 1 || !params) return XTX_E_ARG;
+    try {
+        HIPCHK(hipSetDevice(ctx->device));
+        Server S;
+        S.ctx = ctx;
+        S.p = *params;
+        S.cfg.lds_budget_bytes = INT64_MAX / 4;  // batch: everything resident
+        std::vector<std::unique_ptr<HostInst>> hs;
+        size_t total = 0;
+        for (int q = 0; q < count; ++q) {
+            hs.push_back(make_inst(instances[q], *params));
+            hs.back()->tag = (uint64_t)q;
+            total += hs.back()->bytes;
+        }
+        S.arena.reserve(total + 4096);
+        S.t0 = clk::now();
+        for (auto& h : hs) {
+            h->submit_s = 0;
+            if (!S.try_admit(h)) throw Err{XTX_E_NOMEM, "batch does not fit the device"};
+        }
+        S.rebuild();
+        // epochs per graph launch: about 8 ms of work between polls
+        const int k = 4;
+        Deadline dl(ctx->stop_host, params->max_seconds);
+        int reason = 0;
+        while (S.resident() > 0) {
+            S.run_epochs(k);
+            S.retire(false);
+            if (*(volatile int*)ctx->stop_host) {
+                reason = 1;
+                break;
+            }
+            if (params->max_epochs > 0 && S.epochs >= params->max_epochs) break;
+            S.rebuild();
+        }
+        S.retire(true);
+        if (reason && dl.fired()) *(volatile int*)ctx->stop_host = 0;
+        int solved = 0, verified = 0;
+        for (const auto& r : S.done) {
+            if (results) results[r.tag] = r;
+            solved += r.solved;
+            verified += r.solved && r.verify_status == 0;
+        }
+        if (stats) {
+            xtx_batch_stats bs{};
+            bs.moves = S.moves_retired;
+            bs.accepts = S.accepts_retired;
+            bs.instances = count;
+            bs.solved = solved;
+            bs.verified = verified;
+            bs.epochs = S.epochs;
+            bs.seconds = S.now();
+            bs.kernel_ms = S.kernel_ms;
+            bs.buckets = (int)S.buckets.size();
+            bs.graph_epochs = k;
+            *stats = bs;
+        }
+        return XTX_OK;
+    } catch (const Err& e) {
+        return fail(ctx, e);
+    }
+}
+
+int xtx_serve_open(xtx_ctx* ctx, const xtx_params* params, const xtx_serve_config* cfg) {
+    if (!ctx || !params) return XTX_E_ARG;
+    try {
+        if (ctx->server) throw Err{XTX_E_STATE, "serve already open"};
+        check_params(*params);
+        HIPCHK(hipSetDevice(ctx->device));
+        auto S = std::make_unique<Server>();
+        S->ctx = ctx;
+        S->p = *params;
+        if (cfg) S->cfg = *cfg;
+        const int64_t hbm = S->cfg.hbm_budget_bytes > 0 ? S->cfg.hbm_budget_bytes
+                                                         : (int64_t)8 << 30;
+        S->arena.reserve((size_t)hbm);
+        S->t0 = clk::now();
+        ctx->server = std::move(S);
+        return XTX_OK;
+    } catch (const Err& e) {
+        return fail(ctx, e);
+    }
+}
+
+int xtx_serve_submit(xtx_ctx* ctx, const xtx_instance* inst, uint64_t tag) {
+    if (!ctx || !inst) return XTX_E_ARG;
+    if (!ctx->server) {
+        ctx->err = "serve not open";
+        return XTX_E_STATE;
+    }
+    try {
+        Server& S = *ctx->server;
+        const int qcap = S.cfg.queue_capacity > 0 ? S.cfg.queue_capacity : 65536;
+        if ((int)S.queue.size() >= qcap) throw Err{XTX_E_FULL, "serve queue full"};
+        auto h = make_inst(*inst, S.p);
+        if ((int64_t)h->bytes > (int64_t)S.arena.size())
+            throw Err{XTX_E_NOMEM, "request larger than the HBM budget"};
+        h->tag = tag;
+        h->submit_s = S.now();
+        S.queue.push_back(std::move(h));
+        return XTX_OK;
+    } catch (const Err& e) {
+        return fail(ctx, e);
+    }
+}
+
+int xtx_serve_step(xtx_ctx* ctx, int32_t* resident, int32_t* queued) {
+    if (!ctx) return XTX_E_ARG;
+    if (!ctx->server) {
+        ctx->err = "serve not open";
+        return XTX_E_STATE;
+    }
+    try {
+        HIPCHK(hipSetDevice(ctx->device));
+        Server& S = *ctx->server;
+        S.admit_queued();
+        S.rebuild();
+        if (S.resident() > 0) {
+            S.run_epochs(1);
+            S.retire(false);
+        }
+        S.admit_queued();
+        if (resident) *resident = S.resident();
+        if (queued) *queued = (int32_t)S.queue.size();
+        return XTX_OK;
+    } catch (const Err& e) {
+        return fail(ctx, e);
+    }
+}
+
+int xtx_serve_poll(xtx_ctx* ctx, xtx_result* out, int32_t max, int32_t* count) {
+    if (!ctx || !count || (max > 0 && !out)) return XTX_E_ARG;
+    if (!ctx->server) {
+        ctx->err = "serve not open";
+        return XTX_E_STATE;
+    }
+    Server& S = *ctx->server;
+    const int k = std::min<int>(max, (int)S.done.size());
+    for (int t = 0; t < k; ++t) out[t] = S.done[(size_t)t];
+    S.done.erase(S.done.begin(), S.done.begin() + k);
+    *count = k;
+    return XTX_OK;
+}
+
+int xtx_serve_retire(xtx_ctx* ctx, uint64_t tag) {
+    if (!ctx) return XTX_E_ARG;
+    if (!ctx->server) {
+        ctx->err = "serve not open";
+        return XTX_E_STATE;
+    }
+    try {
+        Server& S = *ctx->server;
+        for (auto it = S.queue.begin(); it != S.queue.end(); ++it) {
+            if ((*it)->tag == tag) {
+                xtx_result res{};
+                res.tag = tag;
+                res.verify_status = -1;
+                res.submit_s = (*it)->submit_s;
+                res.retire_s = S.now();
+                S.done.push_back(res);
+                S.queue.erase(it);
+                return XTX_OK;
+            }
+        }
+        const size_t before = S.done.size();
+        if (!S.h_insts.empty() || S.resident()) {
+            // make sure h_insts reflects the device
+            const int ns = (int)S.slots.size();
+            S.h_insts.resize((size_t)ns);
+            if (ns)
+                HIPCHK(hipMemcpy(S.h_insts.data(), S.d_insts, (size_t)ns * sizeof(Inst),
+                                 hipMemcpyDeviceToHost));
+        }
+        S.retire(false, tag, true);
+        if (S.done.size() == before) throw Err{XTX_E_ARG, "unknown tag"};
+        return XTX_OK;
+    } catch (const Err& e) {
+        return fail(ctx, e);
+    }
+}
+
+int xtx_serve_close(xtx_ctx* ctx) {
+    if (!ctx) return XTX_E_ARG;
+    ctx->server.reset();
+    return XTX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,127 @@
+// engine_types.h -- records shared by the host engine and the gfx950 kernels.
+//
+// Layout in HBM (one instance = one matrix being diagonalized, R replicas):
+//   A, Xt      R * n * n int64   current state of every replica (row-major)
+//   bA, bXt    R * n * n int64   best state of every replica (lazily flushed)
+//   gA, gXt    n * n int64       instance-best snapshot (reseed source)
+//   rs         R RepState        per-replica scalars
+//   temps      R double          tempering ladder, rung -> temperature
+//   act        N uint8           active indices (the reference's `active` list)
+//   log        R * log_cap       accepted-move log per replica (optional)
+// During an epoch a replica's A and Xt live in LDS (row stride P, odd) and its
+// row masses in registers; they return to HBM at the epoch end.
+#pragma once
+#include <stdint.h>
+
+namespace xtx {
+
+constexpr int64_t kLimit = (int64_t)1 << 48;  // mat_io.hpp:35 (strict >)
+constexpr int kShearCap = 64;                 // mat_io.hpp:38
+constexpr int kMaxN = 128;
+constexpr int kMaxSamples = 16;
+constexpr int kRepBits = 20;                  // solve key = move << 20 | replica
+constexpr unsigned long long kNoKey = ~0ull;
+
+enum InstStatus : int32_t {
+    ST_IDLE = 0,
+    ST_RUNNING = 1,
+    ST_SOLVED = 2,
+    ST_EXHAUSTED = 3,  // every replica hit max_moves
+    ST_STOPPED = 4,    // host stop / time budget / forced retire
+};
+
+enum RepFlags : int32_t {
+    RF_CUR_BEST = 1,    // current state is this replica's best (not flushed)
+    RF_STOPPED = 2,
+    RF_SOLVED = 4,
+    RF_BEST_STALE = 8,  // bA/bXt do not hold best_score's state (cannot win)
+    RF_CAPPED = 16,
+};
+
+struct LogEnt {
+    int64_t move;
+    int16_t i, j, s;
+    uint16_t flags;
+};
+
+struct RepState {
+    int64_t score;
+    int64_t offnz;
+    uint64_t moves;
+    int64_t local_best;
+    int64_t msi;           // moves since improvement
+    double T;              // cooling-mode temperature
+    double cooling;        // cooling-mode factor (adaptive)
+    int32_t cool_counter;
+    int32_t win_p;
+    int32_t win_a;
+    int32_t rung;          // tempering rung
+    int64_t best_score;
+    int64_t best_move;
+    int32_t flags;
+    int32_t pad0;
+    double gf;             // per-replica greedy fraction (diversity applied)
+    double tf;             // per-replica target fraction
+    int64_t accepts;
+    int64_t greedy;
+    int64_t infeasible;
+    int64_t sweeps;
+    int64_t log_count;
+    int64_t solve_move;
+};
+
+struct InstParams {
+    double t_init;         // effective (auto already resolved)
+    double t_min;
+    double cooling;
+    double reheat;
+    double reseed_factor;
+    double target_accept;
+    double sweep_fraction;
+    int32_t moves_per_cool;
+    int32_t stuck_threshold;
+    int32_t target_samples;
+    int32_t tempering;
+    int32_t adaptive;
+    int32_t epoch_moves;
+    uint64_t max_moves;
+    int32_t max_epochs;    // 0 = none: the instance is exhausted after this many
+    int32_t pad0;
+};
+
+struct Inst {
+    int32_t n;
+    int32_t N;             // active count
+    int32_t R;
+    int32_t status;
+    uint32_t key0, key1;   // Philox key
+    uint32_t epoch;
+    int32_t log_on;
+    int64_t* A;
+    int64_t* Xt;
+    int64_t* bA;
+    int64_t* bXt;
+    int64_t* gA;
+    int64_t* gXt;
+    RepState* rs;
+    double* temps;
+    uint8_t* act;
+    LogEnt* log;
+    int64_t log_cap;
+    InstParams P;
+    unsigned long long solve_key;
+    int64_t gb_score;      // instance best score at the last epoch end
+    int64_t gb_offnz;
+    int32_t gb_rep;
+    int32_t solve_rep;
+    int64_t solve_move;
+    int64_t exch_tried;
+    int64_t exch_acc;
+};
+
+struct RepRef {
+    int32_t inst;
+    int32_t rep;
+};
+
+}  // namespace xtx
