@@ -156,7 +156,8 @@ typedef struct {
 } xtx_verify_report;
 
 /* One accepted move (16 bytes): the proposal index of the replica, pivot i,
- * target j, shear s; flags bit 0 marks a stall reduction-sweep commit. */
+ * target j, shear s; flags bit 0 marks a stall reduction-sweep commit, bit 1 a
+ * reseed from the instance-best snapshot (cooling mode; i = j = -1, s = 0). */
 typedef struct {
     int64_t move;
     int16_t i, j, s;

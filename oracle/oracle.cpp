@@ -697,6 +697,7 @@ struct Model {
                 }
                 R.T = p.reheat * t_init;
                 if ((double)R.score > (double)gb_score * p.reseed_factor) {
+                    log_commit(R, m, -1, -1, 0, 2);
                     R.A = gbA;
                     R.Xt = gbXt;
                     R.score = orc_score(R.A.data(), n);

@@ -72,9 +72,11 @@ int32_t orc_verify(const int64_t* A0, const int64_t* X, const int64_t* A, int32_
 
 /* ---- move-log replay ----
  * Entry layout (16 bytes, shared with the engine's log format):
- *   int64 move; int16 i; int16 j; int16 s; uint16 flags (bit0 = sweep commit)
+ *   int64 move; int16 i; int16 j; int16 s; uint16 flags (bit0 = sweep commit,
+ *   bit1 = reseed from the instance-best snapshot, i = j = -1)
  * Applies the moves in order to (A, Xt) with the restated primitives; returns
- * the index of the first entry that is infeasible (or count on success).
+ * the index of the first entry that is infeasible or a reseed marker (or count
+ * on success).
  */
 typedef struct {
     int64_t move;

@@ -191,6 +191,8 @@ def test_trajectory_matches_model(eng, orc, case):
         mlog = model["logs"][r]
         assert len(glog) == len(mlog), (r, len(glog), len(mlog))
         assert np.array_equal(glog, mlog), r
+        if (glog["flags"] & 2).any():
+            continue  # a reseed jumps to another replica's state: no single-log replay
         k, A, Xt = orc.replay(A0, eye, glog)
         assert k == len(glog)
         assert np.array_equal(A, gA[r]) and np.array_equal(Xt, gXt[r])

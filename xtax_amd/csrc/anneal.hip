@@ -655,6 +655,7 @@ __global__ void __launch_bounds__(256)
                 T = Pp.reheat * Pp.t_init;
                 if ((double)score > (double)gb_epoch * Pp.reseed_factor) {
                     leave_best();
+                    log_move(m, -1, -1, 0, 2);
                     rp.load(I->gA, I->gXt);
                     rp.compute_row_mass();
                     score = gb_epoch;
@@ -930,14 +931,18 @@ static hipError_t launch_anneal_t(hipStream_t st, int groups_per_block, const Re
     return hipGetLastError();
 }
 
-// Dynamic LDS limits are raised once, outside any stream capture.
-hipError_t init_kernel_attrs() {
-    const int lds = 160 * 1024;
+// Dynamic LDS limits are raised once, outside any stream capture, to the
+// per-block maximum of the device minus each kernel's static LDS.
+hipError_t init_kernel_attrs(int max_lds_per_block) {
     const void* fns[] = {(const void*)k_anneal<16, 1>, (const void*)k_anneal<32, 1>,
                          (const void*)k_anneal<64, 1>, (const void*)k_anneal<64, 2>,
                          (const void*)k_epoch_end};
     for (const void* f : fns) {
-        const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        hipFuncAttributes fa;
+        hipError_t e = hipFuncGetAttributes(&fa, f);
+        if (e != hipSuccess) return e;
+        const int dyn = max_lds_per_block - (int)fa.sharedSizeBytes;
+        e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;

@@ -40,7 +40,7 @@ hipError_t launch_probe(hipStream_t st, int G, int C, const int64_t* A, const in
 hipError_t launch_probe_rng(hipStream_t st, const uint32_t* ck, int count, uint32_t* out,
                             const double* x, int nx, double* ex);
 hipError_t launch_replicate(hipStream_t st, int64_t* dst, const int64_t* src, int64_t nn, int R);
-hipError_t init_kernel_attrs();
+hipError_t init_kernel_attrs(int max_lds_per_block);
 }  // namespace xtx
 
 using namespace xtx;
@@ -94,6 +94,8 @@ Layout layout_for(int n) {
 }
 
 constexpr int kLdsPerCu = 160 * 1024;
+// Largest LDS one block may use on the open device (set by xtx_open).
+int g_lds_block = 64 * 1024;
 
 // ---------------------------------------------------------------- arena
 // First-fit free list over one device allocation (256-byte granules).
@@ -301,7 +303,7 @@ void prepare(HostInst& h, const int64_t* A0, const int64_t* X0, int n, int R, ui
     h.log_on = p.move_log != 0;
     h.log_cap = h.log_on ? p.move_log_cap : 0;
     h.lay = layout_for(n);
-    if (h.lay.rep_bytes > kLdsPerCu)
+    if (h.lay.rep_bytes > g_lds_block)
         throw Err{XTX_E_ARG, "n too large for an LDS-resident replica (A and X)"};
 }
 
@@ -564,7 +566,7 @@ int groups_per_block_for(const Layout& L, int waves) {
     const int per_wave = 64 / L.G;
     int w = waves > 0 ? waves : 1;
     // keep a block within the LDS of one CU
-    while (w > 1 && (int64_t)w * per_wave * L.rep_bytes > kLdsPerCu) --w;
+    while (w > 1 && (int64_t)w * per_wave * L.rep_bytes > g_lds_block) --w;
     return w * per_wave;
 }
 
@@ -630,7 +632,8 @@ int xtx_open(int device, xtx_ctx** out) {
         if (std::strncmp(ctx->prop.gcnArchName, "gfx950", 6) != 0)
             throw Err{XTX_E_DEVICE, std::string("built for gfx950, device is ") +
                                         ctx->prop.gcnArchName};
-        HIPCHK(init_kernel_attrs());
+        g_lds_block = (int)ctx->prop.sharedMemPerBlock;
+        HIPCHK(init_kernel_attrs(g_lds_block));
         HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         HIPCHK(hipEventCreate(&ctx->ev0));
         HIPCHK(hipEventCreate(&ctx->ev1));
@@ -684,7 +687,7 @@ int xtx_query_limits(xtx_ctx* ctx, xtx_limits* out) {
     out->max_n = kMaxN;
     int mx = 0;
     for (int n = 2; n <= kMaxN; ++n)
-        if (layout_for(n).rep_bytes <= kLdsPerCu) mx = n;
+        if (layout_for(n).rep_bytes <= g_lds_block) mx = n;
     out->max_n_lds = mx;
     out->hbm_bytes = (int64_t)ctx->prop.totalGlobalMem;
     std::snprintf(out->arch, sizeof(out->arch), "%s", ctx->prop.gcnArchName);
