@@ -98,43 +98,122 @@ __device__ __forceinline__ int64_t clamp_cap(int64_t v) {
     return v > kShearCap ? kShearCap : (v < -kShearCap ? -kShearCap : v);
 }
 
-// Reductions and broadcasts inside a lane group of G lanes.
+// ---- cross-lane building blocks (DPP within 16-lane rows, v_readlane across
+// rows; ds_bpermute only where each lane reads a different, runtime lane) ----
+constexpr int kDppXor1 = 0xB1;        // quad_perm [1,0,3,2]
+constexpr int kDppXor2 = 0x4E;        // quad_perm [2,3,0,1]
+constexpr int kDppHalfMirror = 0x141; // row_half_mirror
+constexpr int kDppMirror = 0x140;     // row_mirror
+
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ int64_t dpp64(int64_t v) {
+    const uint32_t lo = dpp32<CTRL>((uint32_t)(uint64_t)v);
+    const uint32_t hi = dpp32<CTRL>((uint32_t)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ uint32_t readlane32(uint32_t v, int lane) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
+}
+__device__ __forceinline__ int64_t readlane64(int64_t v, int lane) {
+    const uint32_t lo = readlane32((uint32_t)(uint64_t)v, lane);
+    const uint32_t hi = readlane32((uint32_t)((uint64_t)v >> 32), lane);
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+struct OpAdd {
+    template <class T>
+    __device__ __forceinline__ T operator()(T a, T b) const { return a + b; }
+};
+struct OpMax {
+    template <class T>
+    __device__ __forceinline__ T operator()(T a, T b) const { return a > b ? a : b; }
+};
+
+// Reductions and broadcasts inside a lane group of G lanes (G = 16, 32, 64;
+// a group is 1, 2 or 4 whole 16-lane rows of the wave).
 template <int G>
 struct Grp {
-    static __device__ __forceinline__ int64_t sum(int64_t v) {
-#pragma unroll
-        for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
-        return v;
-    }
-    static __device__ __forceinline__ int32_t sum32(int32_t v) {
-#pragma unroll
-        for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
-        return v;
-    }
-    static __device__ __forceinline__ int64_t max(int64_t v) {
-#pragma unroll
-        for (int o = G / 2; o > 0; o >>= 1) {
-            const int64_t u = __shfl_xor(v, o, G);
-            v = u > v ? u : v;
+    static __device__ __forceinline__ int lane() { return (int)(threadIdx.x & 63u); }
+
+    template <class Op>
+    static __device__ __forceinline__ int64_t all64(int64_t v, Op op) {
+        v = op(v, dpp64<kDppXor1>(v));
+        v = op(v, dpp64<kDppXor2>(v));
+        v = op(v, dpp64<kDppHalfMirror>(v));
+        v = op(v, dpp64<kDppMirror>(v));
+        if constexpr (G == 16) {
+            return v;
+        } else {
+            const int64_t r0 = readlane64(v, 0), r1 = readlane64(v, 16);
+            const int64_t r2 = readlane64(v, 32), r3 = readlane64(v, 48);
+            if constexpr (G == 32) {
+                const int64_t a = op(r0, r1), b = op(r2, r3);
+                return (lane() & 32) ? b : a;
+            } else {
+                return op(op(r0, r1), op(r2, r3));
+            }
         }
-        return v;
     }
-    static __device__ __forceinline__ int32_t min32(int32_t v) {
-#pragma unroll
-        for (int o = G / 2; o > 0; o >>= 1) {
-            const int32_t u = __shfl_xor(v, o, G);
-            v = u < v ? u : v;
+    template <class Op>
+    static __device__ __forceinline__ int32_t all32(int32_t v, Op op) {
+        v = op(v, (int32_t)dpp32<kDppXor1>((uint32_t)v));
+        v = op(v, (int32_t)dpp32<kDppXor2>((uint32_t)v));
+        v = op(v, (int32_t)dpp32<kDppHalfMirror>((uint32_t)v));
+        v = op(v, (int32_t)dpp32<kDppMirror>((uint32_t)v));
+        if constexpr (G == 16) {
+            return v;
+        } else {
+            const int32_t r0 = (int32_t)readlane32((uint32_t)v, 0);
+            const int32_t r1 = (int32_t)readlane32((uint32_t)v, 16);
+            const int32_t r2 = (int32_t)readlane32((uint32_t)v, 32);
+            const int32_t r3 = (int32_t)readlane32((uint32_t)v, 48);
+            if constexpr (G == 32) {
+                const int32_t a = op(r0, r1), b = op(r2, r3);
+                return (lane() & 32) ? b : a;
+            } else {
+                return op(op(r0, r1), op(r2, r3));
+            }
         }
-        return v;
     }
+    static __device__ __forceinline__ int64_t sum(int64_t v) { return all64(v, OpAdd()); }
+    static __device__ __forceinline__ int32_t sum32(int32_t v) { return all32(v, OpAdd()); }
+    static __device__ __forceinline__ int64_t max(int64_t v) { return all64(v, OpMax()); }
     static __device__ __forceinline__ bool any(bool p) {
         const unsigned long long b = __ballot(p);
         if (G == 64) return b != 0ull;
-        const int base = (int)(threadIdx.x & 63u) & ~(G - 1);
+        const int base = lane() & ~(G - 1);
         return ((b >> base) & ((1ull << G) - 1ull)) != 0ull;
     }
+    // value of lane c (compile-time) of the caller's group
+    template <int CC>
+    static __device__ __forceinline__ uint32_t bcast_c(uint32_t v) {
+        if constexpr (G == 64) {
+            return readlane32(v, CC);
+        } else if constexpr (G == 32) {
+            const uint32_t a = readlane32(v, CC), b = readlane32(v, 32 + CC);
+            return (lane() & 32) ? b : a;
+        } else {
+            const uint32_t a = readlane32(v, CC), b = readlane32(v, 16 + CC);
+            const uint32_t c = readlane32(v, 32 + CC), d = readlane32(v, 48 + CC);
+            const int row = lane() >> 4;
+            return row == 0 ? a : (row == 1 ? b : (row == 2 ? c : d));
+        }
+    }
+    // value of lane src (runtime, uniform in the group) of the caller's group
     template <class T>
     static __device__ __forceinline__ T bcast(T v, int src) {
+        if constexpr (G == 64 && sizeof(T) == 4)
+            return (T)readlane32((uint32_t)v, src);
+        else
+            return __shfl(v, src, G);
+    }
+    // value of a lane that differs per lane (always a permute)
+    template <class T>
+    static __device__ __forceinline__ T gather(T v, int src) {
         return __shfl(v, src, G);
     }
 };
@@ -178,6 +257,30 @@ __device__ __forceinline__ void evaluate(const int64_t (&ai)[C], const int64_t (
     e.dnz = dnz;
     e.bad = bad;
     e.delta = 4 * off + (iabs64(new_jj) - iabs64(a_jj));
+}
+
+// Candidate shears in the reference's order (main.cpp:214-231): floor and ceil
+// of the median, then q, q+1, q-1 with q = -round(a_ji/a_ii) when a_ii != 0,
+// each clamped to +/-64; a candidate equal to an earlier one is skipped.
+struct Cands {
+    int64_t v[5];
+    bool ok[5];
+};
+
+__device__ __forceinline__ void make_cands(int64_t cf, int64_t cc, bool has_q, int64_t q,
+                                           Cands& cd) {
+    cd.v[0] = clamp_cap(cf);
+    cd.v[1] = clamp_cap(cc);
+    cd.v[2] = clamp_cap(q);
+    cd.v[3] = clamp_cap(q + 1);
+    cd.v[4] = clamp_cap(q - 1);
+#pragma unroll
+    for (int t = 0; t < 5; ++t) {
+        bool v = t < 2 || has_q;
+#pragma unroll
+        for (int u = 0; u < t; ++u) v &= cd.v[u] != cd.v[t];
+        cd.ok[t] = v;
+    }
 }
 
 // Exact best integer shear (main.cpp:181-242). The |a_ik|-weighted lower median
@@ -236,66 +339,210 @@ __device__ int64_t best_shear(const int64_t (&ai)[C], const int64_t (&aj)[C], in
             cf = (2 * acc >= tot) ? lo - 1 : lo;
         }
     }
-    // Candidate list in the reference's order, deduplicated against every
-    // earlier entry (statically indexed: no scratch).
     const bool has_q = a_ii != 0;
     const int64_t q = has_q ? -rounded_div(a_ji, a_ii) : 0;
-    int64_t cand[5];
-    bool valid[5];
-    cand[0] = clamp_cap(cf);
-    cand[1] = clamp_cap(cc);
-    cand[2] = clamp_cap(q);
-    cand[3] = clamp_cap(q + 1);
-    cand[4] = clamp_cap(q - 1);
-#pragma unroll
-    for (int t = 0; t < 5; ++t) {
-        bool v = t < 2 || has_q;
-#pragma unroll
-        for (int u = 0; u < t; ++u) v &= cand[u] != cand[t];
-        valid[t] = v;
-    }
-    // Score every candidate in one pass (score_delta_l1, main.cpp:150-168).
-    int64_t part[5];
-    bool bad[5];
-#pragma unroll
-    for (int t = 0; t < 5; ++t) {
-        part[t] = 0;
-        bad[t] = false;
-    }
-#pragma unroll
-    for (int c = 0; c < C; ++c) {
-        const int k = g + c * G;
-        if (k < n && k != j) {
-            const int64_t old = aj[c];
-            const int64_t ao = iabs64(old);
-#pragma unroll
-            for (int t = 0; t < 5; ++t) {
-                const int64_t nv = old + cand[t] * ai[c];
-                bad[t] |= over(nv);
-                part[t] += iabs64(nv) - ao;
-            }
-        }
-    }
+    Cands cd;
+    make_cands(cf, cc, has_q, q, cd);
+    // Score every candidate (score_delta_l1, main.cpp:150-168).
     int64_t best_s = 0, best_d = 0;
 #pragma unroll
     for (int t = 0; t < 5; ++t) {
-        if (valid[t]) {
-            const int64_t s = cand[t];
-            const int64_t off = Grp<G>::sum(part[t]);
-            const bool b = Grp<G>::any(bad[t]);
-            if (s != 0) {
-                const int64_t new_jj = a_jj + 2 * s * a_ji + s * s * a_ii;
-                const int64_t d = (b || over(new_jj))
-                                      ? ((int64_t)1 << 62)
-                                      : 4 * off + (iabs64(new_jj) - iabs64(a_jj));
-                if (d < best_d) {
-                    best_d = d;
-                    best_s = s;
+        const int64_t s = cd.v[t];
+        if (cd.ok[t] && s != 0) {
+            int64_t part = 0;
+            bool bad = false;
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const int k = g + c * G;
+                if (k < n && k != j) {
+                    const int64_t nv = aj[c] + s * ai[c];
+                    bad |= over(nv);
+                    part += iabs64(nv) - iabs64(aj[c]);
                 }
+            }
+            const int64_t off = Grp<G>::sum(part);
+            const bool b = Grp<G>::any(bad);
+            const int64_t new_jj = a_jj + 2 * s * a_ji + s * s * a_ii;
+            const int64_t d = (b || over(new_jj)) ? ((int64_t)1 << 62)
+                                                  : 4 * off + (iabs64(new_jj) - iabs64(a_jj));
+            if (d < best_d) {
+                best_d = d;
+                best_s = s;
             }
         }
     }
     return best_s;
+}
+
+// ---- small-entry path ----------------------------------------------------
+// When every entry of rows i and j is below 2^16 in magnitude (checked per
+// move), each new off-diagonal entry is below 2^23, per-group sums below 2^30
+// and no off-diagonal entry can reach the 2^48 bound: the same quantities are
+// computed exactly in int32 with 32-bit reductions. Only the diagonal term
+// (s^2 a_ii up to 2^28) stays int64. The results are identical to the int64
+// path; the two differ only in instruction count.
+constexpr int32_t kSmall = 1 << 16;
+
+template <int G, int C>
+__device__ __forceinline__ bool rows_small(const int64_t (&ai)[C], const int64_t (&aj)[C]) {
+    bool big = false;
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+        big |= ai[c] >= kSmall || ai[c] <= -kSmall || aj[c] >= kSmall || aj[c] <= -kSmall;
+    return !Grp<G>::any(big);
+}
+
+__device__ __forceinline__ int32_t iabs32(int32_t v) { return v < 0 ? -v : v; }
+
+__device__ __forceinline__ int32_t rounded_div32(int32_t a, int32_t b) {
+    int32_t q = a / b;
+    const int32_t r = a - q * b;
+    if (2 * iabs32(r) >= iabs32(b)) q += ((a < 0) == (b < 0)) ? 1 : -1;
+    return q;
+}
+
+template <int G, int C>
+__device__ __forceinline__ void evaluate_small(const int32_t (&ai)[C], const int32_t (&aj)[C],
+                                               int g, int n, int j, int64_t s, int64_t a_ii,
+                                               int64_t a_ji, int64_t a_jj, MoveEval<C>& e) {
+    int32_t off = 0, dnz = 0;
+    const int32_t s32 = (int32_t)s;
+    const int64_t new_jj = a_jj + 2 * s * a_ji + s * s * a_ii;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const int k = g + c * G;
+        e.nv[c] = new_jj;
+        if (k < n && k != j) {
+            const int32_t old = aj[c];
+            const int32_t nv = old + s32 * ai[c];
+            off += iabs32(nv) - iabs32(old);
+            dnz += (old == 0) ? (nv != 0) : -(int32_t)(nv == 0);
+            e.nv[c] = nv;
+        }
+    }
+    off = Grp<G>::sum32(off);
+    dnz = Grp<G>::sum32(dnz);
+    e.off = off;
+    e.dnz = dnz;
+    e.bad = over(new_jj);
+    e.delta = 4 * (int64_t)off + (iabs64(new_jj) - iabs64(a_jj));
+}
+
+// best_shear on the small path. Each lane forms ceil and floor of its
+// breakpoint b_k = -a_jk/a_ik once from a correctly rounded double quotient
+// (exact: b_k is at least 2^-16 from any integer it is not equal to, the
+// quotient is within 2^-37 of b_k) and clamps them to [-65, 65], which leaves
+// every weight sum W(b <= t), t in [-64, 64], unchanged.
+template <int G, int C>
+__device__ int64_t best_shear_small(const int32_t (&ai)[C], const int32_t (&aj)[C], int g, int n,
+                                    int j, int64_t a_ii, int64_t a_ji, int64_t a_jj) {
+    int32_t w[C], ce[C], fl[C];
+    int32_t wl = 0;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const int k = g + c * G;
+        const bool bp = k < n && k != j && ai[c] != 0;
+        w[c] = bp ? iabs32(ai[c]) : 0;
+        wl += w[c];
+        ce[c] = 0;
+        fl[c] = 0;
+        if (bp) {
+            const double b = -(double)aj[c] / (double)ai[c];
+            const int32_t up = (int32_t)__builtin_ceil(b), dn = (int32_t)__builtin_floor(b);
+            ce[c] = up < -65 ? -65 : (up > 65 ? 65 : up);
+            fl[c] = dn < -65 ? -65 : (dn > 65 ? 65 : dn);
+        }
+    }
+    const int32_t tot = Grp<G>::sum32(wl);
+    int cf = 0, cc = 0;
+    if (tot != 0) {
+        int lo = -kShearCap, hi = kShearCap + 1;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            int32_t acc = 0;
+#pragma unroll
+            for (int c = 0; c < C; ++c) acc += ce[c] <= mid ? w[c] : 0;
+            acc = Grp<G>::sum32(acc);
+            if (2 * acc >= tot)
+                hi = mid;
+            else
+                lo = mid + 1;
+        }
+        if (lo <= -kShearCap) {
+            cf = cc = -kShearCap;
+        } else if (lo > kShearCap) {
+            cf = cc = kShearCap;
+        } else {
+            cc = lo;
+            int32_t acc = 0;
+#pragma unroll
+            for (int c = 0; c < C; ++c) acc += fl[c] < lo ? w[c] : 0;
+            acc = Grp<G>::sum32(acc);
+            cf = (2 * acc >= tot) ? lo - 1 : lo;
+        }
+    }
+    const bool has_q = a_ii != 0;
+    const int64_t q = has_q ? -(int64_t)rounded_div32((int32_t)a_ji, (int32_t)a_ii) : 0;
+    Cands cd;
+    make_cands(cf, cc, has_q, q, cd);
+    int64_t best_s = 0, best_d = 0;
+#pragma unroll
+    for (int t = 0; t < 5; ++t) {
+        const int64_t s = cd.v[t];
+        if (cd.ok[t] && s != 0) {
+            const int32_t s32 = (int32_t)s;
+            int32_t part = 0;
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const int k = g + c * G;
+                if (k < n && k != j) part += iabs32(aj[c] + s32 * ai[c]) - iabs32(aj[c]);
+            }
+            const int32_t off = Grp<G>::sum32(part);
+            const int64_t new_jj = a_jj + 2 * s * a_ji + s * s * a_ii;
+            const int64_t d = over(new_jj) ? ((int64_t)1 << 62)
+                                           : 4 * (int64_t)off + (iabs64(new_jj) - iabs64(a_jj));
+            if (d < best_d) {
+                best_d = d;
+                best_s = s;
+            }
+        }
+    }
+    return best_s;
+}
+
+// Both paths behind one call: the int32 path when rows i and j are small.
+template <int G, int C>
+__device__ __forceinline__ int64_t best_shear_any(const int64_t (&ai)[C], const int64_t (&aj)[C],
+                                                  bool small, int g, int n, int j, int64_t a_ii,
+                                                  int64_t a_ji, int64_t a_jj) {
+    if (small) {
+        int32_t si[C], sj[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            si[c] = (int32_t)ai[c];
+            sj[c] = (int32_t)aj[c];
+        }
+        return best_shear_small<G, C>(si, sj, g, n, j, a_ii, a_ji, a_jj);
+    }
+    return best_shear<G, C>(ai, aj, g, n, j, a_ii, a_ji, a_jj);
+}
+
+template <int G, int C>
+__device__ __forceinline__ void evaluate_any(const int64_t (&ai)[C], const int64_t (&aj)[C],
+                                             bool small, int g, int n, int j, int64_t s,
+                                             int64_t a_ii, int64_t a_ji, int64_t a_jj,
+                                             MoveEval<C>& e) {
+    if (small) {
+        int32_t si[C], sj[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            si[c] = (int32_t)ai[c];
+            sj[c] = (int32_t)aj[c];
+        }
+        evaluate_small<G, C>(si, sj, g, n, j, s, a_ii, a_ji, a_jj, e);
+    } else {
+        evaluate<G, C>(ai, aj, g, n, j, s, a_ii, a_ji, a_jj, e);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -304,12 +551,11 @@ __device__ int64_t best_shear(const int64_t (&ai)[C], const int64_t (&aj)[C], in
 
 template <int G, int C>
 struct Replica {
-    // LDS views
-    int64_t* sA;
-    int64_t* sX;
+    int64_t* sA;        // LDS: A, n rows of stride P
+    int64_t* gX;        // HBM: X^T of this replica, n x n row-major (touched on accept)
     const uint8_t* act;
     int P, n, N, g;
-    int64_t rm[C];  // row masses of the lane's rows
+    int64_t rm[C];      // row masses of the lane's rows
 
     __device__ __forceinline__ void load_rows(int i, int j, int64_t (&ai)[C],
                                               int64_t (&aj)[C]) const {
@@ -318,6 +564,18 @@ struct Replica {
             const int k = g + c * G;
             ai[c] = k < n ? sA[i * P + k] : 0;
             aj[c] = k < n ? sA[j * P + k] : 0;
+        }
+    }
+
+    // Rows i and j of X^T, issued early so the HBM latency overlaps the
+    // evaluation of the move (only an accepted move consumes them).
+    __device__ __forceinline__ void prefetch_xt(int i, int j, int64_t (&xi)[C],
+                                                int64_t (&xj)[C]) const {
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const int k = g + c * G;
+            xi[c] = k < n ? gX[i * n + k] : 0;
+            xj[c] = k < n ? gX[j * n + k] : 0;
         }
     }
 
@@ -335,59 +593,65 @@ struct Replica {
 
     // row_mass[row] as seen by any lane of the group
     __device__ __forceinline__ int64_t mass_of(int row) const {
-        int64_t v = Grp<G>::bcast(rm[0], row & (G - 1));
+        int64_t v = Grp<G>::gather(rm[0], row & (G - 1));
         if (C > 1) {
-            const int64_t v1 = Grp<G>::bcast(rm[C > 1 ? 1 : 0], row & (G - 1));
+            const int64_t v1 = Grp<G>::gather(rm[C > 1 ? 1 : 0], row & (G - 1));
             v = (row >= G) ? v1 : v;
         }
         return v;
     }
 
-    // Copy the LDS state to n x n row-major HBM arrays.
-    __device__ __forceinline__ void store(int64_t* gA, int64_t* gX) const {
+    __device__ __forceinline__ void store_A(int64_t* dst) const {
         for (int row = 0; row < n; ++row) {
 #pragma unroll
             for (int c = 0; c < C; ++c) {
                 const int k = g + c * G;
-                if (k < n) {
-                    gA[row * n + k] = sA[row * P + k];
-                    gX[row * n + k] = sX[row * P + k];
-                }
+                if (k < n) dst[row * n + k] = sA[row * P + k];
             }
         }
     }
 
-    __device__ __forceinline__ void load(const int64_t* gA, const int64_t* gX) {
+    __device__ __forceinline__ void load_A(const int64_t* src) {
         for (int row = 0; row < n; ++row) {
 #pragma unroll
             for (int c = 0; c < C; ++c) {
                 const int k = g + c * G;
-                if (k < n) {
-                    sA[row * P + k] = gA[row * n + k];
-                    sX[row * P + k] = gX[row * n + k];
-                }
+                if (k < n) sA[row * P + k] = src[row * n + k];
             }
         }
     }
 
-    // Commit an evaluated Add(i, j, s) after the transform-row bound check
-    // (L1Objective::commit, main.cpp:393-399 -> add_transform_row :97-108,
-    // commit_add :112-133). Returns false (state untouched) when a transform
-    // entry would exceed the bound.
-    __device__ __forceinline__ bool xt_row(int i, int j, int64_t s, int64_t (&nx)[C]) const {
+    // Column-owned copy of an n x n matrix in HBM (each lane copies its columns).
+    __device__ __forceinline__ void copy_cols(int64_t* dst, const int64_t* src) const {
+        for (int row = 0; row < n; ++row) {
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const int k = g + c * G;
+                if (k < n) dst[row * n + k] = src[row * n + k];
+            }
+        }
+    }
+
+    // Transform-row bound check of an evaluated Add(i, j, s)
+    // (add_transform_row, main.cpp:97-108): false leaves the state untouched.
+    __device__ __forceinline__ bool xt_row(int64_t s, const int64_t (&xi)[C],
+                                           const int64_t (&xj)[C], int64_t (&nx)[C]) const {
         bool bad = false;
 #pragma unroll
         for (int c = 0; c < C; ++c) {
             const int k = g + c * G;
             nx[c] = 0;
             if (k < n) {
-                nx[c] = sX[j * P + k] + s * sX[i * P + k];
+                nx[c] = xj[c] + s * xi[c];
                 bad |= over(nx[c]);
             }
         }
         return !Grp<G>::any(bad);
     }
 
+    // commit_add (main.cpp:112-133): row j and its mirror column of A, row j of
+    // X^T, row masses (row j's new mass is its old mass plus the off-diagonal
+    // change).
     __device__ __forceinline__ void write(int j, const int64_t (&aj)[C], const MoveEval<C>& e,
                                           const int64_t (&nx)[C]) {
 #pragma unroll
@@ -402,7 +666,7 @@ struct Replica {
                     sA[j * P + j] = e.nv[c];
                     rm[c] += e.off;
                 }
-                sX[j * P + k] = nx[c];
+                gX[j * n + k] = nx[c];
             }
         }
     }
@@ -427,8 +691,8 @@ __global__ void __launch_bounds__(256)
 
     Replica<G, C> rp;
     rp.sA = (int64_t*)(smem + (size_t)slot * rep_bytes);
-    rp.sX = rp.sA + (size_t)nmax * P;
-    uint8_t* sact = (uint8_t*)(rp.sX + (size_t)nmax * P);
+    uint4* swords = (uint4*)(rp.sA + (size_t)nmax * P);
+    uint8_t* sact = (uint8_t*)(swords + 5);
     rp.act = sact;
     rp.P = P;
     rp.n = I->n;
@@ -436,7 +700,8 @@ __global__ void __launch_bounds__(256)
     rp.g = g;
     const int n = rp.n, N = rp.N;
     const int64_t nn = (int64_t)n * n;
-    rp.load(I->A + (int64_t)r * nn, I->Xt + (int64_t)r * nn);
+    rp.gX = I->Xt + (int64_t)r * nn;
+    rp.load_A(I->A + (int64_t)r * nn);
     for (int q = g; q < N; q += G) sact[q] = I->act[q];
     rp.compute_row_mass();
 
@@ -467,7 +732,8 @@ __global__ void __launch_bounds__(256)
     auto leave_best = [&]() {
         if (flags & RF_CUR_BEST) {
             if (best_score <= gb_epoch) {
-                rp.store(bA, bX);
+                rp.store_A(bA);
+                rp.copy_cols(bX, rp.gX);
                 flags &= ~RF_BEST_STALE;
             } else {
                 flags |= RF_BEST_STALE;
@@ -501,14 +767,16 @@ __global__ void __launch_bounds__(256)
             const int64_t a_ii = rp.sA[pv * P + pv];
             const int64_t a_ji = rp.sA[jj * P + pv];
             const int64_t a_jj = rp.sA[jj * P + jj];
-            const int64_t s = best_shear<G, C>(ai, aj, g, n, jj, a_ii, a_ji, a_jj);
+            const bool small = rows_small<G, C>(ai, aj);
+            const int64_t s = best_shear_any<G, C>(ai, aj, small, g, n, jj, a_ii, a_ji, a_jj);
             ++greedy_n;
             if (s == 0) continue;
             MoveEval<C> e;
-            evaluate<G, C>(ai, aj, g, n, jj, s, a_ii, a_ji, a_jj, e);
+            evaluate_any<G, C>(ai, aj, small, g, n, jj, s, a_ii, a_ji, a_jj, e);
             if (e.bad || e.delta > 0) continue;
-            int64_t nx[C];
-            if (!rp.xt_row(pv, jj, s, nx)) continue;
+            int64_t xi[C], xj[C], nx[C];
+            rp.prefetch_xt(pv, jj, xi, xj);
+            if (!rp.xt_row(s, xi, xj, nx)) continue;
             leave_best();
             rp.write(jj, aj, e, nx);
             score += e.delta;
@@ -529,7 +797,7 @@ __global__ void __launch_bounds__(256)
 
     bool halt = false;
     for (uint32_t it = 0; moves < epoch_end; ++it) {
-        if ((it & 15u) == 0u) {
+        if ((it & 255u) == 0u) {
             const unsigned long long key =
                 __hip_atomic_load(&I->solve_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (key != kNoKey && moves >= (key >> kRepBits)) break;
@@ -541,33 +809,61 @@ __global__ void __launch_bounds__(256)
         const uint64_t m = ++moves;
         const uint4 w = philox10((uint32_t)m, (uint32_t)(m >> 32), (uint32_t)r, (uint32_t)g,
                                  key0, key1);
-        const uint32_t w0z = Gp::bcast(w.z, 0), w0w = Gp::bcast(w.w, 0);
-        const uint32_t w1z = Gp::bcast(w.z, 1), w1w = Gp::bcast(w.w, 1);
-        const uint32_t w2z = Gp::bcast(w.z, 2), w2w = Gp::bcast(w.w, 2);
-        const uint32_t w3z = Gp::bcast(w.z, 3), w3w = Gp::bcast(w.w, 3);
+        // words of blocks 0..4 to every lane of the group: v_readlane for a
+        // whole-wave group, an LDS bounce otherwise
+        uint32_t w0z, w0w, w1z, w1w, w2z, w2w, w3z, w3w, w4z, w4w;
+        if constexpr (G == 64) {
+            w0z = Gp::template bcast_c<0>(w.z);
+            w0w = Gp::template bcast_c<0>(w.w);
+            w1z = Gp::template bcast_c<1>(w.z);
+            w1w = Gp::template bcast_c<1>(w.w);
+            w2z = Gp::template bcast_c<2>(w.z);
+            w2w = Gp::template bcast_c<2>(w.w);
+            w3z = Gp::template bcast_c<3>(w.z);
+            w3w = Gp::template bcast_c<3>(w.w);
+            w4z = Gp::template bcast_c<4>(w.z);
+            w4w = Gp::template bcast_c<4>(w.w);
+        } else {
+            if (g < 5) swords[g] = w;
+            __builtin_amdgcn_wave_barrier();
+            const uint4 b0 = swords[0], b1 = swords[1], b2 = swords[2], b3 = swords[3];
+            const uint4 b4 = swords[4];
+            __builtin_amdgcn_wave_barrier();
+            w0z = b0.z;
+            w0w = b0.w;
+            w1z = b1.z;
+            w1w = b1.w;
+            w2z = b2.z;
+            w2w = b2.w;
+            w3z = b3.z;
+            w3w = b3.w;
+            w4z = b4.z;
+            w4w = b4.w;
+        }
 
         // ---- pair (congruence_anneal.hpp:283-288, :211-228) ----
         int i, j;
         if (u01(w0z) < tf) {
+            // hot-row tournament: first maximum of row_mass over the draws,
+            // as one max over keys (mass << 4 | 15 - draw index)
             const bool in = g < TS;
             const uint32_t p = in ? mulhi32(w.x, (uint32_t)N) : 0u;
             const int row = sact[p];
             const int64_t wt = rp.mass_of(row);
-            const int64_t key_wt = in ? wt : -1;
-            const int64_t mx = Gp::max(key_wt);
-            const int cb = Gp::min32((in && key_wt == mx) ? g : G);
-            const uint32_t tpos = Gp::bcast(p, cb);
+            const int64_t mx = Gp::max(in ? ((wt << 4) | (15 - g)) : (int64_t)-1);
+            const uint32_t tpos = Gp::bcast(p, 15 - (int)(mx & 15));
             const int target = sact[tpos];
+            // pivot tournament: first maximum of |A[target][c]| over draws c != target
             const uint32_t pp = in ? mulhi32(w.y, (uint32_t)N) : 0u;
             const int cand = sact[pp];
-            const int64_t v = (in && cand != target) ? iabs64(rp.sA[target * P + cand]) : -1;
-            const int64_t vmax = Gp::max(v);
+            const bool valid = in && cand != target;
+            const int64_t v = valid ? iabs64(rp.sA[target * P + cand]) : 0;
+            const int64_t vmax = Gp::max(valid ? ((v << 4) | (15 - g)) : (int64_t)-1);
             int ibest;
             if (vmax < 0) {
                 ibest = sact[(tpos + 1u + mulhi32(w3w, (uint32_t)(N - 1))) % (uint32_t)N];
             } else {
-                const int cv = Gp::min32((v == vmax && in && cand != target) ? g : G);
-                ibest = Gp::bcast(cand, cv);
+                ibest = Gp::bcast(cand, 15 - (int)(vmax & 15));
             }
             i = ibest;
             j = target;
@@ -578,30 +874,32 @@ __global__ void __launch_bounds__(256)
             j = sact[jp];
         }
 
-        int64_t ai[C], aj[C];
+        int64_t ai[C], aj[C], xi[C], xj[C];
         rp.load_rows(i, j, ai, aj);
+        rp.prefetch_xt(i, j, xi, xj);
         const int64_t a_ii = rp.sA[i * P + i];
         const int64_t a_ji = rp.sA[j * P + i];
         const int64_t a_jj = rp.sA[j * P + j];
 
         // ---- shear (congruence_anneal.hpp:290-296) ----
+        const bool small = rows_small<G, C>(ai, aj);
         const int64_t pm = (w1z & 1u) ? 1 : -1;
         int64_t s = pm;
         if (u01(w0w) < gf) {
-            s = best_shear<G, C>(ai, aj, g, n, j, a_ii, a_ji, a_jj);
+            s = best_shear_any<G, C>(ai, aj, small, g, n, j, a_ii, a_ji, a_jj);
             ++greedy_n;
             if (s == 0) s = pm;
         }
 
         // ---- attempt_add (congruence_anneal.hpp:232-240) ----
         MoveEval<C> e;
-        evaluate<G, C>(ai, aj, g, n, j, s, a_ii, a_ji, a_jj, e);
+        evaluate_any<G, C>(ai, aj, small, g, n, j, s, a_ii, a_ji, a_jj, e);
         int res = -1;
         if (e.bad) {
             ++infeasible;
         } else if (e.delta <= 0 || u53(w1w, w2z) < xexp(-(double)e.delta / T)) {
             int64_t nx[C];
-            if (rp.xt_row(i, j, s, nx)) {
+            if (rp.xt_row(s, xi, xj, nx)) {
                 const int64_t ns = score + e.delta;
                 if (ns < best_score) {
                     best_score = ns;
@@ -636,8 +934,8 @@ __global__ void __launch_bounds__(256)
         // ---- stall handling (congruence_anneal.hpp:363-412) ----
         if (tempering) {
             if (msi > Pp.stuck_threshold) {
-                if (Pp.sweep_fraction > 0.0 && u01(Gp::bcast(w.z, 4)) < Pp.sweep_fraction) {
-                    sweep(sact[mulhi32(Gp::bcast(w.w, 4), (uint32_t)N)], m);
+                if (Pp.sweep_fraction > 0.0 && u01(w4z) < Pp.sweep_fraction) {
+                    sweep(sact[mulhi32(w4w, (uint32_t)N)], m);
                     track_best(m);
                 }
                 local_best = score;
@@ -645,8 +943,8 @@ __global__ void __launch_bounds__(256)
             }
         } else {
             if (msi > Pp.stuck_threshold) {
-                if (Pp.sweep_fraction > 0.0 && u01(Gp::bcast(w.z, 4)) < Pp.sweep_fraction) {
-                    if (sweep(sact[mulhi32(Gp::bcast(w.w, 4), (uint32_t)N)], m)) {
+                if (Pp.sweep_fraction > 0.0 && u01(w4z) < Pp.sweep_fraction) {
+                    if (sweep(sact[mulhi32(w4w, (uint32_t)N)], m)) {
                         local_best = score;
                         msi = 0;
                     }
@@ -656,7 +954,8 @@ __global__ void __launch_bounds__(256)
                 if ((double)score > (double)gb_epoch * Pp.reseed_factor) {
                     leave_best();
                     log_move(m, -1, -1, 0, 2);
-                    rp.load(I->gA, I->gXt);
+                    rp.load_A(I->gA);
+                    rp.copy_cols(rp.gX, I->gXt);
                     rp.compute_row_mass();
                     score = gb_epoch;
                     offnz = I->gb_offnz;
@@ -693,7 +992,7 @@ __global__ void __launch_bounds__(256)
     }
     (void)halt;
 
-    rp.store(I->A + (int64_t)r * nn, I->Xt + (int64_t)r * nn);
+    rp.store_A(I->A + (int64_t)r * nn);
     if (g == 0) {
         S->moves = moves;
         S->score = score;
@@ -880,9 +1179,10 @@ __global__ void __launch_bounds__(64)
         aj[c] = k < n ? M[j * n + k] : 0;
     }
     const int64_t a_ii = M[i * n + i], a_ji = M[j * n + i], a_jj = M[j * n + j];
+    const bool small = rows_small<G, C>(ai, aj);
     MoveEval<C> e;
-    evaluate<G, C>(ai, aj, g, n, j, s, a_ii, a_ji, a_jj, e);
-    const int64_t b = best_shear<G, C>(ai, aj, g, n, j, a_ii, a_ji, a_jj);
+    evaluate_any<G, C>(ai, aj, small, g, n, j, s, a_ii, a_ji, a_jj, e);
+    const int64_t b = best_shear_any<G, C>(ai, aj, small, g, n, j, a_ii, a_ji, a_jj);
     bool bx = false;
 #pragma unroll
     for (int c = 0; c < C; ++c) {

@@ -88,7 +88,9 @@ Layout layout_for(int n) {
         L.C = 2;
     }
     L.P = L.nmax + 1;
-    const int64_t bytes = 2 * (int64_t)L.nmax * L.P * 8 + L.nmax;
+    // A (nmax rows of stride P), 5 staged Philox blocks, active list; X^T
+    // stays in HBM
+    const int64_t bytes = (int64_t)L.nmax * L.P * 8 + 5 * 16 + L.nmax;
     L.rep_bytes = (int)((bytes + 15) / 16 * 16);
     return L;
 }
@@ -304,7 +306,7 @@ void prepare(HostInst& h, const int64_t* A0, const int64_t* X0, int n, int R, ui
     h.log_cap = h.log_on ? p.move_log_cap : 0;
     h.lay = layout_for(n);
     if (h.lay.rep_bytes > g_lds_block)
-        throw Err{XTX_E_ARG, "n too large for an LDS-resident replica (A and X)"};
+        throw Err{XTX_E_ARG, "n too large for an LDS-resident replica"};
 }
 
 size_t align256(size_t b) { return (b + 255) / 256 * 256; }
