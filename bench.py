@@ -1,28 +1,33 @@
 """Benchmark of the gfx950 xtax annealer (one GPU).
 
-Workload (default, `--workload batch`): 1024 planted instances, n ~ U{8..48},
-A = U^T D U with D = diag(+-U{1..100}) and U a product of 3n random +-1 shears
-(xtax_amd.workloads.batch_config), 32 tempering replicas each on a geometric
-ladder T in [5, 200], greedy fraction 0.5, exchange every 2000 moves, all
-instances resident in one captured launch sequence; an instance retires when a
-replica reaches a diagonal or after `--cap-epochs` epochs (its cap). One step =
-one pass over the whole batch. Every solved result is re-verified exactly on
-the host (X^T A0 X = A, A diagonal, |det X| = 1) and only verified ones count.
+Default workload (`--workload batch`, the metric's configuration): 1024 planted
+instances, n ~ U{8..48}, A = U^T D U with D = diag(+-U{1..100}) and U a product
+of 3n random +-1 shears (xtax_amd.workloads.batch_config), 32 tempering
+replicas each on a geometric ladder T in [5, 200], greedy fraction 0.5,
+exchange every 2000 moves, all instances resident in one captured launch
+sequence. An instance retires when a replica reaches a diagonal or after
+`--cap-epochs` epochs. One step = one pass over the whole batch. Every solved
+result is re-verified exactly on the host (X^T A0 X = A, A diagonal,
+|det X| = 1) and only verified ones count.
 
-Prints one JSON line: metric/value/unit (exact-diagonalized matrices per
-second), ms_per_step, solve latency percentiles, shear moves per second, the
-roofline of the dominant kernel (k_anneal) and a CPU baseline: the reference's
-own xtax binary (oracle/_ref/xtax_ref, compiled from the reference sources) run
-on a bounded sample of the same instances on this host's cores.
+Prints one JSON line: exact-diagonalized matrices per second, ms_per_step,
+solve-latency percentiles, shear moves per second, the roofline of the dominant
+kernel (k_anneal) and a CPU baseline: the reference's own xtax binary
+(oracle/_ref/xtax_ref, compiled from the reference sources) on a bounded sample
+of the same instances on this host's cores.
 
-Other workloads: `--workload plain` (the same instances one per launch),
-`serve` (Poisson request stream, continuous batching), `large` (n = 80
-planted, 4096 replicas), `lattice` (24x24 Gram, 256 replicas),
-`random` (n = 10 random symmetric, 8 replicas).
+Other configurations (`--workload ...`), same line format:
+  plain    the batch instances one per launch (xtx_solve_plain), first --count
+  serve    Poisson stream, n = round(exp(U[ln 6, ln 64])), D = +-U{1..200},
+           64 replicas per request, continuous batching (xtx_serve_*)
+  large    one planted n = 80 instance (D = +-U{1..1000}, 4n shears),
+           4096 replicas, greedy 0.7
+  lattice  24x24 lattice Gram (entries U[-9, 9]), 256 replicas, greedy 0.5,
+           sweep 0.05
+  random   n = 10 random symmetric (U[-20, 20]), 8 replicas, T in [0.5, 50]
 """
 import argparse
 import json
-import math
 import os
 import subprocess
 import sys
@@ -37,7 +42,9 @@ sys.path.insert(0, ROOT)
 from xtax_amd import Engine, EngineParams, workloads  # noqa: E402
 
 REF_BIN = os.path.join(ROOT, "oracle", "_ref", "xtax_ref")
-MI355X_INT_LANE_OPS = 256 * 64 * 2.4e9  # CUs * lanes/clk/CU * clock: 39.3 T int32 lane-ops/s
+# 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz: int32 VALU lane-ops/s of an MI355X
+MI355X_INT_LANE_OPS = 256 * 64 * 2.4e9
+METRIC = "exact-diagonalized matrices/sec + p50/p99 solve latency; shear moves/sec"
 
 
 def log(*a):
@@ -48,125 +55,102 @@ def pct(xs, q):
     return float(np.percentile(np.asarray(xs, float), q)) if len(xs) else None
 
 
-def batch_params(args):
-    return EngineParams(replicas=args.replicas, t_init=args.t_init, t_min=args.t_min,
-                        greedy_fraction=0.5, exchange_interval=args.exchange_interval,
-                        max_epochs=args.cap_epochs, seed=args.seed, verify=True)
+def base_line(args, value, unit, el, config, higher=True):
+    return {"metric": METRIC, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": el * 1e3 / max(1, args.steps),
+            "higher_is_better": higher, "scaling": "weak", "vs_baseline": None,
+            "dtype": "int64", "data": "synthetic", "config": config}
 
 
-def run_batch(eng, mats, args, seeds):
-    p = batch_params(args)
-    res, bst = eng.batch_solve(mats, p, seeds=seeds)
-    return res, bst
+def roofline(kernel_ms, moves, mean_n, greedy):
+    """Algorithmic integer work of k_anneal over its measured time.
 
-
-def cpu_baseline(mats, args, budget_s):
-    """The reference binary on the first instances, one at a time with every
-    core (-t nproc, same ladder), each capped at --cpu-cap seconds, until the
-    budget is spent."""
-    if not os.path.exists(REF_BIN):
+    Per proposal the method evaluates one Add (6n int64 ops over the n
+    columns: multiply, add, bound check, two absolute values, accumulate) and,
+    with probability `greedy`, up to five candidate shears (5 x 6n) plus the
+    weighted-median search (9 weighted counts, 3n each). An int64 op is counted
+    as 2 int32 lane-ops. Peak: int32 VALU lane-ops of the MI355X."""
+    if not kernel_ms:
         return None
-    cores = os.cpu_count() or 1
-    t0 = time.perf_counter()
-    solved = moves = done = 0
-    lat = []
-    with tempfile.TemporaryDirectory() as td:
-        for q, A in enumerate(mats):
-            if time.perf_counter() - t0 > budget_s:
-                break
-            path = os.path.join(td, "A.csv")
-            with open(path, "w") as f:
-                for row in A:
-                    f.write(",".join(str(int(v)) for v in row) + "\n")
-            t1 = time.perf_counter()
-            cp = subprocess.run([REF_BIN, "-A", path, "-t", str(cores), "--seed", str(q + 1),
-                                 "--t-init", str(args.t_init), "--t-min", str(args.t_min),
-                                 "--greedy-fraction", "0.5", "--max-seconds", str(args.cpu_cap)],
-                                cwd=td, capture_output=True, text=True)
-            dt = time.perf_counter() - t1
-            done += 1
-            for line in cp.stdout.splitlines():
-                if line.startswith("done moves="):
-                    moves += int(line.split()[1].split("=")[1])
-            if "matrix is diagonal" in cp.stdout:
-                solved += 1
-                lat.append(dt)
-    el = time.perf_counter() - t0
-    return {"value": solved / el, "unit": "matrices/s", "cores": cores, "kind": "reference",
-            "sample": f"first {done} of the {len(mats)} batch instances, one at a time, "
-                      f"xtax_ref -t {cores} --t-init {args.t_init} --t-min {args.t_min} "
-                      f"--max-seconds {args.cpu_cap}; {solved} solved in {el:.1f} s",
-            "moves_per_sec": moves / el, "p50_latency_s": pct(lat, 50)}
-
-
-def roofline(bst, moves, mean_n, args):
-    """Algorithmic int64 lane-ops of the annealer kernel over its measured time.
-
-    Per proposal the method evaluates 1 Add (6n int64 ops: mul, add, bound,
-    2x abs, accumulate, over the n columns) plus, with probability
-    greedy_fraction, up to 5 candidate shears (5 * 6n) and the median search
-    (9 weighted counts, 3n each). An int64 op is 2 int32 lane-ops."""
-    ops_per_move = 6 * mean_n * (1 + 0.5 * 5) + 0.5 * 9 * 3 * mean_n
-    lane_ops = moves * ops_per_move * 2
-    t = bst["kernel_ms"] / 1e3
-    if t <= 0:
-        return None
-    achieved = lane_ops / t / 1e12
+    ops_per_move = 6 * mean_n * (1 + greedy * 5) + greedy * 9 * 3 * mean_n
+    achieved = moves * ops_per_move * 2 / (kernel_ms / 1e3) / 1e12
     peak = MI355X_INT_LANE_OPS / 1e12
     return {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TOP/s",
             "frac": achieved / peak, "traffic": None,
-            "note": "integer VALU issue bound (no dense contraction, HBM touched only at "
-                    "epoch boundaries); algorithmic int32 lane-ops per k_anneal second"}
+            "kernel": "k_anneal", "kernel_ms": kernel_ms, "moves": moves,
+            "note": "integer VALU issue bound: no dense contraction for MFMA, and HBM is "
+                    "touched only for X^T rows and epoch boundaries"}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
-    ap.add_argument("--workload", default="batch",
-                    choices=["batch", "plain", "serve", "large", "lattice", "random"])
-    ap.add_argument("--count", type=int, default=1024)
-    ap.add_argument("--replicas", type=int, default=32)
-    ap.add_argument("--t-init", type=float, default=200.0)
-    ap.add_argument("--t-min", type=float, default=5.0)
-    ap.add_argument("--exchange-interval", type=int, default=2000)
-    ap.add_argument("--cap-epochs", type=int, default=50)
-    ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--cpu-cap", type=float, default=2.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this engine runs on one GPU; instances are independent, so N GPUs "
-              "would each take 1/N of the batch (see DESIGN.md). Run with --gpus 1.",
-              file=sys.stderr)
-        sys.exit(2)
+def run_ref(A, args_list, timeout):
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "A.csv")
+        with open(path, "w") as f:
+            for row in A:
+                f.write(",".join(str(int(v)) for v in row) + "\n")
+        cp = subprocess.run([REF_BIN, "-A", path] + args_list, cwd=td, capture_output=True,
+                            text=True, timeout=timeout)
+    moves = 0
+    for line in cp.stdout.splitlines():
+        if line.startswith("done moves="):
+            moves = int(line.split()[1].split("=")[1])
+    best = None
+    for line in cp.stdout.splitlines():
+        if line.startswith("Final best score:"):
+            best = int(line.split(":")[1])
+    return "matrix is diagonal" in cp.stdout, moves, best
 
-    log(f"generating workload {args.workload} ...")
-    t = time.perf_counter()
+
+def cpu_sample(mats, extra, budget_s, cap_s, threads=None):
+    """The reference binary on the first instances, one at a time with every
+    core, each capped at cap_s seconds, until budget_s is spent."""
+    if not os.path.exists(REF_BIN):
+        return None
+    cores = threads or os.cpu_count() or 1
+    t0 = time.perf_counter()
+    solved = moves = done = 0
+    lat = []
+    for q, A in enumerate(mats):
+        if time.perf_counter() - t0 > budget_s:
+            break
+        t1 = time.perf_counter()
+        ok, mv, _ = run_ref(A, ["-t", str(cores), "--seed", str(q + 1), "--max-seconds",
+                                str(cap_s)] + extra, timeout=cap_s + 30)
+        done += 1
+        moves += mv
+        if ok:
+            solved += 1
+            lat.append(time.perf_counter() - t1)
+    el = time.perf_counter() - t0
+    return {"value": solved / el, "unit": "matrices/s", "cores": cores, "kind": "reference",
+            "sample": f"first {done} instances one at a time: xtax_ref -t {cores} "
+                      f"{' '.join(extra)} --max-seconds {cap_s}; {solved} solved in {el:.1f} s",
+            "moves_per_sec": moves / el, "p50_latency_s": pct(lat, 50), "solved": solved,
+            "instances": done}
+
+
+def warm(eng):
+    tri = (2 * np.eye(4, dtype=np.int64) + np.eye(4, k=1, dtype=np.int64)
+           + np.eye(4, k=-1, dtype=np.int64))
+    eng.run_annealer(tri, None, EngineParams(replicas=2, max_epochs=1))
+
+
+# --------------------------------------------------------------------- batch
+def bench_batch(args, eng):
     mats = workloads.batch_config(count=args.count, seed=7)
     seeds = [args.seed * 1_000_003 + q for q in range(len(mats))]
-    log(f"  {len(mats)} instances in {time.perf_counter() - t:.1f}s")
     mean_n = float(np.mean([a.shape[0] for a in mats]))
-
-    eng = Engine(0)
-    # device/context warm-up (not a step): one tiny solve
-    tri = 2 * np.eye(4, dtype=np.int64) + np.eye(4, k=1, dtype=np.int64) + np.eye(4, k=-1, dtype=np.int64)
-    eng.run_annealer(tri, None, EngineParams(replicas=2, max_epochs=1))
+    p = EngineParams(replicas=args.replicas, t_init=args.t_init, t_min=args.t_min,
+                     greedy_fraction=0.5, exchange_interval=args.exchange_interval,
+                     max_epochs=args.cap_epochs, seed=args.seed, verify=True)
     for w in range(args.warmup):
-        log(f"warmup step {w}")
-        run_batch(eng, mats, args, seeds)
-
-    solved = verified = 0
-    moves = 0
+        eng.batch_solve(mats, p, seeds=seeds)
+    solved = verified = moves = 0
     lat = []
     kernel_ms = 0.0
     t0 = time.perf_counter()
-    last = None
     for k in range(args.steps):
-        res, bst = run_batch(eng, mats, args, seeds)
-        last = bst
+        res, bst = eng.batch_solve(mats, p, seeds=seeds)
         for A, X, d in res:
             if d["solved"]:
                 solved += 1
@@ -176,45 +160,236 @@ def main():
         moves += bst["moves"]
         kernel_ms += bst["kernel_ms"]
         log(f"step {k}: solved {bst['solved']}/{bst['instances']} verified {bst['verified']} "
-            f"epochs {bst['epochs']} {bst['seconds']:.2f}s kernel {bst['kernel_ms']:.0f}ms "
-            f"moves/s {bst['moves'] / bst['seconds']:.3e}")
+            f"epochs {bst['epochs']} buckets {bst['buckets']} {bst['seconds']:.2f}s "
+            f"kernel {bst['kernel_ms']:.0f}ms moves/s {bst['moves'] / bst['seconds']:.3e}")
     el = time.perf_counter() - t0
-    eng.close()
-
-    value = verified / el
-    out = {
-        "metric": "exact-diagonalized matrices/sec + p50/p99 solve latency; shear moves/sec",
-        "value": value,
-        "unit": "matrices/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": el * 1e3 / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "int64",
-        "data": "synthetic",
-        "config": {"workload": f"batch: {args.count} planted instances n~U{{8..48}}, "
-                               f"D=+-U{{1..100}}, 3n shears, {args.replicas} replicas, "
-                               f"T in [{args.t_min},{args.t_init}], greedy 0.5, exchange "
-                               f"{args.exchange_interval}, cap {args.cap_epochs} epochs",
-                   "instances": args.count, "replicas": args.replicas,
-                   "cap_moves_per_replica": args.cap_epochs * args.exchange_interval},
-        "solved": solved,
-        "verified": verified,
-        "solved_fraction": solved / (args.count * args.steps),
-        "p50_latency_s": pct(lat, 50),
-        "p95_latency_s": pct(lat, 95),
-        "p99_latency_s": pct(lat, 99),
-        "moves_per_sec": moves / el,
-        "kernel_moves_per_sec": moves / (kernel_ms / 1e3) if kernel_ms else None,
-        "roofline": roofline({"kernel_ms": kernel_ms}, moves, mean_n, args),
-        "cpu_baseline": None,
-    }
+    cfg = {"workload": f"batch: {args.count} planted instances n~U{{8..48}}, D=+-U{{1..100}}, "
+                       f"3n shears, {args.replicas} replicas, T in [{args.t_min},{args.t_init}], "
+                       f"greedy 0.5, exchange {args.exchange_interval}, cap {args.cap_epochs} "
+                       f"epochs",
+           "instances": args.count, "replicas": args.replicas,
+           "cap_moves_per_replica": args.cap_epochs * args.exchange_interval}
+    out = base_line(args, verified / el, "matrices/s", el, cfg)
+    out.update({"solved": solved, "verified": verified,
+                "solved_fraction": solved / (args.count * args.steps),
+                "p50_latency_s": pct(lat, 50), "p95_latency_s": pct(lat, 95),
+                "p99_latency_s": pct(lat, 99), "moves_per_sec": moves / el,
+                "kernel_moves_per_sec": moves / (kernel_ms / 1e3) if kernel_ms else None,
+                "roofline": roofline(kernel_ms, moves, mean_n, 0.5), "cpu_baseline": None})
     if not args.no_cpu:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(mats, args, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_sample(
+            mats, ["--t-init", str(args.t_init), "--t-min", str(args.t_min),
+                   "--greedy-fraction", "0.5"], args.cpu_seconds, args.cpu_cap)
+    return out
+
+
+# --------------------------------------------------------------------- plain
+def bench_plain(args, eng):
+    mats = workloads.batch_config(count=1024, seed=7)[:args.count]
+    seeds = [args.seed * 1_000_003 + q for q in range(len(mats))]
+    solved = moves = 0
+    lat = []
+    kernel_ms = 0.0
+    t0 = time.perf_counter()
+    for q, A in enumerate(mats):
+        p = EngineParams(replicas=args.replicas, t_init=args.t_init, t_min=args.t_min,
+                         greedy_fraction=0.5, exchange_interval=args.exchange_interval,
+                         max_epochs=args.cap_epochs, seed=seeds[q], verify=True)
+        t1 = time.perf_counter()
+        r = eng.run_annealer(A, None, p)
+        moves += r.stats["moves"]
+        kernel_ms += r.stats["kernel_ms"]
+        if r.stats["solved"] and r.stats["verify_status"] == 0:
+            solved += 1
+            lat.append(time.perf_counter() - t1)
+    el = time.perf_counter() - t0
+    mean_n = float(np.mean([a.shape[0] for a in mats]))
+    cfg = {"workload": f"plain: the first {len(mats)} batch instances one per launch, "
+                       f"{args.replicas} replicas, cap {args.cap_epochs} epochs"}
+    out = base_line(args, solved / el, "matrices/s", el, cfg)
+    out.update({"solved": solved, "instances": len(mats), "p50_latency_s": pct(lat, 50),
+                "p99_latency_s": pct(lat, 99), "moves_per_sec": moves / el,
+                "roofline": roofline(kernel_ms, moves, mean_n, 0.5), "cpu_baseline": None})
+    return out
+
+
+# --------------------------------------------------------------------- serve
+def bench_serve(args, eng):
+    count = args.count
+    reqs = workloads.serve_config(count=count, dmax=200, seed=11, rate=args.rate)
+    p = EngineParams(replicas=64, t_init=args.t_init, t_min=args.t_min, greedy_fraction=0.5,
+                     exchange_interval=args.exchange_interval, max_epochs=args.cap_epochs,
+                     seed=args.seed, verify=True)
+    eng.serve_open(p)
+    lat_solved, lat_all = [], []
+    solved = verified = moves = 0
+    done = 0
+    nxt = 0
+    t0 = time.perf_counter()
+    arrivals = {}
+    while done < count:
+        now = time.perf_counter() - t0
+        while nxt < count and reqs[nxt][0] <= now:
+            eng.serve_submit(reqs[nxt][1], nxt, args.seed * 1_000_003 + nxt)
+            arrivals[nxt] = t0 + reqs[nxt][0]
+            nxt += 1
+        resident, queued = eng.serve_step()
+        for A, X, d in eng.serve_poll():
+            t_ret = time.perf_counter()
+            lat = t_ret - arrivals.pop(d["tag"])
+            lat_all.append(lat)
+            moves += d["moves"]
+            done += 1
+            if d["solved"]:
+                solved += 1
+                if d["verify_status"] == 0:
+                    verified += 1
+                    lat_solved.append(lat)
+        if resident == 0 and queued == 0 and nxt < count:
+            time.sleep(max(0.0, min(0.01, reqs[nxt][0] - (time.perf_counter() - t0))))
+    el = time.perf_counter() - t0
+    eng.serve_close()
+    cfg = {"workload": f"serve: {count} planted requests, Poisson {args.rate}/s, "
+                       f"n=round(exp(U[ln6,ln64])), D=+-U{{1..200}}, 64 replicas, "
+                       f"cap {args.cap_epochs} epochs"}
+    out = base_line(args, verified / el, "matrices/s", el, cfg)
+    out.update({"solved": solved, "verified": verified, "requests": count,
+                "p50_latency_s": pct(lat_solved, 50), "p95_latency_s": pct(lat_solved, 95),
+                "p99_latency_s": pct(lat_solved, 99),
+                "p50_latency_all_s": pct(lat_all, 50), "p99_latency_all_s": pct(lat_all, 99),
+                "moves_per_sec": moves / el, "cpu_baseline": None})
+    return out
+
+
+# ------------------------------------------------------- single-instance runs
+def bench_single(args, eng, name, A0, params, cpu_extra):
+    t0 = time.perf_counter()
+    r = eng.run_annealer(A0, None, params)
+    el = time.perf_counter() - t0
+    n = A0.shape[0]
+    cfg = {"workload": name, "n": n, "replicas": params.replicas}
+    out = base_line(args, r.stats["moves"] / el, "moves/s", el, cfg)
+    out.update({"solved": r.stats["solved"], "best_score": r.stats["best_score"],
+                "start_score": int(2 * np.abs(A0).sum() - np.abs(np.diag(A0)).sum()),
+                "epochs": r.stats["epochs"], "verify_status": r.stats["verify_status"],
+                "roofline": roofline(r.stats["kernel_ms"], r.stats["moves"], n,
+                                     params.greedy_fraction), "cpu_baseline": None})
+    if not args.no_cpu and os.path.exists(REF_BIN):
+        cores = os.cpu_count() or 1
+        ok, mv, best = run_ref(A0, ["-t", str(cores), "--seed", "1", "--max-seconds",
+                                    str(args.cpu_cap)] + cpu_extra, timeout=args.cpu_cap + 30)
+        out["cpu_baseline"] = {"value": mv / args.cpu_cap, "unit": "moves/s", "cores": cores,
+                               "kind": "reference", "best_score": best, "solved": ok,
+                               "sample": f"xtax_ref -t {cores} {' '.join(cpu_extra)} "
+                                         f"--max-seconds {args.cpu_cap}"}
+    return out
+
+
+def bench_large(args, eng):
+    A0 = workloads.planted(80, 1000, 320, seed=5)[0]
+    p = EngineParams(replicas=4096, greedy_fraction=0.7, t_init=args.t_init * 10,
+                     t_min=args.t_min * 10, exchange_interval=args.exchange_interval,
+                     max_seconds=args.seconds, seed=args.seed, verify=True)
+    return bench_single(args, eng, "large: planted n=80, D=+-U{1..1000}, 4n shears, 4096 "
+                        "replicas, greedy 0.7", A0, p,
+                        ["--greedy-fraction", "0.7", "--t-init", str(args.t_init * 10),
+                         "--t-min", str(args.t_min * 10)])
+
+
+def bench_lattice(args, eng):
+    L, G = workloads.lattice(24, 24, seed=2)
+    p = EngineParams(replicas=256, greedy_fraction=0.5, sweep_fraction=0.05,
+                     exchange_interval=args.exchange_interval, max_seconds=args.seconds,
+                     seed=args.seed, verify=True)
+    return bench_single(args, eng, "lattice: 24x24 basis U[-9,9], Gram, 256 replicas, greedy "
+                        "0.5, sweep 0.05", G, p,
+                        ["--greedy-fraction", "0.5", "--sweep-fraction", "0.05"])
+
+
+def bench_random(args, eng):
+    seeds = range(1, args.count + 1)
+    solved = moves = 0
+    t0 = time.perf_counter()
+    for s in seeds:
+        A0 = workloads.random_symmetric(10, -20, 20, seed=s)
+        p = EngineParams(replicas=8, t_init=50.0, t_min=0.5, greedy_fraction=0.5,
+                         exchange_interval=args.exchange_interval, max_epochs=args.cap_epochs,
+                         seed=s, verify=True)
+        r = eng.run_annealer(A0, None, p)
+        solved += r.stats["solved"]
+        moves += r.stats["moves"]
+    el = time.perf_counter() - t0
+    cfg = {"workload": f"random: {args.count} seeds, n=10 symmetric U[-20,20], 8 replicas, "
+                       f"T in [0.5,50], greedy 0.5, cap {args.cap_epochs} epochs"}
+    out = base_line(args, moves / el, "moves/s", el, cfg)
+    out.update({"solved": solved, "instances": args.count, "cpu_baseline": None})
+    return out
+
+
+def bench_copy(args, eng):
+    """Calibration for the HBM traffic counters: --count device-to-device
+    copies of 256 MiB (one copy kernel each), so FETCH_SIZE / WRITE_SIZE per
+    dispatch can be compared with a known 256 MiB read and write."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    nbytes = 256 << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    t0 = time.perf_counter()
+    for _ in range(args.count):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # DeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    el = time.perf_counter() - t0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    out = base_line(args, args.count * 2 * nbytes / el / 1e9, "GB/s", el,
+                    {"workload": f"copy: {args.count} x 256 MiB device-to-device"})
+    out["bytes_per_copy"] = nbytes
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--workload", default="batch",
+                    choices=["batch", "plain", "serve", "large", "lattice", "random", "copy"])
+    ap.add_argument("--count", type=int, default=None)
+    ap.add_argument("--replicas", type=int, default=32)
+    ap.add_argument("--t-init", type=float, default=200.0)
+    ap.add_argument("--t-min", type=float, default=5.0)
+    ap.add_argument("--exchange-interval", type=int, default=2000)
+    ap.add_argument("--cap-epochs", type=int, default=50)
+    ap.add_argument("--rate", type=float, default=200.0)
+    ap.add_argument("--seconds", type=float, default=10.0)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-cap", type=float, default=None)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this engine runs on one GPU; instances are independent, so N GPUs "
+              "would each take 1/N of the batch. Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    if args.cpu_cap is None:
+        # batch: the reference gets the GPU's per-instance budget in moves
+        # (replicas x cap moves = 3.2M at the defaults, about 0.05 s on 16 cores),
+        # doubled; single-instance runs get a fixed 2 s
+        args.cpu_cap = 0.1 if args.workload == "batch" else 2.0
+    if args.count is None:
+        args.count = {"batch": 1024, "plain": 64, "serve": 3000, "random": 64, "copy": 8}.get(
+            args.workload, 1)
+    eng = Engine(0)
+    warm(eng)
+    fn = {"batch": bench_batch, "plain": bench_plain, "serve": bench_serve,
+          "large": bench_large, "lattice": bench_lattice, "random": bench_random,
+          "copy": bench_copy}[args.workload]
+    out = fn(args, eng)
+    eng.close()
     print(json.dumps(out), flush=True)
 
 

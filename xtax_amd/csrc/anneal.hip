@@ -24,6 +24,11 @@
 
 namespace xtx {
 
+// HBM pointers with their address space spelled out: loads and stores through
+// them become global_* instructions (vector-memory counter only) instead of
+// flat_* ones, which would also hold up every LDS wait.
+#define GLOBAL __attribute__((address_space(1)))
+
 // ---------------------------------------------------------------------------
 // building blocks
 // ---------------------------------------------------------------------------
@@ -184,9 +189,12 @@ struct Grp {
     static __device__ __forceinline__ int64_t max(int64_t v) { return all64(v, OpMax()); }
     static __device__ __forceinline__ bool any(bool p) {
         const unsigned long long b = __ballot(p);
-        if (G == 64) return b != 0ull;
-        const int base = lane() & ~(G - 1);
-        return ((b >> base) & ((1ull << G) - 1ull)) != 0ull;
+        if constexpr (G == 64) {
+            return b != 0ull;
+        } else {
+            const int base = lane() & ~(G - 1);
+            return ((b >> base) & ((1ull << G) - 1ull)) != 0ull;
+        }
     }
     // value of lane c (compile-time) of the caller's group
     template <int CC>
@@ -407,14 +415,17 @@ __device__ __forceinline__ void evaluate_small(const int32_t (&ai)[C], const int
                                                int64_t a_ji, int64_t a_jj, MoveEval<C>& e) {
     int32_t off = 0, dnz = 0;
     const int32_t s32 = (int32_t)s;
-    const int64_t new_jj = a_jj + 2 * s * a_ji + s * s * a_ii;
+    // |a_ii|, |a_ji|, |a_jj| < 2^16 here, so |new_jj| < 2^29: int32, in bound
+    const int32_t jj32 = (int32_t)a_jj +
+                         __mul24(s32, 2 * (int32_t)a_ji + __mul24(s32, (int32_t)a_ii));
+    const int64_t new_jj = jj32;
 #pragma unroll
     for (int c = 0; c < C; ++c) {
         const int k = g + c * G;
         e.nv[c] = new_jj;
         if (k < n && k != j) {
             const int32_t old = aj[c];
-            const int32_t nv = old + s32 * ai[c];
+            const int32_t nv = old + __mul24(s32, ai[c]);
             off += iabs32(nv) - iabs32(old);
             dnz += (old == 0) ? (nv != 0) : -(int32_t)(nv == 0);
             e.nv[c] = nv;
@@ -424,8 +435,8 @@ __device__ __forceinline__ void evaluate_small(const int32_t (&ai)[C], const int
     dnz = Grp<G>::sum32(dnz);
     e.off = off;
     e.dnz = dnz;
-    e.bad = over(new_jj);
-    e.delta = 4 * (int64_t)off + (iabs64(new_jj) - iabs64(a_jj));
+    e.bad = false;
+    e.delta = 4 * (int64_t)off + (int64_t)(iabs32(jj32) - iabs32((int32_t)a_jj));
 }
 
 // best_shear on the small path. Each lane forms ceil and floor of its
@@ -495,12 +506,12 @@ __device__ int64_t best_shear_small(const int32_t (&ai)[C], const int32_t (&aj)[
 #pragma unroll
             for (int c = 0; c < C; ++c) {
                 const int k = g + c * G;
-                if (k < n && k != j) part += iabs32(aj[c] + s32 * ai[c]) - iabs32(aj[c]);
+                if (k < n && k != j) part += iabs32(aj[c] + __mul24(s32, ai[c])) - iabs32(aj[c]);
             }
             const int32_t off = Grp<G>::sum32(part);
-            const int64_t new_jj = a_jj + 2 * s * a_ji + s * s * a_ii;
-            const int64_t d = over(new_jj) ? ((int64_t)1 << 62)
-                                           : 4 * (int64_t)off + (iabs64(new_jj) - iabs64(a_jj));
+            const int32_t jj32 = (int32_t)a_jj +
+                                 __mul24(s32, 2 * (int32_t)a_ji + __mul24(s32, (int32_t)a_ii));
+            const int64_t d = 4 * (int64_t)off + (int64_t)(iabs32(jj32) - iabs32((int32_t)a_jj));
             if (d < best_d) {
                 best_d = d;
                 best_s = s;
@@ -552,7 +563,7 @@ __device__ __forceinline__ void evaluate_any(const int64_t (&ai)[C], const int64
 template <int G, int C>
 struct Replica {
     int64_t* sA;        // LDS: A, n rows of stride P
-    int64_t* gX;        // HBM: X^T of this replica, n x n row-major (touched on accept)
+    GLOBAL int64_t* gX; // HBM: X^T of this replica, n x n row-major (touched on accept)
     const uint8_t* act;
     int P, n, N, g;
     int64_t rm[C];      // row masses of the lane's rows
@@ -601,7 +612,7 @@ struct Replica {
         return v;
     }
 
-    __device__ __forceinline__ void store_A(int64_t* dst) const {
+    __device__ __forceinline__ void store_A(GLOBAL int64_t* dst) const {
         for (int row = 0; row < n; ++row) {
 #pragma unroll
             for (int c = 0; c < C; ++c) {
@@ -611,7 +622,7 @@ struct Replica {
         }
     }
 
-    __device__ __forceinline__ void load_A(const int64_t* src) {
+    __device__ __forceinline__ void load_A(const GLOBAL int64_t* src) {
         for (int row = 0; row < n; ++row) {
 #pragma unroll
             for (int c = 0; c < C; ++c) {
@@ -622,7 +633,7 @@ struct Replica {
     }
 
     // Column-owned copy of an n x n matrix in HBM (each lane copies its columns).
-    __device__ __forceinline__ void copy_cols(int64_t* dst, const int64_t* src) const {
+    __device__ __forceinline__ void copy_cols(GLOBAL int64_t* dst, const GLOBAL int64_t* src) const {
         for (int row = 0; row < n; ++row) {
 #pragma unroll
             for (int c = 0; c < C; ++c) {
@@ -682,11 +693,17 @@ __global__ void __launch_bounds__(256)
     const int slot = (int)threadIdx.x / G;
     const int gid = (int)blockIdx.x * ((int)blockDim.x / G) + slot;
     if (gid >= count) return;
-    const RepRef ref = table[gid];
-    Inst* I = insts + ref.inst;
+    // Every group of a wave belongs to the same instance (the host pads each
+    // instance's replica list to whole waves), so the instance record is read
+    // through scalar registers.
+    const int wave_gid0 =
+        __builtin_amdgcn_readfirstlane((int)blockIdx.x * ((int)blockDim.x / G) +
+                                       (int)(threadIdx.x >> 6) * (64 / G));
+    Inst* I = insts + __builtin_amdgcn_readfirstlane(table[wave_gid0].inst);
     if (I->status != ST_RUNNING) return;
-    const int r = ref.rep;
-    RepState* S = I->rs + r;
+    const int r = G == 64 ? __builtin_amdgcn_readfirstlane(table[gid].rep) : table[gid].rep;
+    if (r < 0) return;  // padding group
+    GLOBAL RepState* S = (GLOBAL RepState*)I->rs + r;
     if (S->flags & RF_STOPPED) return;
 
     Replica<G, C> rp;
@@ -700,9 +717,11 @@ __global__ void __launch_bounds__(256)
     rp.g = g;
     const int n = rp.n, N = rp.N;
     const int64_t nn = (int64_t)n * n;
-    rp.gX = I->Xt + (int64_t)r * nn;
-    rp.load_A(I->A + (int64_t)r * nn);
-    for (int q = g; q < N; q += G) sact[q] = I->act[q];
+    GLOBAL int64_t* const gAcur = (GLOBAL int64_t*)I->A + (int64_t)r * nn;
+    rp.gX = (GLOBAL int64_t*)I->Xt + (int64_t)r * nn;
+    rp.load_A(gAcur);
+    const GLOBAL uint8_t* gact = (const GLOBAL uint8_t*)I->act;
+    for (int q = g; q < N; q += G) sact[q] = gact[q];
     rp.compute_row_mass();
 
     const InstParams Pp = I->P;
@@ -711,14 +730,16 @@ __global__ void __launch_bounds__(256)
     const int64_t gb_epoch = I->gb_score;
     const bool tempering = Pp.tempering != 0;
     const int TS = Pp.target_samples;
-    int64_t* bA = I->bA + (int64_t)r * nn;
-    int64_t* bX = I->bXt + (int64_t)r * nn;
-    LogEnt* logp = I->log ? I->log + (int64_t)r * I->log_cap : nullptr;
+    GLOBAL int64_t* bA = (GLOBAL int64_t*)I->bA + (int64_t)r * nn;
+    GLOBAL int64_t* bX = (GLOBAL int64_t*)I->bXt + (int64_t)r * nn;
+    const GLOBAL int64_t* gsnapA = (const GLOBAL int64_t*)I->gA;
+    const GLOBAL int64_t* gsnapX = (const GLOBAL int64_t*)I->gXt;
+    GLOBAL LogEnt* logp = I->log ? (GLOBAL LogEnt*)I->log + (int64_t)r * I->log_cap : nullptr;
     const int64_t log_cap = I->log_cap;
 
     uint64_t moves = S->moves;
     int64_t score = S->score, offnz = S->offnz, local_best = S->local_best, msi = S->msi;
-    double T = tempering ? I->temps[S->rung] : S->T;
+    double T = tempering ? ((const GLOBAL double*)I->temps)[S->rung] : S->T;
     double cooling = S->cooling;
     int32_t cool_counter = S->cool_counter, win_p = S->win_p, win_a = S->win_a;
     int64_t best_score = S->best_score, best_move = S->best_move;
@@ -744,13 +765,12 @@ __global__ void __launch_bounds__(256)
     auto log_move = [&](uint64_t m, int i, int j, int64_t s, uint16_t f) {
         if (logp) {
             if (log_count < log_cap && g == 0) {
-                LogEnt e;
-                e.move = (int64_t)m;
-                e.i = (int16_t)i;
-                e.j = (int16_t)j;
-                e.s = (int16_t)s;
-                e.flags = f;
-                logp[log_count] = e;
+                GLOBAL LogEnt* e = logp + log_count;
+                e->move = (int64_t)m;
+                e->i = (int16_t)i;
+                e->j = (int16_t)j;
+                e->s = (int16_t)s;
+                e->flags = f;
             }
             ++log_count;
         }
@@ -954,8 +974,8 @@ __global__ void __launch_bounds__(256)
                 if ((double)score > (double)gb_epoch * Pp.reseed_factor) {
                     leave_best();
                     log_move(m, -1, -1, 0, 2);
-                    rp.load_A(I->gA);
-                    rp.copy_cols(rp.gX, I->gXt);
+                    rp.load_A(gsnapA);
+                    rp.copy_cols(rp.gX, gsnapX);
                     rp.compute_row_mass();
                     score = gb_epoch;
                     offnz = I->gb_offnz;
@@ -992,7 +1012,7 @@ __global__ void __launch_bounds__(256)
     }
     (void)halt;
 
-    rp.store_A(I->A + (int64_t)r * nn);
+    rp.store_A(gAcur);
     if (g == 0) {
         S->moves = moves;
         S->score = score;

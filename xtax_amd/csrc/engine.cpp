@@ -488,6 +488,13 @@ int verify_outcome(const HostInst& h, const Outcome& o) {
     return o.solved ? rep.status : (rep.status & ~2);
 }
 
+// Pads a replica table to whole waves: the kernel reads the instance of a wave
+// from its first group, so a wave never mixes instances (rep -1 = idle group).
+void pad_to_wave(std::vector<RepRef>& refs, int G, int inst) {
+    const size_t per_wave = (size_t)(64 / G);
+    while (refs.size() % per_wave) refs.push_back(RepRef{inst, -1});
+}
+
 // Per-bucket replica tables for a set of resident instances.
 struct Bucket {
     Layout lay;
@@ -746,6 +753,7 @@ int xtx_solve_plain(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t 
         buckets[0].lay = h.lay;
         buckets[0].groups_per_block = groups_per_block_for(h.lay, p.waves_per_block);
         for (int r = 0; r < h.R; ++r) buckets[0].refs.push_back(RepRef{0, r});
+        pad_to_wave(buckets[0].refs, h.lay.G, 0);
         HIPCHK(hipMalloc(&buckets[0].d_refs, buckets[0].refs.size() * sizeof(RepRef)));
         HIPCHK(hipMemcpyAsync(buckets[0].d_refs, buckets[0].refs.data(),
                               buckets[0].refs.size() * sizeof(RepRef), hipMemcpyHostToDevice,
@@ -1064,6 +1072,7 @@ struct Server {
             b.lay = h.lay;
             b.groups_per_block = groups_per_block_for(h.lay, p.waves_per_block);
             for (int r = 0; r < h.R; ++r) b.refs.push_back(RepRef{s, r});
+            pad_to_wave(b.refs, h.lay.G, s);
         }
         for (auto& kv : buckets) {
             Bucket& b = kv.second;
