@@ -21,8 +21,8 @@
  *     reference's mt19937 replaced by counter-based Philox and its 5 ms monitor
  *     exchange replaced by an exchange every `exchange_interval` moves)
  *
- * Parity status of each function is stated in the oracle section of DESIGN.md
- * and pinned by tests/test_oracle_*.py.
+ * Each function is pinned by tests/test_oracle_golden.py (golden vectors from
+ * the reference's own code, closed forms, known answers).
  */
 #pragma once
 #include <stdint.h>

@@ -1,0 +1,176 @@
+"""The xtax command-line drop-in: input errors and exit codes (CPU; they are
+reported before any device is opened) and end-to-end runs on the GPU whose
+output files are checked with the oracle's exact verifier."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+XTAX = os.path.join(ROOT, "xtax_amd", "xtax")
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def run(args, cwd, timeout=120):
+    return subprocess.run([XTAX] + args, cwd=cwd, capture_output=True, text=True,
+                          timeout=timeout)
+
+
+def write_csv(path, M, sep=","):
+    with open(path, "w") as f:
+        for row in M:
+            f.write(sep.join(str(int(v)) for v in row) + "\n")
+
+
+def read_csv(path):
+    return np.array([[int(x) for x in r.split(",")] for r in open(path).read().split("\n")
+                     if r.strip()], np.int64)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def built():
+    if not os.path.exists(XTAX):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "xtax_amd"), "xtax"], check=True)
+
+
+def test_exactly_one_input(tmp_path):
+    r = run([], tmp_path)
+    assert r.returncode == 1 and "provide exactly one of -A or -L" in r.stderr
+    write_csv(tmp_path / "a.csv", [[1, 0], [0, 1]])
+    r = run(["-A", "a.csv", "-L", "a.csv"], tmp_path)
+    assert r.returncode == 1
+
+
+@pytest.mark.parametrize("args,msg", [
+    (["--deflate", "--deflate-blocks"], "at most one of --deflate"),
+    (["--gram-rows", "0,1", "--rcm"], "--gram-rows cannot be combined with --rcm"),
+    (["--gram-rows", "0,1", "--deflate"], "--gram-rows cannot be combined with --deflate"),
+    (["--lattice-dims", "0"], "--lattice-dims requires -L"),
+])
+def test_flag_conflicts(tmp_path, args, msg):
+    write_csv(tmp_path / "a.csv", [[2, 1], [1, 2]])
+    r = run(["-A", "a.csv"] + args, tmp_path)
+    assert r.returncode == 1 and msg in r.stderr
+
+
+@pytest.mark.parametrize("content,msg", [
+    ("1,2\n3\n", "Bad CSV row length"),
+    ("1,2,3\n4,5,6\n", "Matrix must be square"),
+    ("\n\n", "Empty matrix file"),
+])
+def test_malformed_csv(tmp_path, content, msg):
+    (tmp_path / "a.csv").write_text(content)
+    r = run(["-A", "a.csv"], tmp_path)
+    assert r.returncode == 1 and msg in r.stderr
+
+
+def test_missing_file_and_x_mismatch(tmp_path):
+    r = run(["-A", "nope.csv"], tmp_path)
+    assert r.returncode == 1 and "Cannot open nope.csv" in r.stderr
+    write_csv(tmp_path / "a.csv", [[2, 1], [1, 2]])
+    write_csv(tmp_path / "x.csv", [[1, 0, 0], [0, 1, 0], [0, 0, 1]])
+    r = run(["-A", "a.csv", "-X", "x.csv"], tmp_path)
+    assert r.returncode == 1 and "X dimensions do not match A" in r.stderr
+
+
+def test_bad_index_spec(tmp_path):
+    write_csv(tmp_path / "a.csv", [[2, 1], [1, 2]])
+    r = run(["-A", "a.csv", "--gram-rows", "0,5"], tmp_path)
+    assert r.returncode == 1 and "out of range" in r.stderr
+    r = run(["-A", "a.csv", "--gram-rows", "1..0"], tmp_path)
+    assert r.returncode == 1 and "reversed range" in r.stderr
+    r = run(["-A", "a.csv", "--gram-rows", "0"], tmp_path)
+    assert r.returncode == 1 and "need at least 2 rows" in r.stderr
+
+
+def test_unknown_flag(tmp_path):
+    r = run(["--bogus"], tmp_path)
+    assert r.returncode != 0 and "not expected" in r.stderr
+
+
+# ------------------------------------------------------------------ GPU runs
+def sample_matrix():
+    return np.load(os.path.join(GOLD, "g2_end_to_end.npz"))["r0_A0"]
+
+
+@pytest.mark.gpu
+def test_matrix_mode_solves_sample(tmp_path, orc):
+    A0 = sample_matrix()
+    write_csv(tmp_path / "a.csv", A0, sep=", ")  # the sample's own spacing
+    r = run(["-A", "a.csv", "-t", "16", "--seed", "3", "--t-init", "50", "--t-min", "0.5",
+             "--max-seconds", "60"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert "done moves=" in r.stdout and "Final best score:" in r.stdout
+    assert "matrix is diagonal" in r.stdout
+    A = read_csv(tmp_path / "best_A.csv")
+    X = read_csv(tmp_path / "best_X.csv")
+    status, sign, _ = orc.verify(A0, X, A)
+    assert status == 0 and sorted(np.diag(A).tolist()) == [-1] + [1] * 9
+
+
+@pytest.mark.gpu
+def test_resume_from_x(tmp_path, orc):
+    from xtax_amd import workloads
+    A0, D, U = workloads.planted(8, 9, 16, seed=21)
+    # start from a partial transform: X0 = first column shears of U^{-1} are unknown,
+    # so use a random unimodular X0 and check the invariant against the input
+    rng = np.random.default_rng(2)
+    X0 = np.eye(8, dtype=np.int64)
+    for _ in range(6):
+        i, j = rng.choice(8, 2, replace=False)
+        X0[:, j] += X0[:, i]
+    write_csv(tmp_path / "a.csv", A0)
+    write_csv(tmp_path / "x.csv", X0)
+    r = run(["-A", "a.csv", "-X", "x.csv", "-t", "16", "--seed", "5", "--t-init", "50",
+             "--t-min", "0.5", "--max-seconds", "5"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    A = read_csv(tmp_path / "best_A.csv")
+    X = read_csv(tmp_path / "best_X.csv")
+    status, _, _ = orc.verify(A0, X, A)
+    assert status & ~2 == 0
+
+
+@pytest.mark.gpu
+def test_lattice_mode(tmp_path):
+    from xtax_amd import workloads
+    L, G = workloads.lattice(6, 6, seed=3)
+    write_csv(tmp_path / "l.csv", L)
+    r = run(["-L", "l.csv", "-t", "16", "--seed", "2", "--max-seconds", "2"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert "[lattice] m=6 d=6" in r.stdout and "[warn]" not in r.stdout
+    Lf = read_csv(tmp_path / "final_L.csv")
+    A = read_csv(tmp_path / "best_A.csv")
+    X = read_csv(tmp_path / "best_X.csv")
+    assert (Lf @ Lf.T == A).all() and (X.T @ L == Lf).all()
+    assert round(abs(np.linalg.det(X))) == 1
+
+
+@pytest.mark.gpu
+def test_gram_rows_restricts_moves(tmp_path, orc):
+    A0 = sample_matrix()
+    write_csv(tmp_path / "a.csv", A0)
+    r = run(["-A", "a.csv", "--gram-rows", "0..5", "-t", "8", "--seed", "1",
+             "--max-seconds", "2"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert "[gram-rows] restricting moves to 6 of 10 rows" in r.stdout
+    X = read_csv(tmp_path / "best_X.csv")
+    A = read_csv(tmp_path / "best_A.csv")
+    # rows/columns outside the set are never pivot or target: X is the identity there
+    assert (X[:, 6:] == np.eye(10, dtype=np.int64)[:, 6:]).all()
+    assert (X[6:, :6] == 0).all()
+    assert orc.verify(A0, X, A)[0] & ~2 == 0
+
+
+@pytest.mark.gpu
+def test_modulus(tmp_path, orc):
+    A0 = np.array([[7, 12, 0], [12, -9, 5], [0, 5, 20]], np.int64)
+    write_csv(tmp_path / "a.csv", A0)
+    r = run(["-A", "a.csv", "--modulus", "5", "-t", "8", "--max-seconds", "2"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert "[modulus] reduced entries mod 5" in r.stdout
+    red = ((A0 % 5) + 5) % 5
+    red = np.where(red > 2, red - 5, red)
+    X = read_csv(tmp_path / "best_X.csv")
+    A = read_csv(tmp_path / "best_A.csv")
+    assert orc.verify(red, X, A)[0] & ~2 == 0
