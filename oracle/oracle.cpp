@@ -506,8 +506,9 @@ int64_t orc_replay(int64_t* A, int64_t* Xt, int32_t n, const orc_log_entry* log,
 // ---------------------------------------------------------------------------
 //
 // Per replica r and move m (1-based), the engine draws Philox blocks
-//   W(g) = philox4x32_10(key = {seed_lo, seed_hi}, ctr = {m_lo, m_hi, r, g})
-// for g = 0 .. max(target_samples, 5) - 1 and uses the words as follows:
+//   W(c) = philox4x32_10(key = {seed_lo, seed_hi}, ctr = {q_lo, q_hi, r, 16 t + c})
+// with q = (m - 1) / 4, t = (m - 1) mod 4, for slots c = 0 .. 15 (one block
+// call covers the 64 slots of four consecutive moves), using the words:
 //   W(c).x  row-tournament candidate c       W(c).y  pivot-tournament cand. c
 //   W(0).z  targeted-pick coin               W(0).w  greedy coin
 //   W(1).z  sign of the +/-1 shear (bit 0)   W(1).w  Metropolis uniform, high
@@ -555,8 +556,13 @@ struct Model {
     std::vector<int64_t> gbA, gbXt;
     int64_t log_cap = 0;
 
-    void block(uint64_t m, uint32_t r, uint32_t g, uint32_t w[4]) const {
-        const uint32_t ctr[4] = {(uint32_t)m, (uint32_t)(m >> 32), r, g};
+    // Slot c of move m: Philox block at counter (q, r, 16 t + c) with
+    // q = (m - 1) / 4 and t = (m - 1) mod 4, so one block call can serve
+    // four consecutive moves (16 slots each).
+    void block(uint64_t m, uint32_t r, uint32_t c, uint32_t w[4]) const {
+        const uint64_t q = (m - 1) >> 2;
+        const uint32_t t = (uint32_t)((m - 1) & 3);
+        const uint32_t ctr[4] = {(uint32_t)q, (uint32_t)(q >> 32), r, 16 * t + c};
         orc_philox4x32_10(ctr, key, w);
     }
 

@@ -815,6 +815,8 @@ __global__ void __launch_bounds__(256)
         }
     };
 
+    uint4 w = make_uint4(0u, 0u, 0u, 0u);
+    bool have_w = false;
     bool halt = false;
     for (uint32_t it = 0; moves < epoch_end; ++it) {
         if ((it & 255u) == 0u) {
@@ -827,24 +829,37 @@ __global__ void __launch_bounds__(256)
             }
         }
         const uint64_t m = ++moves;
-        const uint4 w = philox10((uint32_t)m, (uint32_t)(m >> 32), (uint32_t)r, (uint32_t)g,
-                                 key0, key1);
-        // words of blocks 0..4 to every lane of the group: v_readlane for a
+        // Philox slots of move m: counter (q, r, 16 t + c), q = (m-1)/4,
+        // t = (m-1) mod 4, c < 16. A group of G lanes computes G slots per
+        // call (lane = slot - 16 (t - sub)), i.e. one call serves G/16 moves;
+        // this move's slot c sits in lane base + c.
+        constexpr int S = G / 16;
+        const uint32_t t4 = (uint32_t)((m - 1) & 3u);
+        const int sub = (int)(t4 & (uint32_t)(S - 1));
+        if (sub == 0 || !have_w) {
+            const uint64_t q = (m - 1) >> 2;
+            w = philox10((uint32_t)q, (uint32_t)(q >> 32), (uint32_t)r,
+                         16u * (t4 & ~(uint32_t)(S - 1)) + (uint32_t)g, key0, key1);
+            have_w = true;
+        }
+        const int base = 16 * sub;
+        const int cs = g - base;  // this lane's slot for move m (valid in [0, 16))
+        // words of slots 0..4 to every lane of the group: v_readlane for a
         // whole-wave group, an LDS bounce otherwise
         uint32_t w0z, w0w, w1z, w1w, w2z, w2w, w3z, w3w, w4z, w4w;
         if constexpr (G == 64) {
-            w0z = Gp::template bcast_c<0>(w.z);
-            w0w = Gp::template bcast_c<0>(w.w);
-            w1z = Gp::template bcast_c<1>(w.z);
-            w1w = Gp::template bcast_c<1>(w.w);
-            w2z = Gp::template bcast_c<2>(w.z);
-            w2w = Gp::template bcast_c<2>(w.w);
-            w3z = Gp::template bcast_c<3>(w.z);
-            w3w = Gp::template bcast_c<3>(w.w);
-            w4z = Gp::template bcast_c<4>(w.z);
-            w4w = Gp::template bcast_c<4>(w.w);
+            w0z = readlane32(w.z, base + 0);
+            w0w = readlane32(w.w, base + 0);
+            w1z = readlane32(w.z, base + 1);
+            w1w = readlane32(w.w, base + 1);
+            w2z = readlane32(w.z, base + 2);
+            w2w = readlane32(w.w, base + 2);
+            w3z = readlane32(w.z, base + 3);
+            w3w = readlane32(w.w, base + 3);
+            w4z = readlane32(w.z, base + 4);
+            w4w = readlane32(w.w, base + 4);
         } else {
-            if (g < 5) swords[g] = w;
+            if (cs >= 0 && cs < 5) swords[cs] = w;
             __builtin_amdgcn_wave_barrier();
             const uint4 b0 = swords[0], b1 = swords[1], b2 = swords[2], b3 = swords[3];
             const uint4 b4 = swords[4];
@@ -866,24 +881,24 @@ __global__ void __launch_bounds__(256)
         if (u01(w0z) < tf) {
             // hot-row tournament: first maximum of row_mass over the draws,
             // as one max over keys (mass << 4 | 15 - draw index)
-            const bool in = g < TS;
+            const bool in = cs >= 0 && cs < TS;
             const uint32_t p = in ? mulhi32(w.x, (uint32_t)N) : 0u;
             const int row = sact[p];
             const int64_t wt = rp.mass_of(row);
-            const int64_t mx = Gp::max(in ? ((wt << 4) | (15 - g)) : (int64_t)-1);
-            const uint32_t tpos = Gp::bcast(p, 15 - (int)(mx & 15));
+            const int64_t mx = Gp::max(in ? ((wt << 4) | (15 - cs)) : (int64_t)-1);
+            const uint32_t tpos = Gp::bcast(p, base + 15 - (int)(mx & 15));
             const int target = sact[tpos];
             // pivot tournament: first maximum of |A[target][c]| over draws c != target
             const uint32_t pp = in ? mulhi32(w.y, (uint32_t)N) : 0u;
             const int cand = sact[pp];
             const bool valid = in && cand != target;
             const int64_t v = valid ? iabs64(rp.sA[target * P + cand]) : 0;
-            const int64_t vmax = Gp::max(valid ? ((v << 4) | (15 - g)) : (int64_t)-1);
+            const int64_t vmax = Gp::max(valid ? ((v << 4) | (15 - cs)) : (int64_t)-1);
             int ibest;
             if (vmax < 0) {
                 ibest = sact[(tpos + 1u + mulhi32(w3w, (uint32_t)(N - 1))) % (uint32_t)N];
             } else {
-                ibest = Gp::bcast(cand, 15 - (int)(vmax & 15));
+                ibest = Gp::bcast(cand, base + 15 - (int)(vmax & 15));
             }
             i = ibest;
             j = target;
