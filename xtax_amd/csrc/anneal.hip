@@ -109,11 +109,23 @@ constexpr int kDppXor1 = 0xB1;        // quad_perm [1,0,3,2]
 constexpr int kDppXor2 = 0x4E;        // quad_perm [2,3,0,1]
 constexpr int kDppHalfMirror = 0x141; // row_half_mirror
 constexpr int kDppMirror = 0x140;     // row_mirror
+constexpr int kDppShr1 = 0x111;       // row_shr:1 .. row_shr:8
+constexpr int kDppShr2 = 0x112;
+constexpr int kDppShr4 = 0x114;
+constexpr int kDppShr8 = 0x118;
+constexpr int kDppBcast15 = 0x142;    // row_bcast:15
+constexpr int kDppBcast31 = 0x143;    // row_bcast:31
 
 template <int CTRL>
 __device__ __forceinline__ uint32_t dpp32(uint32_t v) {
     return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
 }
+// shifted-in lanes (no source within the row) read 0
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp32z(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, true);
+}
+__device__ __forceinline__ uint32_t umax(uint32_t a, uint32_t b) { return a > b ? a : b; }
 template <int CTRL>
 __device__ __forceinline__ int64_t dpp64(int64_t v) {
     const uint32_t lo = dpp32<CTRL>((uint32_t)(uint64_t)v);
@@ -224,6 +236,50 @@ struct Grp {
     static __device__ __forceinline__ T gather(T v, int src) {
         return __shfl(v, src, G);
     }
+
+    // unsigned maximum over each 16-lane row, in every lane of the row
+    static __device__ __forceinline__ uint32_t rowmax32(uint32_t v) {
+        v = umax(v, dpp32<kDppXor1>(v));
+        v = umax(v, dpp32<kDppXor2>(v));
+        v = umax(v, dpp32<kDppHalfMirror>(v));
+        v = umax(v, dpp32<kDppMirror>(v));
+        return v;
+    }
+    // a row-wide value of the group's row `sub` (group lanes 16 sub .. 16 sub
+    // + 15) in every lane of the group; for G == 64 `sub` is wave-uniform
+    static __device__ __forceinline__ uint32_t from_row(uint32_t v, int sub) {
+        if constexpr (G == 16) {
+            return v;
+        } else if constexpr (G == 64) {
+            return readlane32(v, 16 * sub);
+        } else {
+            // odd rows of the first operand trade places with even rows of
+            // the second: [0] holds the pair's even row, [1] its odd row
+            const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+            return sub ? r[1] : r[0];
+        }
+    }
+    // unsigned 64-bit maximum over the group's row `sub`, in every lane of
+    // the group (high words first, then low words among the lanes that hold
+    // the maximal high word)
+    static __device__ __forceinline__ uint64_t slot_max(uint64_t key, int sub) {
+        const uint32_t hi = (uint32_t)(key >> 32), lo = (uint32_t)key;
+        const uint32_t mh = rowmax32(hi);
+        const uint32_t ml = rowmax32(hi == mh ? lo : 0u);
+        return ((uint64_t)from_row(mh, sub) << 32) | from_row(ml, sub);
+    }
+    // inclusive prefix sum over the group's lanes in lane order
+    static __device__ __forceinline__ uint32_t scan32(uint32_t v) {
+        v += dpp32z<kDppShr1>(v);
+        v += dpp32z<kDppShr2>(v);
+        v += dpp32z<kDppShr4>(v);
+        v += dpp32z<kDppShr8>(v);
+        if constexpr (G >= 32)  // last lane of rows 0 and 2 into rows 1 and 3
+            v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kDppBcast15, 0xA, 0xF, false);
+        if constexpr (G == 64)  // last lane of row 1 into rows 2 and 3
+            v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kDppBcast31, 0xC, 0xF, false);
+        return v;
+    }
 };
 
 // Result of evaluating Add(i, j, s) (main.cpp:46-91) over one group.
@@ -291,25 +347,50 @@ __device__ __forceinline__ void make_cands(int64_t cf, int64_t cc, bool has_q, i
     }
 }
 
+// Ceiling and floor of a breakpoint b = -aj/ai (ai != 0, |ai|, |aj| <= 2^48),
+// clamped to [-65, 65]. For |b| <= 66 the correctly rounded double quotient is
+// within 66 * 2^-53 of b and equals b when b is an integer, so its ceiling is
+// ceil(b) or ceil(b) - 1; one exact comparison (b <= t <=> -aj <= t ai for
+// ai > 0) settles which, and b is an integer iff t ai == -aj.
+__device__ __forceinline__ void breakpoint64(int64_t ai, int64_t aj, int32_t& ce, int32_t& fl) {
+    double qc = __builtin_ceil(-(double)aj / (double)ai);
+    qc = qc < -66.0 ? -66.0 : (qc > 66.0 ? 66.0 : qc);
+    int32_t t = (int32_t)qc;
+    const int64_t ta = (int64_t)t * ai, na = -aj;
+    const bool le = ai > 0 ? na <= ta : na >= ta;
+    const bool exact = le && ta == na;
+    t += le ? 0 : 1;
+    const int32_t f = exact ? t : t - 1;
+    ce = t < -65 ? -65 : (t > 65 ? 65 : t);
+    fl = f < -65 ? -65 : (f > 65 ? 65 : f);
+}
+
 // Exact best integer shear (main.cpp:181-242). The |a_ik|-weighted lower median
 // m of the breakpoints b_k = -a_jk/a_ik is not formed; its floor and ceiling are
-// found exactly with integer comparisons: ceil(m) = min{t : 2 W(b <= t) >= W}
-// (binary search over t in [-64, 64], clamped) and floor(m) = ceil(m) - 1 iff
-// 2 W(b < ceil m) >= W. Since |a| <= 2^48 these equal the reference's double
-// floor/ceil of the median (every breakpoint is at least 1/|a_ik| from an
-// integer while double rounding moves it by at most 2^-5/|a_ik|).
+// found exactly from each breakpoint's ceiling and floor (clamped to [-65, 65],
+// which leaves every weight sum W(b <= t), t in [-64, 64], unchanged):
+// ceil(m) = min{t : 2 W(b <= t) >= W} (binary search over t in [-64, 64],
+// clamped) and floor(m) = ceil(m) - 1 iff 2 W(b < ceil m) >= W. Since
+// |a| <= 2^48 these equal the reference's double floor/ceil of the median
+// (every breakpoint is at least 1/|a_ik| from an integer while double rounding
+// moves it by at most 2^-5/|a_ik|).
 // Candidates, clamping, dedup order and the strict-improvement rule follow the
 // reference; returns 0 when no candidate lowers the score.
 template <int G, int C>
 __device__ int64_t best_shear(const int64_t (&ai)[C], const int64_t (&aj)[C], int g, int n, int j,
                               int64_t a_ii, int64_t a_ji, int64_t a_jj) {
     int64_t w[C];
+    int32_t ce[C], fl[C];
     int64_t wl = 0;
 #pragma unroll
     for (int c = 0; c < C; ++c) {
         const int k = g + c * G;
-        w[c] = (k < n && k != j && ai[c] != 0) ? iabs64(ai[c]) : 0;
+        const bool bp = k < n && k != j && ai[c] != 0;
+        w[c] = bp ? iabs64(ai[c]) : 0;
         wl += w[c];
+        ce[c] = 0;
+        fl[c] = 0;
+        if (bp) breakpoint64(ai[c], aj[c], ce[c], fl[c]);
     }
     const int64_t tot = Grp<G>::sum(wl);
     int64_t cf = 0, cc = 0;
@@ -319,11 +400,7 @@ __device__ int64_t best_shear(const int64_t (&ai)[C], const int64_t (&aj)[C], in
             const int mid = (lo + hi) >> 1;
             int64_t acc = 0;
 #pragma unroll
-            for (int c = 0; c < C; ++c) {
-                const int64_t t = (int64_t)mid * ai[c];
-                const bool le = ai[c] > 0 ? (-aj[c] <= t) : (-aj[c] >= t);
-                acc += le ? w[c] : 0;
-            }
+            for (int c = 0; c < C; ++c) acc += ce[c] <= mid ? w[c] : 0;
             acc = Grp<G>::sum(acc);
             if (2 * acc >= tot)
                 hi = mid;
@@ -338,11 +415,7 @@ __device__ int64_t best_shear(const int64_t (&ai)[C], const int64_t (&aj)[C], in
             cc = lo;
             int64_t acc = 0;
 #pragma unroll
-            for (int c = 0; c < C; ++c) {
-                const int64_t t = (int64_t)lo * ai[c];
-                const bool lt = ai[c] > 0 ? (-aj[c] < t) : (-aj[c] > t);
-                acc += lt ? w[c] : 0;
-            }
+            for (int c = 0; c < C; ++c) acc += fl[c] < lo ? w[c] : 0;
             acc = Grp<G>::sum(acc);
             cf = (2 * acc >= tot) ? lo - 1 : lo;
         }
@@ -443,10 +516,16 @@ __device__ __forceinline__ void evaluate_small(const int32_t (&ai)[C], const int
 // breakpoint b_k = -a_jk/a_ik once from a correctly rounded double quotient
 // (exact: b_k is at least 2^-16 from any integer it is not equal to, the
 // quotient is within 2^-37 of b_k) and clamps them to [-65, 65], which leaves
-// every weight sum W(b <= t), t in [-64, 64], unchanged.
+// every weight sum W(b <= t), t in [-64, 64], unchanged. ceil(m) comes from
+// the replica's LDS histogram (bins t + 64, zero on entry and on return):
+// every breakpoint adds its weight to the bin of its ceiling, each lane scans
+// its consecutive bins, and min{t : 2 W(ce <= t) >= W} - (-64) is the number
+// of bins whose prefix sum stays below W/2 (129 when none reaches it: the
+// sentinel t = 65). Weights are below 2^16 and W below 2^23: all uint32.
 template <int G, int C>
 __device__ int64_t best_shear_small(const int32_t (&ai)[C], const int32_t (&aj)[C], int g, int n,
-                                    int j, int64_t a_ii, int64_t a_ji, int64_t a_jj) {
+                                    int j, int64_t a_ii, int64_t a_ji, int64_t a_jj,
+                                    uint32_t* hist) {
     int32_t w[C], ce[C], fl[C];
     int32_t wl = 0;
 #pragma unroll
@@ -467,18 +546,33 @@ __device__ int64_t best_shear_small(const int32_t (&ai)[C], const int32_t (&aj)[
     const int32_t tot = Grp<G>::sum32(wl);
     int cf = 0, cc = 0;
     if (tot != 0) {
-        int lo = -kShearCap, hi = kShearCap + 1;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            int32_t acc = 0;
+        constexpr int B = med_per_lane(G);
 #pragma unroll
-            for (int c = 0; c < C; ++c) acc += ce[c] <= mid ? w[c] : 0;
-            acc = Grp<G>::sum32(acc);
-            if (2 * acc >= tot)
-                hi = mid;
-            else
-                lo = mid + 1;
+        for (int c = 0; c < C; ++c)
+            if (w[c] != 0 && ce[c] <= kShearCap)
+                __hip_atomic_fetch_add(&hist[(ce[c] < -kShearCap ? -kShearCap : ce[c]) + kShearCap],
+                                       (uint32_t)w[c], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WAVEFRONT);
+        __builtin_amdgcn_wave_barrier();
+        uint32_t h[B];
+        if (g * B < kMedBins) {
+#pragma unroll
+            for (int b = 0; b < B; ++b) h[b] = hist[g * B + b];
+#pragma unroll
+            for (int b = 0; b < B; ++b) hist[g * B + b] = 0u;
+        } else {
+#pragma unroll
+            for (int b = 0; b < B; ++b) h[b] = 0u;
         }
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int b = 1; b < B; ++b) h[b] += h[b - 1];
+        const uint32_t excl = Grp<G>::scan32(h[B - 1]) - h[B - 1];
+        int32_t below = 0;
+#pragma unroll
+        for (int b = 0; b < B; ++b)
+            below += (g * B + b < kMedBins && 2u * (excl + h[b]) < (uint32_t)tot) ? 1 : 0;
+        const int lo = Grp<G>::sum32(below) - kShearCap;
         if (lo <= -kShearCap) {
             cf = cc = -kShearCap;
         } else if (lo > kShearCap) {
@@ -525,7 +619,7 @@ __device__ int64_t best_shear_small(const int32_t (&ai)[C], const int32_t (&aj)[
 template <int G, int C>
 __device__ __forceinline__ int64_t best_shear_any(const int64_t (&ai)[C], const int64_t (&aj)[C],
                                                   bool small, int g, int n, int j, int64_t a_ii,
-                                                  int64_t a_ji, int64_t a_jj) {
+                                                  int64_t a_ji, int64_t a_jj, uint32_t* hist) {
     if (small) {
         int32_t si[C], sj[C];
 #pragma unroll
@@ -533,7 +627,7 @@ __device__ __forceinline__ int64_t best_shear_any(const int64_t (&ai)[C], const 
             si[c] = (int32_t)ai[c];
             sj[c] = (int32_t)aj[c];
         }
-        return best_shear_small<G, C>(si, sj, g, n, j, a_ii, a_ji, a_jj);
+        return best_shear_small<G, C>(si, sj, g, n, j, a_ii, a_ji, a_jj, hist);
     }
     return best_shear<G, C>(ai, aj, g, n, j, a_ii, a_ji, a_jj);
 }
@@ -683,8 +777,9 @@ struct Replica {
     }
 };
 
+// Three waves per SIMD: at most 168 VGPRs (512 per SIMD lane).
 template <int G, int C>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, 3)
     k_anneal(const RepRef* __restrict__ table, int count, Inst* __restrict__ insts, int P,
              int nmax, int rep_bytes, const volatile int* __restrict__ stop) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -710,6 +805,7 @@ __global__ void __launch_bounds__(256)
     rp.sA = (int64_t*)(smem + (size_t)slot * rep_bytes);
     uint4* swords = (uint4*)(rp.sA + (size_t)nmax * P);
     uint8_t* sact = (uint8_t*)(swords + 5);
+    uint32_t* shist = (uint32_t*)(sact + nmax);
     rp.act = sact;
     rp.P = P;
     rp.n = I->n;
@@ -721,7 +817,17 @@ __global__ void __launch_bounds__(256)
     rp.gX = (GLOBAL int64_t*)I->Xt + (int64_t)r * nn;
     rp.load_A(gAcur);
     const GLOBAL uint8_t* gact = (const GLOBAL uint8_t*)I->act;
-    for (int q = g; q < N; q += G) sact[q] = gact[q];
+    bool nonid = false;
+    for (int q = g; q < N; q += G) {
+        const uint8_t a = gact[q];
+        sact[q] = a;
+        nonid |= a != q;
+    }
+    // every group of the wave holds the same list: wave-uniform
+    const bool ident = N == n && __ballot(nonid) == 0ull;
+    // index at position p of the active list
+    auto rowat = [&](uint32_t p) -> int { return ident ? (int)p : (int)sact[p]; };
+    for (int q = g; q < med_slots(G); q += G) shist[q] = 0u;
     rp.compute_row_mass();
 
     const InstParams Pp = I->P;
@@ -788,7 +894,8 @@ __global__ void __launch_bounds__(256)
             const int64_t a_ji = rp.sA[jj * P + pv];
             const int64_t a_jj = rp.sA[jj * P + jj];
             const bool small = rows_small<G, C>(ai, aj);
-            const int64_t s = best_shear_any<G, C>(ai, aj, small, g, n, jj, a_ii, a_ji, a_jj);
+            const int64_t s =
+                best_shear_any<G, C>(ai, aj, small, g, n, jj, a_ii, a_ji, a_jj, shist);
             ++greedy_n;
             if (s == 0) continue;
             MoveEval<C> e;
@@ -880,23 +987,26 @@ __global__ void __launch_bounds__(256)
         int i, j;
         if (u01(w0z) < tf) {
             // hot-row tournament: first maximum of row_mass over the draws,
-            // as one max over keys (mass << 4 | 15 - draw index)
+            // as one max over keys (mass << 5 | 16 | 15 - draw index; 0 for
+            // no draw) within the 16-lane row holding this move's slots
             const bool in = cs >= 0 && cs < TS;
             const uint32_t p = in ? mulhi32(w.x, (uint32_t)N) : 0u;
-            const int row = sact[p];
+            const int row = rowat(p);
             const int64_t wt = rp.mass_of(row);
-            const int64_t mx = Gp::max(in ? ((wt << 4) | (15 - cs)) : (int64_t)-1);
+            const uint64_t mx =
+                Gp::slot_max(in ? (((uint64_t)wt << 5) | 16u | (uint32_t)(15 - cs)) : 0ull, sub);
             const uint32_t tpos = Gp::bcast(p, base + 15 - (int)(mx & 15));
-            const int target = sact[tpos];
+            const int target = rowat(tpos);
             // pivot tournament: first maximum of |A[target][c]| over draws c != target
             const uint32_t pp = in ? mulhi32(w.y, (uint32_t)N) : 0u;
-            const int cand = sact[pp];
+            const int cand = rowat(pp);
             const bool valid = in && cand != target;
             const int64_t v = valid ? iabs64(rp.sA[target * P + cand]) : 0;
-            const int64_t vmax = Gp::max(valid ? ((v << 4) | (15 - cs)) : (int64_t)-1);
+            const uint64_t vmax =
+                Gp::slot_max(valid ? (((uint64_t)v << 5) | 16u | (uint32_t)(15 - cs)) : 0ull, sub);
             int ibest;
-            if (vmax < 0) {
-                ibest = sact[(tpos + 1u + mulhi32(w3w, (uint32_t)(N - 1))) % (uint32_t)N];
+            if (vmax == 0) {
+                ibest = rowat((tpos + 1u + mulhi32(w3w, (uint32_t)(N - 1))) % (uint32_t)N);
             } else {
                 ibest = Gp::bcast(cand, base + 15 - (int)(vmax & 15));
             }
@@ -905,8 +1015,8 @@ __global__ void __launch_bounds__(256)
         } else {
             const uint32_t ip = mulhi32(w2w, (uint32_t)N);
             const uint32_t jp = (ip + 1u + mulhi32(w3z, (uint32_t)(N - 1))) % (uint32_t)N;
-            i = sact[ip];
-            j = sact[jp];
+            i = rowat(ip);
+            j = rowat(jp);
         }
 
         int64_t ai[C], aj[C], xi[C], xj[C];
@@ -921,7 +1031,7 @@ __global__ void __launch_bounds__(256)
         const int64_t pm = (w1z & 1u) ? 1 : -1;
         int64_t s = pm;
         if (u01(w0w) < gf) {
-            s = best_shear_any<G, C>(ai, aj, small, g, n, j, a_ii, a_ji, a_jj);
+            s = best_shear_any<G, C>(ai, aj, small, g, n, j, a_ii, a_ji, a_jj, shist);
             ++greedy_n;
             if (s == 0) s = pm;
         }
@@ -1199,8 +1309,11 @@ template <int G, int C>
 __global__ void __launch_bounds__(64)
     k_probe(const int64_t* __restrict__ A, const int64_t* __restrict__ Xt, int n,
             const int32_t* __restrict__ queries, int nq, int64_t* __restrict__ out) {
+    __shared__ uint32_t hist[(64 / G) * med_slots(G)];
     const int g = (int)threadIdx.x & (G - 1);
     const int q = (int)blockIdx.x * (64 / G) + (int)threadIdx.x / G;
+    for (int t = (int)threadIdx.x; t < (64 / G) * med_slots(G); t += 64) hist[t] = 0u;
+    __syncthreads();
     if (q >= nq) return;
     const int mat = queries[4 * q + 0], i = queries[4 * q + 1], j = queries[4 * q + 2];
     const int64_t s = queries[4 * q + 3];
@@ -1217,7 +1330,8 @@ __global__ void __launch_bounds__(64)
     const bool small = rows_small<G, C>(ai, aj);
     MoveEval<C> e;
     evaluate_any<G, C>(ai, aj, small, g, n, j, s, a_ii, a_ji, a_jj, e);
-    const int64_t b = best_shear_any<G, C>(ai, aj, small, g, n, j, a_ii, a_ji, a_jj);
+    const int64_t b = best_shear_any<G, C>(ai, aj, small, g, n, j, a_ii, a_ji, a_jj,
+                                           hist + ((int)threadIdx.x / G) * med_slots(G));
     bool bx = false;
 #pragma unroll
     for (int c = 0; c < C; ++c) {

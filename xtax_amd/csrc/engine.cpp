@@ -88,9 +88,9 @@ Layout layout_for(int n) {
         L.C = 2;
     }
     L.P = L.nmax + 1;
-    // A (nmax rows of stride P), 5 staged Philox blocks, active list; X^T
-    // stays in HBM
-    const int64_t bytes = (int64_t)L.nmax * L.P * 8 + 5 * 16 + L.nmax;
+    // A (nmax rows of stride P), 5 staged Philox blocks, active list, median
+    // histogram; X^T stays in HBM
+    const int64_t bytes = (int64_t)L.nmax * L.P * 8 + 5 * 16 + L.nmax + 4 * med_slots(L.G);
     L.rep_bytes = (int)((bytes + 15) / 16 * 16);
     return L;
 }

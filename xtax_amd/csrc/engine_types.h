@@ -8,8 +8,8 @@
 //   temps      R double          tempering ladder, rung -> temperature
 //   act        N uint8           active indices (the reference's `active` list)
 //   log        R * log_cap       accepted-move log per replica (optional)
-// During an epoch a replica's A and Xt live in LDS (row stride P, odd) and its
-// row masses in registers; they return to HBM at the epoch end.
+// During an epoch a replica's A lives in LDS (row stride P, odd) and its row
+// masses in registers (A returns to HBM at the epoch end); X^T stays in HBM.
 #pragma once
 #include <stdint.h>
 
@@ -20,6 +20,16 @@ constexpr int kShearCap = 64;                 // mat_io.hpp:38
 constexpr int kMaxN = 128;
 constexpr int kMaxSamples = 16;
 constexpr int kRepBits = 20;                  // solve key = move << 20 | replica
+
+// Weighted-median histogram of the greedy shear (one per replica, LDS): bin
+// t + 64 holds the weight of breakpoints with ceiling t, t in [-64, 64]. A
+// group of G lanes scans it with med_per_lane(G) consecutive bins per lane;
+// med_slots(G) bins are allocated (whole lanes).
+constexpr int kMedBins = 2 * kShearCap + 1;
+constexpr int med_per_lane(int G) { return (kMedBins + G - 1) / G; }
+constexpr int med_slots(int G) {
+    return (kMedBins + med_per_lane(G) - 1) / med_per_lane(G) * med_per_lane(G);
+}
 constexpr unsigned long long kNoKey = ~0ull;
 
 enum InstStatus : int32_t {
