@@ -1,9 +1,10 @@
 // anneal.hip -- gfx950 kernels of the xtax L1 congruence annealer.
 //
 // Mapping: one replica per lane group of G lanes (G = 16, 32 or 64 by n), lane g
-// owning columns k = g + c*G (c < C). A replica's A and X^T sit in LDS with an
-// odd row stride P; its row masses (main.cpp:136-144) in registers of the lanes
-// owning those rows. Every sum over columns is a group reduction; every choice
+// owning columns k = g + c*G (c < C). A replica's A sits in LDS as its upper
+// triangle (half the footprint of the square, so twice the resident replicas),
+// its X^T in HBM (read on a proposal, written on acceptance), its row masses
+// (main.cpp:136-144) in registers of the lanes owning those rows. Every sum over columns is a group reduction; every choice
 // (pair, shear, accept) is made identically by all G lanes from broadcast
 // values, so the group runs the reference's sequential worker loop
 // (congruence_anneal.hpp:276-413) with O(n/G) work per lane per step.
@@ -656,21 +657,41 @@ __device__ __forceinline__ void evaluate_any(const int64_t (&ai)[C], const int64
 
 template <int G, int C>
 struct Replica {
-    int64_t* sA;        // LDS: A, n rows of stride P
+    // LDS: the upper triangle of the symmetric A, row r holding columns
+    // r .. n-1, so that A[a][b] sits at D(min(a, b)) + max(a, b)
+    int64_t* sT;
     GLOBAL int64_t* gX; // HBM: X^T of this replica, n x n row-major (touched on accept)
     const uint8_t* act;
-    int P, n, N, g;
+    int n, N, g;
+    int Dk[C];          // D(k) of the lane's columns
     int64_t rm[C];      // row masses of the lane's rows
+
+    // D(r) = (start of row r) - r; r (2n - r - 1) is even
+    __device__ __forceinline__ int D(int r) const { return (r * (2 * n - r - 1)) >> 1; }
+    __device__ __forceinline__ int at(int a, int b) const {
+        return a <= b ? D(a) + b : D(b) + a;
+    }
+    __device__ __forceinline__ void init_lanes() {
+#pragma unroll
+        for (int c = 0; c < C; ++c) Dk[c] = D(g + c * G);
+    }
+    // A[r][k] for the lane's column k = g + c G, given D(r)
+    __device__ __forceinline__ int at_lane(int r, int Dr, int c) const {
+        const int k = g + c * G;
+        return k >= r ? Dr + k : Dk[c] + r;
+    }
 
     __device__ __forceinline__ void load_rows(int i, int j, int64_t (&ai)[C],
                                               int64_t (&aj)[C]) const {
+        const int Di = D(i), Dj = D(j);
 #pragma unroll
         for (int c = 0; c < C; ++c) {
             const int k = g + c * G;
-            ai[c] = k < n ? sA[i * P + k] : 0;
-            aj[c] = k < n ? sA[j * P + k] : 0;
+            ai[c] = k < n ? sT[at_lane(i, Di, c)] : 0;
+            aj[c] = k < n ? sT[at_lane(j, Dj, c)] : 0;
         }
     }
+    __device__ __forceinline__ int64_t entry(int a, int b) const { return sT[at(a, b)]; }
 
     // Rows i and j of X^T, issued early so the HBM latency overlaps the
     // evaluation of the move (only an accepted move consumes them).
@@ -691,7 +712,7 @@ struct Replica {
             int64_t m = 0;
             if (k < n)
                 for (int t = 0; t < n; ++t)
-                    if (t != k) m += iabs64(sA[k * P + t]);
+                    if (t != k) m += iabs64(sT[at_lane(t, D(t), c)]);
             rm[c] = m;
         }
     }
@@ -708,20 +729,23 @@ struct Replica {
 
     __device__ __forceinline__ void store_A(GLOBAL int64_t* dst) const {
         for (int row = 0; row < n; ++row) {
+            const int Dr = D(row);
 #pragma unroll
             for (int c = 0; c < C; ++c) {
                 const int k = g + c * G;
-                if (k < n) dst[row * n + k] = sA[row * P + k];
+                if (k < n) dst[row * n + k] = sT[at_lane(row, Dr, c)];
             }
         }
     }
 
+    // upper triangle of a full row-major n x n matrix in HBM
     __device__ __forceinline__ void load_A(const GLOBAL int64_t* src) {
         for (int row = 0; row < n; ++row) {
+            const int Dr = D(row);
 #pragma unroll
             for (int c = 0; c < C; ++c) {
                 const int k = g + c * G;
-                if (k < n) sA[row * P + k] = src[row * n + k];
+                if (k >= row && k < n) sT[Dr + k] = src[row * n + k];
             }
         }
     }
@@ -754,23 +778,18 @@ struct Replica {
         return !Grp<G>::any(bad);
     }
 
-    // commit_add (main.cpp:112-133): row j and its mirror column of A, row j of
-    // X^T, row masses (row j's new mass is its old mass plus the off-diagonal
-    // change).
+    // commit_add (main.cpp:112-133): row j of A (with its mirror column: one
+    // triangle entry per lane), row j of X^T, row masses (row j's new mass is
+    // its old mass plus the off-diagonal change).
     __device__ __forceinline__ void write(int j, const int64_t (&aj)[C], const MoveEval<C>& e,
                                           const int64_t (&nx)[C]) {
+        const int Dj = D(j);
 #pragma unroll
         for (int c = 0; c < C; ++c) {
             const int k = g + c * G;
             if (k < n) {
-                if (k != j) {
-                    sA[j * P + k] = e.nv[c];
-                    sA[k * P + j] = e.nv[c];
-                    rm[c] += iabs64(e.nv[c]) - iabs64(aj[c]);
-                } else {
-                    sA[j * P + j] = e.nv[c];
-                    rm[c] += e.off;
-                }
+                sT[at_lane(j, Dj, c)] = e.nv[c];
+                rm[c] += (k != j) ? iabs64(e.nv[c]) - iabs64(aj[c]) : e.off;
                 gX[j * n + k] = nx[c];
             }
         }
@@ -780,8 +799,8 @@ struct Replica {
 // Three waves per SIMD: at most 168 VGPRs (512 per SIMD lane).
 template <int G, int C>
 __global__ void __launch_bounds__(256, 3)
-    k_anneal(const RepRef* __restrict__ table, int count, Inst* __restrict__ insts, int P,
-             int nmax, int rep_bytes, const volatile int* __restrict__ stop) {
+    k_anneal(const RepRef* __restrict__ table, int count, Inst* __restrict__ insts, int nmax,
+             int rep_bytes, const volatile int* __restrict__ stop) {
     extern __shared__ __align__(16) unsigned char smem[];
     using Gp = Grp<G>;
     const int g = (int)threadIdx.x & (G - 1);
@@ -802,15 +821,16 @@ __global__ void __launch_bounds__(256, 3)
     if (S->flags & RF_STOPPED) return;
 
     Replica<G, C> rp;
-    rp.sA = (int64_t*)(smem + (size_t)slot * rep_bytes);
-    uint4* swords = (uint4*)(rp.sA + (size_t)nmax * P);
+    rp.sT = (int64_t*)(smem + (size_t)slot * rep_bytes);
+    // (nmax (nmax + 1) / 2) * 8 bytes: a multiple of 32 for nmax % 8 == 0
+    uint4* swords = (uint4*)(rp.sT + (size_t)nmax * (nmax + 1) / 2);
     uint8_t* sact = (uint8_t*)(swords + 5);
     uint32_t* shist = (uint32_t*)(sact + nmax);
     rp.act = sact;
-    rp.P = P;
     rp.n = I->n;
     rp.N = I->N;
     rp.g = g;
+    rp.init_lanes();
     const int n = rp.n, N = rp.N;
     const int64_t nn = (int64_t)n * n;
     GLOBAL int64_t* const gAcur = (GLOBAL int64_t*)I->A + (int64_t)r * nn;
@@ -890,9 +910,9 @@ __global__ void __launch_bounds__(256, 3)
             if (jj == pv) continue;
             int64_t ai[C], aj[C];
             rp.load_rows(pv, jj, ai, aj);
-            const int64_t a_ii = rp.sA[pv * P + pv];
-            const int64_t a_ji = rp.sA[jj * P + pv];
-            const int64_t a_jj = rp.sA[jj * P + jj];
+            const int64_t a_ii = rp.entry(pv, pv);
+            const int64_t a_ji = rp.entry(jj, pv);
+            const int64_t a_jj = rp.entry(jj, jj);
             const bool small = rows_small<G, C>(ai, aj);
             const int64_t s =
                 best_shear_any<G, C>(ai, aj, small, g, n, jj, a_ii, a_ji, a_jj, shist);
@@ -1001,7 +1021,7 @@ __global__ void __launch_bounds__(256, 3)
             const uint32_t pp = in ? mulhi32(w.y, (uint32_t)N) : 0u;
             const int cand = rowat(pp);
             const bool valid = in && cand != target;
-            const int64_t v = valid ? iabs64(rp.sA[target * P + cand]) : 0;
+            const int64_t v = valid ? iabs64(rp.entry(target, cand)) : 0;
             const uint64_t vmax =
                 Gp::slot_max(valid ? (((uint64_t)v << 5) | 16u | (uint32_t)(15 - cs)) : 0ull, sub);
             int ibest;
@@ -1022,9 +1042,9 @@ __global__ void __launch_bounds__(256, 3)
         int64_t ai[C], aj[C], xi[C], xj[C];
         rp.load_rows(i, j, ai, aj);
         rp.prefetch_xt(i, j, xi, xj);
-        const int64_t a_ii = rp.sA[i * P + i];
-        const int64_t a_ji = rp.sA[j * P + i];
-        const int64_t a_jj = rp.sA[j * P + j];
+        const int64_t a_ii = rp.entry(i, i);
+        const int64_t a_ji = rp.entry(j, i);
+        const int64_t a_jj = rp.entry(j, j);
 
         // ---- shear (congruence_anneal.hpp:290-296) ----
         const bool small = rows_small<G, C>(ai, aj);
@@ -1366,17 +1386,16 @@ __global__ void k_probe_rng(const uint32_t* __restrict__ ck, int count, uint32_t
 // host-side launch helpers
 // ---------------------------------------------------------------------------
 
-typedef void (*AnnealFn)(const RepRef*, int, Inst*, int, int, int, const volatile int*);
 
 template <int G, int C>
 static hipError_t launch_anneal_t(hipStream_t st, int groups_per_block, const RepRef* table,
-                                  int count, Inst* insts, int P, int nmax, int rep_bytes,
+                                  int count, Inst* insts, int nmax, int rep_bytes,
                                   const volatile int* stop) {
     const int threads = groups_per_block * G;
     const int blocks = (count + groups_per_block - 1) / groups_per_block;
     const size_t lds = (size_t)groups_per_block * rep_bytes;
     hipLaunchKernelGGL((k_anneal<G, C>), dim3(blocks), dim3(threads), lds, st, table, count, insts,
-                       P, nmax, rep_bytes, stop);
+                       nmax, rep_bytes, stop);
     return hipGetLastError();
 }
 
@@ -1398,20 +1417,20 @@ hipError_t init_kernel_attrs(int max_lds_per_block) {
 }
 
 hipError_t launch_anneal(hipStream_t st, int G, int C, int groups_per_block, const RepRef* table,
-                         int count, Inst* insts, int P, int nmax, int rep_bytes,
+                         int count, Inst* insts, int nmax, int rep_bytes,
                          const volatile int* stop) {
     if (count <= 0) return hipSuccess;
     if (G == 16 && C == 1)
-        return launch_anneal_t<16, 1>(st, groups_per_block, table, count, insts, P, nmax,
+        return launch_anneal_t<16, 1>(st, groups_per_block, table, count, insts, nmax,
                                       rep_bytes, stop);
     if (G == 32 && C == 1)
-        return launch_anneal_t<32, 1>(st, groups_per_block, table, count, insts, P, nmax,
+        return launch_anneal_t<32, 1>(st, groups_per_block, table, count, insts, nmax,
                                       rep_bytes, stop);
     if (G == 64 && C == 1)
-        return launch_anneal_t<64, 1>(st, groups_per_block, table, count, insts, P, nmax,
+        return launch_anneal_t<64, 1>(st, groups_per_block, table, count, insts, nmax,
                                       rep_bytes, stop);
     if (G == 64 && C == 2)
-        return launch_anneal_t<64, 2>(st, groups_per_block, table, count, insts, P, nmax,
+        return launch_anneal_t<64, 2>(st, groups_per_block, table, count, insts, nmax,
                                       rep_bytes, stop);
     return hipErrorInvalidValue;
 }

@@ -32,7 +32,7 @@
 
 namespace xtx {
 hipError_t launch_anneal(hipStream_t st, int G, int C, int groups_per_block, const RepRef* table,
-                         int count, Inst* insts, int P, int nmax, int rep_bytes,
+                         int count, Inst* insts, int nmax, int rep_bytes,
                          const volatile int* stop);
 hipError_t launch_epoch_end(hipStream_t st, Inst* insts, int count, int max_r);
 hipError_t launch_probe(hipStream_t st, int G, int C, const int64_t* A, const int64_t* Xt, int n,
@@ -65,9 +65,9 @@ inline int64_t iabs(int64_t v) { return v < 0 ? -v : v; }
 
 // ---------------------------------------------------------------- layout
 // LDS bucket of an order n: lane-group width G, columns per lane C, padded
-// order nmax (multiple of 8) and odd row stride P = nmax + 1.
+// order nmax (multiple of 8) and the LDS bytes of one replica.
 struct Layout {
-    int G, C, nmax, P, rep_bytes;
+    int G, C, nmax, rep_bytes;
     bool operator<(const Layout& o) const { return nmax < o.nmax; }
 };
 
@@ -87,10 +87,10 @@ Layout layout_for(int n) {
         L.G = 64;
         L.C = 2;
     }
-    L.P = L.nmax + 1;
-    // A (nmax rows of stride P), 5 staged Philox blocks, active list, median
+    // upper triangle of A, 5 staged Philox blocks, active list, median
     // histogram; X^T stays in HBM
-    const int64_t bytes = (int64_t)L.nmax * L.P * 8 + 5 * 16 + L.nmax + 4 * med_slots(L.G);
+    const int64_t bytes =
+        (int64_t)L.nmax * (L.nmax + 1) / 2 * 8 + 5 * 16 + L.nmax + 4 * med_slots(L.G);
     L.rep_bytes = (int)((bytes + 15) / 16 * 16);
     return L;
 }
@@ -585,7 +585,7 @@ void launch_epoch(hipStream_t st, std::vector<Bucket>& buckets, Inst* d_insts, i
     for (auto& b : buckets) {
         if (b.refs.empty()) continue;
         HIPCHK(launch_anneal(st, b.lay.G, b.lay.C, b.groups_per_block, b.d_refs,
-                             (int)b.refs.size(), d_insts, b.lay.P, b.lay.nmax, b.lay.rep_bytes,
+                             (int)b.refs.size(), d_insts, b.lay.nmax, b.lay.rep_bytes,
                              stop_dev));
     }
     HIPCHK(launch_epoch_end(st, d_insts, n_insts, max_r));
@@ -773,7 +773,7 @@ int xtx_solve_plain(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t 
                 HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
                 for (auto& b : buckets)
                     HIPCHK(launch_anneal(ctx->stream, b.lay.G, b.lay.C, b.groups_per_block,
-                                         b.d_refs, (int)b.refs.size(), ctx->d_plain, b.lay.P,
+                                         b.d_refs, (int)b.refs.size(), ctx->d_plain,
                                          b.lay.nmax, b.lay.rep_bytes, ctx->stop_dev));
                 HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
                 HIPCHK(launch_epoch_end(ctx->stream, ctx->d_plain, 1, h.R));
