@@ -104,6 +104,45 @@ __device__ __forceinline__ int64_t clamp_cap(int64_t v) {
     return v > kShearCap ? kShearCap : (v < -kShearCap ? -kShearCap : v);
 }
 
+// The same pointer, opaque to the optimizer: loads through it stay in the
+// branch that makes them instead of being hoisted out of the move loop, so
+// rarely used values do not hold registers across it.
+template <class T>
+__device__ __forceinline__ T* opaque(T* p) {
+    asm volatile("" : "+s"(p));
+    return p;
+}
+__device__ __forceinline__ void opaque_u32(uint32_t& v) { asm volatile("" : "+s"(v)); }
+
+// A wave-uniform value in scalar registers (whole-wave groups only).
+template <int G>
+__device__ __forceinline__ int32_t uni(int32_t v) {
+    if constexpr (G == 64) return __builtin_amdgcn_readfirstlane(v);
+    return v;
+}
+template <int G>
+__device__ __forceinline__ int64_t uni(int64_t v) {
+    if constexpr (G == 64) {
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(uint64_t)v);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
+        return (int64_t)(((uint64_t)hi << 32) | lo);
+    }
+    return v;
+}
+
+// Coin u01(w) < f as an integer test: w 2^-32 and f 2^32 are exact doubles,
+// so for integer w, u01(w) < f <=> w < ceil(f 2^32); `all` covers f >= 1.
+struct Coin {
+    uint32_t thr;
+    bool all;
+    __device__ __forceinline__ explicit Coin(double f) {
+        const double x = __builtin_ceil(f * 4294967296.0);
+        all = !(x < 4294967296.0);
+        thr = x > 0.0 && !all ? (uint32_t)x : 0u;
+    }
+    __device__ __forceinline__ bool operator()(uint32_t w) const { return all || w < thr; }
+};
+
 // ---- cross-lane building blocks (DPP within 16-lane rows, v_readlane across
 // rows; ds_bpermute only where each lane reads a different, runtime lane) ----
 constexpr int kDppXor1 = 0xB1;        // quad_perm [1,0,3,2]
@@ -850,37 +889,43 @@ __global__ void __launch_bounds__(256, 3)
     for (int q = g; q < med_slots(G); q += G) shist[q] = 0u;
     rp.compute_row_mass();
 
-    const InstParams Pp = I->P;
+    // Values read on every move stay in registers (scalar ones for a
+    // whole-wave group); what only the stall handling, cooling steps, best
+    // flushes, reseeds and the move log need is read from the instance record
+    // where it is used.
     const uint32_t key0 = I->key0, key1 = I->key1;
-    const uint64_t epoch_end = (uint64_t)(I->epoch + 1u) * (uint64_t)Pp.epoch_moves;
-    const int64_t gb_epoch = I->gb_score;
-    const bool tempering = Pp.tempering != 0;
-    const int TS = Pp.target_samples;
-    GLOBAL int64_t* bA = (GLOBAL int64_t*)I->bA + (int64_t)r * nn;
-    GLOBAL int64_t* bX = (GLOBAL int64_t*)I->bXt + (int64_t)r * nn;
-    const GLOBAL int64_t* gsnapA = (const GLOBAL int64_t*)I->gA;
-    const GLOBAL int64_t* gsnapX = (const GLOBAL int64_t*)I->gXt;
-    GLOBAL LogEnt* logp = I->log ? (GLOBAL LogEnt*)I->log + (int64_t)r * I->log_cap : nullptr;
-    const int64_t log_cap = I->log_cap;
+    const uint64_t epoch_end = (uint64_t)(I->epoch + 1u) * (uint64_t)I->P.epoch_moves;
+    const uint64_t cap = I->P.max_moves;  // 0: none
+    const uint64_t stop_at = (cap != 0 && cap < epoch_end) ? cap : epoch_end;
+    const bool tempering = I->P.tempering != 0;
+    const int TS = I->P.target_samples;
+    const int32_t stuck = I->P.stuck_threshold;
+    const int32_t per_cool = I->P.moves_per_cool;
+    const bool logging = I->log != nullptr;
 
-    uint64_t moves = S->moves;
-    int64_t score = S->score, offnz = S->offnz, local_best = S->local_best, msi = S->msi;
+    uint64_t moves = (uint64_t)uni<G>((int64_t)S->moves);
+    int64_t score = uni<G>(S->score), offnz = uni<G>(S->offnz);
+    int64_t local_best = uni<G>(S->local_best), msi = uni<G>(S->msi);
     double T = tempering ? ((const GLOBAL double*)I->temps)[S->rung] : S->T;
     double cooling = S->cooling;
-    int32_t cool_counter = S->cool_counter, win_p = S->win_p, win_a = S->win_a;
-    int64_t best_score = S->best_score, best_move = S->best_move;
-    int32_t flags = S->flags;
-    const double gf = S->gf, tf = S->tf;
-    int64_t accepts = S->accepts, greedy_n = S->greedy, infeasible = S->infeasible;
-    int64_t sweeps = S->sweeps, log_count = S->log_count, solve_move = S->solve_move;
+    int32_t cool_counter = uni<G>(S->cool_counter), win_p = uni<G>(S->win_p);
+    int32_t win_a = uni<G>(S->win_a);
+    int64_t best_score = uni<G>(S->best_score), best_move = uni<G>(S->best_move);
+    int32_t flags = uni<G>(S->flags);
+    const Coin greedy_coin(S->gf), target_coin(S->tf);
+    int64_t accepts = uni<G>(S->accepts), greedy_n = uni<G>(S->greedy);
+    int64_t infeasible = uni<G>(S->infeasible), sweeps = uni<G>(S->sweeps);
+    int64_t log_count = uni<G>(S->log_count), solve_move = uni<G>(S->solve_move);
 
     // Lazy best flush: the pre-change state goes to bA/bXt only when it is the
     // replica's best and could still be the instance's best.
     auto leave_best = [&]() {
         if (flags & RF_CUR_BEST) {
-            if (best_score <= gb_epoch) {
-                rp.store_A(bA);
-                rp.copy_cols(bX, rp.gX);
+            const Inst* Iq = opaque(I);
+            if (best_score <= Iq->gb_score) {
+                const int64_t off = (int64_t)r * n * n;
+                rp.store_A((GLOBAL int64_t*)Iq->bA + off);
+                rp.copy_cols((GLOBAL int64_t*)Iq->bXt + off, rp.gX);
                 flags &= ~RF_BEST_STALE;
             } else {
                 flags |= RF_BEST_STALE;
@@ -889,9 +934,11 @@ __global__ void __launch_bounds__(256, 3)
         }
     };
     auto log_move = [&](uint64_t m, int i, int j, int64_t s, uint16_t f) {
-        if (logp) {
-            if (log_count < log_cap && g == 0) {
-                GLOBAL LogEnt* e = logp + log_count;
+        if (logging) {
+            const Inst* Iq = opaque(I);
+            if (log_count < Iq->log_cap && g == 0) {
+                GLOBAL LogEnt* e =
+                    (GLOBAL LogEnt*)Iq->log + (int64_t)r * Iq->log_cap + log_count;
                 e->move = (int64_t)m;
                 e->i = (int16_t)i;
                 e->j = (int16_t)j;
@@ -945,7 +992,7 @@ __global__ void __launch_bounds__(256, 3)
     uint4 w = make_uint4(0u, 0u, 0u, 0u);
     bool have_w = false;
     bool halt = false;
-    for (uint32_t it = 0; moves < epoch_end; ++it) {
+    for (uint32_t it = 0; moves < stop_at; ++it) {
         if ((it & 255u) == 0u) {
             const unsigned long long key =
                 __hip_atomic_load(&I->solve_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -965,8 +1012,11 @@ __global__ void __launch_bounds__(256, 3)
         const int sub = (int)(t4 & (uint32_t)(S - 1));
         if (sub == 0 || !have_w) {
             const uint64_t q = (m - 1) >> 2;
+            uint32_t k0 = key0, k1 = key1;  // round keys made here, not held
+            opaque_u32(k0);
+            opaque_u32(k1);
             w = philox10((uint32_t)q, (uint32_t)(q >> 32), (uint32_t)r,
-                         16u * (t4 & ~(uint32_t)(S - 1)) + (uint32_t)g, key0, key1);
+                         16u * (t4 & ~(uint32_t)(S - 1)) + (uint32_t)g, k0, k1);
             have_w = true;
         }
         const int base = 16 * sub;
@@ -1005,7 +1055,7 @@ __global__ void __launch_bounds__(256, 3)
 
         // ---- pair (congruence_anneal.hpp:283-288, :211-228) ----
         int i, j;
-        if (u01(w0z) < tf) {
+        if (target_coin(w0z)) {
             // hot-row tournament: first maximum of row_mass over the draws,
             // as one max over keys (mass << 5 | 16 | 15 - draw index; 0 for
             // no draw) within the 16-lane row holding this move's slots
@@ -1050,7 +1100,7 @@ __global__ void __launch_bounds__(256, 3)
         const bool small = rows_small<G, C>(ai, aj);
         const int64_t pm = (w1z & 1u) ? 1 : -1;
         int64_t s = pm;
-        if (u01(w0w) < gf) {
+        if (greedy_coin(w0w)) {
             s = best_shear_any<G, C>(ai, aj, small, g, n, j, a_ii, a_ji, a_jj, shist);
             ++greedy_n;
             if (s == 0) s = pm;
@@ -1098,8 +1148,9 @@ __global__ void __launch_bounds__(256, 3)
 
         // ---- stall handling (congruence_anneal.hpp:363-412) ----
         if (tempering) {
-            if (msi > Pp.stuck_threshold) {
-                if (Pp.sweep_fraction > 0.0 && u01(w4z) < Pp.sweep_fraction) {
+            if (msi > stuck) {
+                const InstParams& Q = opaque(I)->P;
+                if (Q.sweep_fraction > 0.0 && u01(w4z) < Q.sweep_fraction) {
                     sweep(sact[mulhi32(w4w, (uint32_t)N)], m);
                     track_best(m);
                 }
@@ -1107,33 +1158,37 @@ __global__ void __launch_bounds__(256, 3)
                 msi = 0;
             }
         } else {
-            if (msi > Pp.stuck_threshold) {
-                if (Pp.sweep_fraction > 0.0 && u01(w4z) < Pp.sweep_fraction) {
+            if (msi > stuck) {
+                const Inst* Iq = opaque(I);
+                if (Iq->P.sweep_fraction > 0.0 && u01(w4z) < Iq->P.sweep_fraction) {
                     if (sweep(sact[mulhi32(w4w, (uint32_t)N)], m)) {
                         local_best = score;
                         msi = 0;
                     }
                     track_best(m);
                 }
-                T = Pp.reheat * Pp.t_init;
-                if ((double)score > (double)gb_epoch * Pp.reseed_factor) {
+                const Inst* Iv = opaque(I);
+                T = Iv->P.reheat * Iv->P.t_init;
+                const int64_t gb_epoch = Iv->gb_score;
+                if ((double)score > (double)gb_epoch * Iv->P.reseed_factor) {
                     leave_best();
                     log_move(m, -1, -1, 0, 2);
-                    rp.load_A(gsnapA);
-                    rp.copy_cols(rp.gX, gsnapX);
+                    rp.load_A((const GLOBAL int64_t*)Iv->gA);
+                    rp.copy_cols(rp.gX, (const GLOBAL int64_t*)Iv->gXt);
                     rp.compute_row_mass();
                     score = gb_epoch;
-                    offnz = I->gb_offnz;
+                    offnz = Iv->gb_offnz;
                     track_best(m);
                 }
                 local_best = score;
                 msi = 0;
                 cool_counter = 0;
-            } else if (++cool_counter >= Pp.moves_per_cool) {
+            } else if (++cool_counter >= per_cool) {
+                const InstParams& Q = opaque(I)->P;
                 cool_counter = 0;
-                if (Pp.adaptive && win_p > 0) {
+                if (Q.adaptive && win_p > 0) {
                     const double acc = (double)win_a / (double)win_p;
-                    if (acc > Pp.target_accept)
+                    if (acc > Q.target_accept)
                         cooling *= 0.999;
                     else
                         cooling /= 0.999;
@@ -1142,7 +1197,7 @@ __global__ void __launch_bounds__(256, 3)
                 win_p = 0;
                 win_a = 0;
                 T *= cooling;
-                if (T < Pp.t_min) T = Pp.t_min;
+                if (T < Q.t_min) T = Q.t_min;
             }
         }
         if (offnz == 0) {
@@ -1150,12 +1205,9 @@ __global__ void __launch_bounds__(256, 3)
             solve_move = (int64_t)m;
             break;
         }
-        if (Pp.max_moves && moves >= Pp.max_moves) {
-            flags |= RF_STOPPED | RF_CAPPED;
-            break;
-        }
     }
-    (void)halt;
+    // the move cap ends the loop through stop_at
+    if (!(flags & RF_STOPPED) && !halt && cap != 0 && moves >= cap) flags |= RF_STOPPED | RF_CAPPED;
 
     rp.store_A(gAcur);
     if (g == 0) {
