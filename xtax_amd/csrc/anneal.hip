@@ -69,23 +69,30 @@ __device__ __forceinline__ double u53(uint32_t hi, uint32_t lo) {
 // exp(x), x <= 0, with the same IEEE operation sequence on every device:
 // 0 below -40; k = rint(x/ln2); r = x - k ln2 (two fma); degree-13 Taylor in
 // Horner form (fma); times 2^k.
+// A double constant materialized where it is used (two scalar moves) rather
+// than hoisted into registers held across the caller's loop.
+__device__ __forceinline__ double kconst(double c) {
+    asm volatile("" : "+s"(c));
+    return c;
+}
+
 __device__ __forceinline__ double xexp(double x) {
     if (!(x > -40.0)) return 0.0;
     if (x > 0.0) x = 0.0;
-    const double kd = __builtin_rint(x * 1.4426950408889634);
-    double r = __builtin_fma(-kd, 0x1.62e42fefa39efp-1, x);
-    r = __builtin_fma(-kd, 0x1.abc9e3b39803fp-56, r);
-    double p = 1.0 / 6227020800.0;
-    p = __builtin_fma(p, r, 1.0 / 479001600.0);
-    p = __builtin_fma(p, r, 1.0 / 39916800.0);
-    p = __builtin_fma(p, r, 1.0 / 3628800.0);
-    p = __builtin_fma(p, r, 1.0 / 362880.0);
-    p = __builtin_fma(p, r, 1.0 / 40320.0);
-    p = __builtin_fma(p, r, 1.0 / 5040.0);
-    p = __builtin_fma(p, r, 1.0 / 720.0);
-    p = __builtin_fma(p, r, 1.0 / 120.0);
-    p = __builtin_fma(p, r, 1.0 / 24.0);
-    p = __builtin_fma(p, r, 1.0 / 6.0);
+    const double kd = __builtin_rint(x * kconst(1.4426950408889634));
+    double r = __builtin_fma(-kd, kconst(0x1.62e42fefa39efp-1), x);
+    r = __builtin_fma(-kd, kconst(0x1.abc9e3b39803fp-56), r);
+    double p = kconst(1.0 / 6227020800.0);
+    p = __builtin_fma(p, r, kconst(1.0 / 479001600.0));
+    p = __builtin_fma(p, r, kconst(1.0 / 39916800.0));
+    p = __builtin_fma(p, r, kconst(1.0 / 3628800.0));
+    p = __builtin_fma(p, r, kconst(1.0 / 362880.0));
+    p = __builtin_fma(p, r, kconst(1.0 / 40320.0));
+    p = __builtin_fma(p, r, kconst(1.0 / 5040.0));
+    p = __builtin_fma(p, r, kconst(1.0 / 720.0));
+    p = __builtin_fma(p, r, kconst(1.0 / 120.0));
+    p = __builtin_fma(p, r, kconst(1.0 / 24.0));
+    p = __builtin_fma(p, r, kconst(1.0 / 6.0));
     p = __builtin_fma(p, r, 1.0 / 2.0);
     p = __builtin_fma(p, r, 1.0);
     p = __builtin_fma(p, r, 1.0);
@@ -835,9 +842,9 @@ struct Replica {
     }
 };
 
-// Three waves per SIMD: at most 168 VGPRs (512 per SIMD lane).
+// Four waves per SIMD: at most 128 VGPRs (512 per SIMD lane).
 template <int G, int C>
-__global__ void __launch_bounds__(256, 3)
+__global__ void __launch_bounds__(256, 4)
     k_anneal(const RepRef* __restrict__ table, int count, Inst* __restrict__ insts, int nmax,
              int rep_bytes, const volatile int* __restrict__ stop) {
     extern __shared__ __align__(16) unsigned char smem[];
