@@ -120,6 +120,12 @@ __device__ __forceinline__ T* opaque(T* p) {
     return p;
 }
 __device__ __forceinline__ void opaque_u32(uint32_t& v) { asm volatile("" : "+s"(v)); }
+// the same for a pointer that may differ between the groups of a wave
+template <class T>
+__device__ __forceinline__ T* opaque_v(T* p) {
+    asm volatile("" : "+v"(p));
+    return p;
+}
 
 // A wave-uniform value in scalar registers (whole-wave groups only).
 template <int G>
@@ -912,7 +918,9 @@ __global__ void __launch_bounds__(256, 4)
 
     uint64_t moves = (uint64_t)uni<G>((int64_t)S->moves);
     int64_t score = uni<G>(S->score), offnz = uni<G>(S->offnz);
-    int64_t local_best = uni<G>(S->local_best), msi = uni<G>(S->msi);
+    int64_t local_best = uni<G>(S->local_best);
+    // moves since improvement: reset once above the threshold (an int32)
+    uint32_t msi = (uint32_t)uni<G>((int32_t)S->msi);
     double T = tempering ? ((const GLOBAL double*)I->temps)[S->rung] : S->T;
     double cooling = S->cooling;
     int32_t cool_counter = uni<G>(S->cool_counter), win_p = uni<G>(S->win_p);
@@ -920,9 +928,9 @@ __global__ void __launch_bounds__(256, 4)
     int64_t best_score = uni<G>(S->best_score), best_move = uni<G>(S->best_move);
     int32_t flags = uni<G>(S->flags);
     const Coin greedy_coin(S->gf), target_coin(S->tf);
-    int64_t accepts = uni<G>(S->accepts), greedy_n = uni<G>(S->greedy);
-    int64_t infeasible = uni<G>(S->infeasible), sweeps = uni<G>(S->sweeps);
-    int64_t log_count = uni<G>(S->log_count), solve_move = uni<G>(S->solve_move);
+    // statistics counted from the epoch start (an epoch stays far below 2^32
+    // events), added to the replica record at its end
+    uint32_t accepts = 0, greedy_n = 0, infeasible = 0, sweeps = 0, logged = 0;
 
     // Lazy best flush: the pre-change state goes to bA/bXt only when it is the
     // replica's best and could still be the instance's best.
@@ -943,16 +951,16 @@ __global__ void __launch_bounds__(256, 4)
     auto log_move = [&](uint64_t m, int i, int j, int64_t s, uint16_t f) {
         if (logging) {
             const Inst* Iq = opaque(I);
-            if (log_count < Iq->log_cap && g == 0) {
-                GLOBAL LogEnt* e =
-                    (GLOBAL LogEnt*)Iq->log + (int64_t)r * Iq->log_cap + log_count;
+            const int64_t at = opaque_v(S)->log_count + logged;
+            if (at < Iq->log_cap && g == 0) {
+                GLOBAL LogEnt* e = (GLOBAL LogEnt*)Iq->log + (int64_t)r * Iq->log_cap + at;
                 e->move = (int64_t)m;
                 e->i = (int16_t)i;
                 e->j = (int16_t)j;
                 e->s = (int16_t)s;
                 e->flags = f;
             }
-            ++log_count;
+            ++logged;
         }
     };
     // Greedy reduction sweep from a pivot (congruence_anneal.hpp:246-260).
@@ -1149,13 +1157,12 @@ __global__ void __launch_bounds__(256, 4)
         }
         if (offnz == 0) {
             flags |= RF_SOLVED | RF_STOPPED;
-            solve_move = (int64_t)m;
             break;
         }
 
         // ---- stall handling (congruence_anneal.hpp:363-412) ----
         if (tempering) {
-            if (msi > stuck) {
+            if ((int64_t)msi > stuck) {
                 const InstParams& Q = opaque(I)->P;
                 if (Q.sweep_fraction > 0.0 && u01(w4z) < Q.sweep_fraction) {
                     sweep(sact[mulhi32(w4w, (uint32_t)N)], m);
@@ -1165,7 +1172,7 @@ __global__ void __launch_bounds__(256, 4)
                 msi = 0;
             }
         } else {
-            if (msi > stuck) {
+            if ((int64_t)msi > stuck) {
                 const Inst* Iq = opaque(I);
                 if (Iq->P.sweep_fraction > 0.0 && u01(w4z) < Iq->P.sweep_fraction) {
                     if (sweep(sact[mulhi32(w4w, (uint32_t)N)], m)) {
@@ -1209,7 +1216,6 @@ __global__ void __launch_bounds__(256, 4)
         }
         if (offnz == 0) {
             flags |= RF_SOLVED | RF_STOPPED;
-            solve_move = (int64_t)m;
             break;
         }
     }
@@ -1222,7 +1228,7 @@ __global__ void __launch_bounds__(256, 4)
         S->score = score;
         S->offnz = offnz;
         S->local_best = local_best;
-        S->msi = msi;
+        S->msi = (int64_t)msi;
         S->T = T;
         S->cooling = cooling;
         S->cool_counter = cool_counter;
@@ -1231,12 +1237,12 @@ __global__ void __launch_bounds__(256, 4)
         S->best_score = best_score;
         S->best_move = best_move;
         S->flags = flags;
-        S->accepts = accepts;
-        S->greedy = greedy_n;
-        S->infeasible = infeasible;
-        S->sweeps = sweeps;
-        S->log_count = log_count;
-        S->solve_move = solve_move;
+        S->accepts += accepts;
+        S->greedy += greedy_n;
+        S->infeasible += infeasible;
+        S->sweeps += sweeps;
+        S->log_count += logged;
+        if (flags & RF_SOLVED) S->solve_move = (int64_t)moves;
     }
     if ((flags & RF_SOLVED) && g == 0)
         atomicMin(&I->solve_key, ((unsigned long long)moves << kRepBits) | (unsigned)r);
