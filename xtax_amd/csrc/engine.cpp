@@ -509,10 +509,17 @@ struct Bucket {
 // ---------------------------------------------------------------- context
 struct Server;
 
+constexpr int kSideStreams = 3;
+
 struct xtx_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // buckets of one epoch run concurrently: forked from `stream` onto these
+    // (default priority) and joined back before the epoch end
+    hipStream_t side[kSideStreams] = {};
+    hipEvent_t fork_ev = nullptr;
+    hipEvent_t join_ev[kSideStreams] = {};
     int* stop_host = nullptr;  // pinned, mapped
     int* stop_dev = nullptr;
     std::string err;
@@ -579,16 +586,36 @@ int groups_per_block_for(const Layout& L, int waves) {
     return w * per_wave;
 }
 
-// Launches one epoch (all buckets, then k_epoch_end) on a stream.
-void launch_epoch(hipStream_t st, std::vector<Bucket>& buckets, Inst* d_insts, int n_insts,
-                  int max_r, const int* stop_dev) {
-    for (auto& b : buckets) {
-        if (b.refs.empty()) continue;
+// Launches one epoch on ctx->stream: the bucket kernels, independent of one
+// another, spread over the main and side streams (largest work first, so the
+// device fills while the short buckets finish), then k_epoch_end after all of
+// them. Captured, the fork and join become graph edges.
+void launch_epoch(xtx_ctx* ctx, std::vector<Bucket>& buckets, Inst* d_insts, int n_insts,
+                  int max_r) {
+    std::vector<const Bucket*> order;
+    for (auto& b : buckets)
+        if (!b.refs.empty()) order.push_back(&b);
+    std::stable_sort(order.begin(), order.end(), [](const Bucket* a, const Bucket* b) {
+        return (int64_t)a->refs.size() * a->lay.nmax > (int64_t)b->refs.size() * b->lay.nmax;
+    });
+    const int ns = std::min<int>(kSideStreams, std::max<int>(0, (int)order.size() - 1));
+    if (ns > 0) {
+        HIPCHK(hipEventRecord(ctx->fork_ev, ctx->stream));
+        for (int q = 0; q < ns; ++q) HIPCHK(hipStreamWaitEvent(ctx->side[q], ctx->fork_ev, 0));
+    }
+    for (size_t q = 0; q < order.size(); ++q) {
+        const Bucket& b = *order[q];
+        const int lane = (int)(q % (size_t)(ns + 1));
+        hipStream_t st = lane == 0 ? ctx->stream : ctx->side[lane - 1];
         HIPCHK(launch_anneal(st, b.lay.G, b.lay.C, b.groups_per_block, b.d_refs,
                              (int)b.refs.size(), d_insts, b.lay.nmax, b.lay.rep_bytes,
-                             stop_dev));
+                             ctx->stop_dev));
     }
-    HIPCHK(launch_epoch_end(st, d_insts, n_insts, max_r));
+    for (int q = 0; q < ns; ++q) {
+        HIPCHK(hipEventRecord(ctx->join_ev[q], ctx->side[q]));
+        HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->join_ev[q], 0));
+    }
+    HIPCHK(launch_epoch_end(ctx->stream, d_insts, n_insts, max_r));
 }
 
 }  // namespace
@@ -646,6 +673,11 @@ int xtx_open(int device, xtx_ctx** out) {
         HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         HIPCHK(hipEventCreate(&ctx->ev0));
         HIPCHK(hipEventCreate(&ctx->ev1));
+        HIPCHK(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
+        for (int q = 0; q < kSideStreams; ++q) {
+            HIPCHK(hipStreamCreateWithFlags(&ctx->side[q], hipStreamNonBlocking));
+            HIPCHK(hipEventCreateWithFlags(&ctx->join_ev[q], hipEventDisableTiming));
+        }
         HIPCHK(hipHostMalloc((void**)&ctx->stop_host, sizeof(int), hipHostMallocMapped));
         *ctx->stop_host = 0;
         HIPCHK(hipHostGetDevicePointer((void**)&ctx->stop_dev, ctx->stop_host, 0));
@@ -668,6 +700,12 @@ int xtx_close(xtx_ctx* ctx) {
     ctx->plain_arena.release();
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
+    for (int q = 0; q < kSideStreams; ++q) {
+        if (ctx->side[q]) (void)hipStreamSynchronize(ctx->side[q]);
+        if (ctx->side[q]) (void)hipStreamDestroy(ctx->side[q]);
+        if (ctx->join_ev[q]) (void)hipEventDestroy(ctx->join_ev[q]);
+    }
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     if (ctx->stop_host) (void)hipHostFree(ctx->stop_host);
     delete ctx;
@@ -1114,7 +1152,7 @@ struct Server {
                 hipGraph_t graph;
                 HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
                 for (int e = 0; e < k; ++e)
-                    launch_epoch(ctx->stream, bl, d_insts, ns, max_r, ctx->stop_dev);
+                    launch_epoch(ctx, bl, d_insts, ns, max_r);
                 HIPCHK(hipStreamEndCapture(ctx->stream, &graph));
                 HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
                 (void)hipGraphDestroy(graph);
@@ -1125,7 +1163,7 @@ struct Server {
             HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
         } else {
             HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-            launch_epoch(ctx->stream, bl, d_insts, ns, max_r, ctx->stop_dev);
+            launch_epoch(ctx, bl, d_insts, ns, max_r);
             HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
         }
         HIPCHK(hipMemcpyAsync(h_insts.data(), d_insts, (size_t)ns * sizeof(Inst),
