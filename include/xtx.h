@@ -69,8 +69,11 @@ typedef struct {
     uint64_t max_moves;       /* per-replica proposal cap; 0 = none */
     int32_t max_epochs;       /* 0 = none */
     int32_t verify;           /* 1: exact host verification of every result */
-    int32_t waves_per_block;  /* 0 = autotuned */
-    int32_t reserved0;
+    int32_t waves_per_block;  /* waves per workgroup for orders with no tuned
+                                 configuration; 0 = 1 */
+    int32_t autotune;         /* 1: xtx_batch_solve tunes every order of its
+                                 batch that has no configuration yet (on the
+                                 batch itself, before the run) */
 } xtx_params;
 
 typedef struct {
@@ -145,6 +148,9 @@ typedef struct {
     double kernel_ms;
     int32_t buckets;
     int32_t graph_epochs;     /* epochs per captured graph launch */
+    double upload_seconds;    /* admission: host inputs to device-resident
+                                 state; seconds - upload_seconds is the run
+                                 with inputs resident in HBM */
 } xtx_batch_stats;
 
 typedef struct {
@@ -174,6 +180,39 @@ void xtx_params_default(xtx_params* p);
  * (the reference's g_stop, stop_signal.hpp:40-64). */
 void xtx_request_stop(xtx_ctx* ctx);
 void xtx_clear_stop(xtx_ctx* ctx);
+
+/* ---- launch configuration per order bucket ----
+ * No counterpart in the reference (its CPU threads have no launch shape).
+ * Instances are bucketed by padded order (n rounded up to a multiple of 8);
+ * each bucket runs with one configuration: lanes per replica, columns per
+ * lane, LDS layout of A and waves per workgroup. The configuration changes
+ * speed only: every configuration computes the same moves and results. */
+typedef struct {
+    int32_t order;            /* padded order: a multiple of 8 in [8, 128] */
+    int32_t lanes;            /* 16, 32 or 64 lanes per replica */
+    int32_t cols;             /* 1 or 2 columns per lane; lanes * cols >= order */
+    int32_t triangular;       /* 1: A as its upper triangle in LDS, 0: square */
+    int32_t waves_per_block;  /* 1, 2 or 4; 0 = the params' waves_per_block */
+    int32_t tuned;            /* out: 1 if set (by hand or autotuner), 0 = rule */
+    int32_t lds_bytes;        /* out: LDS per replica */
+    int32_t reserved;
+    double moves_per_sec;     /* out of xtx_autotune: measured rate */
+} xtx_launch;
+
+/* Configuration used for instances of order n. */
+int xtx_launch_get(xtx_ctx* ctx, int32_t n, xtx_launch* out);
+/* Sets the configuration of cfg->order (lanes = 0 restores the rule). Affects
+ * instances admitted afterwards. XTX_E_ARG if no kernel has that shape or its
+ * LDS does not fit a workgroup. */
+int xtx_launch_set(xtx_ctx* ctx, const xtx_launch* cfg);
+/* Autotuner: for every order present in sample[0..count), times each valid
+ * configuration on those instances (repeated until they fill the device) for
+ * `moves` proposals per replica and keeps the fastest for the context.
+ * Writes up to max_results measurements (every candidate, the chosen ones
+ * with tuned = 1) to results; *n_results is the number written. */
+int xtx_autotune(xtx_ctx* ctx, const xtx_instance* sample, int32_t count,
+                 const xtx_params* params, int32_t moves, xtx_launch* results,
+                 int32_t max_results, int32_t* n_results);
 
 /* ---- plain form: one instance per launch ----
  * Replaces run_annealer(L1Objective(X0^T A0 X0, X0), params, active)

@@ -11,6 +11,9 @@
 3. Results: every reported A, X passes the exact verifier; the batched and
    served forms return the same result for an instance as the plain form
    (alone vs in the mix).
+4. Launch configurations: every (lanes, columns, LDS layout, waves per
+   workgroup) of an order returns the rule's result; the autotuner keeps its
+   fastest measurement.
 """
 import os
 import sys
@@ -255,3 +258,79 @@ def test_alone_vs_batch_vs_serve(eng):
             assert np.array_equal(A, alone[q][0]) and np.array_equal(X, alone[q][1]), q
     finally:
         eng.serve_close()
+
+
+# ------------------------------------------------------- launch configurations
+def _configs(order):
+    for G in (16, 32, 64):
+        C = -(-order // G)
+        if C > 2 or (G, C) == (16, 2):
+            continue
+        for tri in (1, 0):
+            for waves in (1, 2, 4):
+                yield G, C, tri, waves
+
+
+@pytest.mark.parametrize("n", [6, 12, 20, 33, 47, 70])
+def test_every_launch_configuration_gives_the_same_result(eng, n):
+    """Lanes per replica, columns per lane, LDS layout and waves per workgroup
+    change speed only: the batch result of every configuration of an order
+    equals the rule's, bit for bit (A, X, solve move, best score, and the
+    proposal count of an unsolved instance; after a solve the other replicas
+    stop at their next poll of the solve key, so that count is timing's)."""
+    order = -(-n // 8) * 8
+    mats = [workloads.planted(n, 20, 2 * n, seed=300 + q)[0] for q in range(3)]
+    mats.append(workloads.random_symmetric(n, -20, 20, seed=7))
+    ep = EngineParams(replicas=6, t_init=30, t_min=0.5, exchange_interval=300, max_epochs=6)
+    seeds = [77 + q for q in range(len(mats))]
+    eng.launch_reset(order)
+    ref, _ = eng.batch_solve(mats, ep, seeds=seeds)
+    tried = 0
+    try:
+        for G, C, tri, waves in _configs(order):
+            try:
+                eng.launch_set(order, G, C, tri, waves)
+            except RuntimeError:
+                continue  # no kernel with that shape, or its LDS does not fit
+            cfg = eng.launch_get(n)
+            assert (cfg["lanes"], cfg["cols"], cfg["triangular"], cfg["tuned"]) == (G, C, tri, 1)
+            say(f"n={n} G={G} C={C} tri={tri} waves={waves}")
+            got, _ = eng.batch_solve(mats, ep, seeds=seeds)
+            tried += 1
+            for q in range(len(mats)):
+                A, X, d = got[q]
+                assert d["verify_status"] == 0
+                assert np.array_equal(A, ref[q][0]) and np.array_equal(X, ref[q][1]), (q, G, C)
+                keys = ("solved", "solve_move", "best_score") + (() if d["solved"] else ("moves",))
+                for k in keys:
+                    assert d[k] == ref[q][2][k], (k, q, G, C, tri, waves)
+    finally:
+        eng.launch_reset(order)
+    assert tried >= 2
+
+
+def test_autotune_picks_a_measured_configuration(eng):
+    mats = [workloads.planted(n, 20, 2 * n, seed=400 + n)[0] for n in (7, 14, 22, 40)]
+    ep = EngineParams(replicas=4, t_init=30, t_min=0.5, exchange_interval=200, max_epochs=3)
+    seeds = [5 + q for q in range(len(mats))]
+    ref, _ = eng.batch_solve(mats, ep, seeds=seeds)
+    try:
+        res = eng.autotune(mats, ep, moves=64, seeds=seeds)
+        for n in (7, 14, 22, 40):
+            order = -(-n // 8) * 8
+            rows = [r for r in res if r["order"] == order]
+            assert rows and all(r["moves_per_sec"] > 0 for r in rows)
+            chosen = [r for r in rows if r["tuned"]]
+            assert len(chosen) == 1
+            assert chosen[0]["moves_per_sec"] == max(r["moves_per_sec"] for r in rows)
+            cfg = eng.launch_get(n)
+            assert cfg["tuned"] == 1
+            assert (cfg["lanes"], cfg["cols"], cfg["triangular"], cfg["waves_per_block"]) == (
+                chosen[0]["lanes"], chosen[0]["cols"], chosen[0]["triangular"],
+                chosen[0]["waves_per_block"])
+        got, _ = eng.batch_solve(mats, ep, seeds=seeds)
+        for q in range(len(mats)):
+            assert np.array_equal(got[q][0], ref[q][0]) and np.array_equal(got[q][1], ref[q][1])
+    finally:
+        for order in (8, 16, 24, 40):
+            eng.launch_reset(order)

@@ -31,7 +31,7 @@ class Params(C.Structure):
         ("seed", C.c_uint64),
         ("replicas", C.c_int32), ("move_log", C.c_int32), ("move_log_cap", C.c_int64),
         ("max_moves", C.c_uint64), ("max_epochs", C.c_int32), ("verify", C.c_int32),
-        ("waves_per_block", C.c_int32), ("reserved0", C.c_int32),
+        ("waves_per_block", C.c_int32), ("autotune", C.c_int32),
     ]
 
 
@@ -70,7 +70,7 @@ class BatchStats(C.Structure):
     _fields_ = [("moves", C.c_int64), ("accepts", C.c_int64), ("instances", C.c_int32),
                 ("solved", C.c_int32), ("verified", C.c_int32), ("epochs", C.c_int32),
                 ("seconds", C.c_double), ("kernel_ms", C.c_double), ("buckets", C.c_int32),
-                ("graph_epochs", C.c_int32)]
+                ("graph_epochs", C.c_int32), ("upload_seconds", C.c_double)]
 
 
 class ServeConfig(C.Structure):
@@ -81,6 +81,13 @@ class ServeConfig(C.Structure):
 class VerifyReport(C.Structure):
     _fields_ = [("status", C.c_int32), ("primes_used", C.c_int32), ("det_sign", C.c_int32),
                 ("exact_int128", C.c_int32)]
+
+
+class Launch(C.Structure):
+    _fields_ = [("order", C.c_int32), ("lanes", C.c_int32), ("cols", C.c_int32),
+                ("triangular", C.c_int32), ("waves_per_block", C.c_int32),
+                ("tuned", C.c_int32), ("lds_bytes", C.c_int32), ("reserved", C.c_int32),
+                ("moves_per_sec", C.c_double)]
 
 
 class LogEntry(C.Structure):
@@ -97,6 +104,10 @@ SIGNATURES = {
     "xtx_params_default": (None, [C.POINTER(Params)]),
     "xtx_request_stop": (None, [C.c_void_p]),
     "xtx_clear_stop": (None, [C.c_void_p]),
+    "xtx_launch_get": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Launch)]),
+    "xtx_launch_set": (C.c_int, [C.c_void_p, C.POINTER(Launch)]),
+    "xtx_autotune": (C.c_int, [C.c_void_p, C.POINTER(Instance), C.c_int32, C.POINTER(Params),
+                               C.c_int32, C.POINTER(Launch), C.c_int32, I32P]),
     "xtx_solve_plain": (C.c_int, [C.c_void_p, I64P, I64P, C.c_int32, C.POINTER(Params), I32P,
                                   C.c_int32, I64P, I64P, C.POINTER(Stats)]),
     "xtx_last_replicas": (C.c_int, [C.c_void_p, I64P, I64P, I64P]),

@@ -12,7 +12,7 @@
 // Metropolis test and the ladder, specified bit-exactly (IEEE ops + fma).
 //
 // Kernels:
-//   k_anneal<G,C>   one epoch (exchange_interval moves) of many replicas
+//   k_anneal<G,C,TRI> one epoch (exchange_interval moves) of many replicas
 //   k_epoch_end     per instance: solve retirement, replica exchange sweep,
 //                   instance-best bookkeeping
 //   k_probe<G,C>    the move primitives on given matrices (parity tests)
@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <vector>
 
 #include "engine_types.h"
 
@@ -707,30 +708,41 @@ __device__ __forceinline__ void evaluate_any(const int64_t (&ai)[C], const int64
 // the annealer
 // ---------------------------------------------------------------------------
 
-template <int G, int C>
+// A replica's A in LDS, in one of two layouts (a per-bucket choice that does
+// not change any result):
+//  * TRI: the upper triangle, row r holding columns r .. n-1, so that A[a][b]
+//    sits at D(min(a, b)) + max(a, b) -- half the footprint, twice the
+//    resident replicas where LDS bounds occupancy;
+//  * square: rows of odd stride P = nmax + 1 (column accesses hit distinct
+//    banks), both mirror entries written on commit -- fewer index operations
+//    where occupancy is bound by registers instead.
+template <int G, int C, bool TRI>
 struct Replica {
-    // LDS: the upper triangle of the symmetric A, row r holding columns
-    // r .. n-1, so that A[a][b] sits at D(min(a, b)) + max(a, b)
     int64_t* sT;
     GLOBAL int64_t* gX; // HBM: X^T of this replica, n x n row-major (touched on accept)
     const uint8_t* act;
-    int n, N, g;
-    int Dk[C];          // D(k) of the lane's columns
+    int n, N, g, P;
+    int Dk[C];          // TRI: D(k) of the lane's columns
     int64_t rm[C];      // row masses of the lane's rows
 
-    // D(r) = (start of row r) - r; r (2n - r - 1) is even
-    __device__ __forceinline__ int D(int r) const { return (r * (2 * n - r - 1)) >> 1; }
+    // start of row r, less r for TRI (r (2n - r - 1) is even)
+    __device__ __forceinline__ int D(int r) const {
+        if constexpr (TRI) return (r * (2 * n - r - 1)) >> 1;
+        return r * P;
+    }
     __device__ __forceinline__ int at(int a, int b) const {
-        return a <= b ? D(a) + b : D(b) + a;
+        if constexpr (TRI) return a <= b ? D(a) + b : D(b) + a;
+        return a * P + b;
     }
     __device__ __forceinline__ void init_lanes() {
 #pragma unroll
-        for (int c = 0; c < C; ++c) Dk[c] = D(g + c * G);
+        for (int c = 0; c < C; ++c) Dk[c] = TRI ? D(g + c * G) : 0;
     }
     // A[r][k] for the lane's column k = g + c G, given D(r)
     __device__ __forceinline__ int at_lane(int r, int Dr, int c) const {
         const int k = g + c * G;
-        return k >= r ? Dr + k : Dk[c] + r;
+        if constexpr (TRI) return k >= r ? Dr + k : Dk[c] + r;
+        return Dr + k;
     }
 
     __device__ __forceinline__ void load_rows(int i, int j, int64_t (&ai)[C],
@@ -790,14 +802,14 @@ struct Replica {
         }
     }
 
-    // upper triangle of a full row-major n x n matrix in HBM
+    // a full row-major n x n matrix in HBM (its upper triangle for TRI)
     __device__ __forceinline__ void load_A(const GLOBAL int64_t* src) {
         for (int row = 0; row < n; ++row) {
             const int Dr = D(row);
 #pragma unroll
             for (int c = 0; c < C; ++c) {
                 const int k = g + c * G;
-                if (k >= row && k < n) sT[Dr + k] = src[row * n + k];
+                if ((!TRI || k >= row) && k < n) sT[Dr + k] = src[row * n + k];
             }
         }
     }
@@ -830,8 +842,8 @@ struct Replica {
         return !Grp<G>::any(bad);
     }
 
-    // commit_add (main.cpp:112-133): row j of A (with its mirror column: one
-    // triangle entry per lane), row j of X^T, row masses (row j's new mass is
+    // commit_add (main.cpp:112-133): row j of A and its mirror column (one
+    // entry per lane in TRI), row j of X^T, row masses (row j's new mass is
     // its old mass plus the off-diagonal change).
     __device__ __forceinline__ void write(int j, const int64_t (&aj)[C], const MoveEval<C>& e,
                                           const int64_t (&nx)[C]) {
@@ -841,6 +853,7 @@ struct Replica {
             const int k = g + c * G;
             if (k < n) {
                 sT[at_lane(j, Dj, c)] = e.nv[c];
+                if (!TRI && k != j) sT[k * P + j] = e.nv[c];
                 rm[c] += (k != j) ? iabs64(e.nv[c]) - iabs64(aj[c]) : e.off;
                 gX[j * n + k] = nx[c];
             }
@@ -849,7 +862,7 @@ struct Replica {
 };
 
 // Four waves per SIMD: at most 128 VGPRs (512 per SIMD lane).
-template <int G, int C>
+template <int G, int C, bool TRI>
 __global__ void __launch_bounds__(256, 4)
     k_anneal(const RepRef* __restrict__ table, int count, Inst* __restrict__ insts, int nmax,
              int rep_bytes, const volatile int* __restrict__ stop) {
@@ -872,10 +885,13 @@ __global__ void __launch_bounds__(256, 4)
     GLOBAL RepState* S = (GLOBAL RepState*)I->rs + r;
     if (S->flags & RF_STOPPED) return;
 
-    Replica<G, C> rp;
+    Replica<G, C, TRI> rp;
     rp.sT = (int64_t*)(smem + (size_t)slot * rep_bytes);
-    // (nmax (nmax + 1) / 2) * 8 bytes: a multiple of 32 for nmax % 8 == 0
-    uint4* swords = (uint4*)(rp.sT + (size_t)nmax * (nmax + 1) / 2);
+    rp.P = nmax + 1;
+    // A: nmax (nmax + 1) / 2 or nmax (nmax + 1) int64, a multiple of 32 bytes
+    // for nmax % 8 == 0
+    uint4* swords = (uint4*)(rp.sT + (TRI ? (size_t)nmax * (nmax + 1) / 2
+                                          : (size_t)nmax * (nmax + 1)));
     uint8_t* sact = (uint8_t*)(swords + 5);
     uint32_t* shist = (uint32_t*)(sact + nmax);
     rp.act = sact;
@@ -1452,24 +1468,37 @@ __global__ void k_probe_rng(const uint32_t* __restrict__ ck, int count, uint32_t
 // ---------------------------------------------------------------------------
 
 
-template <int G, int C>
-static hipError_t launch_anneal_t(hipStream_t st, int groups_per_block, const RepRef* table,
-                                  int count, Inst* insts, int nmax, int rep_bytes,
-                                  const volatile int* stop) {
-    const int threads = groups_per_block * G;
-    const int blocks = (count + groups_per_block - 1) / groups_per_block;
-    const size_t lds = (size_t)groups_per_block * rep_bytes;
-    hipLaunchKernelGGL((k_anneal<G, C>), dim3(blocks), dim3(threads), lds, st, table, count, insts,
-                       nmax, rep_bytes, stop);
-    return hipGetLastError();
+typedef void (*AnnealKernel)(const RepRef*, int, Inst*, int, int, const volatile int*);
+
+// Every (G, C, layout) the launcher can run: G lanes per replica, C columns
+// per lane (G C >= n), triangular or square A.
+struct AnnealVariant {
+    int G, C;
+    bool tri;
+    AnnealKernel fn;
+};
+static const AnnealVariant kVariants[] = {
+    {16, 1, true, k_anneal<16, 1, true>},   {16, 1, false, k_anneal<16, 1, false>},
+    {32, 1, true, k_anneal<32, 1, true>},   {32, 1, false, k_anneal<32, 1, false>},
+    {32, 2, true, k_anneal<32, 2, true>},   {32, 2, false, k_anneal<32, 2, false>},
+    {64, 1, true, k_anneal<64, 1, true>},   {64, 1, false, k_anneal<64, 1, false>},
+    {64, 2, true, k_anneal<64, 2, true>},   {64, 2, false, k_anneal<64, 2, false>},
+};
+
+static const AnnealVariant* find_variant(int G, int C, bool tri) {
+    for (const auto& v : kVariants)
+        if (v.G == G && v.C == C && v.tri == tri) return &v;
+    return nullptr;
 }
+
+bool anneal_variant_exists(int G, int C, bool tri) { return find_variant(G, C, tri) != nullptr; }
 
 // Dynamic LDS limits are raised once, outside any stream capture, to the
 // per-block maximum of the device minus each kernel's static LDS.
 hipError_t init_kernel_attrs(int max_lds_per_block) {
-    const void* fns[] = {(const void*)k_anneal<16, 1>, (const void*)k_anneal<32, 1>,
-                         (const void*)k_anneal<64, 1>, (const void*)k_anneal<64, 2>,
-                         (const void*)k_epoch_end};
+    std::vector<const void*> fns;
+    for (const auto& v : kVariants) fns.push_back((const void*)v.fn);
+    fns.push_back((const void*)k_epoch_end);
     for (const void* f : fns) {
         hipFuncAttributes fa;
         hipError_t e = hipFuncGetAttributes(&fa, f);
@@ -1481,23 +1510,18 @@ hipError_t init_kernel_attrs(int max_lds_per_block) {
     return hipSuccess;
 }
 
-hipError_t launch_anneal(hipStream_t st, int G, int C, int groups_per_block, const RepRef* table,
-                         int count, Inst* insts, int nmax, int rep_bytes,
+hipError_t launch_anneal(hipStream_t st, int G, int C, bool tri, int groups_per_block,
+                         const RepRef* table, int count, Inst* insts, int nmax, int rep_bytes,
                          const volatile int* stop) {
     if (count <= 0) return hipSuccess;
-    if (G == 16 && C == 1)
-        return launch_anneal_t<16, 1>(st, groups_per_block, table, count, insts, nmax,
-                                      rep_bytes, stop);
-    if (G == 32 && C == 1)
-        return launch_anneal_t<32, 1>(st, groups_per_block, table, count, insts, nmax,
-                                      rep_bytes, stop);
-    if (G == 64 && C == 1)
-        return launch_anneal_t<64, 1>(st, groups_per_block, table, count, insts, nmax,
-                                      rep_bytes, stop);
-    if (G == 64 && C == 2)
-        return launch_anneal_t<64, 2>(st, groups_per_block, table, count, insts, nmax,
-                                      rep_bytes, stop);
-    return hipErrorInvalidValue;
+    const AnnealVariant* v = find_variant(G, C, tri);
+    if (!v || groups_per_block < 1 || groups_per_block * G > 256) return hipErrorInvalidValue;
+    const int threads = groups_per_block * G;
+    const int blocks = (count + groups_per_block - 1) / groups_per_block;
+    const size_t lds = (size_t)groups_per_block * rep_bytes;
+    hipLaunchKernelGGL(v->fn, dim3(blocks), dim3(threads), lds, st, table, count, insts, nmax,
+                       rep_bytes, stop);
+    return hipGetLastError();
 }
 
 hipError_t launch_epoch_end(hipStream_t st, Inst* insts, int count, int max_r) {

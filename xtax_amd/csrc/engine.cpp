@@ -25,15 +25,17 @@
 #include <string>
 #include <thread>
 #include <unordered_map>
+#include <tuple>
 #include <vector>
 
 #include "engine_types.h"
 #include "xtx.h"
 
 namespace xtx {
-hipError_t launch_anneal(hipStream_t st, int G, int C, int groups_per_block, const RepRef* table,
-                         int count, Inst* insts, int nmax, int rep_bytes,
+hipError_t launch_anneal(hipStream_t st, int G, int C, bool tri, int groups_per_block,
+                         const RepRef* table, int count, Inst* insts, int nmax, int rep_bytes,
                          const volatile int* stop);
+bool anneal_variant_exists(int G, int C, bool tri);
 hipError_t launch_epoch_end(hipStream_t st, Inst* insts, int count, int max_r);
 hipError_t launch_probe(hipStream_t st, int G, int C, const int64_t* A, const int64_t* Xt, int n,
                         const int32_t* q, int nq, int64_t* out);
@@ -64,36 +66,49 @@ struct Err {
 inline int64_t iabs(int64_t v) { return v < 0 ? -v : v; }
 
 // ---------------------------------------------------------------- layout
-// LDS bucket of an order n: lane-group width G, columns per lane C, padded
-// order nmax (multiple of 8) and the LDS bytes of one replica.
+// Launch configuration of an order bucket: padded order nmax (multiple of 8),
+// lane-group width G, columns per lane C (G C >= nmax), A as its upper
+// triangle or square in LDS, waves per workgroup (0: the params' choice) and
+// the LDS bytes of one replica. Every configuration computes the same
+// trajectory (the random stream and the arithmetic do not depend on it); it is
+// a performance choice, made by a rule or by xtx_autotune.
 struct Layout {
-    int G, C, nmax, rep_bytes;
-    bool operator<(const Layout& o) const { return nmax < o.nmax; }
+    int G = 0, C = 0, nmax = 0, rep_bytes = 0;
+    bool tri = true;
+    int waves = 0;
 };
 
-Layout layout_for(int n) {
-    Layout L{};
-    L.nmax = std::max(8, (n + 7) / 8 * 8);
-    if (L.nmax <= 16) {
-        L.G = 16;
-        L.C = 1;
-    } else if (L.nmax <= 32) {
-        L.G = 32;
-        L.C = 1;
-    } else if (L.nmax <= 64) {
-        L.G = 64;
-        L.C = 1;
-    } else {
-        L.G = 64;
-        L.C = 2;
-    }
-    // upper triangle of A, 5 staged Philox blocks, active list, median
-    // histogram; X^T stays in HBM
-    const int64_t bytes =
-        (int64_t)L.nmax * (L.nmax + 1) / 2 * 8 + 5 * 16 + L.nmax + 4 * med_slots(L.G);
-    L.rep_bytes = (int)((bytes + 15) / 16 * 16);
+int order_of(int n) { return std::max(8, (n + 7) / 8 * 8); }
+
+// A, 5 staged Philox blocks, active list, median histogram; X^T stays in HBM
+int rep_bytes_of(int nmax, int G, bool tri) {
+    const int64_t a = tri ? (int64_t)nmax * (nmax + 1) / 2 : (int64_t)nmax * (nmax + 1);
+    const int64_t bytes = a * 8 + 5 * 16 + nmax + 4 * med_slots(G);
+    return (int)((bytes + 15) / 16 * 16);
+}
+
+Layout make_layout(int nmax, int G, int C, bool tri, int waves) {
+    Layout L;
+    L.nmax = nmax;
+    L.G = G;
+    L.C = C;
+    L.tri = tri;
+    L.waves = waves;
+    L.rep_bytes = rep_bytes_of(nmax, G, tri);
     return L;
 }
+
+// the rule used when a bucket has not been tuned
+Layout layout_for(int n) {
+    const int nmax = order_of(n);
+    if (nmax <= 16) return make_layout(nmax, 16, 1, true, 0);
+    if (nmax <= 32) return make_layout(nmax, 32, 1, true, 0);
+    if (nmax <= 64) return make_layout(nmax, 64, 1, true, 0);
+    return make_layout(nmax, 64, 2, true, 0);
+}
+
+using BucketKey = std::tuple<int, int, int, bool, int>;
+BucketKey key_of(const Layout& L) { return {L.nmax, L.G, L.C, L.tri, L.waves}; }
 
 constexpr int kLdsPerCu = 160 * 1024;
 // Largest LDS one block may use on the open device (set by xtx_open).
@@ -533,6 +548,9 @@ struct xtx_ctx {
     std::vector<int64_t> plain_log_count;
     bool plain_log_valid = false;
     std::unique_ptr<Server> server;
+    // launch configuration per padded order, set by xtx_launch_set or
+    // xtx_autotune; other orders use layout_for's rule
+    std::map<int, Layout> tuned;
 };
 
 static std::string g_open_error;
@@ -542,6 +560,17 @@ namespace {
 int fail(xtx_ctx* ctx, const Err& e) {
     if (ctx) ctx->err = e.msg;
     return e.code;
+}
+
+Layout layout_in(const xtx_ctx* ctx, int n) {
+    const auto it = ctx->tuned.find(order_of(n));
+    return it != ctx->tuned.end() ? it->second : layout_for(n);
+}
+
+void apply_layout(const xtx_ctx* ctx, HostInst& h) {
+    h.lay = layout_in(ctx, h.n);
+    if (h.lay.rep_bytes > g_lds_block)
+        throw Err{XTX_E_ARG, "n too large for an LDS-resident replica"};
 }
 
 // Watchdog: raises the stop flag at a wall-clock deadline (the reference's
@@ -580,7 +609,7 @@ class Deadline {
 
 int groups_per_block_for(const Layout& L, int waves) {
     const int per_wave = 64 / L.G;
-    int w = waves > 0 ? waves : 1;
+    int w = L.waves > 0 ? L.waves : (waves > 0 ? waves : 1);
     // keep a block within the LDS of one CU
     while (w > 1 && (int64_t)w * per_wave * L.rep_bytes > g_lds_block) --w;
     return w * per_wave;
@@ -607,7 +636,7 @@ void launch_epoch(xtx_ctx* ctx, std::vector<Bucket>& buckets, Inst* d_insts, int
         const Bucket& b = *order[q];
         const int lane = (int)(q % (size_t)(ns + 1));
         hipStream_t st = lane == 0 ? ctx->stream : ctx->side[lane - 1];
-        HIPCHK(launch_anneal(st, b.lay.G, b.lay.C, b.groups_per_block, b.d_refs,
+        HIPCHK(launch_anneal(st, b.lay.G, b.lay.C, b.lay.tri, b.groups_per_block, b.d_refs,
                              (int)b.refs.size(), d_insts, b.lay.nmax, b.lay.rep_bytes,
                              ctx->stop_dev));
     }
@@ -755,6 +784,7 @@ int xtx_solve_plain(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t 
         HostInst& h = ctx->plain;
         h = HostInst{};
         prepare(h, A0, X0, n, p.replicas, p.seed, active, n_active, p);
+        apply_layout(ctx, h);
         xtx_stats st{};
         st.verify_status = -1;
         if (n < 2 || h.N < 2 || h.offnz0 == 0) {
@@ -810,7 +840,8 @@ int xtx_solve_plain(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t 
             for (;;) {
                 HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
                 for (auto& b : buckets)
-                    HIPCHK(launch_anneal(ctx->stream, b.lay.G, b.lay.C, b.groups_per_block,
+                    HIPCHK(launch_anneal(ctx->stream, b.lay.G, b.lay.C, b.lay.tri,
+                                         b.groups_per_block,
                                          b.d_refs, (int)b.refs.size(), ctx->d_plain,
                                          b.lay.nmax, b.lay.rep_bytes, ctx->stop_dev));
                 HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
@@ -1020,7 +1051,7 @@ struct Server {
     std::vector<Inst> h_insts;
     Inst* d_insts = nullptr;
     int cap_insts = 0;
-    std::map<int, Bucket> buckets;  // key: nmax
+    std::map<BucketKey, Bucket> buckets;
     bool dirty = true;
     int64_t lds_in_use = 0;
     int max_r = 1;
@@ -1028,6 +1059,7 @@ struct Server {
     double kernel_ms = 0;
     int epochs = 0;
     int64_t moves_retired = 0, accepts_retired = 0;
+    int retired_since_rebuild = 0;
     // graph of several epochs, rebuilt when residency changes
     hipGraphExec_t gexec = nullptr;
     int graph_epochs = 0;
@@ -1050,7 +1082,10 @@ struct Server {
         return (int64_t)ctx->prop.multiProcessorCount * kLdsPerCu * 4;
     }
 
-    bool try_admit(std::unique_ptr<HostInst>& h) {
+    // Admits one instance if the budgets allow. With `keep`, its staging
+    // buffers are parked there and the caller synchronizes the stream once
+    // after a run of admissions; without, this call synchronizes.
+    bool try_admit(std::unique_ptr<HostInst>& h, std::vector<Staging>* keep = nullptr) {
         if (cfg.max_resident > 0 && (int)(slots.size() - free_slots.size()) >= cfg.max_resident)
             return false;
         if (lds_in_use + lds_cost(*h) > lds_budget() && lds_in_use > 0) return false;
@@ -1061,7 +1096,10 @@ struct Server {
         set_fractions(*h, stg, p);
         HIPCHK(hipMemcpyAsync(h->rec.rs, stg.rs.data(), (size_t)h->R * sizeof(RepState),
                               hipMemcpyHostToDevice, ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
+        if (keep)
+            keep->push_back(std::move(stg));
+        else
+            HIPCHK(hipStreamSynchronize(ctx->stream));
         h->admit_s = now();
         lds_in_use += lds_cost(*h);
         int slot;
@@ -1106,7 +1144,7 @@ struct Server {
             HostInst& h = *slots[(size_t)s];
             h_insts[(size_t)s] = h.rec;
             max_r = std::max(max_r, h.R);
-            Bucket& b = buckets[h.lay.nmax];
+            Bucket& b = buckets[key_of(h.lay)];
             b.lay = h.lay;
             b.groups_per_block = groups_per_block_for(h.lay, p.waves_per_block);
             for (int r = 0; r < h.R; ++r) b.refs.push_back(RepRef{s, r});
@@ -1132,6 +1170,7 @@ struct Server {
             gexec = nullptr;
         }
         dirty = false;
+        retired_since_rebuild = 0;
     }
 
     std::vector<Bucket> bucket_list() {
@@ -1209,6 +1248,7 @@ struct Server {
             slots[s].reset();
             free_slots.push_back((int)s);
             dirty = true;
+            ++retired_since_rebuild;
         }
         // trim trailing free slots so the instance array stays compact
         while (!slots.empty() && !slots.back()) {
@@ -1224,13 +1264,15 @@ struct Server {
 
 namespace {
 
-std::unique_ptr<HostInst> make_inst(const xtx_instance& in, const xtx_params& p) {
+std::unique_ptr<HostInst> make_inst(const xtx_ctx* ctx, const xtx_instance& in,
+                                    const xtx_params& p) {
     auto h = std::make_unique<HostInst>();
     const int R = in.replicas > 0 ? in.replicas : p.replicas;
     xtx_params q = p;
     q.replicas = R;
     check_params(q);
     prepare(*h, in.A0, in.X0, in.n, R, in.seed, in.active, in.n_active, q);
+    apply_layout(ctx, *h);
     if (h->n < 2 || h->N < 2)
         throw Err{XTX_E_ARG, "instance needs n >= 2 and at least two active indices"};
     h->bytes = device_bytes(*h);
@@ -1241,6 +1283,212 @@ std::unique_ptr<HostInst> make_inst(const xtx_instance& in, const xtx_params& p)
 
 }  // namespace
 
+// ---------------------------------------------------------------- launch shapes
+namespace {
+
+void fill_launch(const Layout& L, bool tuned, xtx_launch* out) {
+    *out = xtx_launch{};
+    out->order = L.nmax;
+    out->lanes = L.G;
+    out->cols = L.C;
+    out->triangular = L.tri ? 1 : 0;
+    out->waves_per_block = L.waves;
+    out->tuned = tuned ? 1 : 0;
+    out->lds_bytes = L.rep_bytes;
+}
+
+// A configuration the kernels can run for this order, or an error message.
+const char* launch_invalid(int nmax, int G, int C, bool tri, int waves) {
+    if (nmax < 8 || nmax > kMaxN || nmax % 8) return "order must be a multiple of 8 in [8, 128]";
+    if (G * C < nmax) return "lanes * cols must cover the order";
+    if (!anneal_variant_exists(G, C, tri)) return "no kernel with these lanes/cols/layout";
+    if (waves < 0 || waves > 4 || waves == 3) return "waves_per_block must be 0, 1, 2 or 4";
+    const int rb = rep_bytes_of(nmax, G, tri);
+    if ((int64_t)(64 / G) * rb > g_lds_block) return "replica LDS exceeds a workgroup";
+    return nullptr;
+}
+
+// Every valid configuration of an order: G with C = ceil(nmax / G) <= 2,
+// both layouts, 1, 2 or 4 waves per workgroup (those the LDS allows).
+std::vector<Layout> candidates(int nmax) {
+    std::vector<Layout> v;
+    for (int G : {16, 32, 64}) {
+        const int C = (nmax + G - 1) / G;
+        for (bool tri : {true, false})
+            for (int w : {1, 2, 4}) {
+                if (launch_invalid(nmax, G, C, tri, w)) continue;
+                const Layout L = make_layout(nmax, G, C, tri, w);
+                if ((int64_t)w * (64 / G) * L.rep_bytes > g_lds_block) continue;
+                v.push_back(L);
+            }
+    }
+    return v;
+}
+
+// One timed launch of `moves` proposals per replica over copies of the given
+// instances in layout L; returns the kernel milliseconds.
+double time_layout(xtx_ctx* ctx, const std::vector<const HostInst*>& proto, const Layout& L,
+                   const xtx_params& p) {
+    std::vector<HostInst> hs;
+    hs.reserve(proto.size());
+    size_t total = 0;
+    for (const HostInst* h : proto) {
+        hs.push_back(*h);
+        hs.back().lay = L;
+        total += h->bytes;
+    }
+    Arena arena;
+    arena.reserve(total + 4096);
+    std::vector<Staging> keep(hs.size());
+    std::vector<Inst> recs(hs.size());
+    Bucket b;
+    b.lay = L;
+    b.groups_per_block = groups_per_block_for(L, p.waves_per_block);
+    for (size_t q = 0; q < hs.size(); ++q) {
+        HostInst& h = hs[q];
+        upload(h, arena.alloc(h.bytes), ctx->stream, keep[q]);
+        set_fractions(h, keep[q], p);
+        HIPCHK(hipMemcpyAsync(h.rec.rs, keep[q].rs.data(), (size_t)h.R * sizeof(RepState),
+                              hipMemcpyHostToDevice, ctx->stream));
+        recs[q] = h.rec;
+        for (int r = 0; r < h.R; ++r) b.refs.push_back(RepRef{(int)q, r});
+        pad_to_wave(b.refs, L.G, (int)q);
+    }
+    Inst* d_insts = nullptr;
+    HIPCHK(hipMalloc(&d_insts, recs.size() * sizeof(Inst)));
+    HIPCHK(hipMalloc(&b.d_refs, b.refs.size() * sizeof(RepRef)));
+    double ms_out = 0;
+    try {
+        HIPCHK(hipMemcpyAsync(d_insts, recs.data(), recs.size() * sizeof(Inst),
+                              hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(b.d_refs, b.refs.data(), b.refs.size() * sizeof(RepRef),
+                              hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+        HIPCHK(launch_anneal(ctx->stream, L.G, L.C, L.tri, b.groups_per_block, b.d_refs,
+                             (int)b.refs.size(), d_insts, L.nmax, L.rep_bytes, ctx->stop_dev));
+        HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        ms_out = ms;
+    } catch (...) {
+        (void)hipFree(d_insts);
+        (void)hipFree(b.d_refs);
+        throw;
+    }
+    (void)hipFree(d_insts);
+    (void)hipFree(b.d_refs);
+    return ms_out;
+}
+
+// Tunes every order of `sample` not yet in ctx->tuned (all of them when
+// `all`), appending measurements to `out`.
+void autotune_orders(xtx_ctx* ctx, const xtx_instance* sample, int count,
+                     const xtx_params& params, int moves, bool all,
+                     std::vector<xtx_launch>* out) {
+    xtx_params p = params;
+    p.exchange_interval = moves;
+    p.max_epochs = 1;
+    p.max_moves = 0;
+    p.move_log = 0;
+    p.verify = 0;
+    // Each sample instance is timed with 256 replicas (independent chains on
+    // the instance's ladder, the same move mix), repeated until the copies
+    // fill the device twice over in the narrowest shape (16 resident waves
+    // per CU, 4 replicas per wave) or take 8 GiB.
+    constexpr int kTuneReplicas = 256;
+    std::map<int, std::vector<std::unique_ptr<HostInst>>> by_order;
+    for (int q = 0; q < count; ++q) {
+        const int nmax = order_of(sample[q].n);
+        if (!all && ctx->tuned.count(nmax)) continue;
+        xtx_instance in = sample[q];
+        in.replicas = kTuneReplicas;
+        by_order[nmax].push_back(make_inst(ctx, in, p));
+    }
+    const int64_t target_replicas = (int64_t)ctx->prop.multiProcessorCount * 16 * 4 * 2;
+    for (auto& kv : by_order) {
+        const int nmax = kv.first;
+        std::vector<const HostInst*> proto;
+        int64_t reps = 0;
+        size_t bytes = 0;
+        for (size_t q = 0; reps < target_replicas && bytes < ((size_t)8 << 30);
+             q = (q + 1) % kv.second.size()) {
+            proto.push_back(kv.second[q].get());
+            reps += kv.second[q]->R;
+            bytes += kv.second[q]->bytes;
+        }
+        double best_rate = -1;
+        Layout best;
+        size_t first = out->size();
+        for (const Layout& L : candidates(nmax)) {
+            const double ms = time_layout(ctx, proto, L, p);
+            const double rate = ms > 0 ? (double)reps * moves / (ms * 1e-3) : 0.0;
+            xtx_launch r;
+            fill_launch(L, false, &r);
+            r.moves_per_sec = rate;
+            out->push_back(r);
+            if (rate > best_rate) {
+                best_rate = rate;
+                best = L;
+            }
+        }
+        if (best_rate > 0) {
+            ctx->tuned[nmax] = best;
+            for (size_t q = first; q < out->size(); ++q) {
+                const xtx_launch& r = (*out)[q];
+                if (r.lanes == best.G && r.cols == best.C && r.triangular == (best.tri ? 1 : 0) &&
+                    r.waves_per_block == best.waves)
+                    (*out)[q].tuned = 1;
+            }
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int xtx_launch_get(xtx_ctx* ctx, int32_t n, xtx_launch* out) {
+    if (!ctx || !out || n < 2 || n > kMaxN) return XTX_E_ARG;
+    const bool tuned = ctx->tuned.count(order_of(n)) != 0;
+    fill_launch(layout_in(ctx, n), tuned, out);
+    return XTX_OK;
+}
+
+int xtx_launch_set(xtx_ctx* ctx, const xtx_launch* cfg) {
+    if (!ctx || !cfg) return XTX_E_ARG;
+    if (cfg->lanes == 0) {
+        ctx->tuned.erase(cfg->order);
+        return XTX_OK;
+    }
+    if (const char* why = launch_invalid(cfg->order, cfg->lanes, cfg->cols, cfg->triangular != 0,
+                                         cfg->waves_per_block))
+        return fail(ctx, Err{XTX_E_ARG, why});
+    ctx->tuned[cfg->order] = make_layout(cfg->order, cfg->lanes, cfg->cols,
+                                         cfg->triangular != 0, cfg->waves_per_block);
+    return XTX_OK;
+}
+
+int xtx_autotune(xtx_ctx* ctx, const xtx_instance* sample, int32_t count,
+                 const xtx_params* params, int32_t moves, xtx_launch* results,
+                 int32_t max_results, int32_t* n_results) {
+    if (!ctx || !sample || count < 1 || !params || moves < 1) return XTX_E_ARG;
+    try {
+        HIPCHK(hipSetDevice(ctx->device));
+        check_params(*params);
+        std::vector<xtx_launch> out;
+        autotune_orders(ctx, sample, count, *params, moves, true, &out);
+        const int w = results ? std::min<int>(max_results, (int)out.size()) : 0;
+        for (int q = 0; q < w; ++q) results[q] = out[(size_t)q];
+        if (n_results) *n_results = w;
+        return XTX_OK;
+    } catch (const Err& e) {
+        return fail(ctx, e);
+    }
+}
+
+}  // extern "C"
+
 extern "C" {
 
 int xtx_batch_solve(xtx_ctx* ctx, const xtx_instance* instances, int32_t count,
@@ -1248,6 +1496,11 @@ int xtx_batch_solve(xtx_ctx* ctx, const xtx_instance* instances, int32_t count,
     if (!ctx || !instances || count < 1 || !params) return XTX_E_ARG;
     try {
         HIPCHK(hipSetDevice(ctx->device));
+        if (params->autotune) {
+            std::vector<xtx_launch> tuned;
+            autotune_orders(ctx, instances, count, *params,
+                            std::min(params->exchange_interval, 512), false, &tuned);
+        }
         Server S;
         S.ctx = ctx;
         S.p = *params;
@@ -1255,17 +1508,22 @@ int xtx_batch_solve(xtx_ctx* ctx, const xtx_instance* instances, int32_t count,
         std::vector<std::unique_ptr<HostInst>> hs;
         size_t total = 0;
         for (int q = 0; q < count; ++q) {
-            hs.push_back(make_inst(instances[q], *params));
+            hs.push_back(make_inst(ctx, instances[q], *params));
             hs.back()->tag = (uint64_t)q;
             total += hs.back()->bytes;
         }
         S.arena.reserve(total + 4096);
         S.t0 = clk::now();
+        std::vector<Staging> keep;
+        keep.reserve(hs.size());
         for (auto& h : hs) {
             h->submit_s = 0;
-            if (!S.try_admit(h)) throw Err{XTX_E_NOMEM, "batch does not fit the device"};
+            if (!S.try_admit(h, &keep)) throw Err{XTX_E_NOMEM, "batch does not fit the device"};
         }
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        keep.clear();
         S.rebuild();
+        const double upload_s = S.now();
         // epochs per graph launch: about 8 ms of work between polls
         const int k = 4;
         Deadline dl(ctx->stop_host, params->max_seconds);
@@ -1278,7 +1536,10 @@ int xtx_batch_solve(xtx_ctx* ctx, const xtx_instance* instances, int32_t count,
                 break;
             }
             if (params->max_epochs > 0 && S.epochs >= params->max_epochs) break;
-            S.rebuild();
+            // retired instances stay in the launch tables (their waves exit at
+            // once) until a quarter of the residents have gone: each rebuild
+            // costs a graph capture
+            if (4 * S.retired_since_rebuild > S.resident()) S.rebuild();
         }
         S.retire(true);
         if (reason && dl.fired()) *(volatile int*)ctx->stop_host = 0;
@@ -1300,6 +1561,7 @@ int xtx_batch_solve(xtx_ctx* ctx, const xtx_instance* instances, int32_t count,
             bs.kernel_ms = S.kernel_ms;
             bs.buckets = (int)S.buckets.size();
             bs.graph_epochs = k;
+            bs.upload_seconds = upload_s;
             *stats = bs;
         }
         return XTX_OK;
@@ -1339,7 +1601,7 @@ int xtx_serve_submit(xtx_ctx* ctx, const xtx_instance* inst, uint64_t tag) {
         Server& S = *ctx->server;
         const int qcap = S.cfg.queue_capacity > 0 ? S.cfg.queue_capacity : 65536;
         if ((int)S.queue.size() >= qcap) throw Err{XTX_E_FULL, "serve queue full"};
-        auto h = make_inst(*inst, S.p);
+        auto h = make_inst(ctx, *inst, S.p);
         if ((int64_t)h->bytes > (int64_t)S.arena.size())
             throw Err{XTX_E_NOMEM, "request larger than the HBM budget"};
         h->tag = tag;

@@ -46,6 +46,7 @@ class EngineParams:
     max_epochs: int = 0
     verify: bool = True
     waves_per_block: int = 0
+    autotune: bool = False
 
     def to_c(self):
         p = N.Params()
@@ -187,6 +188,47 @@ class Engine:
         self._check(self._lib.xtx_batch_solve(self._h, insts, count, C.byref(p), res,
                                               C.byref(bs)))
         return [(outs[q][0], outs[q][1], _stats(res[q])) for q in range(count)], _stats(bs)
+
+    # -- launch configuration per order bucket -----------------------------
+    def launch_get(self, n):
+        out = N.Launch()
+        self._check(self._lib.xtx_launch_get(self._h, int(n), C.byref(out)))
+        return _stats(out)
+
+    def launch_set(self, order, lanes, cols, triangular, waves_per_block=0):
+        cfg = N.Launch(order, lanes, cols, int(triangular), waves_per_block, 0, 0, 0, 0.0)
+        self._check(self._lib.xtx_launch_set(self._h, C.byref(cfg)))
+
+    def launch_reset(self, order):
+        cfg = N.Launch(order, 0, 0, 0, 0, 0, 0, 0, 0.0)
+        self._check(self._lib.xtx_launch_set(self._h, C.byref(cfg)))
+
+    def autotune(self, mats, params=None, moves=512, seeds=None):
+        """Times every launch configuration of each order in `mats` and keeps
+        the fastest; returns every measurement (chosen ones with tuned = 1)."""
+        params = params or EngineParams()
+        count = len(mats)
+        insts = (N.Instance * count)()
+        keep = []
+        for q, A in enumerate(mats):
+            A0 = _i64(A)
+            keep.append(A0)
+            insts[q].A0 = _p64(A0)
+            insts[q].X0 = None
+            insts[q].n = A0.shape[0]
+            insts[q].replicas = 0
+            insts[q].seed = int(seeds[q]) if seeds is not None else params.seed + q
+            insts[q].active = None
+            insts[q].n_active = 0
+            insts[q].A_out = None
+            insts[q].X_out = None
+        cap = 64 * 64
+        res = (N.Launch * cap)()
+        got = C.c_int32(0)
+        p = params.to_c()
+        self._check(self._lib.xtx_autotune(self._h, insts, count, C.byref(p), int(moves), res,
+                                           cap, C.byref(got)))
+        return [_stats(res[q]) for q in range(got.value)]
 
     # -- served form ------------------------------------------------------
     def serve_open(self, params=None, lds_budget=0, hbm_budget=0, max_resident=0,
