@@ -266,6 +266,28 @@ int xtx_serve_close(xtx_ctx* ctx);
 int xtx_verify(const int64_t* A0, const int64_t* X, const int64_t* A, int32_t n,
                xtx_verify_report* report);
 
+/* ---- host-side parts of xtax around the annealer (no device needed) ---- */
+#define XTX_REORDER_RCM 1       /* --rcm */
+#define XTX_REORDER_CENTROID 2  /* --centroid */
+/* Publish-time symmetric reordering, in place: A <- A[p][p], X <- X[:, p]
+ * (X may be NULL), Reverse Cuthill-McKee first, then the iterated row centre
+ * of mass, as flags ask. Score- and nonzero-preserving. perm (nullable)
+ * receives the composite permutation (new index a holds old index perm[a]).
+ * Replaces L1Objective::reorder_for_publish (main.cpp:407-411) with
+ * rcm_order (main.cpp:249-286), centroid_order (:292-324) and permute_state
+ * (:330-350). */
+int xtx_reorder(int64_t* A, int64_t* X, int32_t n, int32_t flags, int32_t* perm);
+/* det(A) = +-1 by the reference's test: det mod 1000000007 and mod 998244353
+ * both 1 or both -1 (is_unimodular, mat_io.hpp:635-644). */
+int xtx_is_unimodular(const int64_t* A, int32_t n, int32_t* result);
+/* Strict-deflation pivot clearing (deflate_index, main.cpp:435-455): with
+ * M_ii = +-1, shear every active j != i by s = -M_ji / M_ii so that row and
+ * column i of M become zero off the diagonal, updating M = X^T A0 X and
+ * Xt = X^T. If any entry would exceed 2^48, M and Xt are restored and
+ * *cleared = 0; otherwise *cleared = 1. */
+int xtx_deflate_pivot(int64_t* M, int64_t* Xt, int32_t n, int32_t i, const int32_t* active,
+                      int32_t n_active, int32_t* cleared);
+
 /* ---- device primitives, for parity tests of the kernels' building blocks ----
  * Each query q is (matrix index, i, j, s); the device evaluates it with the
  * same device functions the annealer uses. out per query:

@@ -98,4 +98,44 @@ void ref_xt_times(const int64_t* X, const int64_t* L, int m, int d, int64_t* out
     std::memcpy(out, R.data.data(), sizeof(int64_t) * R.data.size());
 }
 
+// ---- publish-time reordering and deflation (main.cpp:244-350, :435-455;
+// mat_io.hpp:597-644) ----
+
+void ref_rcm_order(const int64_t* A, int n, int* perm) {
+    const std::vector<int> p = rcm_order(to_matrix(A, n));
+    std::memcpy(perm, p.data(), sizeof(int) * p.size());
+}
+
+void ref_centroid_order(const int64_t* A, int n, int* perm) {
+    const std::vector<int> p = centroid_order(to_matrix(A, n));
+    std::memcpy(perm, p.data(), sizeof(int) * p.size());
+}
+
+// L1Objective::reorder_for_publish(true) on (A, Xt) with the given flags.
+void ref_reorder_publish(int64_t* A, int64_t* Xt, int n, int band, int centroid) {
+    Matrix MA = to_matrix(A, n), MX = to_matrix(Xt, n);
+    std::vector<int64_t> mass((size_t)n, 0);
+    compute_row_mass(MA, mass);
+    if (band) permute_state(MA, MX, rcm_order(MA), mass);
+    if (centroid) permute_state(MA, MX, centroid_order(MA), mass);
+    from_matrix(MA, A);
+    from_matrix(MX, Xt);
+}
+
+int ref_is_unimodular(const int64_t* A, int n) { return is_unimodular(to_matrix(A, n)) ? 1 : 0; }
+
+int64_t ref_det_mod_prime(const int64_t* A, int n, int64_t p) {
+    return det_mod_prime(to_matrix(A, n), p);
+}
+
+int ref_deflate_index(int64_t* M, int64_t* Xt, int n, int i, const int* active, int n_active) {
+    Matrix MM = to_matrix(M, n), MX = to_matrix(Xt, n);
+    std::vector<int> act(active, active + n_active);
+    std::vector<int64_t> row((size_t)n), xrow((size_t)n), mass((size_t)n, 0);
+    const bool ok = deflate_index(MM, MX, i, act, row, xrow, mass);
+    from_matrix(MM, M);
+    from_matrix(MX, Xt);
+    return ok ? 1 : 0;
+}
+
 }  // extern "C"

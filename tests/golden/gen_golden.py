@@ -17,6 +17,12 @@ input (10x10.csv, stored here as data) and on small planted forms, single
 threaded with fixed seeds. Runs that stop on a diagonal are deterministic and
 record moves and the final best_A/best_X; every run's (A0, X, A) triple is also
 kept as a verifier fixture.
+
+G3 (publish-time reordering and deflation): rcm_order, centroid_order and
+reorder_for_publish (src/main.cpp:244-350, :407-411), det_mod_prime and
+is_unimodular (src/mat_io.hpp:597-644), deflate_index (src/main.cpp:435-455),
+through the same harness, into g3_publish_deflate.npz. `gen_golden.py g3`
+regenerates only that file.
 """
 import ctypes as C
 import os
@@ -60,6 +66,19 @@ def load_harness():
     L.ref_gram_of.argtypes = [I64P, C.c_int, C.c_int, I64P]
     L.ref_xt_times.restype = None
     L.ref_xt_times.argtypes = [I64P, I64P, C.c_int, C.c_int, I64P]
+    I32P = C.POINTER(C.c_int)
+    L.ref_rcm_order.restype = None
+    L.ref_rcm_order.argtypes = [I64P, C.c_int, I32P]
+    L.ref_centroid_order.restype = None
+    L.ref_centroid_order.argtypes = [I64P, C.c_int, I32P]
+    L.ref_reorder_publish.restype = None
+    L.ref_reorder_publish.argtypes = [I64P, I64P, C.c_int, C.c_int, C.c_int]
+    L.ref_is_unimodular.restype = C.c_int
+    L.ref_is_unimodular.argtypes = [I64P, C.c_int]
+    L.ref_det_mod_prime.restype = C.c_int64
+    L.ref_det_mod_prime.argtypes = [I64P, C.c_int, C.c_int64]
+    L.ref_deflate_index.restype = C.c_int
+    L.ref_deflate_index.argtypes = [I64P, I64P, C.c_int, C.c_int, I32P, C.c_int]
     return L
 
 
@@ -300,10 +319,159 @@ def g2():
     np.savez_compressed(os.path.join(OUT, "g2_end_to_end.npz"), **out)
 
 
+def gen_pattern(rng, n, kind):
+    """Symmetric matrices for the reordering fixtures: sparsity patterns that
+    exercise RCM's root choice, degree ties and components, and magnitude
+    patterns that exercise the centroid keys (ties included)."""
+    if kind == "sparse":
+        A = rng.integers(-9, 10, (n, n)) * (rng.random((n, n)) < rng.uniform(0.05, 0.5))
+    elif kind == "blocks":  # disconnected components, isolated indices
+        A = np.zeros((n, n), np.int64)
+        cuts = sorted(set(int(c) for c in rng.integers(1, max(2, n), 3)))
+        lo = 0
+        for hi in cuts + [n]:
+            if hi - lo >= 2 and rng.random() < 0.8:
+                B = rng.integers(-4, 5, (hi - lo, hi - lo)) * (rng.random((hi - lo,) * 2) < 0.6)
+                A[lo:hi, lo:hi] = B
+            lo = hi
+        perm = rng.permutation(n)
+        A = A[perm][:, perm]
+    elif kind == "path":  # a permuted path / band: RCM recovers a narrow band
+        A = np.zeros((n, n), np.int64)
+        for i in range(n - 1):
+            A[i, i + 1] = int(rng.integers(1, 5))
+        A += np.diag(rng.integers(1, 5, n))
+        perm = rng.permutation(n)
+        A = A[perm][:, perm]
+    elif kind == "ties":  # equal magnitudes: equal degrees and equal centroid keys
+        A = rng.choice([-1, 0, 1], (n, n), p=[0.25, 0.5, 0.25])
+    elif kind == "dense":
+        A = rng.integers(-1000, 1001, (n, n))
+    elif kind == "star":
+        A = np.zeros((n, n), np.int64)
+        c = int(rng.integers(0, n))
+        A[c, :] = rng.integers(1, 7, n)
+    else:
+        raise ValueError(kind)
+    return sym(np.asarray(A, dtype=np.int64))
+
+
+def unimodular(rng, n, shears, sign=None):
+    """U^T D U with D = diag(+-1): det = +-1."""
+    d = rng.choice([-1, 1], n) if sign is None else np.full(n, sign)
+    U = np.eye(n, dtype=np.int64)
+    for _ in range(shears if n >= 2 else 0):
+        i, j = rng.choice(n, 2, replace=False)
+        U[:, j] += int(rng.choice([-1, 1])) * U[:, i]
+    return np.asarray((U.T * d) @ U, dtype=np.int64), U
+
+
+def g3(L):
+    """Publish-time reordering (rcm_order, centroid_order, reorder_for_publish),
+    is_unimodular / det_mod_prime and deflate_index, from the reference."""
+    rng = np.random.default_rng(20261203)
+    out = {}
+    t = 0
+    for kind in ["sparse", "blocks", "path", "ties", "dense", "star"]:
+        for n in [1, 2, 3, 5, 8, 13, 17, 24, 31, 40]:
+            A = gen_pattern(rng, n, kind)
+            Xt = rng.integers(-5, 6, (n, n)).astype(np.int64)
+            rc = np.zeros(n, np.int32)
+            ce = np.zeros(n, np.int32)
+            L.ref_rcm_order(P(A), n, rc.ctypes.data_as(C.POINTER(C.c_int)))
+            L.ref_centroid_order(P(A), n, ce.ctypes.data_as(C.POINTER(C.c_int)))
+            out[f"r{t}_A"] = A
+            out[f"r{t}_Xt"] = Xt
+            out[f"r{t}_rcm"] = rc
+            out[f"r{t}_centroid"] = ce
+            for flags in (1, 2, 3):
+                A2, X2 = A.copy(), Xt.copy()
+                L.ref_reorder_publish(P(A2), P(X2), n, flags & 1, (flags >> 1) & 1)
+                out[f"r{t}_A{flags}"] = A2
+                out[f"r{t}_Xt{flags}"] = X2
+            t += 1
+    out["reorder_cases"] = np.array(t)
+    # is_unimodular and det mod each of its primes
+    u = 0
+    for n in [1, 2, 3, 4, 6, 9, 14, 20, 33]:
+        for variant in ["uni", "uni_neg", "det2", "det3", "singular", "random", "bigent"]:
+            if variant == "uni":
+                A, _ = unimodular(rng, n, 3 * n)
+            elif variant == "uni_neg":
+                A, _ = unimodular(rng, n, 3 * n, sign=-1)
+            elif variant in ("det2", "det3"):
+                _, U = unimodular(rng, n, 2 * n)
+                k = int(rng.integers(0, n))
+                d = rng.choice([-1, 1], n)
+                d[k] *= int(variant[-1])
+                A = np.asarray((U.T * d) @ U, dtype=np.int64)
+            elif variant == "singular":
+                B = rng.integers(-4, 5, (n, max(1, n - 1)))
+                A = np.asarray(B @ B.T, dtype=np.int64)
+            elif variant == "random":
+                A = sym(rng.integers(-30, 31, (n, n)).astype(np.int64))
+            else:
+                A = sym(rng.integers(-(1 << 40), 1 << 40, (n, n)).astype(np.int64))
+            dets = [L.ref_det_mod_prime(P(A), n, p) for p in (1000000007, 998244353)]
+            out[f"u{u}_A"] = A
+            out[f"u{u}_uni"] = np.array(L.ref_is_unimodular(P(A), n))
+            out[f"u{u}_det"] = np.array(dets, np.int64)
+            u += 1
+    out["unimodular_cases"] = np.array(u)
+    # deflate_index: a +/-1 pivot cleared over an active set, or rolled back
+    d = 0
+    for n in [2, 3, 5, 8, 12, 20, 32]:
+        for variant in ["plain", "subset", "near_bound", "xt_bound", "neg", "m_bound"]:
+            M = sym(rng.integers(-50, 51, (n, n)).astype(np.int64))
+            i = int(rng.integers(0, n))
+            M[i, i] = -1 if variant == "neg" else int(rng.choice([-1, 1]))
+            Xt = np.eye(n, dtype=np.int64) if variant != "xt_bound" else \
+                rng.integers(-3, 4, (n, n)).astype(np.int64)
+            act = list(range(n))
+            if variant == "subset":
+                act = sorted(set(int(v) for v in rng.choice(n, max(2, n // 2), replace=False))
+                             | {i})
+            if variant == "near_bound" and n > 2:
+                k = (i + 1) % n
+                M[k, :] = M[:, k] = rng.integers(-(1 << 47), 1 << 47, n)
+                M[k, i] = M[i, k] = int(rng.integers(1 << 20, 1 << 22))
+                M[i, i] = 1
+            if variant == "xt_bound":
+                Xt[i, :] = (1 << 47) + int(rng.integers(0, 1 << 40))
+            if variant == "m_bound":
+                # earlier targets clear, a later one overflows: full rollback
+                k = n - 1 if i != n - 1 else 0
+                M[i, :] = M[:, i] = 1
+                M[i, i] = 1
+                for c in range(n):
+                    if c not in (i, k):
+                        M[k, c] = M[c, k] = LIM
+                M[k, i] = M[i, k] = -1
+            a = np.array(act, np.int32)
+            M2, X2 = M.copy(), Xt.copy()
+            ok = L.ref_deflate_index(P(M2), P(X2), n, i, a.ctypes.data_as(C.POINTER(C.c_int)),
+                                     len(act))
+            out[f"d{d}_M"] = M
+            out[f"d{d}_Xt"] = Xt
+            out[f"d{d}_i"] = np.array(i)
+            out[f"d{d}_active"] = a
+            out[f"d{d}_ok"] = np.array(ok)
+            out[f"d{d}_M2"] = M2
+            out[f"d{d}_Xt2"] = X2
+            d += 1
+    out["deflate_cases"] = np.array(d)
+    np.savez_compressed(os.path.join(OUT, "g3_publish_deflate.npz"), **out)
+
+
 if __name__ == "__main__":
     L = load_harness()
-    g1(L)
-    g2()
+    which = sys.argv[1:] or ["g1", "g2", "g3"]
+    if "g1" in which:
+        g1(L)
+    if "g2" in which:
+        g2()
+    if "g3" in which:
+        g3(L)
     for f in sorted(os.listdir(OUT)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(OUT, f)), file=sys.stderr)

@@ -174,3 +174,79 @@ def test_modulus(tmp_path, orc):
     X = read_csv(tmp_path / "best_X.csv")
     A = read_csv(tmp_path / "best_A.csv")
     assert orc.verify(red, X, A)[0] & ~2 == 0
+
+
+def test_deflate_needs_a_unimodular_matrix(tmp_path):
+    write_csv(tmp_path / "a.csv", [[2, 1], [1, 2]])   # det 3
+    r = run(["-A", "a.csv", "--deflate"], tmp_path)
+    assert r.returncode == 1
+    assert "[deflate] verifying the matrix is unimodular" in r.stdout
+    assert "--deflate requires a unimodular matrix" in r.stderr
+
+
+def _unimodular(n, shears, seed):
+    rng = np.random.default_rng(seed)
+    d = rng.choice([-1, 1], n)
+    U = np.eye(n, dtype=np.int64)
+    for _ in range(shears):
+        i, j = rng.choice(n, 2, replace=False)
+        U[:, j] += int(rng.choice([-1, 1])) * U[:, i]
+    return np.asarray((U.T * d) @ U, dtype=np.int64), d
+
+
+@pytest.mark.gpu
+def test_strict_deflation_diagonalizes_a_unimodular_form(tmp_path, orc):
+    A0, d = _unimodular(12, 30, seed=4)
+    write_csv(tmp_path / "a.csv", A0)
+    r = run(["-A", "a.csv", "--deflate", "-t", "16", "--seed", "3", "--deflate-slice", "0.3",
+             "--max-seconds", "60"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert "[deflate] start mode=strict" in r.stdout and "[deflate] done" in r.stdout
+    A = read_csv(tmp_path / "best_A.csv")
+    X = read_csv(tmp_path / "best_X.csv")
+    status, _, _ = orc.verify(A0, X, A)
+    assert status == 0   # X^T A0 X = A, A diagonal, |det X| = 1
+    assert sorted(np.diag(A).tolist()) == sorted(d.tolist())   # Sylvester inertia
+
+
+@pytest.mark.gpu
+def test_relaxed_deflation_keeps_the_congruence(tmp_path, orc):
+    from xtax_amd import workloads
+    A0 = workloads.planted(10, 6, 20, seed=8)[0]
+    write_csv(tmp_path / "a.csv", A0)
+    r = run(["-A", "a.csv", "--deflate-blocks", "-t", "16", "--seed", "2", "--deflate-slice",
+             "0.2", "--max-seconds", "3"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert "[deflate] start mode=blocks" in r.stdout and "[deflate] done active=" in r.stdout
+    A = read_csv(tmp_path / "best_A.csv")
+    X = read_csv(tmp_path / "best_X.csv")
+    assert orc.verify(A0, X, A)[0] & ~2 == 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("flag,bits", [("--rcm", 1), ("--centroid", 2)])
+def test_publish_reorder(tmp_path, flag, bits):
+    """With a fixed epoch budget the run is deterministic: the reordered run's
+    output is the plain run's best with the reference's permutation applied."""
+    import ctypes as C
+    from xtax_amd import _native as N
+    from xtax_amd import workloads
+    A0 = workloads.planted(14, 20, 30, seed=12)[0]
+    write_csv(tmp_path / "a.csv", A0)
+    args = ["-A", "a.csv", "-t", "8", "--seed", "5", "--max-epochs", "3",
+            "--exchange-interval", "500"]
+    plain = tmp_path / "plain"
+    plain.mkdir()
+    write_csv(plain / "a.csv", A0)
+    assert run(args, plain).returncode == 0
+    r = run(args + [flag], tmp_path)
+    assert r.returncode == 0, r.stderr
+    Ap, Xp = read_csv(plain / "best_A.csv"), read_csv(plain / "best_X.csv")
+    A, X = read_csv(tmp_path / "best_A.csv"), read_csv(tmp_path / "best_X.csv")
+    Ae, Xe = np.ascontiguousarray(Ap).copy(), np.ascontiguousarray(Xp).copy()
+    perm = np.zeros(14, np.int32)
+    assert N.lib().xtx_reorder(Ae.ctypes.data_as(N.I64P), Xe.ctypes.data_as(N.I64P), 14, bits,
+                               perm.ctypes.data_as(N.I32P)) == 0
+    assert np.array_equal(A, Ae) and np.array_equal(X, Xe)
+    assert np.array_equal(A, Ap[np.ix_(perm, perm)]) and np.array_equal(X, Xp[:, perm])
+    assert (X.T @ A0 @ X == A).all()

@@ -124,6 +124,9 @@ SIGNATURES = {
     "xtx_serve_retire": (C.c_int, [C.c_void_p, C.c_uint64]),
     "xtx_serve_close": (C.c_int, [C.c_void_p]),
     "xtx_verify": (C.c_int, [I64P, I64P, I64P, C.c_int32, C.POINTER(VerifyReport)]),
+    "xtx_reorder": (C.c_int, [I64P, I64P, C.c_int32, C.c_int32, I32P]),
+    "xtx_is_unimodular": (C.c_int, [I64P, C.c_int32, I32P]),
+    "xtx_deflate_pivot": (C.c_int, [I64P, I64P, C.c_int32, C.c_int32, I32P, C.c_int32, I32P]),
     "xtx_probe_moves": (C.c_int, [C.c_void_p, I64P, I64P, C.c_int32, C.c_int32, I32P,
                                   C.c_int32, I64P]),
     "xtx_probe_rng": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.c_int32,

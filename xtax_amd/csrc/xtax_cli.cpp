@@ -6,9 +6,14 @@
 // xtx_solve_plain. `-t/--threads` sets the replica count (one tempering rung
 // per replica, as one worker per thread in the reference); the GPU default is
 // 64. Flags that only tune CPU threads (--use-hyperthreads, --no-pin) are
-// accepted and have no effect.
+// accepted and have no effect. --deflate / --deflate-blocks run the
+// reference's deflation outer loop over the engine; --rcm / --centroid reorder
+// the published best. GPU-only additions: --device, --max-moves (proposals
+// per replica), --max-epochs (epochs of exchange-interval moves; a run with
+// it and no time limit is deterministic).
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <csignal>
 #include <cstdint>
 #include <cstdio>
@@ -199,6 +204,148 @@ int64_t score_of(const Mat& A) {
     return 2 * full - diag;
 }
 
+int64_t offnz_of(const Mat& A) {
+    int64_t c = 0;
+    for (int i = 0; i < A.rows; ++i)
+        for (int j = i + 1; j < A.rows; ++j) c += A.at(i, j) != 0;
+    return c;
+}
+
+Mat transposed(const Mat& M) {
+    Mat T;
+    T.rows = M.cols;
+    T.cols = M.rows;
+    T.v.assign(M.v.size(), 0);
+    for (int r = 0; r < M.rows; ++r)
+        for (int c = 0; c < M.cols; ++c) T.at(c, r) = M.at(r, c);
+    return T;
+}
+
+Mat square(int n) {
+    Mat M;
+    M.rows = M.cols = n;
+    M.v.assign((size_t)n * n, 0);
+    return M;
+}
+
+// X^T A X (entries within the 2^48 bound: the engine has checked the start
+// state), accumulated in 128 bits
+Mat congruence(const Mat& A, const Mat& X) {
+    const int n = A.rows;
+    std::vector<__int128> AX((size_t)n * n, 0);
+    for (int r = 0; r < n; ++r)
+        for (int k = 0; k < n; ++k) {
+            const int64_t a = A.at(r, k);
+            if (!a) continue;
+            for (int c = 0; c < n; ++c) AX[(size_t)r * n + c] += (__int128)a * X.at(k, c);
+        }
+    Mat C = square(n);
+    for (int r = 0; r < n; ++r)
+        for (int c = 0; c < n; ++c) {
+            __int128 v = 0;
+            for (int k = 0; k < n; ++k) v += (__int128)X.at(k, r) * AX[(size_t)k * n + c];
+            C.at(r, c) = (int64_t)v;
+        }
+    return C;
+}
+
+constexpr int64_t kDeflateOffdiagCap = 1 << 13;  // main.cpp:30
+
+// The deflation outer loop (solve_with_deflation, main.cpp:460-555): anneal
+// the active sub-problem for a slice (through the engine with the active
+// list, from the current transform), then lock pivots and drop them from the
+// active set -- strict: a +-1 pivot whose column is at most 2^13 in
+// magnitude is cleared exactly (xtx_deflate_pivot); relaxed: a nonzero pivot
+// whose column is already zero off the diagonal -- until the active set has
+// fewer than two indices, the matrix is diagonal, time runs out or the user
+// interrupts. Starts from the identity transform; writes the CSVs on each
+// report. Returns false (message printed) on an engine error.
+bool solve_with_deflation(xtx_ctx* ctx, const Mat& A, const xtx_params& params, bool relaxed,
+                          double slice, bool verbose, Mat& M, Mat& Xt) {
+    const int n = A.rows;
+    M = A;
+    Xt = square(n);
+    for (int i = 0; i < n; ++i) Xt.at(i, i) = 1;
+    std::vector<int32_t> active((size_t)n);
+    std::iota(active.begin(), active.end(), 0);
+    const auto t0 = std::chrono::steady_clock::now();
+    auto elapsed = [&]() {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    };
+    std::cout << "[deflate] start mode=" << (relaxed ? "blocks" : "strict")
+              << " score=" << score_of(M) << " n=" << n << " slice=" << slice
+              << "s threads=" << params.replicas << "\n";
+    xtx_params sp = params;
+    const double report_every = std::max(params.save_interval, slice);
+    double last_report = -1e18;
+    Mat X = square(n), bA = square(n), bX = square(n);
+    while (active.size() >= 2) {
+        if (g_interrupted) break;
+        const double el = elapsed();
+        if (params.max_seconds > 0.0) {
+            if (el >= params.max_seconds) break;
+            sp.max_seconds = std::min(slice, params.max_seconds - el);
+        } else {
+            sp.max_seconds = slice;
+        }
+        X = transposed(Xt);
+        xtx_stats st{};
+        if (xtx_solve_plain(ctx, A.v.data(), X.v.data(), n, &sp, active.data(),
+                            (int32_t)active.size(), bA.v.data(), bX.v.data(), &st) != XTX_OK) {
+            std::cerr << "Error: " << xtx_last_error(ctx) << "\n";
+            return false;
+        }
+        if (verbose)
+            std::cout << "[deflate] slice moves=" << st.moves << " best=" << st.best_score
+                      << " seconds=" << st.seconds << "\n";
+        M = bA;
+        Xt = transposed(bX);
+        int locked = 0;
+        const std::vector<int32_t> snapshot = active;
+        for (int i : snapshot) {
+            const int64_t a_ii = M.at(i, i);
+            if (relaxed) {
+                if (a_ii == 0) continue;
+                bool orthogonal = true;
+                for (int j : active)
+                    if (j != i && M.at(j, i) != 0) {
+                        orthogonal = false;
+                        break;
+                    }
+                if (!orthogonal) continue;
+            } else {
+                if (std::llabs(a_ii) != 1) continue;
+                int64_t max_off = 0;
+                for (int j : active)
+                    if (j != i) max_off = std::max<int64_t>(max_off, std::llabs(M.at(j, i)));
+                if (max_off > kDeflateOffdiagCap) continue;
+                int32_t cleared = 0;
+                if (xtx_deflate_pivot(M.v.data(), Xt.v.data(), n, i, active.data(),
+                                      (int32_t)active.size(), &cleared) != XTX_OK ||
+                    !cleared)
+                    continue;
+            }
+            active.erase(std::find(active.begin(), active.end(), i));
+            ++locked;
+        }
+        const bool diagonal = offnz_of(M) == 0;
+        const double now = elapsed();
+        if (locked > 0 || diagonal || now - last_report >= report_every) {
+            std::cout << "[deflate] t=" << now << "s score=" << score_of(M)
+                      << " active=" << active.size() << " offdiag=" << offnz_of(M);
+            if (locked > 0) std::cout << " (locked " << locked << ")";
+            std::cout << "\n";
+            write_csv(transposed(Xt), "best_X.csv");
+            write_csv(M, "best_A.csv");
+            last_report = now;
+        }
+        if (diagonal) break;
+    }
+    std::cout << "[deflate] done active=" << active.size() << " score=" << score_of(M)
+              << " seconds=" << elapsed() << "\n";
+    return true;
+}
+
 // ---- minimal flag parser for the reference's option surface ----
 struct Opt {
     std::vector<std::string> names;
@@ -261,6 +408,7 @@ int main(int argc, char** argv) {
         {{"--verbose"}, true, [&](const std::string&) { verbose = true; }},
         {{"--device"}, false, I(device)},
         {{"--max-moves"}, false, [&](const std::string& s) { p.max_moves = std::stoull(s); }},
+        {{"--max-epochs"}, false, I(p.max_epochs)},
     };
     try {
         for (int a = 1; a < argc; ++a) {
@@ -298,8 +446,6 @@ int main(int argc, char** argv) {
         return 109;  // CLI11's parse-error exit code
     }
     if (p.replicas < 1) p.replicas = 1;
-    (void)verbose;
-    (void)deflate_slice;
 
     const bool have_A = !a_csv.empty(), have_L = !l_csv.empty();
     if (have_A == have_L) {
@@ -322,11 +468,6 @@ int main(int argc, char** argv) {
     if (!lattice_dims.empty() && !have_L) {
         std::cerr << "Error: --lattice-dims requires -L (it selects lattice columns for the Gram "
                      "calculation)\n";
-        return 1;
-    }
-    if (deflate || deflate_blocks || rcm || centroid) {
-        std::cerr << "Error: --deflate, --deflate-blocks, --rcm and --centroid are not supported "
-                     "by this engine yet\n";
         return 1;
     }
 
@@ -375,33 +516,69 @@ int main(int argc, char** argv) {
         return 1;
     }
 
+    if (deflate) {
+        std::cout << (have_A ? "[deflate] verifying the matrix is unimodular (det = +/-1)...\n"
+                             : "[deflate] verifying the Gram matrix is unimodular (det = +/-1)...\n");
+        int32_t uni = 0;
+        if (xtx_is_unimodular(A.v.data(), A.rows, &uni) != XTX_OK || !uni) {
+            if (have_A)
+                std::cerr << "Error: --deflate requires a unimodular matrix (det = +/-1); this one "
+                             "is not.\n";
+            else
+                std::cerr << "Error: --deflate requires a unimodular matrix (det = +/-1); this Gram "
+                             "is not\n       (a lattice Gram is unimodular only when the lattice is "
+                             "unimodular).\n";
+            std::cerr << "       Use --deflate-blocks for a relaxed (orthogonal-summand) deflation,\n"
+                      << "       or drop --deflate to run the plain annealer.\n";
+            return 1;
+        }
+    }
+
     if (xtx_open(device, &g_ctx) != XTX_OK) {
         std::cerr << "Error: " << xtx_last_error(nullptr) << "\n";
         return 1;
     }
     const int n = A.rows;
-    std::cout << "[t=0s] start score=" << score_of(A) << " n=" << n << " threads=" << p.replicas
-              << ((p.tempering && p.replicas >= 2) ? " (parallel tempering)" : "") << "\n";
-    Mat bA, bX;
-    bA.rows = bA.cols = bX.rows = bX.cols = n;
-    bA.v.assign((size_t)n * n, 0);
-    bX.v.assign((size_t)n * n, 0);
-    xtx_stats st{};
+    Mat bA = square(n), bX = square(n);
     p.verify = 1;
-    const int rc = xtx_solve_plain(g_ctx, A.v.data(), (have_A && !x_csv.empty()) ? X.v.data() : nullptr,
-                                   n, &p, rows.empty() ? nullptr : rows.data(), (int32_t)rows.size(),
-                                   bA.v.data(), bX.v.data(), &st);
-    if (rc != XTX_OK) {
-        std::cerr << "Error: " << xtx_last_error(g_ctx) << "\n";
-        xtx_close(g_ctx);
-        return 1;
+    int64_t final_score = 0;
+    if (deflate || deflate_blocks) {
+        Mat Xt;
+        if (!solve_with_deflation(g_ctx, A, p, deflate_blocks, deflate_slice, verbose, bA, Xt)) {
+            xtx_close(g_ctx);
+            return 1;
+        }
+        bX = transposed(Xt);
+        final_score = score_of(bA);
+    } else {
+        std::cout << "[t=0s] start score=" << score_of(A) << " n=" << n << " threads="
+                  << p.replicas << ((p.tempering && p.replicas >= 2) ? " (parallel tempering)" : "")
+                  << "\n";
+        xtx_stats st{};
+        const int rc = xtx_solve_plain(
+            g_ctx, A.v.data(), (have_A && !x_csv.empty()) ? X.v.data() : nullptr, n, &p,
+            rows.empty() ? nullptr : rows.data(), (int32_t)rows.size(), bA.v.data(), bX.v.data(),
+            &st);
+        if (rc != XTX_OK) {
+            std::cerr << "Error: " << xtx_last_error(g_ctx) << "\n";
+            xtx_close(g_ctx);
+            return 1;
+        }
+        if (st.solved)
+            std::cout << "[t=" << st.seconds << "s] matrix is diagonal, stopping (thread "
+                      << st.solve_replica << ")\n";
+        std::cout << "done moves=" << st.moves << " seconds=" << st.seconds << " moves_per_sec="
+                  << (st.seconds > 0 ? (double)st.moves / st.seconds : 0.0) << "\n";
+        // --rcm / --centroid reorder every published best (main.cpp:407-411,
+        // congruence_anneal.hpp:319); the start state is never published, so
+        // a run that found no better state returns it unordered
+        const int flags = (rcm ? XTX_REORDER_RCM : 0) | (centroid ? XTX_REORDER_CENTROID : 0);
+        const int64_t start_score = st.best_score;
+        if (flags && start_score < score_of(have_A && !x_csv.empty() ? congruence(A, X) : A))
+            xtx_reorder(bA.v.data(), bX.v.data(), n, flags, nullptr);
+        final_score = st.best_score;
     }
-    if (st.solved)
-        std::cout << "[t=" << st.seconds << "s] matrix is diagonal, stopping (thread "
-                  << st.solve_replica << ")\n";
-    std::cout << "done moves=" << st.moves << " seconds=" << st.seconds
-              << " moves_per_sec=" << (st.seconds > 0 ? (double)st.moves / st.seconds : 0.0) << "\n";
-    std::cout << "Final best score: " << st.best_score << "\n";
+    std::cout << "Final best score: " << final_score << "\n";
     try {
         if (have_A) {
             if (n <= 20) {
