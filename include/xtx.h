@@ -290,8 +290,10 @@ int xtx_deflate_pivot(int64_t* M, int64_t* Xt, int32_t n, int32_t i, const int32
 
 /* ---- device primitives, for parity tests of the kernels' building blocks ----
  * Each query q is (matrix index, i, j, s); the device evaluates it with the
- * same device functions the annealer uses. out per query:
- *   {feasible, delta, d_nonzero, best_shear(i,j), xt_feasible} */
+ * same device functions the annealer uses. out per query (6 values):
+ *   {feasible, delta, d_nonzero, best_shear(i,j), xt_feasible,
+ *    delta of Add(i, j, best_shear) as the annealer evaluates it after a
+ *    greedy call (0 when best_shear is 0)} */
 int xtx_probe_moves(xtx_ctx* ctx, const int64_t* A, const int64_t* Xt, int32_t n,
                     int32_t n_mats, const int32_t* queries, int32_t n_queries,
                     int64_t* out);

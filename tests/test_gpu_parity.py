@@ -64,8 +64,13 @@ def _check_probe(eng, orc, mats, xts, queries):
         assert got[0] == int(ok), (mi, i, j, s)
         if ok:
             assert got[1] == d and got[2] == dnz, (mi, i, j, s, got, d, dnz)
-        assert got[3] == orc.best_shear_l1(A, i, j), (mi, i, j, got[3])
+        b = orc.best_shear_l1(A, i, j)
+        assert got[3] == b, (mi, i, j, got[3])
         assert got[4] == int(okt)
+        if b != 0:  # the annealer's evaluation of the greedy shear (known off sum)
+            okb, db, _, _, _ = orc.try_add(A, Xt, i, j, b)
+            if okb:
+                assert got[5] == db, (mi, i, j, b, got[5], db)
 
 
 def test_move_primitives_on_golden_matrices(eng, orc):

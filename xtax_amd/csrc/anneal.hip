@@ -350,7 +350,8 @@ struct MoveEval {
 template <int G, int C>
 __device__ __forceinline__ void evaluate(const int64_t (&ai)[C], const int64_t (&aj)[C], int g,
                                          int n, int j, int64_t s, int64_t a_ii, int64_t a_ji,
-                                         int64_t a_jj, MoveEval<C>& e) {
+                                         int64_t a_jj, MoveEval<C>& e, bool know = false,
+                                         int64_t known_off = 0) {
     int64_t off = 0;
     int32_t dnz = 0;
     bool bad = false;
@@ -368,7 +369,7 @@ __device__ __forceinline__ void evaluate(const int64_t (&ai)[C], const int64_t (
             e.nv[c] = nv;
         }
     }
-    off = Grp<G>::sum(off);
+    off = know ? known_off : Grp<G>::sum(off);
     dnz = Grp<G>::sum32(dnz);
     bad = Grp<G>::any(bad) || over(new_jj);
     e.off = off;
@@ -432,7 +433,7 @@ __device__ __forceinline__ void breakpoint64(int64_t ai, int64_t aj, int32_t& ce
 // reference; returns 0 when no candidate lowers the score.
 template <int G, int C>
 __device__ int64_t best_shear(const int64_t (&ai)[C], const int64_t (&aj)[C], int g, int n, int j,
-                              int64_t a_ii, int64_t a_ji, int64_t a_jj) {
+                              int64_t a_ii, int64_t a_ji, int64_t a_jj, int64_t* off_out) {
     int64_t w[C];
     int32_t ce[C], fl[C];
     int64_t wl = 0;
@@ -479,7 +480,7 @@ __device__ int64_t best_shear(const int64_t (&ai)[C], const int64_t (&aj)[C], in
     Cands cd;
     make_cands(cf, cc, has_q, q, cd);
     // Score every candidate (score_delta_l1, main.cpp:150-168).
-    int64_t best_s = 0, best_d = 0;
+    int64_t best_s = 0, best_d = 0, best_off = 0;
 #pragma unroll
     for (int t = 0; t < 5; ++t) {
         const int64_t s = cd.v[t];
@@ -503,9 +504,11 @@ __device__ int64_t best_shear(const int64_t (&ai)[C], const int64_t (&aj)[C], in
             if (d < best_d) {
                 best_d = d;
                 best_s = s;
+                best_off = off;
             }
         }
     }
+    *off_out = best_off;
     return best_s;
 }
 
@@ -536,10 +539,13 @@ __device__ __forceinline__ int32_t rounded_div32(int32_t a, int32_t b) {
     return q;
 }
 
+// With `know`, the off-diagonal sum of s is already known (from the best-shear
+// candidates) and is not summed again.
 template <int G, int C>
 __device__ __forceinline__ void evaluate_small(const int32_t (&ai)[C], const int32_t (&aj)[C],
                                                int g, int n, int j, int64_t s, int64_t a_ii,
-                                               int64_t a_ji, int64_t a_jj, MoveEval<C>& e) {
+                                               int64_t a_ji, int64_t a_jj, MoveEval<C>& e,
+                                               bool know, int64_t known_off) {
     int32_t off = 0, dnz = 0;
     const int32_t s32 = (int32_t)s;
     // |a_ii|, |a_ji|, |a_jj| < 2^16 here, so |new_jj| < 2^29: int32, in bound
@@ -558,7 +564,7 @@ __device__ __forceinline__ void evaluate_small(const int32_t (&ai)[C], const int
             e.nv[c] = nv;
         }
     }
-    off = Grp<G>::sum32(off);
+    off = know ? (int32_t)known_off : Grp<G>::sum32(off);
     dnz = Grp<G>::sum32(dnz);
     e.off = off;
     e.dnz = dnz;
@@ -576,10 +582,11 @@ __device__ __forceinline__ void evaluate_small(const int32_t (&ai)[C], const int
 // its consecutive bins, and min{t : 2 W(ce <= t) >= W} - (-64) is the number
 // of bins whose prefix sum stays below W/2 (129 when none reaches it: the
 // sentinel t = 65). Weights are below 2^16 and W below 2^23: all uint32.
+// *off_out receives the off-diagonal change of the returned shear.
 template <int G, int C>
 __device__ int64_t best_shear_small(const int32_t (&ai)[C], const int32_t (&aj)[C], int g, int n,
                                     int j, int64_t a_ii, int64_t a_ji, int64_t a_jj,
-                                    uint32_t* hist) {
+                                    uint32_t* hist, int32_t* off_out) {
     int32_t w[C], ce[C], fl[C];
     int32_t wl = 0;
 #pragma unroll
@@ -645,6 +652,7 @@ __device__ int64_t best_shear_small(const int32_t (&ai)[C], const int32_t (&aj)[
     Cands cd;
     make_cands(cf, cc, has_q, q, cd);
     int64_t best_s = 0, best_d = 0;
+    int32_t best_off = 0;
 #pragma unroll
     for (int t = 0; t < 5; ++t) {
         const int64_t s = cd.v[t];
@@ -663,17 +671,22 @@ __device__ int64_t best_shear_small(const int32_t (&ai)[C], const int32_t (&aj)[
             if (d < best_d) {
                 best_d = d;
                 best_s = s;
+                best_off = off;
             }
         }
     }
+    *off_out = best_off;
     return best_s;
 }
 
 // Both paths behind one call: the int32 path when rows i and j are small.
 template <int G, int C>
+// *off_out: the off-diagonal change sum_{k != j} |new| - |old| of the returned
+// shear (when it is not 0), so its evaluation needs no second sum.
 __device__ __forceinline__ int64_t best_shear_any(const int64_t (&ai)[C], const int64_t (&aj)[C],
                                                   bool small, int g, int n, int j, int64_t a_ii,
-                                                  int64_t a_ji, int64_t a_jj, uint32_t* hist) {
+                                                  int64_t a_ji, int64_t a_jj,
+                                                  uint32_t* hist, int64_t* off_out) {
     if (small) {
         int32_t si[C], sj[C];
 #pragma unroll
@@ -681,16 +694,20 @@ __device__ __forceinline__ int64_t best_shear_any(const int64_t (&ai)[C], const 
             si[c] = (int32_t)ai[c];
             sj[c] = (int32_t)aj[c];
         }
-        return best_shear_small<G, C>(si, sj, g, n, j, a_ii, a_ji, a_jj, hist);
+        int32_t off = 0;
+        const int64_t s = best_shear_small<G, C>(si, sj, g, n, j, a_ii, a_ji, a_jj, hist, &off);
+        *off_out = off;
+        return s;
     }
-    return best_shear<G, C>(ai, aj, g, n, j, a_ii, a_ji, a_jj);
+    return best_shear<G, C>(ai, aj, g, n, j, a_ii, a_ji, a_jj, off_out);
 }
 
 template <int G, int C>
 __device__ __forceinline__ void evaluate_any(const int64_t (&ai)[C], const int64_t (&aj)[C],
                                              bool small, int g, int n, int j, int64_t s,
                                              int64_t a_ii, int64_t a_ji, int64_t a_jj,
-                                             MoveEval<C>& e) {
+                                             MoveEval<C>& e, bool know = false,
+                                             int64_t known_off = 0) {
     if (small) {
         int32_t si[C], sj[C];
 #pragma unroll
@@ -698,9 +715,9 @@ __device__ __forceinline__ void evaluate_any(const int64_t (&ai)[C], const int64
             si[c] = (int32_t)ai[c];
             sj[c] = (int32_t)aj[c];
         }
-        evaluate_small<G, C>(si, sj, g, n, j, s, a_ii, a_ji, a_jj, e);
+        evaluate_small<G, C>(si, sj, g, n, j, s, a_ii, a_ji, a_jj, e, know, known_off);
     } else {
-        evaluate<G, C>(ai, aj, g, n, j, s, a_ii, a_ji, a_jj, e);
+        evaluate<G, C>(ai, aj, g, n, j, s, a_ii, a_ji, a_jj, e, know, known_off);
     }
 }
 
@@ -992,12 +1009,13 @@ __global__ void __launch_bounds__(256, 4)
             const int64_t a_ji = rp.entry(jj, pv);
             const int64_t a_jj = rp.entry(jj, jj);
             const bool small = rows_small<G, C>(ai, aj);
+            int64_t soff = 0;
             const int64_t s =
-                best_shear_any<G, C>(ai, aj, small, g, n, jj, a_ii, a_ji, a_jj, shist);
+                best_shear_any<G, C>(ai, aj, small, g, n, jj, a_ii, a_ji, a_jj, shist, &soff);
             ++greedy_n;
             if (s == 0) continue;
             MoveEval<C> e;
-            evaluate_any<G, C>(ai, aj, small, g, n, jj, s, a_ii, a_ji, a_jj, e);
+            evaluate_any<G, C>(ai, aj, small, g, n, jj, s, a_ii, a_ji, a_jj, e, true, soff);
             if (e.bad || e.delta > 0) continue;
             int64_t xi[C], xj[C], nx[C];
             rp.prefetch_xt(pv, jj, xi, xj);
@@ -1130,16 +1148,18 @@ __global__ void __launch_bounds__(256, 4)
         // ---- shear (congruence_anneal.hpp:290-296) ----
         const bool small = rows_small<G, C>(ai, aj);
         const int64_t pm = (w1z & 1u) ? 1 : -1;
-        int64_t s = pm;
+        int64_t s = pm, soff = 0;
+        bool know = false;
         if (greedy_coin(w0w)) {
-            s = best_shear_any<G, C>(ai, aj, small, g, n, j, a_ii, a_ji, a_jj, shist);
+            s = best_shear_any<G, C>(ai, aj, small, g, n, j, a_ii, a_ji, a_jj, shist, &soff);
             ++greedy_n;
+            know = s != 0;
             if (s == 0) s = pm;
         }
 
         // ---- attempt_add (congruence_anneal.hpp:232-240) ----
         MoveEval<C> e;
-        evaluate_any<G, C>(ai, aj, small, g, n, j, s, a_ii, a_ji, a_jj, e);
+        evaluate_any<G, C>(ai, aj, small, g, n, j, s, a_ii, a_ji, a_jj, e, know, soff);
         int res = -1;
         if (e.bad) {
             ++infeasible;
@@ -1405,7 +1425,8 @@ __global__ void __launch_bounds__(256) k_epoch_end(Inst* __restrict__ insts) {
 // ---------------------------------------------------------------------------
 
 // One query per group: q = {mat, i, j, s}; out = {feasible, delta, dnz,
-// best_shear(i, j), xt_feasible}.
+// best_shear(i, j), xt_feasible, delta of the best shear evaluated with the
+// off-diagonal sum its search returned (0 when it is 0)}.
 template <int G, int C>
 __global__ void __launch_bounds__(64)
     k_probe(const int64_t* __restrict__ A, const int64_t* __restrict__ Xt, int n,
@@ -1431,8 +1452,12 @@ __global__ void __launch_bounds__(64)
     const bool small = rows_small<G, C>(ai, aj);
     MoveEval<C> e;
     evaluate_any<G, C>(ai, aj, small, g, n, j, s, a_ii, a_ji, a_jj, e);
+    int64_t boff = 0;
     const int64_t b = best_shear_any<G, C>(ai, aj, small, g, n, j, a_ii, a_ji, a_jj,
-                                           hist + ((int)threadIdx.x / G) * med_slots(G));
+                                           hist + ((int)threadIdx.x / G) * med_slots(G), &boff);
+    MoveEval<C> eb;
+    eb.delta = 0;
+    if (b != 0) evaluate_any<G, C>(ai, aj, small, g, n, j, b, a_ii, a_ji, a_jj, eb, true, boff);
     bool bx = false;
 #pragma unroll
     for (int c = 0; c < C; ++c) {
@@ -1441,11 +1466,12 @@ __global__ void __launch_bounds__(64)
     }
     bx = Grp<G>::any(bx);
     if (g == 0) {
-        out[5 * q + 0] = !e.bad;
-        out[5 * q + 1] = e.delta;
-        out[5 * q + 2] = e.dnz;
-        out[5 * q + 3] = b;
-        out[5 * q + 4] = !bx;
+        out[6 * q + 0] = !e.bad;
+        out[6 * q + 1] = e.delta;
+        out[6 * q + 2] = e.dnz;
+        out[6 * q + 3] = b;
+        out[6 * q + 4] = !bx;
+        out[6 * q + 5] = b != 0 ? eb.delta : 0;
     }
 }
 

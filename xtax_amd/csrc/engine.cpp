@@ -988,14 +988,14 @@ int xtx_probe_moves(xtx_ctx* ctx, const int64_t* A, const int64_t* Xt, int32_t n
         HIPCHK(hipMalloc(&dA, nn * 8));
         HIPCHK(hipMalloc(&dX, nn * 8));
         HIPCHK(hipMalloc(&dq, (size_t)n_queries * 16));
-        HIPCHK(hipMalloc(&dout, (size_t)n_queries * 40));
+        HIPCHK(hipMalloc(&dout, (size_t)n_queries * 48));
         HIPCHK(hipMemcpy(dA, A, nn * 8, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(dX, Xt, nn * 8, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(dq, queries, (size_t)n_queries * 16, hipMemcpyHostToDevice));
         const Layout L = layout_for(n);
         HIPCHK(launch_probe(ctx->stream, L.G, L.C, dA, dX, n, dq, n_queries, dout));
         HIPCHK(hipStreamSynchronize(ctx->stream));
-        HIPCHK(hipMemcpy(out, dout, (size_t)n_queries * 40, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(out, dout, (size_t)n_queries * 48, hipMemcpyDeviceToHost));
         (void)hipFree(dA);
         (void)hipFree(dX);
         (void)hipFree(dq);

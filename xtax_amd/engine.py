@@ -288,7 +288,7 @@ class Engine:
             A = A[None]
             Xt = Xt[None]
         q = np.ascontiguousarray(queries, dtype=np.int32).reshape(-1, 4)
-        out = np.zeros((len(q), 5), np.int64)
+        out = np.zeros((len(q), 6), np.int64)
         self._check(self._lib.xtx_probe_moves(self._h, _p64(A), _p64(Xt), A.shape[1], A.shape[0],
                                               q.ctypes.data_as(N.I32P), len(q), _p64(out)))
         return out
