@@ -663,7 +663,9 @@ struct Model {
             std::vector<int64_t> row((size_t)n);
             int64_t d = 0, dnz = 0;
             if (orc_try_add(R.A.data(), nullptr, n, i, j, s, row.data(), nullptr, &d, &dnz)) {
-                const bool ok = d <= 0 || u53(W[1][3], W[2][2]) < orc_xexp(-(double)d / R.T);
+                // exp argument: d times -1/T (the reciprocal formed once per
+                // temperature, as the device keeps it)
+                const bool ok = d <= 0 || u53(W[1][3], W[2][2]) < orc_xexp((double)d * (-1.0 / R.T));
                 if (ok && commit_move(R, i, j, s, d, dnz, row, m, 0)) res = d < 0 ? 1 : 0;
             }
         }

@@ -955,6 +955,7 @@ __global__ void __launch_bounds__(256, 4)
     // moves since improvement: reset once above the threshold (an int32)
     uint32_t msi = (uint32_t)uni<G>((int32_t)S->msi);
     double T = tempering ? ((const GLOBAL double*)I->temps)[S->rung] : S->T;
+    double nbeta = -1.0 / T;  // the Metropolis exponent is delta * (-1/T)
     double cooling = S->cooling;
     int32_t cool_counter = uni<G>(S->cool_counter), win_p = uni<G>(S->win_p);
     int32_t win_a = uni<G>(S->win_a);
@@ -1071,19 +1072,16 @@ __global__ void __launch_bounds__(256, 4)
         const int base = 16 * sub;
         const int cs = g - base;  // this lane's slot for move m (valid in [0, 16))
         // words of slots 0..4 to every lane of the group: v_readlane for a
-        // whole-wave group, an LDS bounce otherwise
-        uint32_t w0z, w0w, w1z, w1w, w2z, w2w, w3z, w3w, w4z, w4w;
+        // whole-wave group (the ones only some moves need are read where they
+        // are used), an LDS bounce otherwise
+        uint32_t w0z, w0w, w1z, w2w, w3z;
+        uint32_t l1w = 0, l2z = 0, l3w = 0, l4z = 0, l4w = 0;
         if constexpr (G == 64) {
             w0z = readlane32(w.z, base + 0);
             w0w = readlane32(w.w, base + 0);
             w1z = readlane32(w.z, base + 1);
-            w1w = readlane32(w.w, base + 1);
-            w2z = readlane32(w.z, base + 2);
             w2w = readlane32(w.w, base + 2);
             w3z = readlane32(w.z, base + 3);
-            w3w = readlane32(w.w, base + 3);
-            w4z = readlane32(w.z, base + 4);
-            w4w = readlane32(w.w, base + 4);
         } else {
             if (cs >= 0 && cs < 5) swords[cs] = w;
             __builtin_amdgcn_wave_barrier();
@@ -1093,14 +1091,23 @@ __global__ void __launch_bounds__(256, 4)
             w0z = b0.z;
             w0w = b0.w;
             w1z = b1.z;
-            w1w = b1.w;
-            w2z = b2.z;
+            l1w = b1.w;
+            l2z = b2.z;
             w2w = b2.w;
             w3z = b3.z;
-            w3w = b3.w;
-            w4z = b4.z;
-            w4w = b4.w;
+            l3w = b3.w;
+            l4z = b4.z;
+            l4w = b4.w;
         }
+        auto word = [&](int slot, bool zw, uint32_t staged) -> uint32_t {
+            if constexpr (G == 64) return readlane32(zw ? w.w : w.z, base + slot);
+            return staged;
+        };
+        auto w1w = [&]() { return word(1, true, l1w); };
+        auto w2z = [&]() { return word(2, false, l2z); };
+        auto w3w = [&]() { return word(3, true, l3w); };
+        auto w4z = [&]() { return word(4, false, l4z); };
+        auto w4w = [&]() { return word(4, true, l4w); };
 
         // ---- pair (congruence_anneal.hpp:283-288, :211-228) ----
         int i, j;
@@ -1125,7 +1132,7 @@ __global__ void __launch_bounds__(256, 4)
                 Gp::slot_max(valid ? (((uint64_t)v << 5) | 16u | (uint32_t)(15 - cs)) : 0ull, sub);
             int ibest;
             if (vmax == 0) {
-                ibest = rowat((tpos + 1u + mulhi32(w3w, (uint32_t)(N - 1))) % (uint32_t)N);
+                ibest = rowat((tpos + 1u + mulhi32(w3w(), (uint32_t)(N - 1))) % (uint32_t)N);
             } else {
                 ibest = Gp::bcast(cand, base + 15 - (int)(vmax & 15));
             }
@@ -1163,7 +1170,7 @@ __global__ void __launch_bounds__(256, 4)
         int res = -1;
         if (e.bad) {
             ++infeasible;
-        } else if (e.delta <= 0 || u53(w1w, w2z) < xexp(-(double)e.delta / T)) {
+        } else if (e.delta <= 0 || u53(w1w(), w2z()) < xexp((double)e.delta * nbeta)) {
             int64_t nx[C];
             if (rp.xt_row(s, xi, xj, nx)) {
                 const int64_t ns = score + e.delta;
@@ -1200,8 +1207,8 @@ __global__ void __launch_bounds__(256, 4)
         if (tempering) {
             if ((int64_t)msi > stuck) {
                 const InstParams& Q = opaque(I)->P;
-                if (Q.sweep_fraction > 0.0 && u01(w4z) < Q.sweep_fraction) {
-                    sweep(sact[mulhi32(w4w, (uint32_t)N)], m);
+                if (Q.sweep_fraction > 0.0 && u01(w4z()) < Q.sweep_fraction) {
+                    sweep(sact[mulhi32(w4w(), (uint32_t)N)], m);
                     track_best(m);
                 }
                 local_best = score;
@@ -1210,8 +1217,8 @@ __global__ void __launch_bounds__(256, 4)
         } else {
             if ((int64_t)msi > stuck) {
                 const Inst* Iq = opaque(I);
-                if (Iq->P.sweep_fraction > 0.0 && u01(w4z) < Iq->P.sweep_fraction) {
-                    if (sweep(sact[mulhi32(w4w, (uint32_t)N)], m)) {
+                if (Iq->P.sweep_fraction > 0.0 && u01(w4z()) < Iq->P.sweep_fraction) {
+                    if (sweep(sact[mulhi32(w4w(), (uint32_t)N)], m)) {
                         local_best = score;
                         msi = 0;
                     }
@@ -1219,6 +1226,7 @@ __global__ void __launch_bounds__(256, 4)
                 }
                 const Inst* Iv = opaque(I);
                 T = Iv->P.reheat * Iv->P.t_init;
+                nbeta = -1.0 / T;
                 const int64_t gb_epoch = Iv->gb_score;
                 if ((double)score > (double)gb_epoch * Iv->P.reseed_factor) {
                     leave_best();
@@ -1248,6 +1256,7 @@ __global__ void __launch_bounds__(256, 4)
                 win_a = 0;
                 T *= cooling;
                 if (T < Q.t_min) T = Q.t_min;
+                nbeta = -1.0 / T;
             }
         }
         if (offnz == 0) {
