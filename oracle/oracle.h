@@ -7,6 +7,8 @@
  * Contents, each restating the reference (flinty-mathematics/xtax) in code of
  * our own:
  *   - exact int64 move primitives of the L1 objective    (src/main.cpp:46-242)
+ *   - xdual's primal/dual move primitives and dual     (src/xdual_main.cpp:67-364,
+ *     inverse, in the engine's double specification      src/mat_io.hpp:297-337)
  *   - Matrix helpers: score, nonzero count, congruence,  (src/mat_io.hpp:41-48,
  *     Gram, X^T L                                         219-235, 340-372, 545-580)
  *   - Philox4x32-10 (Salmon et al., SC'11, "Parallel random numbers: as easy as
@@ -52,6 +54,19 @@ int64_t orc_best_shear_l1(const int64_t* A, int32_t n, int32_t i, int32_t j);
 void orc_congruence_of(const int64_t* A, const int64_t* X, int32_t n, int64_t* out);
 void orc_gram_of(const int64_t* L, int32_t m, int32_t d, int64_t* out);
 void orc_xt_times(const int64_t* X, const int64_t* L, int32_t m, int32_t d, int64_t* out);
+
+/* ---- xdual primal/dual objective (oracle_dual.cpp), in the engine's double
+ * specification: pairwise column sums, exact primal squares rounded once ---- */
+double orc_pairsum(const double* x, int32_t n);
+int32_t orc_dual_eval(const int64_t* P, const double* Q, int32_t n, int32_t i, int32_t j,
+                      int64_t s, int64_t* new_p_row, double* new_q_row, double* d_off_p,
+                      double* d_off_q, int64_t* d_nonzero);
+int64_t orc_dual_best_shear(const int64_t* P, const double* Q, int32_t n, double c, int32_t i,
+                            int32_t j);
+int32_t orc_invert(const int64_t* P, int32_t n, double* Q);
+double orc_frob2_off_p(const int64_t* P, int32_t n);
+double orc_frob2_off_q(const double* Q, int32_t n);
+double orc_dual_residual(const int64_t* P, const double* Q, int32_t n, int32_t k);
 
 /* ---- engine building blocks ---- */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

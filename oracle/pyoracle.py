@@ -107,6 +107,22 @@ def lib():
                                     C.POINTER(C.c_double)]
         L.orc_suggest_t_init.restype = C.c_double
         L.orc_suggest_t_init.argtypes = [I64P, C.c_int32]
+        DP = C.POINTER(C.c_double)
+        L.orc_pairsum.restype = C.c_double
+        L.orc_pairsum.argtypes = [DP, C.c_int32]
+        L.orc_dual_eval.restype = C.c_int32
+        L.orc_dual_eval.argtypes = [I64P, DP, C.c_int32, C.c_int32, C.c_int32, C.c_int64, I64P,
+                                    DP, DP, DP, I64P]
+        L.orc_dual_best_shear.restype = C.c_int64
+        L.orc_dual_best_shear.argtypes = [I64P, DP, C.c_int32, C.c_double, C.c_int32, C.c_int32]
+        L.orc_invert.restype = C.c_int32
+        L.orc_invert.argtypes = [I64P, C.c_int32, DP]
+        L.orc_frob2_off_p.restype = C.c_double
+        L.orc_frob2_off_p.argtypes = [I64P, C.c_int32]
+        L.orc_frob2_off_q.restype = C.c_double
+        L.orc_frob2_off_q.argtypes = [DP, C.c_int32]
+        L.orc_dual_residual.restype = C.c_double
+        L.orc_dual_residual.argtypes = [I64P, DP, C.c_int32, C.c_int32]
         _LIB = L
     return _LIB
 
@@ -292,3 +308,55 @@ def model_run(A0, X0, params, max_epochs, active=None, log_cap=0):
     if log_cap:
         out["logs"] = [logs[r * log_cap: r * log_cap + counts[r]].copy() for r in range(R)]
     return out
+
+
+# ---- xdual primal/dual objective (oracle_dual.cpp) ----
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _pd(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def pairsum(x):
+    x = _f64(x)
+    return lib().orc_pairsum(_pd(x), len(x))
+
+
+def dual_eval(P, Q, i, j, s):
+    """(feasible, d_off_p, d_off_q, d_nonzero, new P row j, new Q row i)."""
+    P, Q = _i64(P), _f64(Q)
+    n = P.shape[0]
+    prow, qrow = np.zeros(n, np.int64), np.zeros(n)
+    dp, dq, dnz = C.c_double(), C.c_double(), C.c_int64()
+    ok = lib().orc_dual_eval(_p(P), _pd(Q), n, i, j, s, _p(prow), _pd(qrow), C.byref(dp),
+                             C.byref(dq), C.byref(dnz))
+    return bool(ok), dp.value, dq.value, dnz.value, prow, qrow
+
+
+def dual_best_shear(P, Q, c, i, j):
+    P, Q = _i64(P), _f64(Q)
+    return lib().orc_dual_best_shear(_p(P), _pd(Q), P.shape[0], float(c), i, j)
+
+
+def invert(P):
+    P = _i64(P)
+    Q = np.zeros(P.shape)
+    ok = lib().orc_invert(_p(P), P.shape[0], _pd(Q))
+    return bool(ok), Q
+
+
+def frob2_off_p(P):
+    P = _i64(P)
+    return lib().orc_frob2_off_p(_p(P), P.shape[0])
+
+
+def frob2_off_q(Q):
+    Q = _f64(Q)
+    return lib().orc_frob2_off_q(_pd(Q), Q.shape[0])
+
+
+def dual_residual(P, Q, k):
+    P, Q = _i64(P), _f64(Q)
+    return lib().orc_dual_residual(_p(P), _pd(Q), P.shape[0], k)

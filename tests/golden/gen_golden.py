@@ -23,6 +23,10 @@ reorder_for_publish (src/main.cpp:244-350, :407-411), det_mod_prime and
 is_unimodular (src/mat_io.hpp:597-644), deflate_index (src/main.cpp:435-455),
 through the same harness, into g3_publish_deflate.npz. `gen_golden.py g3`
 regenerates only that file.
+
+G4 (xdual): eval_add / eval_add_d, DualObjective::best_shear, invert_to and
+the off-diagonal norms (src/xdual_main.cpp:67-364, src/mat_io.hpp:297-337)
+through oracle/_ref/libref_dual.so into g4_xdual.npz (`gen_golden.py g4`).
 """
 import ctypes as C
 import os
@@ -366,6 +370,91 @@ def unimodular(rng, n, shears, sign=None):
     return np.asarray((U.T * d) @ U, dtype=np.int64), U
 
 
+def load_dual_harness():
+    L = C.CDLL(os.path.join(REF_DIR, "libref_dual.so"))
+    DP = C.POINTER(C.c_double)
+    L.refd_eval.restype = C.c_int
+    L.refd_eval.argtypes = [I64P, DP, C.c_int, C.c_int, C.c_int, C.c_int64, I64P, DP, DP, DP,
+                            I64P]
+    L.refd_best_shear.restype = C.c_int64
+    L.refd_best_shear.argtypes = [I64P, DP, C.c_int, C.c_double, C.c_int, C.c_int]
+    L.refd_invert.restype = C.c_int
+    L.refd_invert.argtypes = [I64P, C.c_int, DP]
+    L.refd_frob2_off_p.restype = C.c_double
+    L.refd_frob2_off_p.argtypes = [I64P, C.c_int]
+    L.refd_frob2_off_q.restype = C.c_double
+    L.refd_frob2_off_q.argtypes = [DP, C.c_int]
+    return L
+
+
+def g4(D):
+    """xdual: eval_add / eval_add_d, DualObjective::best_shear, invert_to and
+    the off-diagonal norms (src/xdual_main.cpp:67-364, src/mat_io.hpp:297-337)
+    on seeded full-rank Grams."""
+    rng = np.random.default_rng(44)
+    DP = C.POINTER(C.c_double)
+    out = {}
+    t = 0
+    for n in [2, 3, 4, 6, 9, 12, 17, 24, 33, 48]:
+        for kind in ["lattice", "planted", "big"]:
+            if kind == "lattice":
+                while True:
+                    Lm = rng.integers(-6, 7, (n, n))
+                    if abs(np.linalg.det(Lm)) > 0.5:
+                        break
+                P = np.asarray(Lm @ Lm.T, dtype=np.int64)
+            elif kind == "planted":
+                d = rng.integers(1, 30, n) * rng.choice([-1, 1], n)
+                U = np.eye(n, dtype=np.int64)
+                for _ in range(2 * n):
+                    a, b = rng.choice(n, 2, replace=False)
+                    U[:, b] += int(rng.choice([-1, 1])) * U[:, a]
+                P = np.asarray((U.T * d) @ U, dtype=np.int64)
+            else:
+                while True:
+                    Lm = rng.integers(-(1 << 20), 1 << 20, (n, n))
+                    if abs(np.linalg.det(Lm.astype(float))) > 1.0:
+                        break
+                P = np.asarray(Lm.astype(object).dot(Lm.T.astype(object)), dtype=np.int64)
+            Q = np.zeros((n, n))
+            ok = L_inv = D.refd_invert(P.ctypes.data_as(I64P), n, Q.ctypes.data_as(DP))
+            out[f"c{t}_P"] = P
+            out[f"c{t}_ok"] = np.array(ok)
+            out[f"c{t}_Q"] = Q
+            out[f"c{t}_fp"] = np.array(D.refd_frob2_off_p(P.ctypes.data_as(I64P), n))
+            out[f"c{t}_fq"] = np.array(D.refd_frob2_off_q(Q.ctypes.data_as(DP), n))
+            c = float(rng.choice([0.0, 0.5, 1.0, 37.0, 1e-3])) * (
+                out[f"c{t}_fp"] / out[f"c{t}_fq"] if out[f"c{t}_fq"] > 0 else 1.0)
+            out[f"c{t}_c"] = np.array(c)
+            qs, ev, evq, evp = [], [], [], []
+            for _ in range(60):
+                i = int(rng.integers(0, n)); j = int(rng.integers(0, n - 1)); j += j >= i
+                s = int(rng.choice([-64, -3, -1, 1, 2, 64, int(rng.integers(-64, 65))]))
+                prow = np.zeros(n, np.int64); qrow = np.zeros(n)
+                dp = C.c_double(); dq = C.c_double(); dnz = C.c_int64()
+                okm = D.refd_eval(P.ctypes.data_as(I64P), Q.ctypes.data_as(DP), n, i, j, s,
+                                  prow.ctypes.data_as(I64P), qrow.ctypes.data_as(DP),
+                                  C.byref(dp), C.byref(dq), C.byref(dnz))
+                qs.append((i, j, s))
+                ev.append((okm, dnz.value))
+                evq.append((dp.value, dq.value))
+                evp.append(np.concatenate([prow.astype(float), qrow]) if okm else np.zeros(2 * n))
+            out[f"c{t}_q"] = np.array(qs, np.int64)
+            out[f"c{t}_ev"] = np.array(ev, np.int64)
+            out[f"c{t}_evd"] = np.array(evq)
+            out[f"c{t}_rows_q"] = np.array([r[n:] for r in evp])
+            best = np.zeros((n, n), np.int64)
+            for i in range(n):
+                for j in range(n):
+                    if i != j:
+                        best[i, j] = D.refd_best_shear(P.ctypes.data_as(I64P),
+                                                       Q.ctypes.data_as(DP), n, c, i, j)
+            out[f"c{t}_best"] = best
+            t += 1
+    out["cases"] = np.array(t)
+    np.savez_compressed(os.path.join(OUT, "g4_xdual.npz"), **out)
+
+
 def g3(L):
     """Publish-time reordering (rcm_order, centroid_order, reorder_for_publish),
     is_unimodular / det_mod_prime and deflate_index, from the reference."""
@@ -465,13 +554,15 @@ def g3(L):
 
 if __name__ == "__main__":
     L = load_harness()
-    which = sys.argv[1:] or ["g1", "g2", "g3"]
+    which = sys.argv[1:] or ["g1", "g2", "g3", "g4"]
     if "g1" in which:
         g1(L)
     if "g2" in which:
         g2()
     if "g3" in which:
         g3(L)
+    if "g4" in which:
+        g4(load_dual_harness())
     for f in sorted(os.listdir(OUT)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(OUT, f)), file=sys.stderr)
