@@ -66,6 +66,7 @@ int64_t orc_dual_best_shear(const int64_t* P, const double* Q, int32_t n, double
 int32_t orc_invert(const int64_t* P, int32_t n, double* Q);
 double orc_frob2_off_p(const int64_t* P, int32_t n);
 double orc_frob2_off_q(const double* Q, int32_t n);
+double orc_dual_score(const int64_t* P, const double* Q, int32_t n, double c);
 double orc_dual_residual(const int64_t* P, const double* Q, int32_t n, int32_t k);
 
 /* ---- engine building blocks ---- */
@@ -149,6 +150,42 @@ int32_t orc_model_run(const int64_t* A_start, const int64_t* Xt_start, int32_t n
                       int64_t* best_A, int64_t* best_Xt, int64_t* scal_out,
                       orc_log_entry* log_out, int64_t* log_count, int64_t log_cap,
                       orc_model_result* res);
+
+/* ---- trajectory model of the engine on xdual's objective ---- */
+typedef struct {
+    double c;               /* dual weight, already normalized */
+    double dual_tol;        /* residual that triggers an early re-inversion */
+    int32_t dual_refresh;   /* moves between re-inversions (0 = never) */
+    int32_t dual_check;     /* moves between residual checks (0 = never) */
+} orc_dual_params;
+
+typedef struct {
+    int32_t solved;
+    int32_t solve_replica;
+    int64_t solve_move;
+    double best_score;       /* of the returned state */
+    int32_t best_replica;    /* replica whose state is returned */
+    int32_t epochs;
+    int64_t total_moves;
+    int64_t accepts;
+    int64_t exchanges;
+    int64_t refreshes;
+    int64_t checks;
+    int64_t recomputes;
+} orc_dual_result;
+
+/* The model on (P, Q, X^T) for at most max_epochs epochs. Per replica (each
+ * output may be null): P_out/Xt_out R*n*n int64, Q_out R*n*n double,
+ * dscal_out R*4 {score, best_score, local_best, T}, scal_out R*8 {offnz,
+ * moves, best_move, rung, accepts, refreshes, checks, msi}; best_P/best_Q/
+ * best_Xt the returned state (the solving replica's, else the instance best). */
+int32_t orc_dual_model_run(const int64_t* P_start, const double* Q_start,
+                           const int64_t* Xt_start, int32_t n, const int32_t* active,
+                           int32_t n_active, const orc_params* p, const orc_dual_params* dp,
+                           int32_t max_epochs, int64_t* P_out, double* Q_out, int64_t* Xt_out,
+                           double* dscal_out, int64_t* scal_out, int64_t* best_P,
+                           double* best_Q, int64_t* best_Xt, orc_log_entry* log_out,
+                           int64_t* log_count, int64_t log_cap, orc_dual_result* res);
 
 /* Ladder and per-replica fractions exactly as the model derives them (exposed
  * so tests can check the engine's host setup). */

@@ -2,16 +2,22 @@
 //
 // TEST INFRASTRUCTURE ONLY (see oracle.h). Restates, in code of our own, the
 // move primitives of xdual's DualObjective (src/xdual_main.cpp:67-364) in the
-// engine's double specification, and the reference's double Gauss-Jordan
-// inverse (invert_to, src/mat_io.hpp:297-337) exactly:
-//   * every sum over the columns k of a row is the pairwise (binary tree) sum
-//     of its terms in column order, zero-padded to 128 columns;
-//   * a primal term nv^2 - old^2 is formed exactly in 128-bit integers and
-//     rounded once to double; a dual term is nv*nv - old*old in double;
+// engine's double specification, the reference's double Gauss-Jordan inverse
+// (invert_to, src/mat_io.hpp:297-337) exactly, and a trajectory model of the
+// engine running that objective. The specification:
+//   * a sum over the columns k of a row folds column k + 64 onto column k
+//     (zero-padded to 128 columns), then adds the 64 results as a binary tree
+//     of adjacent pairs;
+//   * a primal term nv^2 - old^2 is (nv - old) * (nv + old) with both factors
+//     exact doubles (|.| <= 2^49 for a feasible shear), i.e. the exact integer
+//     rounded once; a dual term is nv*nv - old*old in double;
+//   * a squared off-diagonal norm sums each column sequentially over the rows
+//     (r = 0, 1, ...), then the column sums as above;
 //   * products and sums are single IEEE operations (no fused multiply-add).
 // The reference accumulates in long double; tests/test_oracle_dual.py pins
 // these functions against its outputs to a relative tolerance and the inverse
 // bit for bit.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <vector>
@@ -22,11 +28,13 @@ namespace {
 
 constexpr int kPad = 128;
 
-// pairwise sum of x[0..n) zero-padded to 128 terms
+// x[0..n) zero-padded to 128 terms: x[k] + x[k + 64], then the adjacent-pair
+// binary tree over those 64
 double pairsum(const double* x, int n) {
-    double buf[kPad];
-    for (int k = 0; k < kPad; ++k) buf[k] = k < n ? x[k] : 0.0;
-    for (int w = kPad / 2; w >= 1; w /= 2)
+    double buf[kPad / 2];
+    for (int k = 0; k < kPad / 2; ++k)
+        buf[k] = (k < n ? x[k] : 0.0) + (k + kPad / 2 < n ? x[k + kPad / 2] : 0.0);
+    for (int w = kPad / 4; w >= 1; w /= 2)
         for (int k = 0; k < w; ++k) buf[k] = buf[2 * k] + buf[2 * k + 1];
     return buf[0];
 }
@@ -53,7 +61,8 @@ int32_t orc_dual_eval(const int64_t* P, const double* Q, int32_t n, int32_t i, i
         const __int128 nv = (__int128)old + (__int128)s * P[(size_t)i * n + k];
         if (over(nv)) return 0;
         new_p_row[k] = (int64_t)nv;
-        tp[(size_t)k] = (double)(nv * nv - (__int128)old * old);
+        // (nv - old)(nv + old): exact factors, one rounding
+        tp[(size_t)k] = (double)(int64_t)(nv - old) * (double)(int64_t)(nv + old);
         if (old == 0) {
             if (nv != 0) ++dnz;
         } else if (nv == 0) {
@@ -184,44 +193,49 @@ int32_t orc_invert(const int64_t* P, int32_t n, double* Q) {
     return 1;
 }
 
-// Off-diagonal squared Frobenius norms: per row the pairwise sum of its
-// off-diagonal squares (primal squares exact in 128 bits, rounded once), the
-// rows summed in order.
+// Off-diagonal squared Frobenius norms: per column k the sequential sum over
+// rows r != k of the squares (exact entries, one rounding each), then the
+// column sums pairwise as above.
 double orc_frob2_off_p(const int64_t* P, int32_t n) {
-    std::vector<double> t((size_t)n);
-    double s = 0.0;
-    for (int r = 0; r < n; ++r) {
-        for (int k = 0; k < n; ++k) {
-            const __int128 v = P[(size_t)r * n + k];
-            t[(size_t)k] = k == r ? 0.0 : (double)(v * v);
-        }
-        s += pairsum(t.data(), n);
-    }
-    return s;
+    std::vector<double> col((size_t)n, 0.0);
+    for (int k = 0; k < n; ++k)
+        for (int r = 0; r < n; ++r)
+            if (r != k) {
+                const double v = (double)P[(size_t)r * n + k];
+                col[(size_t)k] += v * v;
+            }
+    return pairsum(col.data(), n);
 }
 
 double orc_frob2_off_q(const double* Q, int32_t n) {
-    std::vector<double> t((size_t)n);
-    double s = 0.0;
-    for (int r = 0; r < n; ++r) {
-        for (int k = 0; k < n; ++k) {
-            const double v = Q[(size_t)r * n + k];
-            t[(size_t)k] = k == r ? 0.0 : v * v;
-        }
-        s += pairsum(t.data(), n);
-    }
-    return s;
+    std::vector<double> col((size_t)n, 0.0);
+    for (int k = 0; k < n; ++k)
+        for (int r = 0; r < n; ++r)
+            if (r != k) {
+                const double v = Q[(size_t)r * n + k];
+                col[(size_t)k] += v * v;
+            }
+    return pairsum(col.data(), n);
+}
+
+// combined_score (xdual_main.cpp:160-162): |offdiag P|^2 + c |offdiag Q|^2
+double orc_dual_score(const int64_t* P, const double* Q, int32_t n, double c) {
+    const double fp = orc_frob2_off_p(P, n), fq = orc_frob2_off_q(Q, n);
+    const double cq = c * fq;
+    return fp + cq;
 }
 
 // Sampled residual of the dual (DualObjective::periodic_maintenance,
 // xdual_main.cpp:316-333): max over rows r of |(P Q)_rk - delta_rk|, each
-// product row a pairwise sum over t of P_rt * Q_tk.
+// product row the sequential sum over t of P_rt * Q_tk.
 double orc_dual_residual(const int64_t* P, const double* Q, int32_t n, int32_t k) {
-    std::vector<double> t((size_t)n);
     double worst = 0.0;
     for (int r = 0; r < n; ++r) {
-        for (int u = 0; u < n; ++u) t[(size_t)u] = (double)P[(size_t)r * n + u] * Q[(size_t)u * n + k];
-        double acc = pairsum(t.data(), n);
+        double acc = 0.0;
+        for (int t = 0; t < n; ++t) {
+            const double pr = (double)P[(size_t)r * n + t] * Q[(size_t)t * n + k];
+            acc += pr;
+        }
         if (r == k) acc -= 1.0;
         const double a = std::fabs(acc);
         if (a > worst) worst = a;
@@ -230,3 +244,501 @@ double orc_dual_residual(const int64_t* P, const double* Q, int32_t n, int32_t k
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Trajectory model of the engine running the primal/dual objective
+// ---------------------------------------------------------------------------
+//
+// The L1 model's per-move algorithm (oracle.cpp) with DualObjective in the
+// objective slot (congruence_anneal.hpp:276-413 over xdual_main.cpp:169-364):
+// the same Philox words, pair draws (row masses and pivot magnitudes of P) and
+// stall handling; the shear from the dual best_shear; the delta
+// d = d_off_p + c * d_off_q; Metropolis on d <= 0 or u < exp(d * (-1/T)).
+// On top of the L1 model:
+//   * a committed downhill move that is a new best of its replica and below
+//     the instance best at the epoch start recomputes the running score from
+//     scratch (the reference's recompute before publishing a global best,
+//     congruence_anneal.hpp:310-315); the replica best records that score;
+//   * at every epoch start, for each running replica, the maintenance of the
+//     moves made since the last one (periodic_maintenance,
+//     xdual_main.cpp:305-334): a re-inversion when the move count crossed a
+//     multiple of dual_refresh, else a residual check of column
+//     (that multiple of dual_check) mod n when it crossed one, re-inverting
+//     when the residual exceeds dual_tol; a re-inversion, or a changed dual
+//     weight, recomputes the running score.
+namespace {
+
+inline uint32_t dmulhi32(uint32_t w, uint32_t n) { return (uint32_t)(((uint64_t)w * n) >> 32); }
+inline double du01(uint32_t w) { return (double)w * 0x1p-32; }
+inline double du53(uint32_t hi, uint32_t lo) {
+    return (double)(((uint64_t)(hi >> 5) << 26) | (lo >> 6)) * 0x1p-53;
+}
+
+struct DualRep {
+    std::vector<int64_t> P, Xt, rm, bestP, bestXt;
+    std::vector<double> Q, bestQ;
+    double score = 0, local_best = 0, best_score = 0, T = 0, cooling = 0, dc = 0;
+    int64_t offnz = 0, msi = 0, best_move = 0, accepts = 0, solve_move = 0;
+    int64_t refreshes = 0, checks = 0, recomputes = 0;
+    uint64_t moves = 0, m_maint = 0;
+    int32_t cool_counter = 0, win_p = 0, win_a = 0, rung = 0;
+    bool stopped = false, solved = false;
+    double gf = 0, tf = 0;
+    std::vector<orc_log_entry> log;
+};
+
+struct DualModel {
+    int n = 0, N = 0;
+    std::vector<int> act;
+    orc_params p{};
+    orc_dual_params dp{};
+    bool tempering = false;
+    double t_init = 0;
+    std::vector<double> temps;
+    uint32_t key[2] = {0, 0};
+    std::vector<DualRep> reps;
+    double gb_score = 0;
+    int64_t gb_offnz = 0;
+    std::vector<int64_t> gbP, gbXt;
+    std::vector<double> gbQ;
+    int64_t log_cap = 0;
+
+    void block(uint64_t m, uint32_t r, uint32_t c, uint32_t w[4]) const {
+        const uint64_t q = (m - 1) >> 2;
+        const uint32_t t = (uint32_t)((m - 1) & 3);
+        const uint32_t ctr[4] = {(uint32_t)q, (uint32_t)(q >> 32), r, 16 * t + c};
+        orc_philox4x32_10(ctr, key, w);
+    }
+    void log_commit(DualRep& R, uint64_t m, int i, int j, int64_t s, uint16_t flags) {
+        if ((int64_t)R.log.size() < log_cap)
+            R.log.push_back(orc_log_entry{(int64_t)m, (int16_t)i, (int16_t)j, (int16_t)s, flags});
+    }
+    double rescore(const DualRep& R) const { return orc_dual_score(R.P.data(), R.Q.data(), n, dp.c); }
+
+    // DualObjective::commit (xdual_main.cpp:236-252): the transform row is
+    // bound-checked first; P row/column j with the row masses, Q row/column
+    // i, X^T row j.
+    bool commit(DualRep& R, int i, int j, int64_t s, const std::vector<int64_t>& prow,
+                const std::vector<double>& qrow) {
+        std::vector<int64_t> xrow((size_t)n);
+        if (!orc_add_transform_row(R.Xt.data(), n, i, j, s, xrow.data())) return false;
+        int64_t off = 0;
+        for (int k = 0; k < n; ++k) {
+            if (k == j) continue;
+            const int64_t o = R.P[(size_t)j * n + k], v = prow[(size_t)k];
+            R.rm[(size_t)k] += std::llabs(v) - std::llabs(o);
+            off += std::llabs(v) - std::llabs(o);
+        }
+        R.rm[(size_t)j] += off;
+        for (int k = 0; k < n; ++k) {
+            R.P[(size_t)j * n + k] = prow[(size_t)k];
+            R.P[(size_t)k * n + j] = prow[(size_t)k];
+            R.Q[(size_t)i * n + k] = qrow[(size_t)k];
+            R.Q[(size_t)k * n + i] = qrow[(size_t)k];
+            R.Xt[(size_t)j * n + k] = xrow[(size_t)k];
+        }
+        return true;
+    }
+    void snapshot(DualRep& R) {
+        R.bestP = R.P;
+        R.bestQ = R.Q;
+        R.bestXt = R.Xt;
+    }
+    void track_best(DualRep& R, uint64_t m) {
+        if (R.score < R.best_score) {
+            R.best_score = R.score;
+            R.best_move = (int64_t)m;
+            snapshot(R);
+        }
+    }
+
+    bool sweep(DualRep& R, int pivot, uint64_t m) {
+        const double before = R.score;
+        std::vector<int64_t> prow((size_t)n);
+        std::vector<double> qrow((size_t)n);
+        for (int q = 0; q < N; ++q) {
+            const int j = act[(size_t)q];
+            if (j == pivot) continue;
+            const int64_t s = orc_dual_best_shear(R.P.data(), R.Q.data(), n, dp.c, pivot, j);
+            if (s == 0) continue;
+            double d_p = 0, d_q = 0;
+            int64_t dnz = 0;
+            if (!orc_dual_eval(R.P.data(), R.Q.data(), n, pivot, j, s, prow.data(), qrow.data(),
+                               &d_p, &d_q, &dnz))
+                continue;
+            const double cq = dp.c * d_q;
+            const double d = d_p + cq;
+            if (d > 0.0) continue;
+            if (!commit(R, pivot, j, s, prow, qrow)) continue;
+            R.score += d;
+            R.offnz += dnz;
+            ++R.accepts;
+            log_commit(R, m, pivot, j, s, 1);
+        }
+        return R.score < before;
+    }
+
+    void step(DualRep& R, uint32_t r) {
+        const uint64_t m = ++R.moves;
+        if (tempering) R.T = temps[(size_t)R.rung];
+        const int G = std::max(p.target_samples, 5);
+        uint32_t W[32][4];
+        for (int g = 0; g < G; ++g) block(m, r, (uint32_t)g, W[g]);
+
+        int i, j;
+        if (du01(W[0][2]) < R.tf) {
+            uint32_t tpos = dmulhi32(W[0][0], (uint32_t)N);
+            for (int c = 1; c < p.target_samples; ++c) {
+                const uint32_t q = dmulhi32(W[c][0], (uint32_t)N);
+                if (R.rm[(size_t)act[q]] > R.rm[(size_t)act[tpos]]) tpos = q;
+            }
+            const int target = act[tpos];
+            int ibest = -1;
+            int64_t vbest = -1;
+            for (int c = 0; c < p.target_samples; ++c) {
+                const int cand = act[dmulhi32(W[c][1], (uint32_t)N)];
+                if (cand == target) continue;
+                const int64_t v = std::llabs(R.P[(size_t)target * n + cand]);
+                if (v > vbest) {
+                    vbest = v;
+                    ibest = cand;
+                }
+            }
+            if (ibest < 0)
+                ibest = act[(tpos + 1 + dmulhi32(W[3][3], (uint32_t)(N - 1))) % (uint32_t)N];
+            i = ibest;
+            j = target;
+        } else {
+            const uint32_t ip = dmulhi32(W[2][3], (uint32_t)N);
+            const uint32_t jp = (ip + 1 + dmulhi32(W[3][2], (uint32_t)(N - 1))) % (uint32_t)N;
+            i = act[ip];
+            j = act[jp];
+        }
+        const int64_t pm = (W[1][2] & 1u) ? 1 : -1;
+        int64_t s = pm;
+        if (du01(W[0][3]) < R.gf) {
+            s = orc_dual_best_shear(R.P.data(), R.Q.data(), n, dp.c, i, j);
+            if (s == 0) s = pm;
+        }
+
+        int res = -1;
+        {
+            std::vector<int64_t> prow((size_t)n);
+            std::vector<double> qrow((size_t)n);
+            double d_p = 0, d_q = 0;
+            int64_t dnz = 0;
+            if (orc_dual_eval(R.P.data(), R.Q.data(), n, i, j, s, prow.data(), qrow.data(), &d_p,
+                              &d_q, &dnz)) {
+                const double cq = dp.c * d_q;
+                const double d = d_p + cq;
+                const bool ok = d <= 0.0 || du53(W[1][3], W[2][2]) < orc_xexp(d * (-1.0 / R.T));
+                if (ok && commit(R, i, j, s, prow, qrow)) {
+                    const double ns = R.score + d;
+                    const bool newbest = ns < R.best_score;
+                    R.score = ns;
+                    R.offnz += dnz;
+                    if (d < 0.0 && newbest && ns < gb_score) {
+                        R.score = rescore(R);
+                        ++R.recomputes;
+                    }
+                    if (newbest) {
+                        R.best_score = R.score;
+                        R.best_move = (int64_t)m;
+                        snapshot(R);
+                    }
+                    ++R.accepts;
+                    log_commit(R, m, i, j, s, 0);
+                    res = d < 0.0 ? 1 : 0;
+                }
+            }
+        }
+        ++R.win_p;
+        if (res >= 0) ++R.win_a;
+        if (R.score < R.local_best) {
+            R.local_best = R.score;
+            R.msi = 0;
+        } else {
+            ++R.msi;
+        }
+        if (R.offnz == 0) {
+            R.solved = R.stopped = true;
+            R.solve_move = (int64_t)m;
+            return;
+        }
+        if (tempering) {
+            if (R.msi > p.stuck_threshold) {
+                if (p.sweep_fraction > 0.0 && du01(W[4][2]) < p.sweep_fraction) {
+                    sweep(R, act[dmulhi32(W[4][3], (uint32_t)N)], m);
+                    track_best(R, m);
+                }
+                R.local_best = R.score;
+                R.msi = 0;
+            }
+        } else {
+            if (R.msi > p.stuck_threshold) {
+                if (p.sweep_fraction > 0.0 && du01(W[4][2]) < p.sweep_fraction) {
+                    if (sweep(R, act[dmulhi32(W[4][3], (uint32_t)N)], m)) {
+                        R.local_best = R.score;
+                        R.msi = 0;
+                    }
+                    track_best(R, m);
+                }
+                R.T = p.reheat * t_init;
+                if (R.score > gb_score * p.reseed_factor) {
+                    log_commit(R, m, -1, -1, 0, 2);
+                    R.P = gbP;
+                    R.Q = gbQ;
+                    R.Xt = gbXt;
+                    orc_compute_row_mass(R.P.data(), n, R.rm.data());
+                    R.score = gb_score;
+                    R.offnz = gb_offnz;
+                    track_best(R, m);
+                }
+                R.local_best = R.score;
+                R.msi = 0;
+                R.cool_counter = 0;
+            } else if (++R.cool_counter >= p.moves_per_cool) {
+                R.cool_counter = 0;
+                if (p.adaptive_cooling && R.win_p > 0) {
+                    const double acc = (double)R.win_a / (double)R.win_p;
+                    if (acc > p.target_accept)
+                        R.cooling *= 0.999;
+                    else
+                        R.cooling /= 0.999;
+                    R.cooling = std::min(0.99999, std::max(0.90, R.cooling));
+                }
+                R.win_p = 0;
+                R.win_a = 0;
+                R.T *= R.cooling;
+                if (R.T < p.t_min) R.T = p.t_min;
+            }
+        }
+        if (R.offnz == 0) {
+            R.solved = R.stopped = true;
+            R.solve_move = (int64_t)m;
+            return;
+        }
+        if (p.max_moves && R.moves >= p.max_moves) R.stopped = true;
+    }
+
+    void maintain(DualRep& R) {
+        if (R.stopped) return;
+        const uint64_t m0 = R.m_maint, m1 = R.moves;
+        R.m_maint = m1;
+        bool refresh = false;
+        if (dp.dual_refresh > 0 && m1 / (uint64_t)dp.dual_refresh > m0 / (uint64_t)dp.dual_refresh) {
+            refresh = true;
+        } else if (dp.dual_check > 0 &&
+                   m1 / (uint64_t)dp.dual_check > m0 / (uint64_t)dp.dual_check) {
+            const uint64_t at = m1 / (uint64_t)dp.dual_check * (uint64_t)dp.dual_check;
+            const int k = (int)(at % (uint64_t)n);
+            ++R.checks;
+            refresh = orc_dual_residual(R.P.data(), R.Q.data(), n, k) > dp.dual_tol;
+        }
+        bool rescore_now = R.dc != dp.c;
+        if (refresh) {
+            std::vector<double> Qn((size_t)n * n);
+            if (orc_invert(R.P.data(), n, Qn.data())) {
+                R.Q.swap(Qn);
+                ++R.refreshes;
+                rescore_now = true;
+            }
+        }
+        if (rescore_now) {
+            R.score = rescore(R);
+            R.dc = dp.c;
+        }
+    }
+
+    int best_rep() const {
+        int b = 0;
+        for (int r = 1; r < (int)reps.size(); ++r) {
+            const auto& x = reps[(size_t)r];
+            const auto& y = reps[(size_t)b];
+            if (x.best_score < y.best_score ||
+                (x.best_score == y.best_score && x.best_move < y.best_move))
+                b = r;
+        }
+        return b;
+    }
+};
+
+}  // namespace
+
+extern "C" int32_t orc_dual_model_run(const int64_t* P_start, const double* Q_start,
+                                      const int64_t* Xt_start, int32_t n, const int32_t* active,
+                                      int32_t n_active, const orc_params* pp,
+                                      const orc_dual_params* dpp, int32_t max_epochs,
+                                      int64_t* P_out, double* Q_out, int64_t* Xt_out,
+                                      double* dscal_out, int64_t* scal_out,
+                                      int64_t* best_P, double* best_Q, int64_t* best_Xt,
+                                      orc_log_entry* log_out, int64_t* log_count,
+                                      int64_t log_cap, orc_dual_result* res) {
+    DualModel M;
+    M.n = n;
+    M.p = *pp;
+    M.dp = *dpp;
+    M.log_cap = log_cap;
+    if (active && n_active > 0) {
+        M.act.assign(active, active + n_active);
+    } else {
+        M.act.resize((size_t)n);
+        for (int k = 0; k < n; ++k) M.act[(size_t)k] = k;
+    }
+    M.N = (int)M.act.size();
+    const int R = std::max(1, pp->replicas);
+    M.tempering = pp->tempering && R >= 2;
+    M.key[0] = (uint32_t)pp->seed;
+    M.key[1] = (uint32_t)(pp->seed >> 32);
+    const size_t nn = (size_t)n * n;
+    const std::vector<int64_t> P0(P_start, P_start + nn), X0(Xt_start, Xt_start + nn);
+    const std::vector<double> Q0(Q_start, Q_start + nn);
+    const double score0 = orc_dual_score(P0.data(), Q0.data(), n, dpp->c);
+    const int64_t offnz0 = orc_offdiag_nonzero(P0.data(), n);
+    // suggest_t_init (xdual_main.cpp:212-219)
+    const double avg = score0 / (double)((int64_t)n * n);
+    M.t_init = pp->t_init > 0.0 ? pp->t_init : std::max(1.0, 2.0 * avg);
+    if (M.tempering) {
+        M.temps.resize((size_t)R);
+        orc_ladder(M.t_init, pp->t_min, R, M.temps.data());
+    }
+    M.reps.resize((size_t)R);
+    for (int r = 0; r < R; ++r) {
+        DualRep& x = M.reps[(size_t)r];
+        x.P = P0;
+        x.Q = Q0;
+        x.Xt = X0;
+        x.rm.resize((size_t)n);
+        orc_compute_row_mass(x.P.data(), n, x.rm.data());
+        x.score = x.local_best = x.best_score = score0;
+        x.dc = dpp->c;
+        x.offnz = offnz0;
+        x.T = M.t_init;
+        x.cooling = pp->cooling;
+        x.rung = r;
+        M.snapshot(x);
+        orc_fractions(pp, r, &x.gf, &x.tf);
+    }
+    M.gb_score = score0;
+    M.gb_offnz = offnz0;
+    M.gbP = P0;
+    M.gbQ = Q0;
+    M.gbXt = X0;
+
+    orc_dual_result out{};
+    out.solve_replica = -1;
+    const bool trivial = n < 2 || M.N < 2;
+    const bool start_diag = offnz0 == 0;
+    if (start_diag) {
+        out.solved = 1;
+        out.solve_replica = 0;
+    }
+    int epochs = 0;
+    int64_t exchanges = 0;
+    const uint64_t E = (uint64_t)std::max(1, pp->exchange_interval);
+    for (int e = 0; !trivial && !start_diag && e < max_epochs; ++e) {
+        for (int r = 0; r < R; ++r) M.maintain(M.reps[(size_t)r]);
+        bool any_running = false;
+        for (int r = 0; r < R; ++r) {
+            DualRep& x = M.reps[(size_t)r];
+            while (!x.stopped && x.moves < (uint64_t)(e + 1) * E) M.step(x, (uint32_t)r);
+            any_running |= !x.stopped;
+        }
+        epochs = e + 1;
+        int sr = -1;
+        for (int r = 0; r < R; ++r) {
+            const DualRep& x = M.reps[(size_t)r];
+            if (x.solved && (sr < 0 || x.solve_move < M.reps[(size_t)sr].solve_move)) sr = r;
+        }
+        if (sr >= 0) {
+            out.solved = 1;
+            out.solve_replica = sr;
+            out.solve_move = M.reps[(size_t)sr].solve_move;
+            break;
+        }
+        if (M.tempering) {
+            std::vector<int> owner((size_t)R);
+            for (int r = 0; r < R; ++r) owner[(size_t)M.reps[(size_t)r].rung] = r;
+            for (int q = 0; q + 1 < R; ++q) {
+                const int wa = owner[(size_t)q], wb = owner[(size_t)q + 1];
+                const double Ta = M.temps[(size_t)q], Tb = M.temps[(size_t)q + 1];
+                const double Sa = M.reps[(size_t)wa].score, Sb = M.reps[(size_t)wb].score;
+                const double arg = (1.0 / Ta - 1.0 / Tb) * (Sa - Sb);
+                uint32_t w[4];
+                const uint32_t ctr[4] = {(uint32_t)e, (uint32_t)((uint64_t)e >> 32), (uint32_t)q,
+                                         0x40000000u};
+                orc_philox4x32_10(ctr, M.key, w);
+                if (arg >= 0.0 || du53(w[0], w[1]) < orc_xexp(arg)) {
+                    M.reps[(size_t)wa].rung = q + 1;
+                    M.reps[(size_t)wb].rung = q;
+                    std::swap(owner[(size_t)q], owner[(size_t)q + 1]);
+                    ++exchanges;
+                }
+            }
+        }
+        const int b = M.best_rep();
+        if (M.reps[(size_t)b].best_score < M.gb_score) {
+            const DualRep& x = M.reps[(size_t)b];
+            M.gb_score = x.best_score;
+            M.gbP = x.bestP;
+            M.gbQ = x.bestQ;
+            M.gbXt = x.bestXt;
+            M.gb_offnz = orc_offdiag_nonzero(M.gbP.data(), n);
+        }
+        if (!any_running) break;
+    }
+    const int b = M.best_rep();
+    out.best_replica = out.solved && out.solve_replica >= 0 ? out.solve_replica : b;
+    const DualRep& win = M.reps[(size_t)out.best_replica];
+    const bool cur = out.solved;
+    out.best_score = cur ? win.score : win.best_score;
+    out.epochs = epochs;
+    out.exchanges = exchanges;
+    for (int r = 0; r < R; ++r) {
+        const DualRep& x = M.reps[(size_t)r];
+        out.total_moves += (int64_t)x.moves;
+        out.accepts += x.accepts;
+        out.refreshes += x.refreshes;
+        out.checks += x.checks;
+        out.recomputes += x.recomputes;
+        if (P_out) std::copy(x.P.begin(), x.P.end(), P_out + (size_t)r * nn);
+        if (Q_out) std::copy(x.Q.begin(), x.Q.end(), Q_out + (size_t)r * nn);
+        if (Xt_out) std::copy(x.Xt.begin(), x.Xt.end(), Xt_out + (size_t)r * nn);
+        if (dscal_out) {
+            double* o = dscal_out + (size_t)r * 4;
+            o[0] = x.score;
+            o[1] = x.best_score;
+            o[2] = x.local_best;
+            o[3] = x.T;
+        }
+        if (scal_out) {
+            int64_t* o = scal_out + (size_t)r * 8;
+            o[0] = x.offnz;
+            o[1] = (int64_t)x.moves;
+            o[2] = x.best_move;
+            o[3] = x.rung;
+            o[4] = x.accepts;
+            o[5] = x.refreshes;
+            o[6] = x.checks;
+            o[7] = x.msi;
+        }
+        if (log_out && log_count) {
+            std::copy(x.log.begin(), x.log.end(), log_out + (size_t)r * log_cap);
+            log_count[r] = (int64_t)x.log.size();
+        }
+    }
+    if (best_P) {
+        const auto& src = cur ? win.P : win.bestP;
+        std::copy(src.begin(), src.end(), best_P);
+    }
+    if (best_Q) {
+        const auto& src = cur ? win.Q : win.bestQ;
+        std::copy(src.begin(), src.end(), best_Q);
+    }
+    if (best_Xt) {
+        const auto& src = cur ? win.Xt : win.bestXt;
+        std::copy(src.begin(), src.end(), best_Xt);
+    }
+    if (res) *res = out;
+    return out.solved;
+}

@@ -42,6 +42,20 @@ class ModelResult(C.Structure):
     ]
 
 
+class DualParams(C.Structure):
+    _fields_ = [("c", C.c_double), ("dual_tol", C.c_double), ("dual_refresh", C.c_int32),
+                ("dual_check", C.c_int32)]
+
+
+class DualResult(C.Structure):
+    _fields_ = [
+        ("solved", C.c_int32), ("solve_replica", C.c_int32), ("solve_move", C.c_int64),
+        ("best_score", C.c_double), ("best_replica", C.c_int32), ("epochs", C.c_int32),
+        ("total_moves", C.c_int64), ("accepts", C.c_int64), ("exchanges", C.c_int64),
+        ("refreshes", C.c_int64), ("checks", C.c_int64), ("recomputes", C.c_int64),
+    ]
+
+
 LOG_DTYPE = np.dtype([("move", "<i8"), ("i", "<i2"), ("j", "<i2"), ("s", "<i2"),
                       ("flags", "<u2")])
 
@@ -121,6 +135,13 @@ def lib():
         L.orc_frob2_off_p.argtypes = [I64P, C.c_int32]
         L.orc_frob2_off_q.restype = C.c_double
         L.orc_frob2_off_q.argtypes = [DP, C.c_int32]
+        L.orc_dual_score.restype = C.c_double
+        L.orc_dual_score.argtypes = [I64P, DP, C.c_int32, C.c_double]
+        VP = C.c_void_p
+        L.orc_dual_model_run.restype = C.c_int32
+        L.orc_dual_model_run.argtypes = [I64P, DP, I64P, C.c_int32, VP, C.c_int32, VP, VP,
+                                         C.c_int32, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP,
+                                         C.c_int64, VP]
         L.orc_dual_residual.restype = C.c_double
         L.orc_dual_residual.argtypes = [I64P, DP, C.c_int32, C.c_int32]
         _LIB = L
@@ -360,3 +381,43 @@ def frob2_off_q(Q):
 def dual_residual(P, Q, k):
     P, Q = _i64(P), _f64(Q)
     return lib().orc_dual_residual(_p(P), _pd(Q), P.shape[0], k)
+
+
+def dual_score(P, Q, c):
+    P, Q = _i64(P), _f64(Q)
+    return lib().orc_dual_score(_p(P), _pd(Q), P.shape[0], float(c))
+
+
+def dual_model_run(P0, Q0, X0, params, dparams, max_epochs, active=None, log_cap=0):
+    """Runs the engine model on the primal/dual objective from (P0, Q0, X0)."""
+    P0, Q0 = _i64(P0), _f64(Q0)
+    n = P0.shape[0]
+    Xt0 = _i64(np.asarray(X0).T)
+    R = params.replicas
+    P = np.zeros((R, n, n), np.int64)
+    Q = np.zeros((R, n, n))
+    Xt = np.zeros((R, n, n), np.int64)
+    dscal = np.zeros((R, 4))
+    scal = np.zeros((R, 8), np.int64)
+    bP = np.zeros((n, n), np.int64)
+    bQ = np.zeros((n, n))
+    bXt = np.zeros((n, n), np.int64)
+    logs = np.zeros(max(1, R * log_cap), LOG_DTYPE)
+    counts = np.zeros(R, np.int64)
+    res = DualResult()
+    act = None
+    na = 0
+    if active is not None:
+        act = np.ascontiguousarray(active, dtype=np.int32)
+        na = len(act)
+    d = lambda a: a.ctypes.data  # noqa: E731
+    lib().orc_dual_model_run(_p(P0), _pd(Q0), _p(Xt0), n, d(act) if act is not None else None,
+                             na, C.addressof(params), C.addressof(dparams), max_epochs, d(P),
+                             d(Q), d(Xt), d(dscal), d(scal), d(bP), d(bQ), d(bXt),
+                             d(logs) if log_cap else None, d(counts) if log_cap else None,
+                             log_cap, C.addressof(res))
+    out = dict(P=P, Q=Q, Xt=Xt, dscal=dscal, scal=scal, best_P=bP, best_Q=bQ, best_Xt=bXt,
+               **{f: getattr(res, f) for f, _ in DualResult._fields_})
+    if log_cap:
+        out["logs"] = [logs[r * log_cap: r * log_cap + counts[r]].copy() for r in range(R)]
+    return out

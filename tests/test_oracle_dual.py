@@ -34,13 +34,18 @@ def cases(g4):
 
 def test_pairwise_sum_closed_form(po):
     rng = np.random.default_rng(3)
-    for n in (1, 2, 3, 7, 64, 100, 128):
+    for n in (1, 2, 3, 7, 64, 65, 100, 128):
         x = rng.standard_normal(n)
-        buf = np.concatenate([x, np.zeros(128 - n)])
+        pad = np.concatenate([x, np.zeros(128 - n)])
+        buf = pad[:64] + pad[64:]          # column k + 64 folds onto column k
         while len(buf) > 1:
-            buf = buf[0::2] + buf[1::2]
+            buf = buf[0::2] + buf[1::2]    # then adjacent pairs
         assert po.pairsum(x) == buf[0]
     assert po.pairsum(np.array([2.0 ** 60, 1.0, -2.0 ** 60, 1.0])) == 0.0  # (a+1)+(-a+1) rounds
+    # the fold: x[0] + x[64] first, so 2^60 absorbs the 1 before -2^60 cancels it
+    x = np.zeros(65)
+    x[0], x[1], x[64] = 2.0 ** 60, -2.0 ** 60, 1.0
+    assert po.pairsum(x) == 0.0
 
 
 def test_invert_is_the_references(g4, po):
@@ -134,3 +139,100 @@ def test_residual_of_a_true_inverse(po):
     Q2 = Q.copy()
     Q2[3, 3] += 1e-3
     assert po.dual_residual(P, Q2, 3) > 1e-6
+
+
+def _brute_best(P, Q, c, i, j):
+    """argmin over s in [-64, 64] of the exact (rational) change f(s) < 0,
+    the smaller s on a tie, else 0: the integer minimizer the best shear's
+    floor/ceil rule must find for a convex quadratic."""
+    best_s, best_f = 0, Fraction(0)
+    for s in range(-64, 65):
+        if s == 0:
+            continue
+        f, _ = _f_exact(P, Q, c, i, j, s)
+        if f < best_f:
+            best_s, best_f = s, f
+    return best_s, best_f
+
+
+def test_best_shear_is_the_integer_minimizer(g4, po):
+    """Independently of the reference: at c = 0 the change is the exact
+    integer change of |offdiag P|^2; at c > 0 the rational change of the
+    double state. A differing choice must tie to rounding."""
+    rng = np.random.default_rng(21)
+    for t in list(cases(g4))[:18]:
+        P, Q = g4[f"c{t}_P"], g4[f"c{t}_Q"]
+        n = P.shape[0]
+        for _ in range(12):
+            i, j = (int(v) for v in rng.choice(n, 2, replace=False))
+            for c in (0.0, float(g4[f"c{t}_c"])):
+                got = po.dual_best_shear(P, Q, c, i, j)
+                want, fw = _brute_best(P, Q, c, i, j)
+                if c == 0.0:
+                    # exact integer objective change for the chosen shear
+                    ok, dp, _, _, prow, _ = po.dual_eval(P, Q, i, j, want) if want else (1, 0, 0, 0, 0, 0)
+                    if want:
+                        exact = 2 * sum(int(prow[k]) ** 2 - int(P[j, k]) ** 2 for k in range(n) if k != j)
+                        assert Fraction(exact) == fw
+                if got != want:
+                    fg, quad = _f_exact(P, Q, c, i, j, got) if got else (Fraction(0), 1)
+                    assert abs(fg - fw) <= Fraction(1, 10 ** 9) * (abs(quad) * 64 * 64 + 1), (t, i, j, c)
+
+
+def _model_case(po, n, seed, tempering, R=4, epochs=4, E=300, refresh=700, check=250, tol=1e-9):
+    rng = np.random.default_rng(seed)
+    B = np.eye(n, dtype=np.int64)
+    for _ in range(3 * n):
+        a, b = rng.choice(n, 2, replace=False)
+        B[a] += int(rng.integers(-2, 3)) * B[b]
+    P0 = (B @ np.diag(rng.integers(1, 4, n)) @ B.T).astype(np.int64)
+    ok, Q0 = po.invert(P0)
+    assert ok
+    p = po.Params(**dict(po.DEFAULTS, replicas=R, seed=seed, tempering=tempering,
+                         exchange_interval=E, stuck_threshold=150, sweep_fraction=0.5,
+                         greedy_fraction=0.6))
+    fp, fq = po.frob2_off_p(P0), po.frob2_off_q(Q0)
+    dp = po.DualParams(c=fp / fq, dual_tol=tol, dual_refresh=refresh, dual_check=check)
+    out = po.dual_model_run(P0, Q0, np.eye(n, dtype=np.int64), p, dp, epochs, log_cap=4096)
+    return P0, Q0, dp, out
+
+
+@pytest.mark.parametrize("tempering", [1, 0])
+def test_dual_model_invariants(po, tempering):
+    """What the mathematics fixes about every replica of the model: P is the
+    exact congruence X^T P0 X by a unimodular X, the running score tracks the
+    recomputed one, Q stays an inverse of P, and (tempering: no reseeds) the
+    accepted-move log replays P exactly through the move primitives."""
+    n = 9
+    P0, Q0, dp, out = _model_case(po, n, 5, tempering)
+    assert out["refreshes"] > 0 and out["checks"] > 0
+    for r in range(len(out["P"])):
+        P, Q, Xt = out["P"][r], out["Q"][r], out["Xt"][r]
+        Xo = Xt.T.astype(object)
+        assert np.array_equal((Xo.T @ P0.astype(object) @ Xo).astype(np.int64), P)
+        assert round(abs(np.linalg.det(Xt.astype(float)))) == 1
+        assert int(out["scal"][r][0]) == int(np.count_nonzero(np.triu(P, 1)))
+        # the dual drifts between re-inversions (condition numbers here reach 1e8)
+        assert np.abs(P.astype(float) @ Q - np.eye(n)).max() < 1e-4
+        score = out["dscal"][r][0]
+        assert score == pytest.approx(po.dual_score(P, Q, dp.c), rel=1e-9)
+        if tempering:
+            Pr, Qr = P0.copy(), Q0.copy()
+            for e in out["logs"][r]:
+                i, j, s = int(e["i"]), int(e["j"]), int(e["s"])
+                ok, _, _, _, prow, qrow = po.dual_eval(Pr, Qr, i, j, s)
+                assert ok
+                Pr[j, :] = prow
+                Pr[:, j] = prow
+                Qr[i, :] = qrow
+                Qr[:, i] = qrow
+            assert np.array_equal(Pr, P)
+
+
+def test_dual_model_is_deterministic_and_solves(po):
+    a = _model_case(po, 7, 3, 1, epochs=30)[3]
+    b = _model_case(po, 7, 3, 1, epochs=30)[3]
+    assert np.array_equal(a["P"], b["P"]) and np.array_equal(a["Q"], b["Q"])
+    assert a["solved"] == 1
+    bp = a["best_P"]
+    assert np.count_nonzero(bp - np.diag(np.diag(bp))) == 0
