@@ -288,6 +288,77 @@ int xtx_is_unimodular(const int64_t* A, int32_t n, int32_t* result);
 int xtx_deflate_pivot(int64_t* M, int64_t* Xt, int32_t n, int32_t i, const int32_t* active,
                       int32_t n_active, int32_t* cleared);
 
+/* ---- xdual: the primal/dual objective on the same engine ----
+ * Replaces run_xdual (src/xdual_main.cpp:378-424), i.e.
+ * run_annealer(DualObjective(P, P^-1, X0, c), params, rows): minimize
+ * |offdiag P|^2 + c |offdiag Q|^2 over P = X^T A0 X (exact int64, the same
+ * shears and 2^48 bound) with the double dual Q = P^-1 updated by the inverse
+ * transpose of every shear. The dual is inverted once on the host (invert_to,
+ * mat_io.hpp:297-337); c = lambda * |offdiag P0|^2 / |offdiag Q0|^2 (long
+ * double, :388-393). Engine parameters are the xtx_params of the L1 engine
+ * (waves_per_block, autotune and the launch configurations do not apply). */
+typedef struct {
+    double lambda;            /* 1.0: equal initial primal/dual weight (--lambda) */
+    double dual_tol;          /* 1e-6: residual that triggers an early re-inversion */
+    double lambda_ramp;       /* 0: seconds to ramp c from 0 to full (--lambda-ramp);
+                                 applied per epoch from the host clock */
+    int32_t dual_refresh;     /* 200000: moves between re-inversions (0 = never) */
+    int32_t dual_check;       /* 25000: moves between sampled residual checks (0 = never) */
+} xtx_dual_params;
+
+typedef struct {
+    int64_t moves;
+    int64_t accepts;
+    int64_t greedy_calls;
+    int64_t infeasible;
+    int64_t sweeps;
+    int64_t exchanges_tried;
+    int64_t exchanges_accepted;
+    int64_t refreshes;        /* dual re-inversions (all replicas) */
+    int64_t checks;           /* sampled residual checks */
+    int64_t recomputes;       /* running-score recomputations before a new best */
+    int32_t solved;           /* the primal P reached a diagonal */
+    int32_t solve_replica;
+    int64_t solve_move;
+    double best_score;        /* |offdiag P|^2 + c |offdiag Q|^2 of the returned
+                                 state at the full weight c */
+    double start_score;       /* the same of the start state */
+    double c;                 /* normalized dual weight */
+    double dual_residual;     /* max |P Q - I| of the returned state */
+    int32_t best_replica;
+    int32_t epochs;
+    int32_t stop_reason;      /* as xtx_stats.stop_reason */
+    int32_t verify_status;    /* X^T A0 X == P and |det X| = 1 (and P diagonal
+                                 when solved); 0 ok, -1 not run */
+    double seconds;
+    double kernel_ms;
+} xtx_dual_stats;
+
+void xtx_dual_params_default(xtx_dual_params* d);
+/* A0 (n x n, symmetric, |entries| <= 2^48), X0 (nullable = identity): the
+ * annealer starts from P = X0^T A0 X0. Writes the best (or solving) P, Q and
+ * X to the caller's buffers (each nullable). XTX_E_ARG if P is singular. */
+int xtx_dual_solve(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t n,
+                   const xtx_params* params, const xtx_dual_params* dparams,
+                   const int32_t* active, int32_t n_active, int64_t* P_out, double* Q_out,
+                   int64_t* X_out, xtx_dual_stats* stats);
+/* Per-replica state of the last xtx_dual_solve: P, Xt (R*n*n int64), Q
+ * (R*n*n double), 4 doubles {score, best_score, local_best, T} and 8 int64
+ * {offnz, moves, best_move, rung, accepts, refreshes, checks, moves since
+ * improvement} per replica (each pointer nullable). Its move log is read with
+ * xtx_movelog_get. */
+int xtx_dual_last_replicas(xtx_ctx* ctx, int64_t* P, double* Q, int64_t* Xt, double* dscal,
+                           int64_t* scal);
+/* The primal/dual move primitives as the device computes them, for queries
+ * (matrix index, i, j, s) on n_mats matrices P (int64) and Q (double): per
+ * query out_i = {feasible, d_nonzero, best_shear(i, j)}, out_d = delta
+ * (d_off_p + c d_off_q), the proposed P row j and Q row i (n each), and the
+ * score |offdiag P|^2 + c |offdiag Q|^2 of the matrix (each pointer nullable
+ * except out_i). */
+int xtx_dual_probe(xtx_ctx* ctx, const int64_t* P, const double* Q, int32_t n, int32_t n_mats,
+                   double c, const int32_t* queries, int32_t n_queries, int64_t* out_i,
+                   double* out_d, int64_t* rows_p, double* rows_q, double* scores);
+
 /* ---- device primitives, for parity tests of the kernels' building blocks ----
  * Each query q is (matrix index, i, j, s); the device evaluates it with the
  * same device functions the annealer uses. out per query (6 values):

@@ -127,6 +127,7 @@ int64_t orc_dual_best_shear(const int64_t* P, const double* Q, int32_t n, double
     const double cd2 = c * d2;
     const double lin = b2 - cd2;
     double s_star = -lin / quad;
+    if (s_star != s_star) return 0;  // no minimizer from a non-finite state
     s_star = s_star > 4e18 ? 4e18 : (s_star < -4e18 ? -4e18 : s_star);
     int64_t lo = (int64_t)std::floor(s_star);
     int64_t hi = lo + 1;
@@ -255,10 +256,11 @@ double orc_dual_residual(const int64_t* P, const double* Q, int32_t n, int32_t k
 // stall handling; the shear from the dual best_shear; the delta
 // d = d_off_p + c * d_off_q; Metropolis on d <= 0 or u < exp(d * (-1/T)).
 // On top of the L1 model:
-//   * a committed downhill move that is a new best of its replica and below
-//     the instance best at the epoch start recomputes the running score from
-//     scratch (the reference's recompute before publishing a global best,
-//     congruence_anneal.hpp:310-315); the replica best records that score;
+//   * a committed downhill move whose running score is below both its
+//     replica's best and the instance best at the epoch start recomputes the
+//     running score from scratch (the reference's recompute before publishing
+//     a global best, congruence_anneal.hpp:310-315), and is a new replica best
+//     if the recomputed score is below the replica's best;
 //   * at every epoch start, for each running replica, the maintenance of the
 //     moves made since the last one (periodic_maintenance,
 //     xdual_main.cpp:305-334): a re-inversion when the move count crossed a
@@ -434,12 +436,14 @@ struct DualModel {
                 const bool ok = d <= 0.0 || du53(W[1][3], W[2][2]) < orc_xexp(d * (-1.0 / R.T));
                 if (ok && commit(R, i, j, s, prow, qrow)) {
                     const double ns = R.score + d;
-                    const bool newbest = ns < R.best_score;
+                    const bool rec = d < 0.0 && ns < R.best_score && ns < gb_score;
+                    bool newbest = ns < R.best_score;
                     R.score = ns;
                     R.offnz += dnz;
-                    if (d < 0.0 && newbest && ns < gb_score) {
+                    if (rec) {
                         R.score = rescore(R);
                         ++R.recomputes;
+                        newbest = R.score < R.best_score;
                     }
                     if (newbest) {
                         R.best_score = R.score;

@@ -4,7 +4,8 @@ The hot path is HIP (xtax_amd/csrc/anneal.hip) behind the C ABI of
 include/xtx.h; this package is the Python host binding used by tests and the
 benchmark, and xtax_amd/xtax is the C++20 command-line drop-in.
 """
-from .engine import Engine, EngineParams, SolveResult, verify  # noqa: F401
+from .engine import DualParams, DualResult, Engine, EngineParams, SolveResult, verify  # noqa: F401
 from . import workloads  # noqa: F401
 
-__all__ = ["Engine", "EngineParams", "SolveResult", "verify", "workloads"]
+__all__ = ["DualParams", "DualResult", "Engine", "EngineParams", "SolveResult", "verify",
+           "workloads"]

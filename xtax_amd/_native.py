@@ -90,6 +90,24 @@ class Launch(C.Structure):
                 ("moves_per_sec", C.c_double)]
 
 
+class DualParams(C.Structure):
+    _fields_ = [("lambda_", C.c_double), ("dual_tol", C.c_double), ("lambda_ramp", C.c_double),
+                ("dual_refresh", C.c_int32), ("dual_check", C.c_int32)]
+
+
+class DualStats(C.Structure):
+    _fields_ = [
+        ("moves", C.c_int64), ("accepts", C.c_int64), ("greedy_calls", C.c_int64),
+        ("infeasible", C.c_int64), ("sweeps", C.c_int64), ("exchanges_tried", C.c_int64),
+        ("exchanges_accepted", C.c_int64), ("refreshes", C.c_int64), ("checks", C.c_int64),
+        ("recomputes", C.c_int64), ("solved", C.c_int32), ("solve_replica", C.c_int32),
+        ("solve_move", C.c_int64), ("best_score", C.c_double), ("start_score", C.c_double),
+        ("c", C.c_double), ("dual_residual", C.c_double), ("best_replica", C.c_int32),
+        ("epochs", C.c_int32), ("stop_reason", C.c_int32), ("verify_status", C.c_int32),
+        ("seconds", C.c_double), ("kernel_ms", C.c_double),
+    ]
+
+
 class LogEntry(C.Structure):
     _fields_ = [("move", C.c_int64), ("i", C.c_int16), ("j", C.c_int16), ("s", C.c_int16),
                 ("flags", C.c_uint16)]
@@ -127,6 +145,15 @@ SIGNATURES = {
     "xtx_reorder": (C.c_int, [I64P, I64P, C.c_int32, C.c_int32, I32P]),
     "xtx_is_unimodular": (C.c_int, [I64P, C.c_int32, I32P]),
     "xtx_deflate_pivot": (C.c_int, [I64P, I64P, C.c_int32, C.c_int32, I32P, C.c_int32, I32P]),
+    "xtx_dual_params_default": (None, [C.POINTER(DualParams)]),
+    "xtx_dual_solve": (C.c_int, [C.c_void_p, I64P, I64P, C.c_int32, C.POINTER(Params),
+                                 C.POINTER(DualParams), I32P, C.c_int32, I64P,
+                                 C.POINTER(C.c_double), I64P, C.POINTER(DualStats)]),
+    "xtx_dual_last_replicas": (C.c_int, [C.c_void_p, I64P, C.POINTER(C.c_double), I64P,
+                                         C.POINTER(C.c_double), I64P]),
+    "xtx_dual_probe": (C.c_int, [C.c_void_p, I64P, C.POINTER(C.c_double), C.c_int32, C.c_int32,
+                                 C.c_double, I32P, C.c_int32, I64P, C.POINTER(C.c_double), I64P,
+                                 C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "xtx_probe_moves": (C.c_int, [C.c_void_p, I64P, I64P, C.c_int32, C.c_int32, I32P,
                                   C.c_int32, I64P]),
     "xtx_probe_rng": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.c_int32,

@@ -10,6 +10,8 @@
 //   log        R * log_cap       accepted-move log per replica (optional)
 // During an epoch a replica's A lives in LDS (row stride P, odd) and its row
 // masses in registers (A returns to HBM at the epoch end); X^T stays in HBM.
+// An instance of the primal/dual objective adds Q, bQ (R * n * n double), gQ
+// (n * n) and a Gauss-Jordan work area (R * n * 2n double).
 #pragma once
 #include <stdint.h>
 
@@ -78,6 +80,15 @@ struct RepState {
     int64_t sweeps;
     int64_t log_count;
     int64_t solve_move;
+    // primal/dual objective (Inst::dual): the score is a double
+    double dscore;
+    double dlocal_best;
+    double dbest_score;
+    double dc;             // dual weight the running score was computed with
+    uint64_t m_maint;      // move count at the last maintenance
+    int64_t refreshes;     // dual re-inversions
+    int64_t checks;        // sampled residual checks
+    int64_t recomputes;    // running-score recomputations before a new best
 };
 
 struct InstParams {
@@ -127,7 +138,32 @@ struct Inst {
     int64_t solve_move;
     int64_t exch_tried;
     int64_t exch_acc;
+    // primal/dual objective (xdual's DualObjective): P lives in A/bA/gA, the
+    // double dual Q = P^-1 in Q/bQ/gQ (R * n * n, n * n), work holds one
+    // n x 2n Gauss-Jordan matrix per replica
+    int32_t dual;
+    int32_t dual_refresh;  // moves between re-inversions (0 = never)
+    int32_t dual_check;    // moves between residual checks (0 = never)
+    int32_t pad1;
+    double dual_tol;
+    double c;              // dual weight (normalized)
+    double gb_dscore;      // instance best score at the last epoch end
+    double* Q;
+    double* bQ;
+    double* gQ;
+    double* work;
 };
+
+// Order-preserving map of a double onto int64 (for a non-NaN x and y,
+// x < y <=> okey(x) < okey(y); -0 and +0 map to the same key).
+#ifdef __HIPCC__
+__host__ __device__
+#endif
+inline int64_t okey(double x) {
+    x = x + 0.0;
+    const int64_t b = __builtin_bit_cast(int64_t, x);
+    return b ^ ((b >> 63) & 0x7FFFFFFFFFFFFFFFll);
+}
 
 struct RepRef {
     int32_t inst;

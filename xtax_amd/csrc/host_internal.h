@@ -1,0 +1,276 @@
+// host_internal.h -- host-side pieces shared by the engine's translation
+// units (engine.cpp: plain, batched and served forms of the L1 objective;
+// dual.cpp: the primal/dual objective). Not part of the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <tuple>
+#include <vector>
+
+#include "engine_types.h"
+#include "xtx.h"
+
+namespace xtx {
+namespace host {
+
+using clk = std::chrono::steady_clock;
+
+struct Err {
+    int code;
+    std::string msg;
+};
+
+#define HIPCHK(expr)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            throw Err{XTX_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)}; \
+    } while (0)
+
+inline int64_t iabs(int64_t v) { return v < 0 ? -v : v; }
+
+// ---------------------------------------------------------------- layout
+// Launch configuration of an order bucket: padded order nmax (multiple of 8),
+// lane-group width G, columns per lane C (G C >= nmax), A as its upper
+// triangle or square in LDS, waves per workgroup (0: the params' choice) and
+// the LDS bytes of one replica. Every configuration computes the same
+// trajectory (the random stream and the arithmetic do not depend on it); it is
+// a performance choice, made by a rule or by xtx_autotune.
+struct Layout {
+    int G = 0, C = 0, nmax = 0, rep_bytes = 0;
+    bool tri = true;
+    int waves = 0;
+};
+
+inline int order_of(int n) { return std::max(8, (n + 7) / 8 * 8); }
+
+// A, 5 staged Philox blocks, active list, median histogram; X^T stays in HBM
+inline int rep_bytes_of(int nmax, int G, bool tri) {
+    const int64_t a = tri ? (int64_t)nmax * (nmax + 1) / 2 : (int64_t)nmax * (nmax + 1);
+    const int64_t bytes = a * 8 + 5 * 16 + nmax + 4 * med_slots(G);
+    return (int)((bytes + 15) / 16 * 16);
+}
+
+inline Layout make_layout(int nmax, int G, int C, bool tri, int waves) {
+    Layout L;
+    L.nmax = nmax;
+    L.G = G;
+    L.C = C;
+    L.tri = tri;
+    L.waves = waves;
+    L.rep_bytes = rep_bytes_of(nmax, G, tri);
+    return L;
+}
+
+// the rule used when a bucket has not been tuned
+inline Layout layout_for(int n) {
+    const int nmax = order_of(n);
+    if (nmax <= 16) return make_layout(nmax, 16, 1, true, 0);
+    if (nmax <= 32) return make_layout(nmax, 32, 1, true, 0);
+    if (nmax <= 64) return make_layout(nmax, 64, 1, true, 0);
+    return make_layout(nmax, 64, 2, true, 0);
+}
+
+using BucketKey = std::tuple<int, int, int, bool, int>;
+inline BucketKey key_of(const Layout& L) { return {L.nmax, L.G, L.C, L.tri, L.waves}; }
+
+constexpr int kLdsPerCu = 160 * 1024;
+// Largest LDS one block may use on the open device (set by xtx_open).
+extern int g_lds_block;
+
+// ---------------------------------------------------------------- arena
+// First-fit free list over one device allocation (256-byte granules).
+class Arena {
+  public:
+    ~Arena() { release(); }
+    void reserve(size_t bytes) {
+        release();
+        bytes = (bytes + 255) / 256 * 256;
+        HIPCHK(hipMalloc(&base_, bytes));
+        size_ = bytes;
+        free_.clear();
+        free_[0] = bytes;
+    }
+    void release() {
+        if (base_) (void)hipFree(base_);
+        base_ = nullptr;
+        size_ = 0;
+        free_.clear();
+    }
+    char* alloc(size_t bytes) {
+        bytes = (bytes + 255) / 256 * 256;
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= bytes) {
+                const size_t off = it->first, len = it->second;
+                free_.erase(it);
+                if (len > bytes) free_[off + bytes] = len - bytes;
+                return base_ + off;
+            }
+        }
+        return nullptr;
+    }
+    void free(char* p, size_t bytes) {
+        bytes = (bytes + 255) / 256 * 256;
+        size_t off = (size_t)(p - base_);
+        auto next = free_.lower_bound(off);
+        if (next != free_.begin()) {
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == off) {
+                off = prev->first;
+                bytes += prev->second;
+                free_.erase(prev);
+            }
+        }
+        next = free_.lower_bound(off);
+        if (next != free_.end() && off + bytes == next->first) {
+            bytes += next->second;
+            free_.erase(next);
+        }
+        free_[off] = bytes;
+    }
+    size_t size() const { return size_; }
+    bool empty() const { return base_ == nullptr; }
+
+  private:
+    char* base_ = nullptr;
+    size_t size_ = 0;
+    std::map<size_t, size_t> free_;
+};
+
+// ---------------------------------------------------------------- instance
+// Host-side description of one instance: validated inputs, start state,
+// ladder, and where its device buffers live.
+struct HostInst {
+    int n = 0, N = 0, R = 0;
+    uint64_t seed = 0;
+    std::vector<int64_t> A0, Astart, Xt0;  // original input, X0^T A0 X0, X0^T
+    std::vector<int32_t> act;
+    int64_t score0 = 0, offnz0 = 0;
+    double t_init = 0;
+    std::vector<double> temps;
+    InstParams P{};
+    bool log_on = false;
+    int64_t log_cap = 0;
+    Layout lay{};
+    size_t bytes = 0;
+    char* dmem = nullptr;
+    Inst rec{};
+    int64_t* A_out = nullptr;
+    int64_t* X_out = nullptr;
+    uint64_t tag = 0;
+    double submit_s = 0, admit_s = 0;
+};
+
+int64_t score_of(const int64_t* A, int n);
+int64_t offnz_of(const int64_t* A, int n);
+void check_params(const xtx_params& p);
+// Validates the inputs and builds the start state A_start = X0^T A0 X0
+// exactly, the active list, the L1 score and temperature setup, and the
+// instance parameters.
+void prepare(HostInst& h, const int64_t* A0, const int64_t* X0, int n, int R, uint64_t seed,
+             const int32_t* active, int n_active, const xtx_params& p);
+// geometric ladder from min(t_min, top) to top (congruence_anneal.hpp:454-468)
+std::vector<double> ladder(double top, double t_min, int R);
+// per-replica greedy/target fractions (congruence_anneal.hpp:200-207)
+void fractions(const xtx_params& p, int r, int R, double* greedy, double* target);
+size_t align256(size_t b);
+
+// Watchdog: raises the stop flag at a wall-clock deadline (the reference's
+// monitor-thread max_seconds check, congruence_anneal.hpp:512-519).
+class Deadline {
+  public:
+    Deadline(int* flag, double seconds) {
+        if (!(seconds > 0.0)) return;
+        th_ = std::thread([this, flag, seconds] {
+            std::unique_lock<std::mutex> lk(m_);
+            if (!cv_.wait_for(lk, std::chrono::duration<double>(seconds), [this] { return done_; })) {
+                fired_ = true;
+                *(volatile int*)flag = 1;
+            }
+        });
+    }
+    ~Deadline() {
+        if (th_.joinable()) {
+            {
+                std::lock_guard<std::mutex> lk(m_);
+                done_ = true;
+            }
+            cv_.notify_all();
+            th_.join();
+        }
+    }
+    bool fired() const { return fired_; }
+
+  private:
+    std::thread th_;
+    std::mutex m_;
+    std::condition_variable cv_;
+    bool done_ = false;
+    std::atomic<bool> fired_{false};
+};
+
+// Host copy of the last primal/dual solve (dual.cpp).
+struct DualLast {
+    bool valid = false;
+    double c = 0.0;
+    std::vector<double> Q0;
+    std::vector<RepState> rs;
+};
+
+}  // namespace host
+}  // namespace xtx
+
+struct Server;
+
+
+constexpr int kSideStreams = 3;
+
+struct xtx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // buckets of one epoch run concurrently: forked from `stream` onto these
+    // (default priority) and joined back before the epoch end
+    hipStream_t side[kSideStreams] = {};
+    hipEvent_t fork_ev = nullptr;
+    hipEvent_t join_ev[kSideStreams] = {};
+    int* stop_host = nullptr;  // pinned, mapped
+    int* stop_dev = nullptr;
+    std::string err;
+    hipDeviceProp_t prop{};
+    // last plain solve
+    xtx::host::Arena plain_arena;
+    xtx::host::HostInst plain;
+    xtx::Inst* d_plain = nullptr;
+    std::vector<xtx::RepState> plain_rs;
+    std::vector<xtx::LogEnt> plain_log;
+    std::vector<int64_t> plain_log_count;
+    bool plain_log_valid = false;
+    std::unique_ptr<Server> server;
+    // launch configuration per padded order, set by xtx_launch_set or
+    // xtx_autotune; other orders use layout_for's rule
+    std::map<int, xtx::host::Layout> tuned;
+    // last solve of the primal/dual objective (its instance is `plain`)
+    xtx::host::DualLast dual;
+};
+
+namespace xtx {
+namespace host {
+
+inline int fail(xtx_ctx* ctx, const Err& e) {
+    if (ctx) ctx->err = e.msg;
+    return e.code;
+}
+
+}  // namespace host
+}  // namespace xtx

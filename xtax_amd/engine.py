@@ -57,6 +57,23 @@ class EngineParams:
         return p
 
 
+@dataclass
+class DualParams:
+    """xdual's Params beyond the engine's (src/xdual_main.cpp:366-373)."""
+    lambda_: float = 1.0
+    dual_refresh: int = 200000
+    dual_check: int = 25000
+    dual_tol: float = 1e-6
+    lambda_ramp: float = 0.0
+
+    def to_c(self):
+        d = N.DualParams()
+        N.lib().xtx_dual_params_default(C.byref(d))
+        for f in dataclasses.fields(self):
+            setattr(d, f.name, getattr(self, f.name))
+        return d
+
+
 def _i64(a):
     return np.ascontiguousarray(a, dtype=np.int64)
 
@@ -65,9 +82,21 @@ def _p64(a):
     return a.ctypes.data_as(N.I64P) if a is not None else None
 
 
+def _pd(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
+
+
 @dataclass
 class SolveResult:
     A: np.ndarray
+    X: np.ndarray
+    stats: dict = field(default_factory=dict)
+
+
+@dataclass
+class DualResult:
+    P: np.ndarray
+    Q: np.ndarray
     X: np.ndarray
     stats: dict = field(default_factory=dict)
 
@@ -155,6 +184,61 @@ class Engine:
             return np.zeros(0, dt), bool(tr.value)
         buf = C.string_at(ptr, cnt.value * 16)
         return np.frombuffer(buf, dtype=dt).copy(), bool(tr.value)
+
+    # -- primal/dual objective (xdual) --------------------------------------
+    def run_xdual(self, A0, X0=None, params=None, dual=None, active=None,
+                  allow_unverified=False):
+        """run_xdual (xdual_main.cpp:378-424): anneal P = X0^T A0 X0 and its
+        double dual Q = P^-1 under |offdiag P|^2 + c |offdiag Q|^2; returns
+        DualResult(P, Q, X, stats)."""
+        params = params or EngineParams()
+        dual = dual or DualParams()
+        A0 = _i64(A0)
+        n = A0.shape[0]
+        X0 = _i64(X0) if X0 is not None else None
+        act = np.ascontiguousarray(active, dtype=np.int32) if active is not None else None
+        P = np.zeros((n, n), np.int64)
+        Q = np.zeros((n, n), np.float64)
+        X = np.zeros((n, n), np.int64)
+        st = N.DualStats()
+        p, d = params.to_c(), dual.to_c()
+        rc = self._lib.xtx_dual_solve(
+            self._h, _p64(A0), _p64(X0), n, C.byref(p), C.byref(d),
+            act.ctypes.data_as(N.I32P) if act is not None else None,
+            len(act) if act is not None else 0, _p64(P), _pd(Q), _p64(X), C.byref(st))
+        if rc == -7 and allow_unverified:
+            return DualResult(P, Q, X, _stats(st))
+        self._check(rc)
+        return DualResult(P, Q, X, _stats(st))
+
+    def dual_last_replicas(self, R, n):
+        P = np.zeros((R, n, n), np.int64)
+        Q = np.zeros((R, n, n), np.float64)
+        Xt = np.zeros((R, n, n), np.int64)
+        dsc = np.zeros((R, 4), np.float64)
+        sc = np.zeros((R, 8), np.int64)
+        self._check(self._lib.xtx_dual_last_replicas(self._h, _p64(P), _pd(Q), _p64(Xt),
+                                                     _pd(dsc), _p64(sc)))
+        return P, Q, Xt, dsc, sc
+
+    def probe_dual(self, P, Q, c, queries):
+        """Device primal/dual primitives: (out_i[q, 3] = {feasible, dnz,
+        best_shear}, delta[q], P rows[q, n], Q rows[q, n], score[q])."""
+        P = _i64(P)
+        Q = np.ascontiguousarray(Q, dtype=np.float64)
+        if P.ndim == 2:
+            P, Q = P[None], Q[None]
+        n = P.shape[1]
+        q = np.ascontiguousarray(queries, dtype=np.int32).reshape(-1, 4)
+        oi = np.zeros((len(q), 3), np.int64)
+        od = np.zeros(len(q))
+        rp = np.zeros((len(q), n), np.int64)
+        rq = np.zeros((len(q), n))
+        sc = np.zeros(len(q))
+        self._check(self._lib.xtx_dual_probe(self._h, _p64(P), _pd(Q), n, P.shape[0], float(c),
+                                             q.ctypes.data_as(N.I32P), len(q), _p64(oi), _pd(od),
+                                             _p64(rp), _pd(rq), _pd(sc)))
+        return oi, od, rp, rq, sc
 
     # -- batched form -----------------------------------------------------
     def batch_solve(self, mats, params=None, seeds=None, replicas=None):
