@@ -323,6 +323,8 @@ typedef struct {
     double best_score;        /* |offdiag P|^2 + c |offdiag Q|^2 of the returned
                                  state at the full weight c */
     double start_score;       /* the same of the start state */
+    double start_primal;      /* |offdiag P|^2 of the start state */
+    double start_dual;        /* |offdiag Q|^2 of the start state */
     double c;                 /* normalized dual weight */
     double dual_residual;     /* max |P Q - I| of the returned state */
     int32_t best_replica;

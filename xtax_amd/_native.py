@@ -102,6 +102,7 @@ class DualStats(C.Structure):
         ("exchanges_accepted", C.c_int64), ("refreshes", C.c_int64), ("checks", C.c_int64),
         ("recomputes", C.c_int64), ("solved", C.c_int32), ("solve_replica", C.c_int32),
         ("solve_move", C.c_int64), ("best_score", C.c_double), ("start_score", C.c_double),
+        ("start_primal", C.c_double), ("start_dual", C.c_double),
         ("c", C.c_double), ("dual_residual", C.c_double), ("best_replica", C.c_int32),
         ("epochs", C.c_int32), ("stop_reason", C.c_int32), ("verify_status", C.c_int32),
         ("seconds", C.c_double), ("kernel_ms", C.c_double),

@@ -213,6 +213,8 @@ int xtx_dual_solve(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t n
         st.verify_status = -1;
         st.c = c_full;
         st.start_score = dual_score(h.Astart.data(), Q0.data(), n, c_full);
+        st.start_primal = dual_score(h.Astart.data(), Q0.data(), n, 0.0);
+        st.start_dual = (double)sq0;
         st.solve_replica = -1;
         auto finish_trivial = [&]() {
             if (P_out) std::memcpy(P_out, h.Astart.data(), nn * 8);

@@ -31,171 +31,12 @@
 
 #include <unistd.h>
 
+#include "cli_common.h"
 #include "xtx.h"
 
 namespace {
 
-std::atomic<bool> g_interrupted{false};
-xtx_ctx* g_ctx = nullptr;
-
-const char kInterruptMsg[] =
-    "\n[interrupt received, finishing and writing the best result so far...]\n";
-
-// First Ctrl-C: stop flag + one acknowledgement; later ones are swallowed
-// (stop_signal.hpp:52-64).
-void on_sigint(int) {
-    if (!g_interrupted.exchange(true)) {
-        if (g_ctx) xtx_request_stop(g_ctx);
-        ssize_t r = write(STDERR_FILENO, kInterruptMsg, sizeof(kInterruptMsg) - 1);
-        (void)r;
-    }
-}
-
-struct Mat {
-    int rows = 0, cols = 0;
-    std::vector<int64_t> v;
-    int64_t& at(int r, int c) { return v[(size_t)r * cols + c]; }
-    int64_t at(int r, int c) const { return v[(size_t)r * cols + c]; }
-};
-
-// CSV reader with the reference's rules (mat_io.hpp:432-489): blank lines
-// skipped, comma-separated std::stoll tokens, equal row lengths.
-Mat read_csv(const std::string& path, bool square) {
-    std::ifstream in(path);
-    if (!in) throw std::runtime_error("Cannot open " + path);
-    Mat M;
-    std::string line;
-    while (std::getline(in, line)) {
-        if (line.empty()) continue;
-        std::stringstream ss(line);
-        std::string tok;
-        int cols = 0;
-        while (std::getline(ss, tok, ',')) {
-            M.v.push_back((int64_t)std::stoll(tok));
-            ++cols;
-        }
-        if (M.rows == 0) M.cols = cols;
-        if (cols != M.cols) throw std::runtime_error("Bad CSV row length in " + path);
-        ++M.rows;
-    }
-    if (M.rows == 0)
-        throw std::runtime_error(std::string(square ? "Empty matrix file: " : "Empty lattice file: ") +
-                                 path);
-    if (square && M.rows != M.cols) throw std::runtime_error("Matrix must be square in " + path);
-    return M;
-}
-
-// Atomic replace: write file.tmp, then rename (mat_io.hpp:389-402).
-void write_csv(const Mat& M, const std::string& path) {
-    const std::string tmp = path + ".tmp";
-    {
-        std::ofstream out(tmp);
-        if (!out) throw std::runtime_error("Failed to open output file: " + tmp);
-        for (int r = 0; r < M.rows; ++r) {
-            for (int c = 0; c < M.cols; ++c) {
-                out << M.at(r, c);
-                if (c + 1 < M.cols) out << ',';
-            }
-            out << '\n';
-        }
-        out.flush();
-        if (!out) throw std::runtime_error("Failed while writing: " + tmp);
-    }
-    std::error_code ec;
-    std::filesystem::rename(tmp, path, ec);
-    if (ec) throw std::runtime_error("Failed to replace " + path + ": " + ec.message());
-}
-
-void print_mat(const Mat& M) {
-    for (int r = 0; r < M.rows; ++r) {
-        for (int c = 0; c < M.cols; ++c) {
-            std::cout << M.at(r, c);
-            if (c + 1 < M.cols) std::cout << ',';
-        }
-        std::cout << '\n';
-    }
-}
-
-// --modulus (mat_io.hpp:75-88): balanced residues in (-m/2, m/2]; m = 1
-// saturates nonzero entries to 1.
-void reduce_mod(std::vector<int64_t>& v, uint64_t m) {
-    if (m == 1) {
-        for (auto& x : v) x = x != 0;
-        return;
-    }
-    const int64_t mm = (int64_t)m, half = mm / 2;
-    for (auto& x : v) {
-        int64_t r = x % mm;
-        if (r < 0) r += mm;
-        if (r > half) r -= mm;
-        x = r;
-    }
-}
-
-std::string modulus_note(uint64_t m) {
-    if (m == 1) return "saturated entries (nonzero -> 1, 0 stays 0)";
-    return "reduced entries mod " + std::to_string(m) + " (balanced residues in (-m/2, m/2])";
-}
-
-// Index lists "0,3..6,9" (mat_io.hpp:97-141): sorted, deduplicated.
-std::vector<int> parse_index_spec(const std::string& spec, int n, const std::string& opt) {
-    auto trim = [](std::string s) {
-        const auto b = s.find_first_not_of(" \t");
-        if (b == std::string::npos) return std::string();
-        const auto e = s.find_last_not_of(" \t");
-        return s.substr(b, e - b + 1);
-    };
-    auto index = [&](const std::string& tok) -> int {
-        size_t pos = 0;
-        long long v = -1;
-        try {
-            v = std::stoll(tok, &pos);
-        } catch (const std::exception&) {
-            pos = 0;
-        }
-        if (tok.empty() || pos != tok.size() || v < 0)
-            throw std::runtime_error(opt + ": invalid index '" + tok + "'");
-        if (v >= n)
-            throw std::runtime_error(opt + ": index " + tok + " out of range (valid indices are 0.." +
-                                     std::to_string(n - 1) + ")");
-        return (int)v;
-    };
-    std::vector<int> out;
-    std::stringstream ss(spec);
-    std::string tok;
-    while (std::getline(ss, tok, ',')) {
-        tok = trim(tok);
-        if (tok.empty()) continue;
-        const size_t dots = tok.find("..");
-        if (dots == std::string::npos) {
-            out.push_back(index(tok));
-        } else {
-            const int lo = index(trim(tok.substr(0, dots)));
-            const int hi = index(trim(tok.substr(dots + 2)));
-            if (lo > hi) throw std::runtime_error(opt + ": reversed range '" + tok + "'");
-            for (int r = lo; r <= hi; ++r) out.push_back(r);
-        }
-    }
-    std::sort(out.begin(), out.end());
-    out.erase(std::unique(out.begin(), out.end()), out.end());
-    return out;
-}
-
-Mat gram(const Mat& L, const std::vector<int>& dims) {
-    Mat G;
-    G.rows = G.cols = L.rows;
-    G.v.assign((size_t)L.rows * L.rows, 0);
-    for (int i = 0; i < L.rows; ++i)
-        for (int j = i; j < L.rows; ++j) {
-            int64_t s = 0;
-            if (dims.empty())
-                for (int k = 0; k < L.cols; ++k) s += L.at(i, k) * L.at(j, k);
-            else
-                for (int k : dims) s += L.at(i, k) * L.at(j, k);
-            G.at(i, j) = G.at(j, i) = s;
-        }
-    return G;
-}
+using namespace cli;
 
 int64_t score_of(const Mat& A) {
     int64_t full = 0, diag = 0;
@@ -209,44 +50,6 @@ int64_t offnz_of(const Mat& A) {
     for (int i = 0; i < A.rows; ++i)
         for (int j = i + 1; j < A.rows; ++j) c += A.at(i, j) != 0;
     return c;
-}
-
-Mat transposed(const Mat& M) {
-    Mat T;
-    T.rows = M.cols;
-    T.cols = M.rows;
-    T.v.assign(M.v.size(), 0);
-    for (int r = 0; r < M.rows; ++r)
-        for (int c = 0; c < M.cols; ++c) T.at(c, r) = M.at(r, c);
-    return T;
-}
-
-Mat square(int n) {
-    Mat M;
-    M.rows = M.cols = n;
-    M.v.assign((size_t)n * n, 0);
-    return M;
-}
-
-// X^T A X (entries within the 2^48 bound: the engine has checked the start
-// state), accumulated in 128 bits
-Mat congruence(const Mat& A, const Mat& X) {
-    const int n = A.rows;
-    std::vector<__int128> AX((size_t)n * n, 0);
-    for (int r = 0; r < n; ++r)
-        for (int k = 0; k < n; ++k) {
-            const int64_t a = A.at(r, k);
-            if (!a) continue;
-            for (int c = 0; c < n; ++c) AX[(size_t)r * n + c] += (__int128)a * X.at(k, c);
-        }
-    Mat C = square(n);
-    for (int r = 0; r < n; ++r)
-        for (int c = 0; c < n; ++c) {
-            __int128 v = 0;
-            for (int k = 0; k < n; ++k) v += (__int128)X.at(k, r) * AX[(size_t)k * n + c];
-            C.at(r, c) = (int64_t)v;
-        }
-    return C;
 }
 
 constexpr int64_t kDeflateOffdiagCap = 1 << 13;  // main.cpp:30
@@ -346,13 +149,6 @@ bool solve_with_deflation(xtx_ctx* ctx, const Mat& A, const xtx_params& params, 
     return true;
 }
 
-// ---- minimal flag parser for the reference's option surface ----
-struct Opt {
-    std::vector<std::string> names;
-    bool is_flag;
-    std::function<void(const std::string&)> set;
-};
-
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -364,87 +160,24 @@ int main(int argc, char** argv) {
     int device = 0;
     bool rcm = false, centroid = false, deflate = false, deflate_blocks = false, verbose = false;
     double deflate_slice = 0.5;
-    auto D = [](double& d) { return [&d](const std::string& s) { d = std::stod(s); }; };
-    auto I = [](int32_t& d) { return [&d](const std::string& s) { d = (int32_t)std::stol(s); }; };
-    auto S = [](std::string& d) { return [&d](const std::string& s) { d = s; }; };
     std::vector<Opt> opts = {
-        {{"-A"}, false, S(a_csv)},
-        {{"-L", "--lattice"}, false, S(l_csv)},
-        {{"-X"}, false, S(x_csv)},
-        {{"--modulus"}, false, [&](const std::string& s) {
-             const long long v = std::stoll(s);
-             if (v < 1) throw std::runtime_error("--modulus: value must be >= 1");
-             modulus = (uint64_t)v;
-         }},
-        {{"--gram-rows"}, false, S(gram_rows)},
-        {{"--lattice-dims"}, false, S(lattice_dims)},
-        {{"-t", "--threads", "--replicas"}, false, I(p.replicas)},
-        {{"--use-hyperthreads"}, true, [](const std::string&) {}},
-        {{"--no-pin"}, true, [](const std::string&) {}},
-        {{"--seed"}, false, [&](const std::string& s) { p.seed = std::stoull(s); }},
-        {{"--stuck-threshold"}, false, I(p.stuck_threshold)},
-        {{"--t-init"}, false, D(p.t_init)},
-        {{"--t-min"}, false, D(p.t_min)},
-        {{"--cooling"}, false, D(p.cooling)},
-        {{"--moves-per-cool"}, false, I(p.moves_per_cool)},
-        {{"--reheat"}, false, D(p.reheat)},
-        {{"--greedy-fraction"}, false, D(p.greedy_fraction)},
-        {{"--target-fraction"}, false, D(p.target_fraction)},
-        {{"--target-samples"}, false, I(p.target_samples)},
-        {{"--reseed-factor"}, false, D(p.reseed_factor)},
-        {{"--sweep-fraction"}, false, D(p.sweep_fraction)},
-        {{"--tempering"}, true, [&](const std::string&) { p.tempering = 1; }},
-        {{"--no-tempering"}, true, [&](const std::string&) { p.tempering = 0; }},
-        {{"--exchange-interval"}, false, I(p.exchange_interval)},
-        {{"--adaptive-cooling"}, true, [&](const std::string&) { p.adaptive_cooling = 1; }},
-        {{"--worker-diversity"}, false, D(p.worker_diversity)},
-        {{"--max-seconds"}, false, D(p.max_seconds)},
-        {{"--save-interval"}, false, D(p.save_interval)},
+        {{"-A"}, false, set_string(a_csv)},
+        {{"-L", "--lattice"}, false, set_string(l_csv)},
+        {{"-X"}, false, set_string(x_csv)},
+        {{"--modulus"}, false, set_modulus(modulus)},
+        {{"--gram-rows"}, false, set_string(gram_rows)},
+        {{"--lattice-dims"}, false, set_string(lattice_dims)},
         {{"--rcm"}, true, [&](const std::string&) { rcm = true; }},
         {{"--centroid"}, true, [&](const std::string&) { centroid = true; }},
         {{"--deflate"}, true, [&](const std::string&) { deflate = true; }},
         {{"--deflate-blocks"}, true, [&](const std::string&) { deflate_blocks = true; }},
-        {{"--deflate-slice"}, false, D(deflate_slice)},
+        {{"--deflate-slice"}, false, set_double(deflate_slice)},
         {{"--verbose"}, true, [&](const std::string&) { verbose = true; }},
-        {{"--device"}, false, I(device)},
-        {{"--max-moves"}, false, [&](const std::string& s) { p.max_moves = std::stoull(s); }},
-        {{"--max-epochs"}, false, I(p.max_epochs)},
     };
-    try {
-        for (int a = 1; a < argc; ++a) {
-            std::string arg = argv[a], val;
-            const size_t eq = arg.find('=');
-            bool has_eq = false;
-            if (arg.rfind("--", 0) == 0 && eq != std::string::npos) {
-                val = arg.substr(eq + 1);
-                arg = arg.substr(0, eq);
-                has_eq = true;
-            }
-            if (arg == "-h" || arg == "--help") {
-                std::cout << "xtax (gfx950 engine): see the reference README for the options\n";
-                return 0;
-            }
-            bool found = false;
-            for (const auto& o : opts) {
-                if (std::find(o.names.begin(), o.names.end(), arg) == o.names.end()) continue;
-                found = true;
-                if (o.is_flag) {
-                    o.set("");
-                } else {
-                    if (!has_eq) {
-                        if (a + 1 >= argc) throw std::runtime_error(arg + " requires an argument");
-                        val = argv[++a];
-                    }
-                    o.set(val);
-                }
-                break;
-            }
-            if (!found) throw std::runtime_error("The following argument was not expected: " + arg);
-        }
-    } catch (const std::exception& e) {
-        std::cerr << e.what() << "\nRun with --help for more information.\n";
-        return 109;  // CLI11's parse-error exit code
-    }
+    for (auto& o : engine_options(p, device)) opts.push_back(std::move(o));
+    const int rc_args =
+        parse_args(argc, argv, opts, "xtax (gfx950 engine): see the reference README for the options\n");
+    if (rc_args >= 0) return rc_args;
     if (p.replicas < 1) p.replicas = 1;
 
     const bool have_A = !a_csv.empty(), have_L = !l_csv.empty();
@@ -593,16 +326,7 @@ int main(int argc, char** argv) {
             write_csv(bA, "best_A.csv");
         } else {
             // final basis X^T L and the Gram consistency check (main.cpp:850-857)
-            Mat Lf;
-            Lf.rows = Lat.rows;
-            Lf.cols = Lat.cols;
-            Lf.v.assign(Lat.v.size(), 0);
-            for (int i = 0; i < Lat.rows; ++i)
-                for (int q = 0; q < Lat.rows; ++q) {
-                    const int64_t x = bX.at(q, i);
-                    if (!x) continue;
-                    for (int k = 0; k < Lat.cols; ++k) Lf.at(i, k) += x * Lat.at(q, k);
-                }
+            const Mat Lf = xt_times(bX, Lat);
             const Mat g2 = gram(Lf, dims);
             if (g2.v != bA.v)
                 std::cout << "[warn] Gram(X^T L) != annealed Gram (transform mismatch, possible "
