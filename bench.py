@@ -28,6 +28,9 @@ Other configurations (`--workload ...`), same line format:
   lattice  24x24 lattice Gram (entries U[-9, 9]), 256 replicas, greedy 0.5,
            sweep 0.05
   random   n = 10 random symmetric (U[-20, 20]), 8 replicas, T in [0.5, 50]
+  dual     xdual's primal/dual objective (xtx_dual_solve) on the lattice
+           workload's Gram, 4096 replicas, greedy 0.5; CPU baseline the
+           reference's xdual binary (oracle/_ref/xdual_ref)
 """
 import argparse
 import json
@@ -42,9 +45,10 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from xtax_amd import Engine, EngineParams, workloads  # noqa: E402
+from xtax_amd import DualParams, Engine, EngineParams, workloads  # noqa: E402
 
 REF_BIN = os.path.join(ROOT, "oracle", "_ref", "xtax_ref")
+REF_DUAL_BIN = os.path.join(ROOT, "oracle", "_ref", "xdual_ref")
 # 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz: int32 VALU lane-ops/s of an MI355X
 MI355X_INT_LANE_OPS = 256 * 64 * 2.4e9
 METRIC = "exact-diagonalized matrices/sec + p50/p99 solve latency; shear moves/sec"
@@ -85,13 +89,13 @@ def roofline(kernel_ms, moves, mean_n, greedy):
                     "touched only for X^T rows and epoch boundaries"}
 
 
-def run_ref(A, args_list, timeout):
+def run_ref(A, args_list, timeout, binary=REF_BIN):
     with tempfile.TemporaryDirectory() as td:
         path = os.path.join(td, "A.csv")
         with open(path, "w") as f:
             for row in A:
                 f.write(",".join(str(int(v)) for v in row) + "\n")
-        cp = subprocess.run([REF_BIN, "-A", path] + args_list, cwd=td, capture_output=True,
+        cp = subprocess.run([binary, "-A", path] + args_list, cwd=td, capture_output=True,
                             text=True, timeout=timeout)
     moves = 0
     for line in cp.stdout.splitlines():
@@ -100,7 +104,7 @@ def run_ref(A, args_list, timeout):
     best = None
     for line in cp.stdout.splitlines():
         if line.startswith("Final best score:"):
-            best = int(line.split(":")[1])
+            best = float(line.split(":")[1])
     return "matrix is diagonal" in cp.stdout, moves, best
 
 
@@ -351,6 +355,57 @@ def bench_random(args, eng):
     return out
 
 
+def dual_roofline(kernel_ms, moves, n, greedy):
+    """Algorithmic VALU work of k_anneal_dual over its measured time. Per
+    proposal and column: the primal Add (multiply, add, bound check, nonzero
+    change, the two exact factors: 5 int64 ops, 2 conversions and a product),
+    the dual Add (shear product, add, two squares, difference: 5 fp64 ops) and
+    the two column sums; with probability `greedy` the best shear's four
+    products (2 conversions) and four sums. An int64 op counts 2 int32
+    lane-ops, an fp64 add/mul 1 (gfx950 issues vector fp64 at the int32 rate)."""
+    if not kernel_ms:
+        return None
+    per_col = 5 * 2 + 3 + 5 + 2 + greedy * (4 + 2 + 4)
+    achieved = moves * per_col * n / (kernel_ms / 1e3) / 1e12
+    peak = MI355X_INT_LANE_OPS / 1e12
+    return {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TOP/s",
+            "frac": achieved / peak, "traffic": None, "kernel": "k_anneal_dual",
+            "kernel_ms": kernel_ms, "moves": moves,
+            "note": "int64 + fp64 VALU issue bound; P and Q live in LDS, HBM is touched for "
+                    "X^T rows and epoch boundaries"}
+
+
+def bench_dual(args, eng):
+    _, G = workloads.lattice(24, 24, seed=2)
+    n = G.shape[0]
+    p = EngineParams(replicas=4096, greedy_fraction=0.5, exchange_interval=args.exchange_interval,
+                     max_seconds=args.seconds, seed=args.seed, verify=True)
+    t0 = time.perf_counter()
+    r = eng.run_xdual(G, None, p, DualParams())
+    el = time.perf_counter() - t0
+    st = r.stats
+    cfg = {"workload": "dual: xdual objective on the 24x24 lattice Gram (basis U[-9,9]), 4096 "
+                       "replicas, greedy 0.5, lambda 1", "n": n, "replicas": 4096}
+    out = base_line(args, st["moves"] / el, "moves/s", el, cfg)
+    out["dtype"] = "int64+f64"
+    out.update({"solved": st["solved"], "best_score": st["best_score"],
+                "start_score": st["start_score"], "c": st["c"], "epochs": st["epochs"],
+                "refreshes": st["refreshes"], "dual_residual": st["dual_residual"],
+                "verify_status": st["verify_status"],
+                "kernel_moves_per_sec": st["moves"] / (st["kernel_ms"] / 1e3),
+                "roofline": dual_roofline(st["kernel_ms"], st["moves"], n, 0.5),
+                "cpu_baseline": None})
+    if not args.no_cpu and os.path.exists(REF_DUAL_BIN):
+        cores = os.cpu_count() or 1
+        ok, mv, best = run_ref(G, ["-t", str(cores), "--seed", "1", "--max-seconds",
+                                   str(args.cpu_cap)], timeout=args.cpu_cap + 30,
+                               binary=REF_DUAL_BIN)
+        out["cpu_baseline"] = {"value": mv / args.cpu_cap, "unit": "moves/s", "cores": cores,
+                               "kind": "reference", "best_score": best, "solved": ok,
+                               "sample": f"xdual_ref -t {cores} --max-seconds {args.cpu_cap}"}
+    return out
+
+
 def bench_copy(args, eng):
     """Calibration for the HBM traffic counters: --count device-to-device
     copies of 256 MiB (one copy kernel each), so FETCH_SIZE / WRITE_SIZE per
@@ -381,7 +436,8 @@ def main():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--workload", default="batch",
-                    choices=["batch", "plain", "serve", "large", "lattice", "random", "copy"])
+                    choices=["batch", "plain", "serve", "large", "lattice", "random", "dual",
+                             "copy"])
     ap.add_argument("--count", type=int, default=None)
     ap.add_argument("--replicas", type=int, default=32)
     ap.add_argument("--t-init", type=float, default=200.0)
@@ -413,7 +469,7 @@ def main():
     warm(eng)
     fn = {"batch": bench_batch, "plain": bench_plain, "serve": bench_serve,
           "large": bench_large, "lattice": bench_lattice, "random": bench_random,
-          "copy": bench_copy}[args.workload]
+          "dual": bench_dual, "copy": bench_copy}[args.workload]
     out = fn(args, eng)
     eng.close()
     print(json.dumps(out), flush=True)

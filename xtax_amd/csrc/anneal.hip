@@ -992,6 +992,8 @@ __global__ void __launch_bounds__(256, 4)
 // owner[q] is the replica on rung q; rungs q and q+1 swap when
 // arg = (1/T_q - 1/T_{q+1}) (S_a - S_b) >= 0 or u < exp(arg), sweeping q
 // upward so a configuration can travel several rungs in one sweep.
+constexpr int kSweepChunk = 512;
+
 __global__ void __launch_bounds__(256) k_epoch_end(Inst* __restrict__ insts) {
     extern __shared__ __align__(16) unsigned char smem[];
     Inst* I = insts + blockIdx.x;
@@ -1092,36 +1094,104 @@ __global__ void __launch_bounds__(256) k_epoch_end(Inst* __restrict__ insts) {
         if (t == 0) I->gb_offnz = s_bs[0];
     }
 
-    // exchange sweep
+    // exchange sweep: the configuration carried upward meets, at each rung
+    // q + 1, the replica that started the sweep there, so everything the
+    // sequential decisions read except the carried score is fixed in advance;
+    // it is staged into LDS in chunks of kSweepChunk rungs by all threads and
+    // thread 0 runs the decisions from LDS
     if (I->P.tempering && R >= 2) {
-        int* owner = (int*)smem;
-        double* u = (double*)(smem + ((size_t)R * sizeof(int) + 7) / 8 * 8);
+        int* owner = (int*)smem;  // rung -> replica; rungs below the chunk: final
+        double* s_sb = (double*)(smem + ((size_t)R * sizeof(int) + 7) / 8 * 8);
+        double* s_db = s_sb + kSweepChunk;
+        double* s_u = s_db + kSweepChunk;
+        double* s_lu = s_u + kSweepChunk;
+        __shared__ int s_carry;
+        __shared__ double s_sa;
+        __shared__ int64_t s_acc;
+        auto score_of = [&](int x) {
+            return dual ? I->rs[x].dscore : (double)I->rs[x].score;
+        };
         for (int r = t; r < R; r += blockDim.x) owner[I->rs[r].rung] = r;
-        const uint32_t e = I->epoch;
-        for (int q = t; q < R - 1; q += blockDim.x) {
-            const uint4 w = philox10(e, 0u, (uint32_t)q, 0x40000000u, I->key0, I->key1);
-            u[q] = u53(w.x, w.y);
-        }
         __syncthreads();
         if (t == 0) {
-            int64_t acc = 0;
-            for (int q = 0; q + 1 < R; ++q) {
-                const int wa = owner[q], wb = owner[q + 1];
-                const double Ta = I->temps[q], Tb = I->temps[q + 1];
-                const double Sa = dual ? I->rs[wa].dscore : (double)I->rs[wa].score;
-                const double Sb = dual ? I->rs[wb].dscore : (double)I->rs[wb].score;
-                const double arg = (1.0 / Ta - 1.0 / Tb) * (Sa - Sb);
-                if (arg >= 0.0 || u[q] < xexp(arg)) {
-                    I->rs[wa].rung = q + 1;
-                    I->rs[wb].rung = q;
-                    owner[q] = wb;
-                    owner[q + 1] = wa;
-                    ++acc;
-                }
-            }
-            I->exch_tried += R - 1;
-            I->exch_acc += acc;
+            s_carry = owner[0];
+            s_sa = score_of(owner[0]);
+            s_acc = 0;
         }
+        const uint32_t e = I->epoch;
+        for (int q0 = 0; q0 < R - 1; q0 += kSweepChunk) {
+            const int qn = R - 1 - q0 < kSweepChunk ? R - 1 - q0 : kSweepChunk;
+            for (int k = t; k < qn; k += blockDim.x) {
+                const int q = q0 + k;
+                s_sb[k] = score_of(owner[q + 1]);
+                const double Ta = I->temps[q], Tb = I->temps[q + 1];
+                s_db[k] = 1.0 / Ta - 1.0 / Tb;
+                const uint4 w = philox10(e, 0u, (uint32_t)q, 0x40000000u, I->key0, I->key1);
+                s_u[k] = u53(w.x, w.y);
+                s_lu[k] = log(s_u[k]);
+            }
+            __syncthreads();
+            if (t == 0) {
+                int wa = s_carry;
+                double Sa = s_sa;
+                int64_t acc = 0;
+                // 8 rungs' inputs in registers at a time, so the LDS latency
+                // is paid once per 8 decisions instead of on every one
+                constexpr int kU = 8;
+                for (int k0 = 0; k0 < qn; k0 += kU) {
+                    int wbv[kU];
+                    double sbv[kU], dbv[kU], luv[kU];
+#pragma unroll
+                    for (int j = 0; j < kU; ++j) {
+                        const int k = k0 + j < qn ? k0 + j : qn - 1;
+                        wbv[j] = owner[q0 + k + 1];
+                        sbv[j] = s_sb[k];
+                        dbv[j] = s_db[k];
+                        luv[j] = s_lu[k];
+                    }
+#pragma unroll
+                    for (int j = 0; j < kU; ++j) {
+                        if (k0 + j >= qn) break;
+                        const double arg = dbv[j] * (Sa - sbv[j]);
+                        // u < xexp(arg) decided from log u where that is
+                        // certain: xexp is within 4e-16 relative of exp above
+                        // -40 (0 at or below), log within a few ulps, so a
+                        // margin of 1e-6 leaves the exact test only for the
+                        // rare close call
+                        bool take;
+                        if (arg >= 0.0)
+                            take = true;
+                        else if (!(arg > -40.0))
+                            take = false;
+                        else if (arg > luv[j] + 1e-6)
+                            take = true;
+                        else if (arg < luv[j] - 1e-6)
+                            take = false;
+                        else
+                            take = s_u[k0 + j] < xexp(arg);
+                        if (take) {
+                            owner[q0 + k0 + j] = wbv[j];  // wb moves down, wa goes on up
+                            ++acc;
+                        } else {
+                            owner[q0 + k0 + j] = wa;
+                            wa = wbv[j];
+                            Sa = sbv[j];
+                        }
+                    }
+                }
+                s_carry = wa;
+                s_sa = Sa;
+                s_acc += acc;
+            }
+            __syncthreads();
+        }
+        if (t == 0) {
+            owner[R - 1] = s_carry;
+            I->exch_tried += R - 1;
+            I->exch_acc += s_acc;
+        }
+        __syncthreads();
+        for (int q = t; q < R; q += blockDim.x) I->rs[owner[q]].rung = q;
     }
     __syncthreads();
     if (t == 0) {
@@ -1263,7 +1333,7 @@ hipError_t launch_anneal(hipStream_t st, int G, int C, bool tri, int groups_per_
 
 hipError_t launch_epoch_end(hipStream_t st, Inst* insts, int count, int max_r) {
     if (count <= 0) return hipSuccess;
-    const size_t lds = ((size_t)max_r * sizeof(int) + 7) / 8 * 8 + (size_t)max_r * sizeof(double);
+    const size_t lds = ((size_t)max_r * sizeof(int) + 7) / 8 * 8 + 4 * kSweepChunk * sizeof(double);
     hipLaunchKernelGGL(k_epoch_end, dim3(count), dim3(256), lds, st, insts);
     return hipGetLastError();
 }
