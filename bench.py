@@ -3,7 +3,7 @@
 Default workload (`--workload batch`, the metric's configuration): 1024 planted
 instances, n ~ U{8..48}, A = U^T D U with D = diag(+-U{1..100}) and U a product
 of 3n random +-1 shears (xtax_amd.workloads.batch_config), 32 tempering
-replicas each on a geometric ladder T in [5, 200], greedy fraction 0.5,
+replicas each on a geometric ladder T in [14, 18], greedy fraction 0.5,
 exchange every 2000 moves, all instances resident in one captured launch
 sequence. An instance retires when a replica reaches a diagonal or after
 `--cap-epochs` epochs. One step = one pass over the whole batch. Every solved
@@ -19,12 +19,17 @@ kernel (k_anneal) and a CPU baseline: the reference's own xtax binary
 (oracle/_ref/xtax_ref, compiled from the reference sources) on a bounded sample
 of the same instances on this host's cores.
 
+The ladders are per workload (--t-init/--t-min override them), chosen by
+solve rate on the workloads' own instances: T in [14, 18] for the batch
+(D up to 100; the former [5, 200] solved 1.7x fewer), [28, 36] for serving
+(D up to 200).
+
 Other configurations (`--workload ...`), same line format:
   plain    the batch instances one per launch (xtx_solve_plain), first --count
   serve    Poisson stream, n = round(exp(U[ln 6, ln 64])), D = +-U{1..200},
            64 replicas per request, continuous batching (xtx_serve_*)
   large    one planted n = 80 instance (D = +-U{1..1000}, 4n shears),
-           4096 replicas, greedy 0.7
+           4096 replicas, greedy 0.7, T in [50, 2000]
   lattice  24x24 lattice Gram (entries U[-9, 9]), 256 replicas, greedy 0.5,
            sweep 0.05
   random   n = 10 random symmetric (U[-20, 20]), 8 replicas, T in [0.5, 50]
@@ -280,7 +285,7 @@ def bench_serve(args, eng):
     eng.serve_close()
     cfg = {"workload": f"serve: {count} planted requests, Poisson {args.rate}/s, "
                        f"n=round(exp(U[ln6,ln64])), D=+-U{{1..200}}, 64 replicas, "
-                       f"cap {args.cap_epochs} epochs"}
+                       f"T in [{args.t_min},{args.t_init}], cap {args.cap_epochs} epochs"}
     out = base_line(args, verified / el, "matrices/s", el, cfg)
     out.update({"solved": solved, "verified": verified, "requests": count,
                 "p50_latency_s": pct(lat_solved, 50), "p95_latency_s": pct(lat_solved, 95),
@@ -316,13 +321,13 @@ def bench_single(args, eng, name, A0, params, cpu_extra):
 
 def bench_large(args, eng):
     A0 = workloads.planted(80, 1000, 320, seed=5)[0]
-    p = EngineParams(replicas=4096, greedy_fraction=0.7, t_init=args.t_init * 10,
-                     t_min=args.t_min * 10, exchange_interval=args.exchange_interval,
+    p = EngineParams(replicas=4096, greedy_fraction=0.7, t_init=args.t_init,
+                     t_min=args.t_min, exchange_interval=args.exchange_interval,
                      max_seconds=args.seconds, seed=args.seed, verify=True)
     return bench_single(args, eng, "large: planted n=80, D=+-U{1..1000}, 4n shears, 4096 "
                         "replicas, greedy 0.7", A0, p,
-                        ["--greedy-fraction", "0.7", "--t-init", str(args.t_init * 10),
-                         "--t-min", str(args.t_min * 10)])
+                        ["--greedy-fraction", "0.7", "--t-init", str(args.t_init),
+                         "--t-min", str(args.t_min)])
 
 
 def bench_lattice(args, eng):
@@ -440,8 +445,8 @@ def main():
                              "copy"])
     ap.add_argument("--count", type=int, default=None)
     ap.add_argument("--replicas", type=int, default=32)
-    ap.add_argument("--t-init", type=float, default=200.0)
-    ap.add_argument("--t-min", type=float, default=5.0)
+    ap.add_argument("--t-init", type=float, default=None)
+    ap.add_argument("--t-min", type=float, default=None)
     ap.add_argument("--exchange-interval", type=int, default=2000)
     ap.add_argument("--cap-epochs", type=int, default=50)
     ap.add_argument("--rate", type=float, default=200.0)
@@ -462,6 +467,12 @@ def main():
         # (replicas x cap moves = 3.2M at the defaults, about 0.05 s on 16 cores),
         # doubled; single-instance runs get a fixed 2 s
         args.cpu_cap = 0.1 if args.workload == "batch" else 2.0
+    ladder = {"batch": (18.0, 14.0), "plain": (18.0, 14.0), "serve": (36.0, 28.0),
+              "large": (2000.0, 50.0)}.get(args.workload, (200.0, 5.0))
+    if args.t_init is None:
+        args.t_init = ladder[0]
+    if args.t_min is None:
+        args.t_min = ladder[1]
     if args.count is None:
         args.count = {"batch": 1024, "plain": 64, "serve": 3000, "random": 64, "copy": 8}.get(
             args.workload, 1)
