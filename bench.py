@@ -4,6 +4,7 @@ Default workload (`--workload batch`, the metric's configuration): 1024 planted
 instances, n ~ U{8..48}, A = U^T D U with D = diag(+-U{1..100}) and U a product
 of 3n random +-1 shears (xtax_amd.workloads.batch_config), 32 tempering
 replicas each on a geometric ladder T in [14, 18], greedy fraction 0.5,
+target fraction 1.0,
 exchange every 2000 moves, all instances resident in one captured launch
 sequence. An instance retires when a replica reaches a diagonal or after
 `--cap-epochs` epochs. One step = one pass over the whole batch. Every solved
@@ -19,10 +20,12 @@ kernel (k_anneal) and a CPU baseline: the reference's own xtax binary
 (oracle/_ref/xtax_ref, compiled from the reference sources) on a bounded sample
 of the same instances on this host's cores.
 
-The ladders are per workload (--t-init/--t-min override them), chosen by
-solve rate on the workloads' own instances: T in [14, 18] for the batch
-(D up to 100; the former [5, 200] solved 1.7x fewer), [28, 36] for serving
-(D up to 200).
+The ladders and the target fraction are per workload (--t-init, --t-min and
+--target-fraction override them), chosen by solve rate on the workloads' own
+instances: T in [14, 18] for the batch (D up to 100; [5, 200] solved 1.75x
+fewer), [36, 50] for serving (D up to 200), and every proposal targeted
+(--target-fraction 1: 1.4x the solves of the reference default 0.5). The CPU
+baseline runs the reference with the same settings.
 
 Other configurations (`--workload ...`), same line format:
   plain    the batch instances one per launch (xtx_solve_plain), first --count
@@ -153,7 +156,8 @@ def bench_batch(args, eng):
     seeds = [args.seed * 1_000_003 + q for q in range(len(mats))]
     mean_n = float(np.mean([a.shape[0] for a in mats]))
     p = EngineParams(replicas=args.replicas, t_init=args.t_init, t_min=args.t_min,
-                     greedy_fraction=0.5, exchange_interval=args.exchange_interval,
+                     greedy_fraction=0.5, target_fraction=args.target_fraction,
+                     exchange_interval=args.exchange_interval,
                      max_epochs=args.cap_epochs, seed=args.seed, verify=True)
     tune = None
     if not args.no_tune:
@@ -193,8 +197,8 @@ def bench_batch(args, eng):
     el = time.perf_counter() - t0
     cfg = {"workload": f"batch: {args.count} planted instances n~U{{8..48}}, D=+-U{{1..100}}, "
                        f"3n shears, {args.replicas} replicas, T in [{args.t_min},{args.t_init}], "
-                       f"greedy 0.5, exchange {args.exchange_interval}, cap {args.cap_epochs} "
-                       f"epochs",
+                       f"greedy 0.5, target {args.target_fraction}, exchange "
+                       f"{args.exchange_interval}, cap {args.cap_epochs} epochs",
            "instances": args.count, "replicas": args.replicas,
            "cap_moves_per_replica": args.cap_epochs * args.exchange_interval}
     # value: inputs resident in HBM when the clock starts (each step's own
@@ -213,7 +217,8 @@ def bench_batch(args, eng):
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_sample(
             mats, ["--t-init", str(args.t_init), "--t-min", str(args.t_min),
-                   "--greedy-fraction", "0.5"], args.cpu_seconds, args.cpu_cap)
+                   "--greedy-fraction", "0.5", "--target-fraction", str(args.target_fraction)],
+            args.cpu_seconds, args.cpu_cap)
     return out
 
 
@@ -227,7 +232,8 @@ def bench_plain(args, eng):
     t0 = time.perf_counter()
     for q, A in enumerate(mats):
         p = EngineParams(replicas=args.replicas, t_init=args.t_init, t_min=args.t_min,
-                         greedy_fraction=0.5, exchange_interval=args.exchange_interval,
+                         greedy_fraction=0.5, target_fraction=args.target_fraction,
+                         exchange_interval=args.exchange_interval,
                          max_epochs=args.cap_epochs, seed=seeds[q], verify=True)
         t1 = time.perf_counter()
         r = eng.run_annealer(A, None, p)
@@ -239,7 +245,8 @@ def bench_plain(args, eng):
     el = time.perf_counter() - t0
     mean_n = float(np.mean([a.shape[0] for a in mats]))
     cfg = {"workload": f"plain: the first {len(mats)} batch instances one per launch, "
-                       f"{args.replicas} replicas, cap {args.cap_epochs} epochs"}
+                       f"{args.replicas} replicas, T in [{args.t_min},{args.t_init}], target "
+                       f"{args.target_fraction}, cap {args.cap_epochs} epochs"}
     out = base_line(args, solved / el, "matrices/s", el, cfg)
     out.update({"solved": solved, "instances": len(mats), "p50_latency_s": pct(lat, 50),
                 "p99_latency_s": pct(lat, 99), "moves_per_sec": moves / el,
@@ -252,6 +259,7 @@ def bench_serve(args, eng):
     count = args.count
     reqs = workloads.serve_config(count=count, dmax=200, seed=11, rate=args.rate)
     p = EngineParams(replicas=64, t_init=args.t_init, t_min=args.t_min, greedy_fraction=0.5,
+                     target_fraction=args.target_fraction,
                      exchange_interval=args.exchange_interval, max_epochs=args.cap_epochs,
                      seed=args.seed, verify=True)
     eng.serve_open(p)
@@ -285,7 +293,8 @@ def bench_serve(args, eng):
     eng.serve_close()
     cfg = {"workload": f"serve: {count} planted requests, Poisson {args.rate}/s, "
                        f"n=round(exp(U[ln6,ln64])), D=+-U{{1..200}}, 64 replicas, "
-                       f"T in [{args.t_min},{args.t_init}], cap {args.cap_epochs} epochs"}
+                       f"T in [{args.t_min},{args.t_init}], target {args.target_fraction}, "
+                       f"cap {args.cap_epochs} epochs"}
     out = base_line(args, verified / el, "matrices/s", el, cfg)
     out.update({"solved": solved, "verified": verified, "requests": count,
                 "p50_latency_s": pct(lat_solved, 50), "p95_latency_s": pct(lat_solved, 95),
@@ -447,6 +456,7 @@ def main():
     ap.add_argument("--replicas", type=int, default=32)
     ap.add_argument("--t-init", type=float, default=None)
     ap.add_argument("--t-min", type=float, default=None)
+    ap.add_argument("--target-fraction", type=float, default=None)
     ap.add_argument("--exchange-interval", type=int, default=2000)
     ap.add_argument("--cap-epochs", type=int, default=50)
     ap.add_argument("--rate", type=float, default=200.0)
@@ -467,8 +477,10 @@ def main():
         # (replicas x cap moves = 3.2M at the defaults, about 0.05 s on 16 cores),
         # doubled; single-instance runs get a fixed 2 s
         args.cpu_cap = 0.1 if args.workload == "batch" else 2.0
-    ladder = {"batch": (18.0, 14.0), "plain": (18.0, 14.0), "serve": (36.0, 28.0),
+    ladder = {"batch": (18.0, 14.0), "plain": (18.0, 14.0), "serve": (50.0, 36.0),
               "large": (2000.0, 50.0)}.get(args.workload, (200.0, 5.0))
+    if args.target_fraction is None:
+        args.target_fraction = 1.0 if args.workload in ("batch", "plain", "serve") else 0.5
     if args.t_init is None:
         args.t_init = ladder[0]
     if args.t_min is None:
