@@ -100,29 +100,38 @@ def test_primitives_every_layout(eng, po, n):
     _check(eng, po, big, Q, c, [(0, 0, 1, 1), (0, 1, 0, -1), (0, 2, 1, 3), (0, 1, 2, 64)])
 
 
-def _params(R, seed, tempering, E=300, epochs=4):
-    return EngineParams(replicas=R, seed=seed, tempering=tempering, exchange_interval=E,
-                        stuck_threshold=150, sweep_fraction=0.5, greedy_fraction=0.6,
-                        max_epochs=epochs, move_log=True, move_log_cap=4096, verify=True)
+TRAJ = [  # n, R, tempering, extra engine settings, dual_refresh, dual_check
+    (9, 4, 1, {}, 700, 250),
+    (9, 4, 0, {}, 700, 250),
+    (20, 8, 1, {}, 700, 250),
+    (40, 4, 0, {}, 700, 250),
+    (70, 2, 1, {}, 700, 250),
+    # cold replicas that sit on their best while the dual is re-inverted
+    # every epoch: the best keeps the dual it was found with
+    # (both end unsolved on a best found before the last re-inversion)
+    (20, 4, 0, dict(t_init=0.05, t_min=0.01, stuck_threshold=100000), 300, 0),
+    (20, 4, 1, dict(t_init=0.5, t_min=0.1, stuck_threshold=100000), 300, 0),
+]
 
 
-@pytest.mark.parametrize("n,R,tempering", [(9, 4, 1), (9, 4, 0), (20, 8, 1), (40, 4, 0),
-                                           (70, 2, 1)])
-def test_trajectory_equals_model(eng, po, n, R, tempering):
+@pytest.mark.parametrize("n,R,tempering,extra,refresh,check", TRAJ)
+def test_trajectory_equals_model(eng, po, n, R, tempering, extra, refresh, check):
     seed = 100 + n
     A0 = gram(n, seed)
-    ep = _params(R, seed, tempering)
-    dpar = DualParams(dual_refresh=700, dual_check=250, dual_tol=1e-9)
-    say("dual trajectory n", n, "R", R, "tempering", tempering)
+    kw = dict(replicas=R, seed=seed, tempering=tempering, exchange_interval=300,
+              stuck_threshold=150, sweep_fraction=0.5, greedy_fraction=0.6)
+    kw.update(extra)
+    ep = EngineParams(max_epochs=6 if extra else 4, move_log=True, move_log_cap=4096,
+                      verify=True, **kw)
+    dpar = DualParams(dual_refresh=refresh, dual_check=check, dual_tol=1e-9)
+    say("dual trajectory n", n, "R", R, "tempering", tempering, extra)
     res = eng.run_xdual(A0, None, ep, dpar, allow_unverified=False)
     st = res.stats
     P, Q, Xt, dsc, sc = eng.dual_last_replicas(R, n)
     ok, Q0 = po.invert(A0)
     assert ok
-    mp = po.Params(**dict(po.DEFAULTS, replicas=R, seed=seed, tempering=tempering,
-                          exchange_interval=300, stuck_threshold=150, sweep_fraction=0.5,
-                          greedy_fraction=0.6))
-    md = po.DualParams(c=st["c"], dual_tol=1e-9, dual_refresh=700, dual_check=250)
+    mp = po.Params(**dict(po.DEFAULTS, **kw))
+    md = po.DualParams(c=st["c"], dual_tol=1e-9, dual_refresh=refresh, dual_check=check)
     m = po.dual_model_run(A0, Q0, np.eye(n, dtype=np.int64), mp, md, st["epochs"], log_cap=4096)
     assert st["c"] == pytest.approx(po.frob2_off_p(A0) / po.frob2_off_q(Q0), rel=1e-12)
     assert st["solved"] == m["solved"]
@@ -155,6 +164,10 @@ def test_trajectory_equals_model(eng, po, n, R, tempering):
         assert np.array_equal(Pr, P[r]) and np.array_equal(Xr, Xt[r])
     assert np.array_equal(res.P, m["best_P"]) and np.array_equal(res.Q, m["best_Q"])
     assert np.array_equal(res.X.T, m["best_Xt"])
+    if extra:
+        b = m["best_replica"]
+        assert not m["solved"] and np.array_equal(m["best_P"], m["P"][b])
+        assert not np.array_equal(m["best_Q"], m["Q"][b])  # re-inverted since
 
 
 def test_solves_and_verifies(eng, po):

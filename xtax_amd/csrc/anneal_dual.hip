@@ -851,6 +851,23 @@ __global__ void __launch_bounds__(256) k_dual_maint(Inst* __restrict__ insts) {
             __syncthreads();
         }
         if (!singular) {
+            // the current state may be the replica's unflushed best: the
+            // best keeps the dual it was found with, so flush it first
+            const int32_t fl = S->flags;
+            if (fl & RF_CUR_BEST) {
+                const bool keep = S->dbest_score <= I->gb_dscore;
+                if (keep) {
+                    const int64_t off = (int64_t)r * nn;
+                    for (int q = t; q < (int)nn; q += 256) {
+                        I->bA[off + q] = P[q];
+                        I->bQ[off + q] = Q[q];
+                        I->bXt[off + q] = I->Xt[off + q];
+                    }
+                }
+                __syncthreads();
+                if (t == 0)
+                    S->flags = (fl & ~(RF_CUR_BEST | RF_BEST_STALE)) | (keep ? 0 : RF_BEST_STALE);
+            }
             for (int q = t; q < (int)nn; q += 256) {
                 const int a = q / n, b = q % n;
                 const double x = A[(int64_t)a * w + n + b];
