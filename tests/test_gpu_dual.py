@@ -211,3 +211,27 @@ def test_singular_gram_is_refused(eng):
     A0 = np.array([[1, 2, 0], [2, 4, 0], [0, 0, 1]], np.int64)
     with pytest.raises(XtxError, match="invert"):
         eng.run_xdual(A0, None, EngineParams(replicas=2, max_epochs=1), DualParams())
+
+
+def test_limits_and_small_orders(eng, po):
+    """max_moves caps every replica, max_seconds stops the run, n = 2 and a
+    two-index active list work, an already diagonal start is reported solved."""
+    A0 = gram(14, 31)
+    r = eng.run_xdual(A0, None, EngineParams(replicas=4, seed=2, max_moves=700,
+                                             exchange_interval=300), DualParams())
+    assert r.stats["stop_reason"] == 3 and r.stats["moves"] == 4 * 700
+    r = eng.run_xdual(A0, None, EngineParams(replicas=8, seed=2, max_seconds=0.3,
+                                             exchange_interval=2000), DualParams())
+    assert r.stats["stop_reason"] in (0, 1) and r.stats["verify_status"] == 0
+    A2 = np.array([[2, 2], [2, 5]], np.int64)   # U^T diag(2, 3) U, U = [[1, 1], [0, 1]]
+    r = eng.run_xdual(A2, None, EngineParams(replicas=2, seed=1, max_epochs=20), DualParams())
+    assert r.stats["solved"] == 1 and r.P[0, 1] == 0
+    assert np.array_equal(r.X.T @ A2 @ r.X, r.P)
+    r = eng.run_xdual(A0, None, EngineParams(replicas=2, seed=1, max_epochs=3), DualParams(),
+                      active=[3, 9])
+    keep = [k for k in range(14) if k not in (3, 9)]
+    assert np.array_equal(r.X[:, keep], np.eye(14, dtype=np.int64)[:, keep])
+    D = np.diag([3, -1, 2]).astype(np.int64)
+    r = eng.run_xdual(D, None, EngineParams(replicas=2), DualParams())
+    assert r.stats["solved"] == 1 and r.stats["stop_reason"] == 4
+    assert np.allclose(r.Q, np.diag([1 / 3, -1, 0.5]))
