@@ -203,15 +203,18 @@ struct Grp {
         v = op(v, dpp64<kDppMirror>(v));
         if constexpr (G == 16) {
             return v;
+        } else if constexpr (G == 32) {
+            // the group's two rows trade places: even row's value op odd row's
+            const uint32_t lo = (uint32_t)(uint64_t)v, hi = (uint32_t)((uint64_t)v >> 32);
+            const auto pl = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+            const auto ph = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+            const int64_t ev = (int64_t)(((uint64_t)ph[0] << 32) | pl[0]);
+            const int64_t od = (int64_t)(((uint64_t)ph[1] << 32) | pl[1]);
+            return op(ev, od);
         } else {
             const int64_t r0 = readlane64(v, 0), r1 = readlane64(v, 16);
             const int64_t r2 = readlane64(v, 32), r3 = readlane64(v, 48);
-            if constexpr (G == 32) {
-                const int64_t a = op(r0, r1), b = op(r2, r3);
-                return (lane() & 32) ? b : a;
-            } else {
-                return op(op(r0, r1), op(r2, r3));
-            }
+            return op(op(r0, r1), op(r2, r3));
         }
     }
     template <class Op>
@@ -222,17 +225,15 @@ struct Grp {
         v = op(v, (int32_t)dpp32<kDppMirror>((uint32_t)v));
         if constexpr (G == 16) {
             return v;
+        } else if constexpr (G == 32) {
+            const auto pr = __builtin_amdgcn_permlane16_swap((uint32_t)v, (uint32_t)v, false, false);
+            return op((int32_t)pr[0], (int32_t)pr[1]);
         } else {
             const int32_t r0 = (int32_t)readlane32((uint32_t)v, 0);
             const int32_t r1 = (int32_t)readlane32((uint32_t)v, 16);
             const int32_t r2 = (int32_t)readlane32((uint32_t)v, 32);
             const int32_t r3 = (int32_t)readlane32((uint32_t)v, 48);
-            if constexpr (G == 32) {
-                const int32_t a = op(r0, r1), b = op(r2, r3);
-                return (lane() & 32) ? b : a;
-            } else {
-                return op(op(r0, r1), op(r2, r3));
-            }
+            return op(op(r0, r1), op(r2, r3));
         }
     }
     // Sum of a double over the group as the binary tree of adjacent lanes
@@ -250,18 +251,22 @@ struct Grp {
         v = v + dppf(v, std::integral_constant<int, kDppMirror>());
         if constexpr (G == 16) {
             return v;
+        } else if constexpr (G == 32) {
+            // even row + odd row of the group, in every lane
+            const uint64_t b = __builtin_bit_cast(uint64_t, v);
+            const uint32_t lo = (uint32_t)b, hi = (uint32_t)(b >> 32);
+            const auto pl = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+            const auto ph = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+            const double ev = __builtin_bit_cast(double, ((uint64_t)ph[0] << 32) | pl[0]);
+            const double od = __builtin_bit_cast(double, ((uint64_t)ph[1] << 32) | pl[1]);
+            return ev + od;
         } else {
             const int64_t b = __builtin_bit_cast(int64_t, v);
             const double r0 = __builtin_bit_cast(double, readlane64(b, 0));
             const double r1 = __builtin_bit_cast(double, readlane64(b, 16));
             const double r2 = __builtin_bit_cast(double, readlane64(b, 32));
             const double r3 = __builtin_bit_cast(double, readlane64(b, 48));
-            if constexpr (G == 32) {
-                const double a = r0 + r1, c = r2 + r3;
-                return (lane() & 32) ? c : a;
-            } else {
-                return (r0 + r1) + (r2 + r3);
-            }
+            return (r0 + r1) + (r2 + r3);
         }
     }
     static __device__ __forceinline__ int64_t sum(int64_t v) { return all64(v, OpAdd()); }
