@@ -328,6 +328,16 @@ int xtx_dual_solve(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t n
         for (int r = 0; r < R; ++r) refs.push_back(RepRef{0, r});
         while (refs.size() % (size_t)(64 / L.G)) refs.push_back(RepRef{0, -1});
         RepRef* d_refs = nullptr;
+        Inst* pin = nullptr;
+        // released on every exit, including an error thrown below
+        struct Release {
+            RepRef*& refs;
+            Inst*& pin;
+            ~Release() {
+                if (refs) (void)hipFree(refs);
+                if (pin) (void)hipHostFree(pin);
+            }
+        } release{d_refs, pin};
         HIPCHK(hipMalloc(&d_refs, refs.size() * sizeof(RepRef)));
         HIPCHK(hipMemcpyAsync(d_refs, refs.data(), refs.size() * sizeof(RepRef),
                               hipMemcpyHostToDevice, sm));
@@ -335,7 +345,6 @@ int xtx_dual_solve(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t n
         int waves = p.waves_per_block > 0 ? p.waves_per_block : 1;
         while (waves > 1 && (int64_t)waves * per_wave * L.rep_bytes > g_lds_block) --waves;
         const int gpb = waves * per_wave;
-        Inst* pin = nullptr;
         HIPCHK(hipHostMalloc((void**)&pin, sizeof(Inst), hipHostMallocDefault));
         HIPCHK(hipStreamSynchronize(sm));
 
@@ -387,8 +396,6 @@ int xtx_dual_solve(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t n
         }
         if (reason == 1) *(volatile int*)ctx->stop_host = 0;
         I = *pin;
-        (void)hipHostFree(pin);
-        (void)hipFree(d_refs);
 
         // outcome: the solving replica's state, else the instance best
         rs.resize((size_t)R);

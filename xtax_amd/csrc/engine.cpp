@@ -619,13 +619,22 @@ int xtx_solve_plain(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t 
         buckets[0].groups_per_block = groups_per_block_for(h.lay, p.waves_per_block);
         for (int r = 0; r < h.R; ++r) buckets[0].refs.push_back(RepRef{0, r});
         pad_to_wave(buckets[0].refs, h.lay.G, 0);
+        Inst* pin = nullptr;
+        // released on every exit, including an error thrown below
+        struct Release {
+            RepRef*& refs;
+            Inst*& pin;
+            ~Release() {
+                if (refs) (void)hipFree(refs);
+                if (pin) (void)hipHostFree(pin);
+            }
+        } release{buckets[0].d_refs, pin};
         HIPCHK(hipMalloc(&buckets[0].d_refs, buckets[0].refs.size() * sizeof(RepRef)));
         HIPCHK(hipMemcpyAsync(buckets[0].d_refs, buckets[0].refs.data(),
                               buckets[0].refs.size() * sizeof(RepRef), hipMemcpyHostToDevice,
                               ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
 
-        Inst* pin = nullptr;
         HIPCHK(hipHostMalloc((void**)&pin, sizeof(Inst), hipHostMallocDefault));
         const int stop_before = *(volatile int*)ctx->stop_host;
         (void)stop_before;
@@ -670,8 +679,6 @@ int xtx_solve_plain(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t 
         }
         if (reason == 1) *(volatile int*)ctx->stop_host = 0;  // deadline flag is ours
         h.rec = *pin;
-        (void)hipHostFree(pin);
-        (void)hipFree(buckets[0].d_refs);
         Outcome o;
         fetch_outcome(h, h.rec, ctx->stream, o);
         ctx->plain_rs = o.rs;
