@@ -74,6 +74,13 @@ typedef struct {
     int32_t autotune;         /* 1: xtx_batch_solve tunes every order of its
                                  batch that has no configuration yet (on the
                                  batch itself, before the run) */
+    /* Plain forms (xtx_solve_plain, xtx_dual_solve): called on the calling
+     * thread at an epoch boundary whenever the instance best improves -- the
+     * reference's "new best" report (congruence_anneal.hpp:309-352). seconds
+     * since the call began; score (L1) or dscore (xdual; score = 0); the
+     * replica holding the best. NULL: no reports. */
+    void (*on_best)(void* user, double seconds, int64_t score, double dscore, int32_t replica);
+    void* on_best_user;
 } xtx_params;
 
 typedef struct {

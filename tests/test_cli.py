@@ -103,6 +103,11 @@ def test_matrix_mode_solves_sample(tmp_path, orc):
     assert r.returncode == 0, r.stderr
     assert "done moves=" in r.stdout and "Final best score:" in r.stdout
     assert "matrix is diagonal" in r.stdout
+    # "new best" progress lines: strictly decreasing scores, below the start
+    start = int(r.stdout.split("[t=0s] start score=")[1].split()[0])
+    best = [int(l.split("new best score=")[1].split()[0]) for l in r.stdout.splitlines()
+            if "] new best score=" in l]
+    assert all(b < a for a, b in zip([start] + best, best))
     A = read_csv(tmp_path / "best_A.csv")
     X = read_csv(tmp_path / "best_X.csv")
     status, sign, _ = orc.verify(A0, X, A)

@@ -235,3 +235,29 @@ def test_limits_and_small_orders(eng, po):
     r = eng.run_xdual(D, None, EngineParams(replicas=2), DualParams())
     assert r.stats["solved"] == 1 and r.stats["stop_reason"] == 4
     assert np.allclose(r.Q, np.diag([1 / 3, -1, 0.5]))
+
+
+def test_new_best_reports(eng, po):
+    """on_best reports the xdual instance best (dscore) as it improves."""
+    import ctypes as C
+    from xtax_amd import _native as N
+    n = 20
+    A0 = gram(n, 120)
+    seen = []
+    cb = N.ON_BEST(lambda user, t, score, dscore, rep: seen.append((score, dscore, rep)))
+    p = EngineParams(replicas=8, seed=3, max_epochs=8, exchange_interval=300).to_c()
+    p.on_best = C.cast(cb, C.c_void_p)
+    d = DualParams().to_c()
+    P = np.zeros((n, n), np.int64)
+    Q = np.zeros((n, n))
+    X = np.zeros((n, n), np.int64)
+    st = N.DualStats()
+    A0c = np.ascontiguousarray(A0, np.int64)
+    dp = C.POINTER(C.c_double)
+    rc = N.lib().xtx_dual_solve(eng._h, A0c.ctypes.data_as(N.I64P), None, n, C.byref(p),
+                                C.byref(d), None, 0, P.ctypes.data_as(N.I64P),
+                                Q.ctypes.data_as(dp), X.ctypes.data_as(N.I64P), C.byref(st))
+    assert rc == 0 and seen
+    scores = [st.start_score] + [ds for _, ds, _ in seen]
+    assert all(b < a for a, b in zip(scores, scores[1:]))
+    assert all(s == 0 and 0 <= r < 8 for s, _, r in seen)

@@ -339,3 +339,35 @@ def test_autotune_picks_a_measured_configuration(eng):
     finally:
         for order in (8, 16, 24, 40):
             eng.launch_reset(order)
+
+
+def test_new_best_reports(eng, orc):
+    """on_best: one report per epoch whose instance best improved, strictly
+    decreasing, each equal to the best-so-far score the replicas record."""
+    import ctypes as C
+    from xtax_amd import _native as N
+    A0 = workloads.planted(24, 100, 72, seed=3)[0]
+    seen = []
+    cb = N.ON_BEST(lambda user, t, score, dscore, rep: seen.append((t, score, dscore, rep)))
+    ep = EngineParams(replicas=16, seed=4, max_epochs=12, exchange_interval=500, t_init=18.0,
+                      t_min=14.0)
+    p = ep.to_c()
+    p.on_best = C.cast(cb, C.c_void_p)
+    n = A0.shape[0]
+    A = np.zeros((n, n), np.int64)
+    X = np.zeros((n, n), np.int64)
+    st = N.Stats()
+    A0c = np.ascontiguousarray(A0, np.int64)
+    rc = N.lib().xtx_solve_plain(eng._h, A0c.ctypes.data_as(N.I64P), None, n, C.byref(p), None,
+                                 0, A.ctypes.data_as(N.I64P), X.ctypes.data_as(N.I64P),
+                                 C.byref(st))
+    assert rc == 0
+    start = int(2 * np.abs(A0).sum() - np.abs(np.diag(A0)).sum())
+    assert seen, "no report"
+    scores = [start] + [s for _, s, _, _ in seen]
+    assert all(b < a for a, b in zip(scores, scores[1:]))
+    assert all(float(s) == d for _, s, d, _ in seen)
+    assert all(0 <= r < 16 for _, _, _, r in seen)
+    assert len(seen) <= st.epochs
+    if not st.solved:
+        assert seen[-1][1] == st.best_score

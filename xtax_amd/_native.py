@@ -32,7 +32,12 @@ class Params(C.Structure):
         ("replicas", C.c_int32), ("move_log", C.c_int32), ("move_log_cap", C.c_int64),
         ("max_moves", C.c_uint64), ("max_epochs", C.c_int32), ("verify", C.c_int32),
         ("waves_per_block", C.c_int32), ("autotune", C.c_int32),
+        ("on_best", C.c_void_p), ("on_best_user", C.c_void_p),
     ]
+
+
+# void (*on_best)(void* user, double seconds, int64_t score, double dscore, int32_t replica)
+ON_BEST = C.CFUNCTYPE(None, C.c_void_p, C.c_double, C.c_int64, C.c_double, C.c_int32)
 
 
 class Limits(C.Structure):

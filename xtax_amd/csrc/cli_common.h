@@ -315,6 +315,17 @@ inline std::vector<Opt> engine_options(xtx_params& p, int& device) {
     };
 }
 
+// The reference's "new best" progress line (congruence_anneal.hpp:331-334)
+// from the engine's on_best report; the replica stands in for the thread.
+inline void print_new_best_l1(void*, double seconds, int64_t score, double, int32_t replica) {
+    std::cout << "[t=" << seconds << "s] new best score=" << score << " (thread " << replica
+              << ")\n";
+}
+inline void print_new_best_dual(void*, double seconds, int64_t, double score, int32_t replica) {
+    std::cout << "[t=" << seconds << "s] new best score=" << score << " (thread " << replica
+              << ")\n";
+}
+
 // --modulus value (>= 1)
 inline std::function<void(const std::string&)> set_modulus(uint64_t& m) {
     return [&m](const std::string& s) {

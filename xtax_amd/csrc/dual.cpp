@@ -351,6 +351,7 @@ int xtx_dual_solve(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t n
         double kernel_ms = 0;
         int epochs = 0;
         int reason = 3;
+        double reported = score0;  // the start state is never reported
         const auto ramp_t0 = clk::now();
         {
             Deadline dl(ctx->stop_host, p.max_seconds);
@@ -376,6 +377,12 @@ int xtx_dual_solve(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t n
                 HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
                 kernel_ms += ms;
                 ++epochs;
+                if (p.on_best && pin->status != ST_SOLVED && pin->gb_dscore < reported) {
+                    reported = pin->gb_dscore;
+                    p.on_best(p.on_best_user,
+                              std::chrono::duration<double>(clk::now() - t0).count(), 0, reported,
+                              pin->gb_rep);
+                }
                 if (pin->status == ST_SOLVED) {
                     reason = 0;
                     break;

@@ -641,6 +641,7 @@ int xtx_solve_plain(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t 
         double kernel_ms = 0;
         int epochs = 0;
         int reason = 3;
+        int64_t reported = h.score0;  // the start state is never reported
         {
             Deadline dl(ctx->stop_host, p.max_seconds);
             for (;;) {
@@ -659,6 +660,12 @@ int xtx_solve_plain(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t 
                 HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
                 kernel_ms += ms;
                 ++epochs;
+                if (p.on_best && pin->status != ST_SOLVED && pin->gb_score < reported) {
+                    reported = pin->gb_score;
+                    p.on_best(p.on_best_user,
+                              std::chrono::duration<double>(clk::now() - t0).count(), reported,
+                              (double)reported, pin->gb_rep);
+                }
                 if (pin->status == ST_SOLVED) {
                     reason = 0;
                     break;

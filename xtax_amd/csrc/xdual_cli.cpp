@@ -153,6 +153,7 @@ int main(int argc, char** argv) {
     Mat bP = square(n), bX = square(n);
     std::vector<double> bQ((size_t)n * n);
     p.verify = 1;
+    p.on_best = print_new_best_dual;
     std::cout << "[xdual] inverting the Gram (double-precision dual P^-1)...\n";
     xtx_dual_stats st{};
     const int rc = xtx_dual_solve(g_ctx, A.v.data(), (have_A && !x_csv.empty()) ? X.v.data() : nullptr,

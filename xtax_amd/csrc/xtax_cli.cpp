@@ -79,6 +79,7 @@ bool solve_with_deflation(xtx_ctx* ctx, const Mat& A, const xtx_params& params, 
               << " score=" << score_of(M) << " n=" << n << " slice=" << slice
               << "s threads=" << params.replicas << "\n";
     xtx_params sp = params;
+    sp.on_best = verbose ? print_new_best_l1 : nullptr;  // slices are quiet (main.cpp:482)
     const double report_every = std::max(params.save_interval, slice);
     double last_report = -1e18;
     Mat X = square(n), bA = square(n), bX = square(n);
@@ -288,6 +289,7 @@ int main(int argc, char** argv) {
                   << p.replicas << ((p.tempering && p.replicas >= 2) ? " (parallel tempering)" : "")
                   << "\n";
         xtx_stats st{};
+        p.on_best = print_new_best_l1;
         const int rc = xtx_solve_plain(
             g_ctx, A.v.data(), (have_A && !x_csv.empty()) ? X.v.data() : nullptr, n, &p,
             rows.empty() ? nullptr : rows.data(), (int32_t)rows.size(), bA.v.data(), bX.v.data(),
