@@ -230,6 +230,12 @@ int xtx_solve_plain(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t 
                     const xtx_params* params, const int32_t* active, int32_t n_active,
                     int64_t* A_out, int64_t* X_out, xtx_stats* stats);
 
+/* The instance best as of the last epoch boundary of the running plain solve
+ * (L1: A and X; xdual: P, Q and X; each pointer nullable). Valid only from
+ * inside an on_best report (the reference's throttled best_*.csv writes,
+ * congruence_anneal.hpp:327-340); XTX_E_STATE elsewhere. */
+int xtx_best_so_far(xtx_ctx* ctx, int64_t* A_out, double* Q_out, int64_t* X_out);
+
 /* Per-replica state of the last plain solve (for replay and trajectory checks):
  * A, Xt (R*n*n each, nullable) and 8 scalars per replica
  * {score, offnz, moves, best_score, best_move, rung, accepts, local_best}. */

@@ -244,7 +244,19 @@ def test_new_best_reports(eng, po):
     n = 20
     A0 = gram(n, 120)
     seen = []
-    cb = N.ON_BEST(lambda user, t, score, dscore, rep: seen.append((score, dscore, rep)))
+    snaps = []
+
+    def on_best(user, t, score, dscore, rep):
+        seen.append((score, dscore, rep))
+        P1 = np.zeros((n, n), np.int64)
+        Q1 = np.zeros((n, n))
+        X1 = np.zeros((n, n), np.int64)
+        assert N.lib().xtx_best_so_far(eng._h, P1.ctypes.data_as(N.I64P),
+                                       Q1.ctypes.data_as(C.POINTER(C.c_double)),
+                                       X1.ctypes.data_as(N.I64P)) == 0
+        snaps.append((P1, Q1, X1))
+
+    cb = N.ON_BEST(on_best)
     p = EngineParams(replicas=8, seed=3, max_epochs=8, exchange_interval=300).to_c()
     p.on_best = C.cast(cb, C.c_void_p)
     d = DualParams().to_c()
@@ -261,3 +273,7 @@ def test_new_best_reports(eng, po):
     scores = [st.start_score] + [ds for _, ds, _ in seen]
     assert all(b < a for a, b in zip(scores, scores[1:]))
     assert all(s == 0 and 0 <= r < 8 for s, _, r in seen)
+    for P1, Q1, X1 in snaps:
+        Xo = X1.astype(object)
+        assert np.array_equal((Xo.T @ A0.astype(object) @ Xo).astype(np.int64), P1)
+        assert np.abs(P1.astype(float) @ Q1 - np.eye(n)).max() < 1e-4

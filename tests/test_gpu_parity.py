@@ -348,12 +348,22 @@ def test_new_best_reports(eng, orc):
     from xtax_amd import _native as N
     A0 = workloads.planted(24, 100, 72, seed=3)[0]
     seen = []
-    cb = N.ON_BEST(lambda user, t, score, dscore, rep: seen.append((t, score, dscore, rep)))
+    snaps = []
+
+    def on_best(user, t, score, dscore, rep):
+        seen.append((t, score, dscore, rep))
+        A1 = np.zeros((n, n), np.int64)
+        X1 = np.zeros((n, n), np.int64)
+        assert N.lib().xtx_best_so_far(eng._h, A1.ctypes.data_as(N.I64P), None,
+                                       X1.ctypes.data_as(N.I64P)) == 0
+        snaps.append((score, A1, X1))
+
+    n = A0.shape[0]
+    cb = N.ON_BEST(on_best)
     ep = EngineParams(replicas=16, seed=4, max_epochs=12, exchange_interval=500, t_init=18.0,
                       t_min=14.0)
     p = ep.to_c()
     p.on_best = C.cast(cb, C.c_void_p)
-    n = A0.shape[0]
     A = np.zeros((n, n), np.int64)
     X = np.zeros((n, n), np.int64)
     st = N.Stats()
@@ -371,3 +381,9 @@ def test_new_best_reports(eng, orc):
     assert len(seen) <= st.epochs
     if not st.solved:
         assert seen[-1][1] == st.best_score
+    # each report's snapshot is that best: X^T A0 X = A exactly, with its score
+    for score, A1, X1 in snaps:
+        status, sign, _ = orc.verify(A0, X1, A1)
+        assert status & ~2 == 0
+        assert int(2 * np.abs(A1).sum() - np.abs(np.diag(A1)).sum()) == score
+    assert N.lib().xtx_best_so_far(eng._h, None, None, None) != 0   # outside a report

@@ -135,6 +135,7 @@ SIGNATURES = {
     "xtx_solve_plain": (C.c_int, [C.c_void_p, I64P, I64P, C.c_int32, C.POINTER(Params), I32P,
                                   C.c_int32, I64P, I64P, C.POINTER(Stats)]),
     "xtx_last_replicas": (C.c_int, [C.c_void_p, I64P, I64P, I64P]),
+    "xtx_best_so_far": (C.c_int, [C.c_void_p, I64P, C.POINTER(C.c_double), I64P]),
     "xtx_movelog_get": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.POINTER(LogEntry)),
                                   I64P, I32P]),
     "xtx_movelog_free": (C.c_int, [C.c_void_p]),

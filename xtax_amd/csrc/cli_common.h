@@ -315,15 +315,36 @@ inline std::vector<Opt> engine_options(xtx_params& p, int& device) {
     };
 }
 
+// What a "new best" report does besides printing: the reference rewrites the
+// best_*.csv files on a new best at most every save_interval seconds
+// (congruence_anneal.hpp:327-340); `save` writes them from the state
+// xtx_best_so_far returns.
+struct BestReports {
+    xtx_ctx* ctx = nullptr;
+    double save_interval = 2.0;
+    double last_save = -1e300;
+    std::function<void()> save;
+};
+
+inline void maybe_save(BestReports* b, double seconds) {
+    if (b && b->save && seconds - b->last_save >= b->save_interval) {
+        b->save();
+        b->last_save = seconds;
+    }
+}
+
 // The reference's "new best" progress line (congruence_anneal.hpp:331-334)
 // from the engine's on_best report; the replica stands in for the thread.
-inline void print_new_best_l1(void*, double seconds, int64_t score, double, int32_t replica) {
+inline void print_new_best_l1(void* user, double seconds, int64_t score, double, int32_t replica) {
     std::cout << "[t=" << seconds << "s] new best score=" << score << " (thread " << replica
               << ")\n";
+    maybe_save(static_cast<BestReports*>(user), seconds);
 }
-inline void print_new_best_dual(void*, double seconds, int64_t, double score, int32_t replica) {
+inline void print_new_best_dual(void* user, double seconds, int64_t, double score,
+                                int32_t replica) {
     std::cout << "[t=" << seconds << "s] new best score=" << score << " (thread " << replica
               << ")\n";
+    maybe_save(static_cast<BestReports*>(user), seconds);
 }
 
 // --modulus value (>= 1)

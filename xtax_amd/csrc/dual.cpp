@@ -348,6 +348,54 @@ int xtx_dual_solve(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t n
         HIPCHK(hipHostMalloc((void**)&pin, sizeof(Inst), hipHostMallocDefault));
         HIPCHK(hipStreamSynchronize(sm));
 
+        // the solving replica's state if solved, else the instance best
+        // (argmin of best score, best move, replica): P, Q and X; returns the
+        // replica
+        auto pick = [&](const Inst& J, std::vector<RepState>& rsv, std::vector<int64_t>& Pv,
+                        std::vector<double>& Qv, std::vector<int64_t>& Xv) -> int {
+            rsv.resize((size_t)R);
+            HIPCHK(hipMemcpy(rsv.data(), J.rs, (size_t)R * sizeof(RepState),
+                             hipMemcpyDeviceToHost));
+            int w = 0;
+            bool cur = true;
+            if (J.status == ST_SOLVED) {
+                w = J.solve_rep;
+            } else {
+                for (int r = 1; r < R; ++r) {
+                    const RepState& x = rsv[(size_t)r];
+                    const RepState& y = rsv[(size_t)w];
+                    if (okey(x.dbest_score) < okey(y.dbest_score) ||
+                        (okey(x.dbest_score) == okey(y.dbest_score) && x.best_move < y.best_move))
+                        w = r;
+                }
+                const RepState& x = rsv[(size_t)w];
+                if (!(x.flags & RF_CUR_BEST) && (x.flags & RF_BEST_STALE))
+                    throw Err{XTX_E_STATE, "internal: best replica has a stale snapshot"};
+                cur = (x.flags & RF_CUR_BEST) != 0;
+            }
+            std::vector<int64_t> Xt(nn);
+            const size_t off = (size_t)w * nn;
+            HIPCHK(hipMemcpy(Pv.data(), (cur ? J.A : J.bA) + off, nn * 8, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(Xt.data(), (cur ? J.Xt : J.bXt) + off, nn * 8, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(Qv.data(), (cur ? J.Q : J.bQ) + off, nn * 8, hipMemcpyDeviceToHost));
+            for (int r = 0; r < n; ++r)
+                for (int c = 0; c < n; ++c) Xv[(size_t)r * n + c] = Xt[(size_t)c * n + r];
+            return w;
+        };
+        ctx->best_so_far = [&](int64_t* Po, double* Qo, int64_t* Xo) {
+            std::vector<RepState> r2;
+            std::vector<int64_t> P2(nn), X2(nn);
+            std::vector<double> Q2(nn);
+            pick(*pin, r2, P2, Q2, X2);
+            if (Po) std::memcpy(Po, P2.data(), nn * 8);
+            if (Qo) std::memcpy(Qo, Q2.data(), nn * 8);
+            if (Xo) std::memcpy(Xo, X2.data(), nn * 8);
+        };
+        struct Clear {
+            xtx_ctx* c;
+            ~Clear() { c->best_so_far = nullptr; }
+        } clear_snapshot{ctx};
+
         double kernel_ms = 0;
         int epochs = 0;
         int reason = 3;
@@ -405,39 +453,14 @@ int xtx_dual_solve(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t n
         I = *pin;
 
         // outcome: the solving replica's state, else the instance best
-        rs.resize((size_t)R);
-        HIPCHK(hipMemcpy(rs.data(), I.rs, (size_t)R * sizeof(RepState), hipMemcpyDeviceToHost));
-        int win;
-        bool current;
+        std::vector<int64_t> Pw(nn), X(nn);
+        std::vector<double> Qw(nn);
+        const int win = pick(I, rs, Pw, Qw, X);
         if (I.status == ST_SOLVED) {
-            win = I.solve_rep;
-            current = true;
             st.solved = 1;
             st.solve_replica = win;
             st.solve_move = I.solve_move;
-        } else {
-            win = 0;
-            for (int r = 1; r < R; ++r) {
-                const RepState& x = rs[(size_t)r];
-                const RepState& y = rs[(size_t)win];
-                if (okey(x.dbest_score) < okey(y.dbest_score) ||
-                    (okey(x.dbest_score) == okey(y.dbest_score) && x.best_move < y.best_move))
-                    win = r;
-            }
-            const RepState& x = rs[(size_t)win];
-            if (!(x.flags & RF_CUR_BEST) && (x.flags & RF_BEST_STALE))
-                throw Err{XTX_E_STATE, "internal: best replica has a stale snapshot"};
-            current = (x.flags & RF_CUR_BEST) != 0;
         }
-        std::vector<int64_t> Pw(nn), Xtw(nn);
-        std::vector<double> Qw(nn);
-        const size_t off = (size_t)win * nn;
-        HIPCHK(hipMemcpy(Pw.data(), (current ? I.A : I.bA) + off, nn * 8, hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(Xtw.data(), (current ? I.Xt : I.bXt) + off, nn * 8, hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(Qw.data(), (current ? I.Q : I.bQ) + off, nn * 8, hipMemcpyDeviceToHost));
-        std::vector<int64_t> X(nn);
-        for (int r = 0; r < n; ++r)
-            for (int c = 0; c < n; ++c) X[(size_t)r * n + c] = Xtw[(size_t)c * n + r];
         if (P_out) std::memcpy(P_out, Pw.data(), nn * 8);
         if (Q_out) std::memcpy(Q_out, Qw.data(), nn * 8);
         if (X_out) std::memcpy(X_out, X.data(), nn * 8);

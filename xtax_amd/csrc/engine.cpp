@@ -642,6 +642,15 @@ int xtx_solve_plain(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t 
         int epochs = 0;
         int reason = 3;
         int64_t reported = h.score0;  // the start state is never reported
+        ctx->best_so_far = [ctx, &h, &pin](int64_t* A, double*, int64_t* X) {
+            Outcome o;
+            fetch_outcome(h, *pin, ctx->stream, o);
+            write_outputs(h, o, A, X);
+        };
+        struct Clear {
+            xtx_ctx* c;
+            ~Clear() { c->best_so_far = nullptr; }
+        } clear_snapshot{ctx};
         {
             Deadline dl(ctx->stop_host, p.max_seconds);
             for (;;) {
@@ -720,6 +729,18 @@ int xtx_solve_plain(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t 
             ctx->err = "result failed exact verification";
             return XTX_E_VERIFY;
         }
+        return XTX_OK;
+    } catch (const Err& e) {
+        return fail(ctx, e);
+    }
+}
+
+int xtx_best_so_far(xtx_ctx* ctx, int64_t* A_out, double* Q_out, int64_t* X_out) {
+    if (!ctx) return XTX_E_ARG;
+    try {
+        if (!ctx->best_so_far)
+            throw Err{XTX_E_STATE, "xtx_best_so_far is valid only inside an on_best report"};
+        ctx->best_so_far(A_out, Q_out, X_out);
         return XTX_OK;
     } catch (const Err& e) {
         return fail(ctx, e);

@@ -9,6 +9,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <cstdint>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -257,6 +258,9 @@ struct xtx_ctx {
     std::vector<int64_t> plain_log_count;
     bool plain_log_valid = false;
     std::unique_ptr<Server> server;
+    // set by a running plain solve (L1 or xdual) for xtx_best_so_far: writes
+    // the instance best as of the last epoch boundary (A or P, Q, X)
+    std::function<void(int64_t*, double*, int64_t*)> best_so_far;
     // launch configuration per padded order, set by xtx_launch_set or
     // xtx_autotune; other orders use layout_for's rule
     std::map<int, xtx::host::Layout> tuned;

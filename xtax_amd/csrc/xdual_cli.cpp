@@ -153,7 +153,21 @@ int main(int argc, char** argv) {
     Mat bP = square(n), bX = square(n);
     std::vector<double> bQ((size_t)n * n);
     p.verify = 1;
+    // mid-run saves of the best (DualObjective::publish_files,
+    // xdual_main.cpp:351-355)
+    BestReports reports;
+    reports.ctx = g_ctx;
+    reports.save_interval = p.save_interval;
+    reports.save = [&]() {
+        Mat sP = square(n), sX = square(n);
+        std::vector<double> sQ((size_t)n * n);
+        if (xtx_best_so_far(g_ctx, sP.v.data(), sQ.data(), sX.v.data()) != XTX_OK) return;
+        write_csv(sX, "best_X.csv");
+        write_csv(sP, "best_P.csv");
+        write_csv_d(sQ, n, "best_Q.csv");
+    };
     p.on_best = print_new_best_dual;
+    p.on_best_user = &reports;
     std::cout << "[xdual] inverting the Gram (double-precision dual P^-1)...\n";
     xtx_dual_stats st{};
     const int rc = xtx_dual_solve(g_ctx, A.v.data(), (have_A && !x_csv.empty()) ? X.v.data() : nullptr,

@@ -289,7 +289,21 @@ int main(int argc, char** argv) {
                   << p.replicas << ((p.tempering && p.replicas >= 2) ? " (parallel tempering)" : "")
                   << "\n";
         xtx_stats st{};
+        // mid-run saves of the best (publish_files, main.cpp:413-416), reordered
+        // like every published best under --rcm/--centroid
+        const int pub_flags = (rcm ? XTX_REORDER_RCM : 0) | (centroid ? XTX_REORDER_CENTROID : 0);
+        BestReports reports;
+        reports.ctx = g_ctx;
+        reports.save_interval = p.save_interval;
+        reports.save = [&]() {
+            Mat sA = square(n), sX = square(n);
+            if (xtx_best_so_far(g_ctx, sA.v.data(), nullptr, sX.v.data()) != XTX_OK) return;
+            if (pub_flags) xtx_reorder(sA.v.data(), sX.v.data(), n, pub_flags, nullptr);
+            write_csv(sX, "best_X.csv");
+            write_csv(sA, "best_A.csv");
+        };
         p.on_best = print_new_best_l1;
+        p.on_best_user = &reports;
         const int rc = xtx_solve_plain(
             g_ctx, A.v.data(), (have_A && !x_csv.empty()) ? X.v.data() : nullptr, n, &p,
             rows.empty() ? nullptr : rows.data(), (int32_t)rows.size(), bA.v.data(), bX.v.data(),
