@@ -158,7 +158,8 @@ def bench_batch(args, eng):
     p = EngineParams(replicas=args.replicas, t_init=args.t_init, t_min=args.t_min,
                      greedy_fraction=0.5, target_fraction=args.target_fraction,
                      exchange_interval=args.exchange_interval,
-                     max_epochs=args.cap_epochs, seed=args.seed, verify=True)
+                     max_epochs=args.cap_epochs, seed=args.seed, verify=True,
+                     lookahead=args.lookahead)
     tune = None
     if not args.no_tune:
         # launch configuration per order bucket, measured once per device
@@ -234,7 +235,8 @@ def bench_plain(args, eng):
         p = EngineParams(replicas=args.replicas, t_init=args.t_init, t_min=args.t_min,
                          greedy_fraction=0.5, target_fraction=args.target_fraction,
                          exchange_interval=args.exchange_interval,
-                         max_epochs=args.cap_epochs, seed=seeds[q], verify=True)
+                         max_epochs=args.cap_epochs, seed=seeds[q], verify=True,
+                         lookahead=args.lookahead)
         t1 = time.perf_counter()
         r = eng.run_annealer(A, None, p)
         moves += r.stats["moves"]
@@ -261,7 +263,7 @@ def bench_serve(args, eng):
     p = EngineParams(replicas=64, t_init=args.t_init, t_min=args.t_min, greedy_fraction=0.5,
                      target_fraction=args.target_fraction,
                      exchange_interval=args.exchange_interval, max_epochs=args.cap_epochs,
-                     seed=args.seed, verify=True)
+                     seed=args.seed, verify=True, lookahead=args.lookahead)
     eng.serve_open(p)
     lat_solved, lat_all = [], []
     solved = verified = moves = 0
@@ -332,7 +334,8 @@ def bench_large(args, eng):
     A0 = workloads.planted(80, 1000, 320, seed=5)[0]
     p = EngineParams(replicas=4096, greedy_fraction=0.7, t_init=args.t_init,
                      t_min=args.t_min, exchange_interval=args.exchange_interval,
-                     max_seconds=args.seconds, seed=args.seed, verify=True)
+                     max_seconds=args.seconds, seed=args.seed, verify=True,
+                     lookahead=args.lookahead)
     return bench_single(args, eng, "large: planted n=80, D=+-U{1..1000}, 4n shears, 4096 "
                         "replicas, greedy 0.7", A0, p,
                         ["--greedy-fraction", "0.7", "--t-init", str(args.t_init),
@@ -343,7 +346,7 @@ def bench_lattice(args, eng):
     L, G = workloads.lattice(24, 24, seed=2)
     p = EngineParams(replicas=256, greedy_fraction=0.5, sweep_fraction=0.05,
                      exchange_interval=args.exchange_interval, max_seconds=args.seconds,
-                     seed=args.seed, verify=True)
+                     seed=args.seed, verify=True, lookahead=args.lookahead)
     return bench_single(args, eng, "lattice: 24x24 basis U[-9,9], Gram, 256 replicas, greedy "
                         "0.5, sweep 0.05", G, p,
                         ["--greedy-fraction", "0.5", "--sweep-fraction", "0.05"])
@@ -357,7 +360,7 @@ def bench_random(args, eng):
         A0 = workloads.random_symmetric(10, -20, 20, seed=s)
         p = EngineParams(replicas=8, t_init=50.0, t_min=0.5, greedy_fraction=0.5,
                          exchange_interval=args.exchange_interval, max_epochs=args.cap_epochs,
-                         seed=s, verify=True)
+                         seed=s, verify=True, lookahead=args.lookahead)
         r = eng.run_annealer(A0, None, p)
         solved += r.stats["solved"]
         moves += r.stats["moves"]
@@ -462,6 +465,8 @@ def main():
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--seconds", type=float, default=10.0)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--lookahead", type=int, default=0,
+                    help="proposals of a replica evaluated at once (0: the engine's rule)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--cpu-cap", type=float, default=None)
     ap.add_argument("--no-cpu", action="store_true")

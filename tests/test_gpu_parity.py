@@ -159,15 +159,19 @@ def _matrix(name):
     raise KeyError(name)
 
 
+# lookahead 0: the plain form's rule (4 groups per replica here: few
+# replicas), 1: one group per replica, 2: two groups
+@pytest.mark.parametrize("look", [0, 1, 2], ids=["auto", "K1", "K2"])
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
-def test_trajectory_matches_model(eng, orc, case):
+def test_trajectory_matches_model(eng, orc, case, look):
     name, mat, kw = case
     A0 = _matrix(mat)
     n = A0.shape[0]
     epochs = 3
     cap = 200000
     ep, op = _both(max_epochs=epochs, move_log=True, move_log_cap=cap, **kw)
-    say(f"trajectory {name} n={n} R={ep.replicas}")
+    ep.lookahead = look
+    say(f"trajectory {name} n={n} R={ep.replicas} lookahead={look}")
     res = eng.run_annealer(A0, None, ep)
     model = orc.model_run(A0, np.eye(n, dtype=np.int64), op, epochs, log_cap=cap)
     st = res.stats
@@ -312,6 +316,33 @@ def test_every_launch_configuration_gives_the_same_result(eng, n):
     finally:
         eng.launch_reset(order)
     assert tried >= 2
+
+
+def test_lookahead_changes_speed_only(eng):
+    """Evaluating a replica's next proposals at once (lookahead 2 or 4: one
+    wave per replica, its lane groups resolving the proposals in move order)
+    gives the batch result of one group per replica, bit for bit, at every
+    order (above 64 the engine keeps one group), with stall sweeps among the
+    moves; a lookahead other than 0, 1, 2 or 4 is refused."""
+    from xtax_amd._native import XtxError
+    mats = [workloads.planted(n, 20, 2 * n, seed=500 + n)[0] for n in (5, 12, 24, 31, 40, 60, 70)]
+    mats.append(workloads.random_symmetric(16, -20, 20, seed=8))
+    seeds = [900 + q for q in range(len(mats))]
+    base = dict(replicas=6, t_init=30, t_min=0.5, exchange_interval=300, max_epochs=6,
+                sweep_fraction=0.3, stuck_threshold=200)
+    ref, _ = eng.batch_solve(mats, EngineParams(lookahead=1, **base), seeds=seeds)
+    for look in (2, 4):
+        say(f"batch lookahead={look}")
+        got, _ = eng.batch_solve(mats, EngineParams(lookahead=look, **base), seeds=seeds)
+        for q in range(len(mats)):
+            A, X, d = got[q]
+            assert d["verify_status"] == 0
+            assert np.array_equal(A, ref[q][0]) and np.array_equal(X, ref[q][1]), (q, look)
+            keys = ("solved", "solve_move", "best_score") + (() if d["solved"] else ("moves",))
+            for k in keys:
+                assert d[k] == ref[q][2][k], (k, q, look)
+    with pytest.raises(XtxError):
+        eng.run_annealer(mats[0], None, EngineParams(lookahead=3, replicas=2, max_epochs=1))
 
 
 def test_autotune_picks_a_measured_configuration(eng):

@@ -31,7 +31,7 @@ class Params(C.Structure):
         ("seed", C.c_uint64),
         ("replicas", C.c_int32), ("move_log", C.c_int32), ("move_log_cap", C.c_int64),
         ("max_moves", C.c_uint64), ("max_epochs", C.c_int32), ("verify", C.c_int32),
-        ("waves_per_block", C.c_int32), ("autotune", C.c_int32),
+        ("waves_per_block", C.c_int32), ("autotune", C.c_int32), ("lookahead", C.c_int32),
         ("on_best", C.c_void_p), ("on_best_user", C.c_void_p),
     ]
 

@@ -569,26 +569,55 @@ struct Replica {
     }
 };
 
-// Four waves per SIMD: at most 128 VGPRs (512 per SIMD lane).
-template <int G, int C, bool TRI>
-__global__ void __launch_bounds__(256, 4)
+// int64 value of lane `src` (runtime, per lane) of the wave, in every lane
+__device__ __forceinline__ int64_t bperm64(int64_t v, int src) {
+    const int lo = __builtin_amdgcn_ds_bpermute(src << 2, (int)(uint32_t)(uint64_t)v);
+    const int hi = __builtin_amdgcn_ds_bpermute(src << 2, (int)(uint32_t)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+    return __builtin_bit_cast(double, readlane64(__builtin_bit_cast(int64_t, v), lane));
+}
+
+// One epoch of many replicas.
+//
+// K == 1: one replica per lane group, 64/G replicas per wave; four waves per
+// SIMD (at most 128 VGPRs).
+//
+// K > 1 (G K == 64, lookahead): one replica per wave. Its K groups evaluate the
+// replica's next K proposals (moves m+1 .. m+K) on its current state at once,
+// and the moves are then resolved in order: each is decided exactly as the
+// one-group loop decides it, and the first that changes the state (a
+// committed move, a sweep commit, a reseed) ends the step; the proposals after
+// it are made again, on the new state, in the next step. A move that leaves
+// the state unchanged leaves every later proposal valid, so each resolved move
+// is evaluated on exactly the state, and decided with exactly the temperature,
+// that the sequential loop would use: the trajectory is the same, K moves per
+// step while proposals are rejected. Used where replicas are too few to fill
+// the device (the annealer rejects most proposals: 0.2% to 5% are accepted at
+// the workloads' settings).
+template <int G, int C, bool TRI, int K>
+__global__ void __launch_bounds__(256, K == 1 ? 4 : 2)
     k_anneal(const RepRef* __restrict__ table, int count, Inst* __restrict__ insts, int nmax,
              int rep_bytes, const volatile int* __restrict__ stop) {
+    static_assert(K == 1 || G * K == 64, "lookahead groups fill one wave");
     extern __shared__ __align__(16) unsigned char smem[];
     using Gp = Grp<G>;
+    constexpr int U = G * K;  // lanes per replica
     const int g = (int)threadIdx.x & (G - 1);
-    const int slot = (int)threadIdx.x / G;
-    const int gid = (int)blockIdx.x * ((int)blockDim.x / G) + slot;
+    const int kk = K == 1 ? 0 : (int)(threadIdx.x & 63u) / G;  // lookahead index of the group
+    const int slot = (int)threadIdx.x / U;
+    const int gid = (int)blockIdx.x * ((int)blockDim.x / U) + slot;
     if (gid >= count) return;
     // Every group of a wave belongs to the same instance (the host pads each
     // instance's replica list to whole waves), so the instance record is read
     // through scalar registers.
     const int wave_gid0 =
-        __builtin_amdgcn_readfirstlane((int)blockIdx.x * ((int)blockDim.x / G) +
-                                       (int)(threadIdx.x >> 6) * (64 / G));
+        __builtin_amdgcn_readfirstlane((int)blockIdx.x * ((int)blockDim.x / U) +
+                                       (int)(threadIdx.x >> 6) * (64 / U));
     Inst* I = insts + __builtin_amdgcn_readfirstlane(table[wave_gid0].inst);
     if (I->status != ST_RUNNING) return;
-    const int r = G == 64 ? __builtin_amdgcn_readfirstlane(table[gid].rep) : table[gid].rep;
+    const int r = U == 64 ? __builtin_amdgcn_readfirstlane(table[gid].rep) : table[gid].rep;
     if (r < 0) return;  // padding group
     GLOBAL RepState* S = (GLOBAL RepState*)I->rs + r;
     if (S->flags & RF_STOPPED) return;
@@ -597,11 +626,13 @@ __global__ void __launch_bounds__(256, 4)
     rp.sT = (int64_t*)(smem + (size_t)slot * rep_bytes);
     rp.P = nmax + 1;
     // A: nmax (nmax + 1) / 2 or nmax (nmax + 1) int64, a multiple of 32 bytes
-    // for nmax % 8 == 0
-    uint4* swords = (uint4*)(rp.sT + (TRI ? (size_t)nmax * (nmax + 1) / 2
-                                          : (size_t)nmax * (nmax + 1)));
-    uint8_t* sact = (uint8_t*)(swords + 5);
-    uint32_t* shist = (uint32_t*)(sact + nmax);
+    // for nmax % 8 == 0; then five staged Philox blocks and a median
+    // histogram per group, and the active list
+    uint4* const words0 = (uint4*)(rp.sT + (TRI ? (size_t)nmax * (nmax + 1) / 2
+                                                : (size_t)nmax * (nmax + 1)));
+    uint4* const swords = words0 + 5 * kk;
+    uint8_t* sact = (uint8_t*)(words0 + 5 * K);
+    uint32_t* shist = (uint32_t*)(sact + nmax) + kk * med_slots(G);
     rp.act = sact;
     rp.n = I->n;
     rp.N = I->N;
@@ -640,18 +671,18 @@ __global__ void __launch_bounds__(256, 4)
     const int32_t per_cool = I->P.moves_per_cool;
     const bool logging = I->log != nullptr;
 
-    uint64_t moves = (uint64_t)uni<G>((int64_t)S->moves);
-    int64_t score = uni<G>(S->score), offnz = uni<G>(S->offnz);
-    int64_t local_best = uni<G>(S->local_best);
+    uint64_t moves = (uint64_t)uni<U>((int64_t)S->moves);
+    int64_t score = uni<U>(S->score), offnz = uni<U>(S->offnz);
+    int64_t local_best = uni<U>(S->local_best);
     // moves since improvement: reset once above the threshold (an int32)
-    uint32_t msi = (uint32_t)uni<G>((int32_t)S->msi);
+    uint32_t msi = (uint32_t)uni<U>((int32_t)S->msi);
     double T = tempering ? ((const GLOBAL double*)I->temps)[S->rung] : S->T;
     double nbeta = -1.0 / T;  // the Metropolis exponent is delta * (-1/T)
     double cooling = S->cooling;
-    int32_t cool_counter = uni<G>(S->cool_counter), win_p = uni<G>(S->win_p);
-    int32_t win_a = uni<G>(S->win_a);
-    int64_t best_score = uni<G>(S->best_score), best_move = uni<G>(S->best_move);
-    int32_t flags = uni<G>(S->flags);
+    int32_t cool_counter = uni<U>(S->cool_counter), win_p = uni<U>(S->win_p);
+    int32_t win_a = uni<U>(S->win_a);
+    int64_t best_score = uni<U>(S->best_score), best_move = uni<U>(S->best_move);
+    int32_t flags = uni<U>(S->flags);
     const Coin greedy_coin(S->gf), target_coin(S->tf);
     // statistics counted from the epoch start (an epoch stays far below 2^32
     // events), added to the replica record at its end
@@ -664,8 +695,10 @@ __global__ void __launch_bounds__(256, 4)
             const Inst* Iq = opaque(I);
             if (best_score <= Iq->gb_score) {
                 const int64_t off = (int64_t)r * n * n;
-                rp.store_A((GLOBAL int64_t*)Iq->bA + off);
-                rp.copy_cols((GLOBAL int64_t*)Iq->bXt + off, rp.gX);
+                if (kk == 0) {  // lookahead groups hold the same state
+                    rp.store_A((GLOBAL int64_t*)Iq->bA + off);
+                    rp.copy_cols((GLOBAL int64_t*)Iq->bXt + off, rp.gX);
+                }
                 flags &= ~RF_BEST_STALE;
             } else {
                 flags |= RF_BEST_STALE;
@@ -677,7 +710,7 @@ __global__ void __launch_bounds__(256, 4)
         if (logging) {
             const Inst* Iq = opaque(I);
             const int64_t at = opaque_v(S)->log_count + logged;
-            if (at < Iq->log_cap && g == 0) {
+            if (at < Iq->log_cap && g == 0 && kk == 0) {
                 GLOBAL LogEnt* e = (GLOBAL LogEnt*)Iq->log + (int64_t)r * Iq->log_cap + at;
                 e->move = (int64_t)m;
                 e->i = (int16_t)i;
@@ -689,7 +722,7 @@ __global__ void __launch_bounds__(256, 4)
         }
     };
     // Greedy reduction sweep from a pivot (congruence_anneal.hpp:246-260).
-    auto sweep = [&](int pv, uint64_t m) -> bool {
+    auto sweep = [&](int pv, uint64_t m) __attribute__((always_inline)) -> bool {
         const int64_t before = score;
         ++sweeps;
         for (int q = 0; q < N; ++q) {
@@ -702,16 +735,26 @@ __global__ void __launch_bounds__(256, 4)
             const int64_t a_jj = rp.entry(jj, jj);
             const bool small = rows_small<G, C>(ai, aj);
             int64_t soff = 0;
-            const int64_t s =
+            int64_t s =
                 best_shear_any<G, C>(ai, aj, small, g, n, jj, a_ii, a_ji, a_jj, shist, &soff);
+            // lookahead groups all run the same sweep: their decisions are
+            // wave-uniform
+            if constexpr (K > 1) s = uni<64>(s);
             ++greedy_n;
             if (s == 0) continue;
             MoveEval<C> e;
             evaluate_any<G, C>(ai, aj, small, g, n, jj, s, a_ii, a_ji, a_jj, e, true, soff);
+            if constexpr (K > 1) {
+                e.bad = uni<64>((int32_t)e.bad) != 0;
+                e.delta = uni<64>(e.delta);
+                e.dnz = uni<64>(e.dnz);
+            }
             if (e.bad || e.delta > 0) continue;
             int64_t xi[C], xj[C], nx[C];
             rp.prefetch_xt(pv, jj, xi, xj);
-            if (!rp.xt_row(s, xi, xj, nx)) continue;
+            bool fits = rp.xt_row(s, xi, xj, nx);
+            if constexpr (K > 1) fits = uni<64>((int32_t)fits) != 0;
+            if (!fits) continue;
             leave_best();
             rp.write(jj, aj, e, nx);
             score += e.delta;
@@ -730,78 +773,14 @@ __global__ void __launch_bounds__(256, 4)
         }
     };
 
-    uint4 w = make_uint4(0u, 0u, 0u, 0u);
-    bool have_w = false;
-    bool halt = false;
-    for (uint32_t it = 0; moves < stop_at; ++it) {
-        if ((it & 255u) == 0u) {
-            const unsigned long long key =
-                __hip_atomic_load(&I->solve_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (key != kNoKey && moves >= (key >> kRepBits)) break;
-            if (*stop) {
-                halt = true;
-                break;
-            }
-        }
-        const uint64_t m = ++moves;
-        // Philox slots of move m: counter (q, r, 16 t + c), q = (m-1)/4,
-        // t = (m-1) mod 4, c < 16. A group of G lanes computes G slots per
-        // call (lane = slot - 16 (t - sub)), i.e. one call serves G/16 moves;
-        // this move's slot c sits in lane base + c.
-        constexpr int S = G / 16;
-        const uint32_t t4 = (uint32_t)((m - 1) & 3u);
-        const int sub = (int)(t4 & (uint32_t)(S - 1));
-        if (sub == 0 || !have_w) {
-            const uint64_t q = (m - 1) >> 2;
-            uint32_t k0 = key0, k1 = key1;  // round keys made here, not held
-            opaque_u32(k0);
-            opaque_u32(k1);
-            w = philox10((uint32_t)q, (uint32_t)(q >> 32), (uint32_t)r,
-                         16u * (t4 & ~(uint32_t)(S - 1)) + (uint32_t)g, k0, k1);
-            have_w = true;
-        }
-        const int base = 16 * sub;
-        const int cs = g - base;  // this lane's slot for move m (valid in [0, 16))
-        // words of slots 0..4 to every lane of the group: v_readlane for a
-        // whole-wave group (the ones only some moves need are read where they
-        // are used), an LDS bounce otherwise
-        uint32_t w0z, w0w, w1z, w2w, w3z;
-        uint32_t l1w = 0, l2z = 0, l3w = 0, l4z = 0, l4w = 0;
-        if constexpr (G == 64) {
-            w0z = readlane32(w.z, base + 0);
-            w0w = readlane32(w.w, base + 0);
-            w1z = readlane32(w.z, base + 1);
-            w2w = readlane32(w.w, base + 2);
-            w3z = readlane32(w.z, base + 3);
-        } else {
-            if (cs >= 0 && cs < 5) swords[cs] = w;
-            __builtin_amdgcn_wave_barrier();
-            const uint4 b0 = swords[0], b1 = swords[1], b2 = swords[2], b3 = swords[3];
-            const uint4 b4 = swords[4];
-            __builtin_amdgcn_wave_barrier();
-            w0z = b0.z;
-            w0w = b0.w;
-            w1z = b1.z;
-            l1w = b1.w;
-            l2z = b2.z;
-            w2w = b2.w;
-            w3z = b3.z;
-            l3w = b3.w;
-            l4z = b4.z;
-            l4w = b4.w;
-        }
-        auto word = [&](int slot, bool zw, uint32_t staged) -> uint32_t {
-            if constexpr (G == 64) return readlane32(zw ? w.w : w.z, base + slot);
-            return staged;
-        };
-        auto w1w = [&]() { return word(1, true, l1w); };
-        auto w2z = [&]() { return word(2, false, l2z); };
-        auto w3w = [&]() { return word(3, true, l3w); };
-        auto w4z = [&]() { return word(4, false, l4z); };
-        auto w4w = [&]() { return word(4, true, l4w); };
-
-        // ---- pair (congruence_anneal.hpp:283-288, :211-228) ----
-        int i, j;
+    // Steps of the lookahead loop; the one-group loop spells the same steps
+    // out inline.
+    // ---- pair (congruence_anneal.hpp:283-288, :211-228) ----
+    // `w` is this lane's Philox block; the move's slot c sits in lane base + c
+    // (cs = g - base), in the group's 16-lane row `sub`.
+    auto choose_pair = [&](const uint4& w, int base, int cs, int sub, uint32_t w0z,
+                           uint32_t w2w, uint32_t w3z, auto w3w, int& i,
+                           int& j) __attribute__((always_inline)) {
         if (target_coin(w0z)) {
             // hot-row tournament: first maximum of row_mass over the draws,
             // as one max over keys (mass << 5 | 16 | 15 - draw index; 0 for
@@ -835,52 +814,14 @@ __global__ void __launch_bounds__(256, 4)
             i = rowat(ip);
             j = rowat(jp);
         }
+    };
 
-        int64_t ai[C], aj[C], xi[C], xj[C];
-        rp.load_rows(i, j, ai, aj);
-        rp.prefetch_xt(i, j, xi, xj);
-        const int64_t a_ii = rp.entry(i, i);
-        const int64_t a_ji = rp.entry(j, i);
-        const int64_t a_jj = rp.entry(j, j);
-
-        // ---- shear (congruence_anneal.hpp:290-296) ----
-        const bool small = rows_small<G, C>(ai, aj);
-        const int64_t pm = (w1z & 1u) ? 1 : -1;
-        int64_t s = pm, soff = 0;
-        bool know = false;
-        if (greedy_coin(w0w)) {
-            s = best_shear_any<G, C>(ai, aj, small, g, n, j, a_ii, a_ji, a_jj, shist, &soff);
-            ++greedy_n;
-            know = s != 0;
-            if (s == 0) s = pm;
-        }
-
-        // ---- attempt_add (congruence_anneal.hpp:232-240) ----
-        MoveEval<C> e;
-        evaluate_any<G, C>(ai, aj, small, g, n, j, s, a_ii, a_ji, a_jj, e, know, soff);
-        int res = -1;
-        if (e.bad) {
-            ++infeasible;
-        } else if (e.delta <= 0 || u53(w1w(), w2z()) < xexp((double)e.delta * nbeta)) {
-            int64_t nx[C];
-            if (rp.xt_row(s, xi, xj, nx)) {
-                const int64_t ns = score + e.delta;
-                if (ns < best_score) {
-                    best_score = ns;
-                    best_move = (int64_t)m;
-                    flags |= RF_CUR_BEST;
-                    flags &= ~RF_BEST_STALE;
-                } else {
-                    leave_best();
-                }
-                rp.write(j, aj, e, nx);
-                score = ns;
-                offnz += e.dnz;
-                ++accepts;
-                log_move(m, i, j, s, 0);
-                res = e.delta < 0 ? 1 : 0;
-            }
-        }
+    // Bookkeeping after move m (res: -1 rejected, 0 committed, 1 improving)
+    // and the stall handling that follows it (congruence_anneal.hpp:363-412).
+    // True when the replica is solved; `changed` is set when a sweep commit or
+    // a reseed changed the state.
+    auto finish = [&](uint64_t m, int res, auto w4z, auto w4w,
+                      bool& changed) __attribute__((always_inline)) -> bool {
         ++win_p;
         if (res >= 0) ++win_a;
         if (score < local_best) {
@@ -891,10 +832,9 @@ __global__ void __launch_bounds__(256, 4)
         }
         if (offnz == 0) {
             flags |= RF_SOLVED | RF_STOPPED;
-            break;
+            return true;
         }
-
-        // ---- stall handling (congruence_anneal.hpp:363-412) ----
+        const uint32_t accepts0 = accepts;
         if (tempering) {
             if ((int64_t)msi > stuck) {
                 const InstParams& Q = opaque(I)->P;
@@ -928,6 +868,7 @@ __global__ void __launch_bounds__(256, 4)
                     score = gb_epoch;
                     offnz = Iv->gb_offnz;
                     track_best(m);
+                    changed = true;
                 }
                 local_best = score;
                 msi = 0;
@@ -950,14 +891,395 @@ __global__ void __launch_bounds__(256, 4)
                 nbeta = -1.0 / T;
             }
         }
+        changed |= accepts != accepts0;
         if (offnz == 0) {
             flags |= RF_SOLVED | RF_STOPPED;
-            break;
+            return true;
+        }
+        return false;
+    };
+
+    bool halt = false;
+    // the solve key and the stop flag, every 256 steps
+    auto poll = [&](uint32_t it) -> bool {
+        if ((it & 255u) == 0u) {
+            const unsigned long long key =
+                __hip_atomic_load(&I->solve_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (key != kNoKey && moves >= (key >> kRepBits)) return true;
+            if (*stop) {
+                halt = true;
+                return true;
+            }
+        }
+        return false;
+    };
+    // Philox slots of move m: counter (q, r, 16 t + c), q = (m-1)/4,
+    // t = (m-1) mod 4, c < 16. A group of G lanes computes G slots per call
+    // (lane = slot - 16 (t - sub)), i.e. one call serves G/16 moves; the
+    // move's slot c sits in lane base + c, base = 16 sub.
+    constexpr int SB = G / 16;
+
+    if constexpr (K == 1) {
+        // (the steps inline: composed from the helpers above, this loop
+        // needs more registers than four waves per SIMD allow)
+        uint4 w = make_uint4(0u, 0u, 0u, 0u);
+        bool have_w = false;
+        for (uint32_t it = 0; moves < stop_at; ++it) {
+            if ((it & 255u) == 0u) {
+                const unsigned long long key =
+                    __hip_atomic_load(&I->solve_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (key != kNoKey && moves >= (key >> kRepBits)) break;
+                if (*stop) {
+                    halt = true;
+                    break;
+                }
+            }
+            const uint64_t m = ++moves;
+            // Philox slots of move m: counter (q, r, 16 t + c), q = (m-1)/4,
+            // t = (m-1) mod 4, c < 16. A group of G lanes computes G slots per
+            // call (lane = slot - 16 (t - sub)), i.e. one call serves G/16 moves;
+            // this move's slot c sits in lane base + c.
+            constexpr int S = G / 16;
+            const uint32_t t4 = (uint32_t)((m - 1) & 3u);
+            const int sub = (int)(t4 & (uint32_t)(S - 1));
+            if (sub == 0 || !have_w) {
+                const uint64_t q = (m - 1) >> 2;
+                uint32_t k0 = key0, k1 = key1;  // round keys made here, not held
+                opaque_u32(k0);
+                opaque_u32(k1);
+                w = philox10((uint32_t)q, (uint32_t)(q >> 32), (uint32_t)r,
+                             16u * (t4 & ~(uint32_t)(S - 1)) + (uint32_t)g, k0, k1);
+                have_w = true;
+            }
+            const int base = 16 * sub;
+            const int cs = g - base;  // this lane's slot for move m (valid in [0, 16))
+            // words of slots 0..4 to every lane of the group: v_readlane for a
+            // whole-wave group (the ones only some moves need are read where they
+            // are used), an LDS bounce otherwise
+            uint32_t w0z, w0w, w1z, w2w, w3z;
+            uint32_t l1w = 0, l2z = 0, l3w = 0, l4z = 0, l4w = 0;
+            if constexpr (G == 64) {
+                w0z = readlane32(w.z, base + 0);
+                w0w = readlane32(w.w, base + 0);
+                w1z = readlane32(w.z, base + 1);
+                w2w = readlane32(w.w, base + 2);
+                w3z = readlane32(w.z, base + 3);
+            } else {
+                if (cs >= 0 && cs < 5) swords[cs] = w;
+                __builtin_amdgcn_wave_barrier();
+                const uint4 b0 = swords[0], b1 = swords[1], b2 = swords[2], b3 = swords[3];
+                const uint4 b4 = swords[4];
+                __builtin_amdgcn_wave_barrier();
+                w0z = b0.z;
+                w0w = b0.w;
+                w1z = b1.z;
+                l1w = b1.w;
+                l2z = b2.z;
+                w2w = b2.w;
+                w3z = b3.z;
+                l3w = b3.w;
+                l4z = b4.z;
+                l4w = b4.w;
+            }
+            auto word = [&](int slot, bool zw, uint32_t staged) -> uint32_t {
+                if constexpr (G == 64) return readlane32(zw ? w.w : w.z, base + slot);
+                return staged;
+            };
+            auto w1w = [&]() { return word(1, true, l1w); };
+            auto w2z = [&]() { return word(2, false, l2z); };
+            auto w3w = [&]() { return word(3, true, l3w); };
+            auto w4z = [&]() { return word(4, false, l4z); };
+            auto w4w = [&]() { return word(4, true, l4w); };
+
+            // ---- pair (congruence_anneal.hpp:283-288, :211-228) ----
+            int i, j;
+            if (target_coin(w0z)) {
+                // hot-row tournament: first maximum of row_mass over the draws,
+                // as one max over keys (mass << 5 | 16 | 15 - draw index; 0 for
+                // no draw) within the 16-lane row holding this move's slots
+                const bool in = cs >= 0 && cs < TS;
+                const uint32_t p = in ? mulhi32(w.x, (uint32_t)N) : 0u;
+                const int row = rowat(p);
+                const int64_t wt = rp.mass_of(row);
+                const uint64_t mx =
+                    Gp::slot_max(in ? (((uint64_t)wt << 5) | 16u | (uint32_t)(15 - cs)) : 0ull, sub);
+                const uint32_t tpos = Gp::bcast(p, base + 15 - (int)(mx & 15));
+                const int target = rowat(tpos);
+                // pivot tournament: first maximum of |A[target][c]| over draws c != target
+                const uint32_t pp = in ? mulhi32(w.y, (uint32_t)N) : 0u;
+                const int cand = rowat(pp);
+                const bool valid = in && cand != target;
+                const int64_t v = valid ? iabs64(rp.entry(target, cand)) : 0;
+                const uint64_t vmax =
+                    Gp::slot_max(valid ? (((uint64_t)v << 5) | 16u | (uint32_t)(15 - cs)) : 0ull, sub);
+                int ibest;
+                if (vmax == 0) {
+                    ibest = rowat((tpos + 1u + mulhi32(w3w(), (uint32_t)(N - 1))) % (uint32_t)N);
+                } else {
+                    ibest = Gp::bcast(cand, base + 15 - (int)(vmax & 15));
+                }
+                i = ibest;
+                j = target;
+            } else {
+                const uint32_t ip = mulhi32(w2w, (uint32_t)N);
+                const uint32_t jp = (ip + 1u + mulhi32(w3z, (uint32_t)(N - 1))) % (uint32_t)N;
+                i = rowat(ip);
+                j = rowat(jp);
+            }
+
+            int64_t ai[C], aj[C], xi[C], xj[C];
+            rp.load_rows(i, j, ai, aj);
+            rp.prefetch_xt(i, j, xi, xj);
+            const int64_t a_ii = rp.entry(i, i);
+            const int64_t a_ji = rp.entry(j, i);
+            const int64_t a_jj = rp.entry(j, j);
+
+            // ---- shear (congruence_anneal.hpp:290-296) ----
+            const bool small = rows_small<G, C>(ai, aj);
+            const int64_t pm = (w1z & 1u) ? 1 : -1;
+            int64_t s = pm, soff = 0;
+            bool know = false;
+            if (greedy_coin(w0w)) {
+                s = best_shear_any<G, C>(ai, aj, small, g, n, j, a_ii, a_ji, a_jj, shist, &soff);
+                ++greedy_n;
+                know = s != 0;
+                if (s == 0) s = pm;
+            }
+
+            // ---- attempt_add (congruence_anneal.hpp:232-240) ----
+            MoveEval<C> e;
+            evaluate_any<G, C>(ai, aj, small, g, n, j, s, a_ii, a_ji, a_jj, e, know, soff);
+            int res = -1;
+            if (e.bad) {
+                ++infeasible;
+            } else if (e.delta <= 0 || u53(w1w(), w2z()) < xexp((double)e.delta * nbeta)) {
+                int64_t nx[C];
+                if (rp.xt_row(s, xi, xj, nx)) {
+                    const int64_t ns = score + e.delta;
+                    if (ns < best_score) {
+                        best_score = ns;
+                        best_move = (int64_t)m;
+                        flags |= RF_CUR_BEST;
+                        flags &= ~RF_BEST_STALE;
+                    } else {
+                        leave_best();
+                    }
+                    rp.write(j, aj, e, nx);
+                    score = ns;
+                    offnz += e.dnz;
+                    ++accepts;
+                    log_move(m, i, j, s, 0);
+                    res = e.delta < 0 ? 1 : 0;
+                }
+            }
+            ++win_p;
+            if (res >= 0) ++win_a;
+            if (score < local_best) {
+                local_best = score;
+                msi = 0;
+            } else {
+                ++msi;
+            }
+            if (offnz == 0) {
+                flags |= RF_SOLVED | RF_STOPPED;
+                break;
+            }
+
+            // ---- stall handling (congruence_anneal.hpp:363-412) ----
+            if (tempering) {
+                if ((int64_t)msi > stuck) {
+                    const InstParams& Q = opaque(I)->P;
+                    if (Q.sweep_fraction > 0.0 && u01(w4z()) < Q.sweep_fraction) {
+                        sweep(sact[mulhi32(w4w(), (uint32_t)N)], m);
+                        track_best(m);
+                    }
+                    local_best = score;
+                    msi = 0;
+                }
+            } else {
+                if ((int64_t)msi > stuck) {
+                    const Inst* Iq = opaque(I);
+                    if (Iq->P.sweep_fraction > 0.0 && u01(w4z()) < Iq->P.sweep_fraction) {
+                        if (sweep(sact[mulhi32(w4w(), (uint32_t)N)], m)) {
+                            local_best = score;
+                            msi = 0;
+                        }
+                        track_best(m);
+                    }
+                    const Inst* Iv = opaque(I);
+                    T = Iv->P.reheat * Iv->P.t_init;
+                    nbeta = -1.0 / T;
+                    const int64_t gb_epoch = Iv->gb_score;
+                    if ((double)score > (double)gb_epoch * Iv->P.reseed_factor) {
+                        leave_best();
+                        log_move(m, -1, -1, 0, 2);
+                        rp.load_A((const GLOBAL int64_t*)Iv->gA);
+                        rp.copy_cols(rp.gX, (const GLOBAL int64_t*)Iv->gXt);
+                        rp.compute_row_mass();
+                        score = gb_epoch;
+                        offnz = Iv->gb_offnz;
+                        track_best(m);
+                    }
+                    local_best = score;
+                    msi = 0;
+                    cool_counter = 0;
+                } else if (++cool_counter >= per_cool) {
+                    const InstParams& Q = opaque(I)->P;
+                    cool_counter = 0;
+                    if (Q.adaptive && win_p > 0) {
+                        const double acc = (double)win_a / (double)win_p;
+                        if (acc > Q.target_accept)
+                            cooling *= 0.999;
+                        else
+                            cooling /= 0.999;
+                        cooling = cooling < 0.90 ? 0.90 : (cooling > 0.99999 ? 0.99999 : cooling);
+                    }
+                    win_p = 0;
+                    win_a = 0;
+                    T *= cooling;
+                    if (T < Q.t_min) T = Q.t_min;
+                    nbeta = -1.0 / T;
+                }
+            }
+            if (offnz == 0) {
+                flags |= RF_SOLVED | RF_STOPPED;
+                break;
+            }
+        }
+    } else {
+        // lookahead: group kk proposes move m0 + kk
+        for (uint32_t it = 0; moves < stop_at; ++it) {
+            if (poll(it)) break;
+            const uint64_t m0 = moves + 1;
+            const uint64_t mg = m0 + (uint64_t)kk;
+            const uint32_t t4 = (uint32_t)((mg - 1) & 3u);
+            const int sub = (int)(t4 & (uint32_t)(SB - 1));
+            const uint64_t q = (mg - 1) >> 2;
+            // the key as scalars (its load may have landed in vector registers)
+            uint32_t k0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)key0);
+            uint32_t k1 = (uint32_t)__builtin_amdgcn_readfirstlane((int)key1);
+            opaque_u32(k0);
+            opaque_u32(k1);
+            const uint4 w = philox10((uint32_t)q, (uint32_t)(q >> 32), (uint32_t)r,
+                                     16u * (t4 & ~(uint32_t)(SB - 1)) + (uint32_t)g, k0, k1);
+            const int base = 16 * sub;
+            const int cs = g - base;
+            if (cs >= 0 && cs < 5) swords[cs] = w;
+            __builtin_amdgcn_wave_barrier();
+            const uint4 b0 = swords[0], b1 = swords[1], b2 = swords[2], b3 = swords[3];
+            const uint4 b4 = swords[4];
+            __builtin_amdgcn_wave_barrier();
+
+            int i, j;
+            choose_pair(w, base, cs, sub, b0.z, b2.w, b3.z, [&]() { return b3.w; }, i, j);
+            int64_t ai[C], aj[C], xi[C], xj[C];
+            rp.load_rows(i, j, ai, aj);
+            rp.prefetch_xt(i, j, xi, xj);
+            const int64_t a_ii = rp.entry(i, i);
+            const int64_t a_ji = rp.entry(j, i);
+            const int64_t a_jj = rp.entry(j, j);
+            const bool small = rows_small<G, C>(ai, aj);
+            const int64_t pm = (b1.z & 1u) ? 1 : -1;
+            int64_t s = pm, soff = 0;
+            bool know = false;
+            const bool gr = greedy_coin(b0.w);
+            if (gr) {
+                s = best_shear_any<G, C>(ai, aj, small, g, n, j, a_ii, a_ji, a_jj, shist, &soff);
+                know = s != 0;
+                if (s == 0) s = pm;
+            }
+            MoveEval<C> e;
+            evaluate_any<G, C>(ai, aj, small, g, n, j, s, a_ii, a_ji, a_jj, e, know, soff);
+            // Metropolis at the step's temperature; a move after a cooling
+            // step inside this step is decided again at the new one
+            const double u = u53(b1.w, b2.z);
+            const double nb0 = nbeta;
+            const bool pass = !e.bad && (e.delta <= 0 || u < xexp((double)e.delta * nb0));
+
+            // moves m0 .. m0 + kn - 1 are resolvable (the epoch ends at stop_at)
+            const int kn = (stop_at - moves) < (uint64_t)K ? (int)(stop_at - moves) : K;
+            const uint64_t live = kn == K ? ~0ull : ((1ull << (kn * G)) - 1ull);
+            if ((__ballot(pass) & live) == 0ull && score >= local_best &&
+                (int64_t)msi + kn <= (int64_t)stuck && (tempering || cool_counter + kn < per_cool)) {
+                // every resolvable move is rejected and none of them reaches a
+                // stall or a cooling step: the sequential bookkeeping, kn times
+                const uint64_t lead = __ballot(g == 0) & live;
+                greedy_n += (uint32_t)__popcll(__ballot(gr) & lead);
+                infeasible += (uint32_t)__popcll(__ballot(e.bad) & lead);
+                moves += (uint64_t)kn;
+                win_p += kn;
+                msi += (uint32_t)kn;
+                if (!tempering) cool_counter += kn;
+                continue;
+            }
+
+            int64_t nx[C];
+            bool feas = false, have_nx = false;
+            bool solved = false;
+            for (int k = 0; k < kn; ++k) {
+                const int L = k * G;  // first lane of group k
+                const uint64_t m = m0 + (uint64_t)k;
+                moves = m;
+                if (readlane32((uint32_t)gr, L)) ++greedy_n;
+                int res = -1;
+                bool changed = false;
+                if (readlane32((uint32_t)e.bad, L)) {
+                    ++infeasible;
+                } else {
+                    const int64_t d = readlane64(e.delta, L);
+                    bool acc = readlane32((uint32_t)pass, L) != 0;
+                    if (nbeta != nb0) acc = d <= 0 || readlane_f64(u, L) < xexp((double)d * nbeta);
+                    if (acc && !have_nx) {
+                        feas = rp.xt_row(s, xi, xj, nx);  // every group's, once
+                        have_nx = true;
+                    }
+                    if (acc && readlane32((uint32_t)feas, L)) {
+                        const int64_t ns = score + d;
+                        if (ns < best_score) {
+                            best_score = ns;
+                            best_move = (int64_t)m;
+                            flags |= RF_CUR_BEST;
+                            flags &= ~RF_BEST_STALE;
+                        } else {
+                            leave_best();
+                        }
+                        // group k's move, written by every group: each keeps
+                        // its own row masses and X^T columns
+                        const int src = L + g;
+                        const int jk = (int)readlane32((uint32_t)j, L);
+                        int64_t cj[C], cx[C];
+                        MoveEval<C> ce;
+#pragma unroll
+                        for (int c = 0; c < C; ++c) {
+                            cj[c] = bperm64(aj[c], src);
+                            ce.nv[c] = bperm64(e.nv[c], src);
+                            cx[c] = bperm64(nx[c], src);
+                        }
+                        ce.off = readlane64(e.off, L);
+                        rp.write(jk, cj, ce, cx);
+                        score = ns;
+                        offnz += (int32_t)readlane32((uint32_t)e.dnz, L);
+                        ++accepts;
+                        log_move(m, (int)readlane32((uint32_t)i, L), jk, readlane64(s, L), 0);
+                        res = d < 0 ? 1 : 0;
+                        changed = true;
+                    }
+                }
+                auto k4z = [&]() { return readlane32(b4.z, L); };
+                auto k4w = [&]() { return readlane32(b4.w, L); };
+                if (finish(m, res, k4z, k4w, changed)) {
+                    solved = true;
+                    break;
+                }
+                if (changed) break;
+            }
+            if (solved) break;
         }
     }
     // the move cap ends the loop through stop_at
     if (!(flags & RF_STOPPED) && !halt && cap != 0 && moves >= cap) flags |= RF_STOPPED | RF_CAPPED;
 
+    if (kk != 0) return;  // lookahead: group 0 writes the replica back
     rp.store_A(gAcur);
     if (g == 0) {
         S->moves = moves;
@@ -1277,28 +1599,33 @@ __global__ void k_probe_rng(const uint32_t* __restrict__ ck, int count, uint32_t
 
 typedef void (*AnnealKernel)(const RepRef*, int, Inst*, int, int, const volatile int*);
 
-// Every (G, C, layout) the launcher can run: G lanes per replica, C columns
-// per lane (G C >= n), triangular or square A.
+// Every (G, C, layout, lookahead) the launcher can run: G lanes per group, C
+// columns per lane (G C >= n), triangular or square A, K groups per replica.
 struct AnnealVariant {
     int G, C;
     bool tri;
+    int K;
     AnnealKernel fn;
 };
 static const AnnealVariant kVariants[] = {
-    {16, 1, true, k_anneal<16, 1, true>},   {16, 1, false, k_anneal<16, 1, false>},
-    {32, 1, true, k_anneal<32, 1, true>},   {32, 1, false, k_anneal<32, 1, false>},
-    {32, 2, true, k_anneal<32, 2, true>},   {32, 2, false, k_anneal<32, 2, false>},
-    {64, 1, true, k_anneal<64, 1, true>},   {64, 1, false, k_anneal<64, 1, false>},
-    {64, 2, true, k_anneal<64, 2, true>},   {64, 2, false, k_anneal<64, 2, false>},
+    {16, 1, true, 1, k_anneal<16, 1, true, 1>},   {16, 1, false, 1, k_anneal<16, 1, false, 1>},
+    {32, 1, true, 1, k_anneal<32, 1, true, 1>},   {32, 1, false, 1, k_anneal<32, 1, false, 1>},
+    {32, 2, true, 1, k_anneal<32, 2, true, 1>},   {32, 2, false, 1, k_anneal<32, 2, false, 1>},
+    {64, 1, true, 1, k_anneal<64, 1, true, 1>},   {64, 1, false, 1, k_anneal<64, 1, false, 1>},
+    {64, 2, true, 1, k_anneal<64, 2, true, 1>},   {64, 2, false, 1, k_anneal<64, 2, false, 1>},
+    {16, 1, true, 4, k_anneal<16, 1, true, 4>},   {16, 2, true, 4, k_anneal<16, 2, true, 4>},
+    {32, 1, true, 2, k_anneal<32, 1, true, 2>},   {32, 2, true, 2, k_anneal<32, 2, true, 2>},
 };
 
-static const AnnealVariant* find_variant(int G, int C, bool tri) {
+static const AnnealVariant* find_variant(int G, int C, bool tri, int K) {
     for (const auto& v : kVariants)
-        if (v.G == G && v.C == C && v.tri == tri) return &v;
+        if (v.G == G && v.C == C && v.tri == tri && v.K == K) return &v;
     return nullptr;
 }
 
-bool anneal_variant_exists(int G, int C, bool tri) { return find_variant(G, C, tri) != nullptr; }
+bool anneal_variant_exists(int G, int C, bool tri, int K) {
+    return find_variant(G, C, tri, K) != nullptr;
+}
 
 // Dynamic LDS limits are raised once, outside any stream capture, to the
 // per-block maximum of the device minus each kernel's static LDS.
@@ -1317,13 +1644,15 @@ hipError_t init_kernel_attrs(int max_lds_per_block) {
     return hipSuccess;
 }
 
-hipError_t launch_anneal(hipStream_t st, int G, int C, bool tri, int groups_per_block,
+// units_per_block: table entries (replicas, or padding) per block, G K lanes each
+hipError_t launch_anneal(hipStream_t st, int G, int C, bool tri, int K, int units_per_block,
                          const RepRef* table, int count, Inst* insts, int nmax, int rep_bytes,
                          const volatile int* stop) {
     if (count <= 0) return hipSuccess;
-    const AnnealVariant* v = find_variant(G, C, tri);
-    if (!v || groups_per_block < 1 || groups_per_block * G > 256) return hipErrorInvalidValue;
-    const int threads = groups_per_block * G;
+    const AnnealVariant* v = find_variant(G, C, tri, K);
+    if (!v || units_per_block < 1 || units_per_block * G * K > 256) return hipErrorInvalidValue;
+    const int groups_per_block = units_per_block;
+    const int threads = units_per_block * G * K;
     const int blocks = (count + groups_per_block - 1) / groups_per_block;
     const size_t lds = (size_t)groups_per_block * rep_bytes;
     hipLaunchKernelGGL(v->fn, dim3(blocks), dim3(threads), lds, st, table, count, insts, nmax,

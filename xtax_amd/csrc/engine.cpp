@@ -33,10 +33,10 @@
 #include "xtx.h"
 
 namespace xtx {
-hipError_t launch_anneal(hipStream_t st, int G, int C, bool tri, int groups_per_block,
+hipError_t launch_anneal(hipStream_t st, int G, int C, bool tri, int K, int units_per_block,
                          const RepRef* table, int count, Inst* insts, int nmax, int rep_bytes,
                          const volatile int* stop);
-bool anneal_variant_exists(int G, int C, bool tri);
+bool anneal_variant_exists(int G, int C, bool tri, int K);
 hipError_t launch_epoch_end(hipStream_t st, Inst* insts, int count, int max_r);
 hipError_t launch_probe(hipStream_t st, int G, int C, const int64_t* A, const int64_t* Xt, int n,
                         const int32_t* q, int nq, int64_t* out);
@@ -80,6 +80,8 @@ void check_params(const xtx_params& p) {
     if (!(p.cooling > 0.0)) bad("cooling must be > 0");
     if (p.move_log && p.move_log_cap < 1) bad("move_log_cap must be >= 1 when move_log is set");
     if (p.tempering && p.replicas > 8000) bad("tempering supports at most 8000 replicas");
+    if (p.lookahead != 0 && p.lookahead != 1 && p.lookahead != 2 && p.lookahead != 4)
+        bad("lookahead must be 0, 1, 2 or 4");
 }
 
 // Builds the start state exactly: A_start = X0^T A0 X0 in __int128, rejecting
@@ -372,8 +374,9 @@ int verify_outcome(const HostInst& h, const Outcome& o) {
 
 // Pads a replica table to whole waves: the kernel reads the instance of a wave
 // from its first group, so a wave never mixes instances (rep -1 = idle group).
-void pad_to_wave(std::vector<RepRef>& refs, int G, int inst) {
-    const size_t per_wave = (size_t)(64 / G);
+// U: lanes per replica (G K).
+void pad_to_wave(std::vector<RepRef>& refs, int U, int inst) {
+    const size_t per_wave = (size_t)(64 / U);
     while (refs.size() % per_wave) refs.push_back(RepRef{inst, -1});
 }
 
@@ -383,7 +386,7 @@ struct Bucket {
     std::vector<RepRef> refs;
     RepRef* d_refs = nullptr;
     int cap = 0;
-    int groups_per_block = 1;
+    int groups_per_block = 1;  // replicas (table entries) per block
 };
 
 }  // namespace
@@ -397,14 +400,15 @@ Layout layout_in(const xtx_ctx* ctx, int n) {
     return it != ctx->tuned.end() ? it->second : layout_for(n);
 }
 
-void apply_layout(const xtx_ctx* ctx, HostInst& h) {
-    h.lay = layout_in(ctx, h.n);
+// lookahead: the most groups per replica wanted (1: the tuned or default layout)
+void apply_layout(const xtx_ctx* ctx, HostInst& h, int lookahead) {
+    h.lay = lookahead > 1 ? lookahead_layout(order_of(h.n), lookahead) : layout_in(ctx, h.n);
     if (h.lay.rep_bytes > g_lds_block)
         throw Err{XTX_E_ARG, "n too large for an LDS-resident replica"};
 }
 
 int groups_per_block_for(const Layout& L, int waves) {
-    const int per_wave = 64 / L.G;
+    const int per_wave = 64 / (L.G * L.K);
     int w = L.waves > 0 ? L.waves : (waves > 0 ? waves : 1);
     // keep a block within the LDS of one CU
     while (w > 1 && (int64_t)w * per_wave * L.rep_bytes > g_lds_block) --w;
@@ -432,7 +436,7 @@ void launch_epoch(xtx_ctx* ctx, std::vector<Bucket>& buckets, Inst* d_insts, int
         const Bucket& b = *order[q];
         const int lane = (int)(q % (size_t)(ns + 1));
         hipStream_t st = lane == 0 ? ctx->stream : ctx->side[lane - 1];
-        HIPCHK(launch_anneal(st, b.lay.G, b.lay.C, b.lay.tri, b.groups_per_block, b.d_refs,
+        HIPCHK(launch_anneal(st, b.lay.G, b.lay.C, b.lay.tri, b.lay.K, b.groups_per_block, b.d_refs,
                              (int)b.refs.size(), d_insts, b.lay.nmax, b.lay.rep_bytes,
                              ctx->stop_dev));
     }
@@ -477,6 +481,7 @@ void xtx_params_default(xtx_params* p) {
     p->max_epochs = 0;
     p->verify = 1;
     p->waves_per_block = 0;
+    p->lookahead = 0;
 }
 
 int xtx_open(int device, xtx_ctx** out) {
@@ -581,7 +586,11 @@ int xtx_solve_plain(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t 
         HostInst& h = ctx->plain;
         h = HostInst{};
         prepare(h, A0, X0, n, p.replicas, p.seed, active, n_active, p);
-        apply_layout(ctx, h);
+        // Few replicas leave most of the device idle: each then gets a wave
+        // whose lane groups evaluate its next proposals at once.
+        const int look = p.lookahead != 0 ? p.lookahead
+                         : (h.R <= 4 * ctx->prop.multiProcessorCount ? 4 : 1);
+        apply_layout(ctx, h, look);
         xtx_stats st{};
         st.verify_status = -1;
         if (n < 2 || h.N < 2 || h.offnz0 == 0) {
@@ -618,7 +627,7 @@ int xtx_solve_plain(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t 
         buckets[0].lay = h.lay;
         buckets[0].groups_per_block = groups_per_block_for(h.lay, p.waves_per_block);
         for (int r = 0; r < h.R; ++r) buckets[0].refs.push_back(RepRef{0, r});
-        pad_to_wave(buckets[0].refs, h.lay.G, 0);
+        pad_to_wave(buckets[0].refs, h.lay.G * h.lay.K, 0);
         Inst* pin = nullptr;
         // released on every exit, including an error thrown below
         struct Release {
@@ -656,7 +665,7 @@ int xtx_solve_plain(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t 
             for (;;) {
                 HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
                 for (auto& b : buckets)
-                    HIPCHK(launch_anneal(ctx->stream, b.lay.G, b.lay.C, b.lay.tri,
+                    HIPCHK(launch_anneal(ctx->stream, b.lay.G, b.lay.C, b.lay.tri, b.lay.K,
                                          b.groups_per_block,
                                          b.d_refs, (int)b.refs.size(), ctx->d_plain,
                                          b.lay.nmax, b.lay.rep_bytes, ctx->stop_dev));
@@ -980,7 +989,7 @@ struct Server {
             b.lay = h.lay;
             b.groups_per_block = groups_per_block_for(h.lay, p.waves_per_block);
             for (int r = 0; r < h.R; ++r) b.refs.push_back(RepRef{s, r});
-            pad_to_wave(b.refs, h.lay.G, s);
+            pad_to_wave(b.refs, h.lay.G * h.lay.K, s);
         }
         for (auto& kv : buckets) {
             Bucket& b = kv.second;
@@ -1104,7 +1113,7 @@ std::unique_ptr<HostInst> make_inst(const xtx_ctx* ctx, const xtx_instance& in,
     q.replicas = R;
     check_params(q);
     prepare(*h, in.A0, in.X0, in.n, R, in.seed, in.active, in.n_active, q);
-    apply_layout(ctx, *h);
+    apply_layout(ctx, *h, p.lookahead);
     if (h->n < 2 || h->N < 2)
         throw Err{XTX_E_ARG, "instance needs n >= 2 and at least two active indices"};
     h->bytes = device_bytes(*h);
@@ -1133,7 +1142,7 @@ void fill_launch(const Layout& L, bool tuned, xtx_launch* out) {
 const char* launch_invalid(int nmax, int G, int C, bool tri, int waves) {
     if (nmax < 8 || nmax > kMaxN || nmax % 8) return "order must be a multiple of 8 in [8, 128]";
     if (G * C < nmax) return "lanes * cols must cover the order";
-    if (!anneal_variant_exists(G, C, tri)) return "no kernel with these lanes/cols/layout";
+    if (!anneal_variant_exists(G, C, tri, 1)) return "no kernel with these lanes/cols/layout";
     if (waves < 0 || waves > 4 || waves == 3) return "waves_per_block must be 0, 1, 2 or 4";
     const int rb = rep_bytes_of(nmax, G, tri);
     if ((int64_t)(64 / G) * rb > g_lds_block) return "replica LDS exceeds a workgroup";
@@ -1184,7 +1193,7 @@ double time_layout(xtx_ctx* ctx, const std::vector<const HostInst*>& proto, cons
                               hipMemcpyHostToDevice, ctx->stream));
         recs[q] = h.rec;
         for (int r = 0; r < h.R; ++r) b.refs.push_back(RepRef{(int)q, r});
-        pad_to_wave(b.refs, L.G, (int)q);
+        pad_to_wave(b.refs, L.G * L.K, (int)q);
     }
     Inst* d_insts = nullptr;
     HIPCHK(hipMalloc(&d_insts, recs.size() * sizeof(Inst)));
@@ -1196,7 +1205,7 @@ double time_layout(xtx_ctx* ctx, const std::vector<const HostInst*>& proto, cons
         HIPCHK(hipMemcpyAsync(b.d_refs, b.refs.data(), b.refs.size() * sizeof(RepRef),
                               hipMemcpyHostToDevice, ctx->stream));
         HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-        HIPCHK(launch_anneal(ctx->stream, L.G, L.C, L.tri, b.groups_per_block, b.d_refs,
+        HIPCHK(launch_anneal(ctx->stream, L.G, L.C, L.tri, L.K, b.groups_per_block, b.d_refs,
                              (int)b.refs.size(), d_insts, L.nmax, L.rep_bytes, ctx->stop_dev));
         HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));

@@ -51,25 +51,28 @@ struct Layout {
     int G = 0, C = 0, nmax = 0, rep_bytes = 0;
     bool tri = true;
     int waves = 0;
+    int K = 1;  // lookahead: lane groups per replica (G K == 64 when K > 1)
 };
 
 inline int order_of(int n) { return std::max(8, (n + 7) / 8 * 8); }
 
-// A, 5 staged Philox blocks, active list, median histogram; X^T stays in HBM
-inline int rep_bytes_of(int nmax, int G, bool tri) {
+// A, then per lane group 5 staged Philox blocks and a median histogram, and the
+// active list; X^T stays in HBM
+inline int rep_bytes_of(int nmax, int G, bool tri, int K = 1) {
     const int64_t a = tri ? (int64_t)nmax * (nmax + 1) / 2 : (int64_t)nmax * (nmax + 1);
-    const int64_t bytes = a * 8 + 5 * 16 + nmax + 4 * med_slots(G);
+    const int64_t bytes = a * 8 + K * (5 * 16 + 4 * med_slots(G)) + nmax;
     return (int)((bytes + 15) / 16 * 16);
 }
 
-inline Layout make_layout(int nmax, int G, int C, bool tri, int waves) {
+inline Layout make_layout(int nmax, int G, int C, bool tri, int waves, int K = 1) {
     Layout L;
     L.nmax = nmax;
     L.G = G;
     L.C = C;
     L.tri = tri;
     L.waves = waves;
-    L.rep_bytes = rep_bytes_of(nmax, G, tri);
+    L.K = K;
+    L.rep_bytes = rep_bytes_of(nmax, G, tri, K);
     return L;
 }
 
@@ -82,8 +85,19 @@ inline Layout layout_for(int n) {
     return make_layout(nmax, 64, 2, true, 0);
 }
 
-using BucketKey = std::tuple<int, int, int, bool, int>;
-inline BucketKey key_of(const Layout& L) { return {L.nmax, L.G, L.C, L.tri, L.waves}; }
+// The lookahead layout of order nmax: one wave per replica, K groups of G
+// lanes, K the largest of 4 and 2 that is at most `want` and that a group can
+// hold the order with (C <= 2); K = 1 (the order's default layout) otherwise.
+inline Layout lookahead_layout(int nmax, int want) {
+    if (want >= 4 && nmax <= 16) return make_layout(nmax, 16, 1, true, 0, 4);
+    if (want >= 4 && nmax <= 32) return make_layout(nmax, 16, 2, true, 0, 4);
+    if (want >= 2 && nmax <= 32) return make_layout(nmax, 32, 1, true, 0, 2);
+    if (want >= 2 && nmax <= 64) return make_layout(nmax, 32, 2, true, 0, 2);
+    return layout_for(nmax);
+}
+
+using BucketKey = std::tuple<int, int, int, bool, int, int>;
+inline BucketKey key_of(const Layout& L) { return {L.nmax, L.G, L.C, L.tri, L.waves, L.K}; }
 
 constexpr int kLdsPerCu = 160 * 1024;
 // Largest LDS one block may use on the open device (set by xtx_open).

@@ -47,6 +47,7 @@ class EngineParams:
     verify: bool = True
     waves_per_block: int = 0
     autotune: bool = False
+    lookahead: int = 0
 
     def to_c(self):
         p = N.Params()
