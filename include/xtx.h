@@ -75,13 +75,15 @@ typedef struct {
                                  batch that has no configuration yet (on the
                                  batch itself, before the run) */
     int32_t lookahead;        /* proposals of one replica evaluated at once
-                                 (1, 2 or 4; orders up to 32 allow 4, up to 64
-                                 allow 2, larger ones 1): the replica takes a
-                                 whole wave and resolves them in order, the
+                                 (1, 2, 4, 8 or 16; orders up to 32 allow 16,
+                                 up to 64 allow 8, larger ones 1): the replica
+                                 takes a whole wave (16 or 8: a workgroup of
+                                 four waves) and resolves them in order, the
                                  same trajectory at up to that many moves per
-                                 step. 0 = auto: xtx_solve_plain uses 4 when
-                                 replicas <= 4 per compute unit, else 1; the
-                                 batch and serve forms use 1 */
+                                 step. 0 = auto: xtx_solve_plain uses 16 when
+                                 replicas <= compute units, 4 when <= 4 per
+                                 compute unit, else 1; the batch and serve
+                                 forms use 1 */
     /* Plain forms (xtx_solve_plain, xtx_dual_solve): called on the calling
      * thread at an epoch boundary whenever the instance best improves -- the
      * reference's "new best" report (congruence_anneal.hpp:309-352). seconds

@@ -159,9 +159,10 @@ def _matrix(name):
     raise KeyError(name)
 
 
-# lookahead 0: the plain form's rule (4 groups per replica here: few
-# replicas), 1: one group per replica, 2: two groups
-@pytest.mark.parametrize("look", [0, 1, 2], ids=["auto", "K1", "K2"])
+# lookahead 0: the plain form's rule (here, with few replicas, 16 groups per
+# replica over four waves up to order 32, 8 up to 64), 1: one group per
+# replica, 4: four groups in one wave
+@pytest.mark.parametrize("look", [0, 1, 4], ids=["auto", "K1", "K4"])
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
 def test_trajectory_matches_model(eng, orc, case, look):
     name, mat, kw = case
@@ -320,10 +321,11 @@ def test_every_launch_configuration_gives_the_same_result(eng, n):
 
 def test_lookahead_changes_speed_only(eng):
     """Evaluating a replica's next proposals at once (lookahead 2 or 4: one
-    wave per replica, its lane groups resolving the proposals in move order)
-    gives the batch result of one group per replica, bit for bit, at every
-    order (above 64 the engine keeps one group), with stall sweeps among the
-    moves; a lookahead other than 0, 1, 2 or 4 is refused."""
+    wave per replica; 8 or 16: a workgroup of four waves; the lane groups
+    resolve the proposals in move order) gives the batch result of one group
+    per replica, bit for bit, at every order (above 64 the engine keeps one
+    group), with stall sweeps among the moves; a lookahead other than 0, 1, 2,
+    4, 8 or 16 is refused."""
     from xtax_amd._native import XtxError
     mats = [workloads.planted(n, 20, 2 * n, seed=500 + n)[0] for n in (5, 12, 24, 31, 40, 60, 70)]
     mats.append(workloads.random_symmetric(16, -20, 20, seed=8))
@@ -331,7 +333,7 @@ def test_lookahead_changes_speed_only(eng):
     base = dict(replicas=6, t_init=30, t_min=0.5, exchange_interval=300, max_epochs=6,
                 sweep_fraction=0.3, stuck_threshold=200)
     ref, _ = eng.batch_solve(mats, EngineParams(lookahead=1, **base), seeds=seeds)
-    for look in (2, 4):
+    for look in (2, 4, 8, 16):
         say(f"batch lookahead={look}")
         got, _ = eng.batch_solve(mats, EngineParams(lookahead=look, **base), seeds=seeds)
         for q in range(len(mats)):

@@ -579,6 +579,18 @@ __device__ __forceinline__ double readlane_f64(double v, int lane) {
     return __builtin_bit_cast(double, readlane64(__builtin_bit_cast(int64_t, v), lane));
 }
 
+// One lookahead proposal as the other waves of its replica read it (LDS).
+struct alignas(16) Proposal {
+    int64_t delta;
+    int64_t s;
+    double u;         // the Metropolis draw
+    int32_t i, j;
+    uint32_t w4z, w4w;  // the stall words
+    uint32_t flags;     // 1: over the bound, 2: passes at the step's temperature, 4: greedy
+    uint32_t pad[3];
+};
+static_assert(sizeof(Proposal) == 64, "two proposals per 128 bytes");
+
 // One epoch of many replicas.
 //
 // K == 1: one replica per lane group, 64/G replicas per wave; four waves per
@@ -596,16 +608,27 @@ __device__ __forceinline__ double readlane_f64(double v, int lane) {
 // step while proposals are rejected. Used where replicas are too few to fill
 // the device (the annealer rejects most proposals: 0.2% to 5% are accepted at
 // the workloads' settings).
+//
+// K > 64 / G: the replica's K groups span W = G K / 64 waves of one workgroup
+// (one wave per SIMD of a CU). Each group publishes its proposal to LDS; after
+// a workgroup barrier every wave reads all K and resolves them identically; a
+// committed move is evaluated again by every group from the shared state and
+// written by all of them (each its own columns and row masses), after a
+// barrier that lets every wave finish reading the state it changes. Stall
+// sweeps and reseeds write the same way. Every wave thus takes the same
+// branches and meets the same barriers.
 template <int G, int C, bool TRI, int K>
 __global__ void __launch_bounds__(256, K == 1 ? 4 : 2)
     k_anneal(const RepRef* __restrict__ table, int count, Inst* __restrict__ insts, int nmax,
              int rep_bytes, const volatile int* __restrict__ stop) {
-    static_assert(K == 1 || G * K == 64, "lookahead groups fill one wave");
+    constexpr int W = K == 1 ? 1 : G * K / 64;  // waves per replica
+    static_assert(K == 1 || (G * K) % 64 == 0, "lookahead groups fill whole waves");
+    static_assert(W <= 4, "a workgroup of at most 256 lanes");
     extern __shared__ __align__(16) unsigned char smem[];
     using Gp = Grp<G>;
     constexpr int U = G * K;  // lanes per replica
     const int g = (int)threadIdx.x & (G - 1);
-    const int kk = K == 1 ? 0 : (int)(threadIdx.x & 63u) / G;  // lookahead index of the group
+    const int kk = K == 1 ? 0 : (int)(threadIdx.x % (unsigned)U) / G;  // lookahead index
     const int slot = (int)threadIdx.x / U;
     const int gid = (int)blockIdx.x * ((int)blockDim.x / U) + slot;
     if (gid >= count) return;
@@ -633,6 +656,10 @@ __global__ void __launch_bounds__(256, K == 1 ? 4 : 2)
     uint4* const swords = words0 + 5 * kk;
     uint8_t* sact = (uint8_t*)(words0 + 5 * K);
     uint32_t* shist = (uint32_t*)(sact + nmax) + kk * med_slots(G);
+    // W > 1: two buffers of K proposals (by step parity) and the poll word
+    Proposal* const props =
+        (Proposal*)((((uintptr_t)((uint32_t*)(sact + nmax) + K * med_slots(G))) + 15) & ~(uintptr_t)15);
+    volatile uint32_t* const spoll = (volatile uint32_t*)(props + 2 * K);
     rp.act = sact;
     rp.n = I->n;
     rp.N = I->N;
@@ -756,6 +783,7 @@ __global__ void __launch_bounds__(256, K == 1 ? 4 : 2)
             if constexpr (K > 1) fits = uni<64>((int32_t)fits) != 0;
             if (!fits) continue;
             leave_best();
+            if constexpr (W > 1) __syncthreads();  // every wave has read what changes
             rp.write(jj, aj, e, nx);
             score += e.delta;
             offnz += e.dnz;
@@ -862,6 +890,7 @@ __global__ void __launch_bounds__(256, K == 1 ? 4 : 2)
                 if ((double)score > (double)gb_epoch * Iv->P.reseed_factor) {
                     leave_best();
                     log_move(m, -1, -1, 0, 2);
+                    if constexpr (W > 1) __syncthreads();
                     rp.load_A((const GLOBAL int64_t*)Iv->gA);
                     rp.copy_cols(rp.gX, (const GLOBAL int64_t*)Iv->gXt);
                     rp.compute_row_mass();
@@ -1146,8 +1175,8 @@ __global__ void __launch_bounds__(256, K == 1 ? 4 : 2)
                 break;
             }
         }
-    } else {
-        // lookahead: group kk proposes move m0 + kk
+    } else if constexpr (W == 1) {
+        // lookahead in one wave: group kk proposes move m0 + kk
         for (uint32_t it = 0; moves < stop_at; ++it) {
             if (poll(it)) break;
             const uint64_t m0 = moves + 1;
@@ -1267,6 +1296,152 @@ __global__ void __launch_bounds__(256, K == 1 ? 4 : 2)
                 }
                 auto k4z = [&]() { return readlane32(b4.z, L); };
                 auto k4w = [&]() { return readlane32(b4.w, L); };
+                if (finish(m, res, k4z, k4w, changed)) {
+                    solved = true;
+                    break;
+                }
+                if (changed) break;
+            }
+            if (solved) break;
+        }
+    } else {
+        // lookahead over W waves: group kk proposes move m0 + kk and publishes it
+        const int ln = (int)(threadIdx.x & 63u);
+        auto uni_f64 = [](double v) {
+            return __builtin_bit_cast(double, uni<64>(__builtin_bit_cast(int64_t, v)));
+        };
+        for (uint32_t it = 0; moves < stop_at; ++it) {
+            const uint64_t m0 = moves + 1;
+            const uint64_t mg = m0 + (uint64_t)kk;
+            const uint32_t t4 = (uint32_t)((mg - 1) & 3u);
+            const int sub = (int)(t4 & (uint32_t)(SB - 1));
+            const uint64_t q = (mg - 1) >> 2;
+            uint32_t k0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)key0);
+            uint32_t k1 = (uint32_t)__builtin_amdgcn_readfirstlane((int)key1);
+            opaque_u32(k0);
+            opaque_u32(k1);
+            const uint4 w = philox10((uint32_t)q, (uint32_t)(q >> 32), (uint32_t)r,
+                                     16u * (t4 & ~(uint32_t)(SB - 1)) + (uint32_t)g, k0, k1);
+            const int base = 16 * sub;
+            const int cs = g - base;
+            if (cs >= 0 && cs < 5) swords[cs] = w;
+            __builtin_amdgcn_wave_barrier();
+            const uint4 b0 = swords[0], b1 = swords[1], b2 = swords[2], b3 = swords[3];
+            const uint4 b4 = swords[4];
+            __builtin_amdgcn_wave_barrier();
+
+            int i, j;
+            choose_pair(w, base, cs, sub, b0.z, b2.w, b3.z, [&]() { return b3.w; }, i, j);
+            int64_t ai[C], aj[C];
+            rp.load_rows(i, j, ai, aj);
+            const int64_t a_ii = rp.entry(i, i);
+            const int64_t a_ji = rp.entry(j, i);
+            const int64_t a_jj = rp.entry(j, j);
+            const bool small = rows_small<G, C>(ai, aj);
+            const int64_t pm = (b1.z & 1u) ? 1 : -1;
+            int64_t s = pm, soff = 0;
+            bool know = false;
+            const bool gr = greedy_coin(b0.w);
+            if (gr) {
+                s = best_shear_any<G, C>(ai, aj, small, g, n, j, a_ii, a_ji, a_jj, shist, &soff);
+                know = s != 0;
+                if (s == 0) s = pm;
+            }
+            MoveEval<C> e;
+            evaluate_any<G, C>(ai, aj, small, g, n, j, s, a_ii, a_ji, a_jj, e, know, soff);
+            const double u = u53(b1.w, b2.z);
+            const double nb0 = nbeta;
+            const bool pass = !e.bad && (e.delta <= 0 || u < xexp((double)e.delta * nb0));
+            Proposal* const pb = props + (it & 1u) * K;
+            if (g == 0) {
+                Proposal P;
+                P.delta = e.delta;
+                P.s = s;
+                P.u = u;
+                P.i = i;
+                P.j = j;
+                P.w4z = b4.z;
+                P.w4w = b4.w;
+                P.flags = (e.bad ? 1u : 0u) | (pass ? 2u : 0u) | (gr ? 4u : 0u);
+                pb[kk] = P;
+            }
+            // the solve key and the stop flag, read once for every wave
+            if ((it & 255u) == 0u && threadIdx.x == 0) {
+                const unsigned long long key =
+                    __hip_atomic_load(&I->solve_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                *spoll = (key != kNoKey && moves >= (key >> kRepBits)) ? 1u : (*stop ? 2u : 0u);
+            }
+            __syncthreads();
+            if ((it & 255u) == 0u) {
+                const uint32_t pr = (uint32_t)uni<64>((int32_t)*spoll);
+                if (pr == 2u) halt = true;
+                if (pr != 0u) break;
+            }
+
+            // moves m0 .. m0 + kn - 1 are resolvable (the epoch ends at stop_at)
+            const int kn = (stop_at - moves) < (uint64_t)K ? (int)(stop_at - moves) : K;
+            const uint32_t fl = ln < kn ? pb[ln].flags : 0u;  // proposal ln's flags in lane ln
+            if (__ballot((fl & 2u) != 0u) == 0ull && score >= local_best &&
+                (int64_t)msi + kn <= (int64_t)stuck && (tempering || cool_counter + kn < per_cool)) {
+                greedy_n += (uint32_t)__popcll(__ballot((fl & 4u) != 0u));
+                infeasible += (uint32_t)__popcll(__ballot((fl & 1u) != 0u));
+                moves += (uint64_t)kn;
+                win_p += kn;
+                msi += (uint32_t)kn;
+                if (!tempering) cool_counter += kn;
+                continue;
+            }
+
+            bool solved = false;
+            for (int k = 0; k < kn; ++k) {
+                const Proposal* const P = pb + k;
+                const uint32_t f = (uint32_t)uni<64>((int32_t)P->flags);
+                const uint64_t m = m0 + (uint64_t)k;
+                moves = m;
+                if (f & 4u) ++greedy_n;
+                int res = -1;
+                bool changed = false;
+                if (f & 1u) {
+                    ++infeasible;
+                } else {
+                    const int64_t d = uni<64>(P->delta);
+                    bool acc = (f & 2u) != 0u;
+                    if (nbeta != nb0) acc = d <= 0 || uni_f64(P->u) < xexp((double)d * nbeta);
+                    if (acc) {
+                        // every group evaluates move k again from the shared
+                        // state: its rows, its X^T columns
+                        const int ik = uni<64>(P->i), jk = uni<64>(P->j);
+                        const int64_t sk = uni<64>(P->s);
+                        int64_t ci[C], cj[C], xi[C], xj[C], nx[C];
+                        rp.load_rows(ik, jk, ci, cj);
+                        rp.prefetch_xt(ik, jk, xi, xj);
+                        MoveEval<C> ce;
+                        evaluate<G, C>(ci, cj, g, n, jk, sk, rp.entry(ik, ik), rp.entry(jk, ik),
+                                       rp.entry(jk, jk), ce);
+                        const bool fits = uni<64>((int32_t)rp.xt_row(sk, xi, xj, nx)) != 0;
+                        if (fits) {
+                            const int64_t ns = score + d;
+                            if (ns < best_score) {
+                                best_score = ns;
+                                best_move = (int64_t)m;
+                                flags |= RF_CUR_BEST;
+                                flags &= ~RF_BEST_STALE;
+                            } else {
+                                leave_best();
+                            }
+                            __syncthreads();  // every wave has read what changes
+                            rp.write(jk, cj, ce, nx);
+                            score = ns;
+                            offnz += uni<64>(ce.dnz);
+                            ++accepts;
+                            log_move(m, ik, jk, sk, 0);
+                            res = d < 0 ? 1 : 0;
+                            changed = true;
+                        }
+                    }
+                }
+                auto k4z = [&]() { return (uint32_t)uni<64>((int32_t)P->w4z); };
+                auto k4w = [&]() { return (uint32_t)uni<64>((int32_t)P->w4w); };
                 if (finish(m, res, k4z, k4w, changed)) {
                     solved = true;
                     break;
@@ -1615,6 +1790,8 @@ static const AnnealVariant kVariants[] = {
     {64, 2, true, 1, k_anneal<64, 2, true, 1>},   {64, 2, false, 1, k_anneal<64, 2, false, 1>},
     {16, 1, true, 4, k_anneal<16, 1, true, 4>},   {16, 2, true, 4, k_anneal<16, 2, true, 4>},
     {32, 1, true, 2, k_anneal<32, 1, true, 2>},   {32, 2, true, 2, k_anneal<32, 2, true, 2>},
+    {16, 1, true, 16, k_anneal<16, 1, true, 16>}, {16, 2, true, 16, k_anneal<16, 2, true, 16>},
+    {32, 2, true, 8, k_anneal<32, 2, true, 8>},
 };
 
 static const AnnealVariant* find_variant(int G, int C, bool tri, int K) {

@@ -80,8 +80,8 @@ void check_params(const xtx_params& p) {
     if (!(p.cooling > 0.0)) bad("cooling must be > 0");
     if (p.move_log && p.move_log_cap < 1) bad("move_log_cap must be >= 1 when move_log is set");
     if (p.tempering && p.replicas > 8000) bad("tempering supports at most 8000 replicas");
-    if (p.lookahead != 0 && p.lookahead != 1 && p.lookahead != 2 && p.lookahead != 4)
-        bad("lookahead must be 0, 1, 2 or 4");
+    if (p.lookahead < 0 || p.lookahead > 16 || (p.lookahead & (p.lookahead - 1)) != 0)
+        bad("lookahead must be 0, 1, 2, 4, 8 or 16");
 }
 
 // Builds the start state exactly: A_start = X0^T A0 X0 in __int128, rejecting
@@ -376,7 +376,7 @@ int verify_outcome(const HostInst& h, const Outcome& o) {
 // from its first group, so a wave never mixes instances (rep -1 = idle group).
 // U: lanes per replica (G K).
 void pad_to_wave(std::vector<RepRef>& refs, int U, int inst) {
-    const size_t per_wave = (size_t)(64 / U);
+    const size_t per_wave = (size_t)std::max(1, 64 / U);
     while (refs.size() % per_wave) refs.push_back(RepRef{inst, -1});
 }
 
@@ -408,6 +408,7 @@ void apply_layout(const xtx_ctx* ctx, HostInst& h, int lookahead) {
 }
 
 int groups_per_block_for(const Layout& L, int waves) {
+    if (L.G * L.K > 64) return 1;  // a replica over several waves: one per block
     const int per_wave = 64 / (L.G * L.K);
     int w = L.waves > 0 ? L.waves : (waves > 0 ? waves : 1);
     // keep a block within the LDS of one CU
@@ -587,9 +588,11 @@ int xtx_solve_plain(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t 
         h = HostInst{};
         prepare(h, A0, X0, n, p.replicas, p.seed, active, n_active, p);
         // Few replicas leave most of the device idle: each then gets a wave
+        // (at most one replica per compute unit: a workgroup of four waves)
         // whose lane groups evaluate its next proposals at once.
+        const int cus = ctx->prop.multiProcessorCount;
         const int look = p.lookahead != 0 ? p.lookahead
-                         : (h.R <= 4 * ctx->prop.multiProcessorCount ? 4 : 1);
+                         : (h.R <= cus ? 16 : (h.R <= 4 * cus ? 4 : 1));
         apply_layout(ctx, h, look);
         xtx_stats st{};
         st.verify_status = -1;

@@ -57,10 +57,12 @@ struct Layout {
 inline int order_of(int n) { return std::max(8, (n + 7) / 8 * 8); }
 
 // A, then per lane group 5 staged Philox blocks and a median histogram, and the
-// active list; X^T stays in HBM
+// active list; a replica over several waves adds two buffers of K 64-byte
+// proposals and a poll word (16-byte aligned). X^T stays in HBM.
 inline int rep_bytes_of(int nmax, int G, bool tri, int K = 1) {
     const int64_t a = tri ? (int64_t)nmax * (nmax + 1) / 2 : (int64_t)nmax * (nmax + 1);
-    const int64_t bytes = a * 8 + K * (5 * 16 + 4 * med_slots(G)) + nmax;
+    int64_t bytes = a * 8 + K * (5 * 16 + 4 * med_slots(G)) + nmax;
+    if (G * K > 64) bytes = (bytes + 15) / 16 * 16 + 2 * K * 64 + 16;
     return (int)((bytes + 15) / 16 * 16);
 }
 
@@ -85,10 +87,14 @@ inline Layout layout_for(int n) {
     return make_layout(nmax, 64, 2, true, 0);
 }
 
-// The lookahead layout of order nmax: one wave per replica, K groups of G
-// lanes, K the largest of 4 and 2 that is at most `want` and that a group can
-// hold the order with (C <= 2); K = 1 (the order's default layout) otherwise.
+// The lookahead layout of order nmax: K groups of G lanes per replica (one
+// wave for K G = 64, a workgroup of K G / 64 waves above), K the largest of
+// 16, 8, 4 and 2 that is at most `want` and has a kernel for the order (C <= 2);
+// K = 1 (the order's default layout) otherwise.
 inline Layout lookahead_layout(int nmax, int want) {
+    if (want >= 16 && nmax <= 16) return make_layout(nmax, 16, 1, true, 0, 16);
+    if (want >= 16 && nmax <= 32) return make_layout(nmax, 16, 2, true, 0, 16);
+    if (want >= 8 && nmax <= 64) return make_layout(nmax, 32, 2, true, 0, 8);
     if (want >= 4 && nmax <= 16) return make_layout(nmax, 16, 1, true, 0, 4);
     if (want >= 4 && nmax <= 32) return make_layout(nmax, 16, 2, true, 0, 4);
     if (want >= 2 && nmax <= 32) return make_layout(nmax, 32, 1, true, 0, 2);
