@@ -321,11 +321,11 @@ def test_every_launch_configuration_gives_the_same_result(eng, n):
 
 def test_lookahead_changes_speed_only(eng):
     """Evaluating a replica's next proposals at once (lookahead 2 or 4: one
-    wave per replica; 8 or 16: a workgroup of four waves; the lane groups
+    wave per replica; 8, 16 or 32: a workgroup of four or eight waves; the lane groups
     resolve the proposals in move order) gives the batch result of one group
     per replica, bit for bit, at every order (above 64 the engine keeps one
     group), with stall sweeps among the moves; a lookahead other than 0, 1, 2,
-    4, 8 or 16 is refused."""
+    4, 8, 16 or 32 is refused."""
     from xtax_amd._native import XtxError
     mats = [workloads.planted(n, 20, 2 * n, seed=500 + n)[0] for n in (5, 12, 24, 31, 40, 60, 70)]
     mats.append(workloads.random_symmetric(16, -20, 20, seed=8))
@@ -333,7 +333,7 @@ def test_lookahead_changes_speed_only(eng):
     base = dict(replicas=6, t_init=30, t_min=0.5, exchange_interval=300, max_epochs=6,
                 sweep_fraction=0.3, stuck_threshold=200)
     ref, _ = eng.batch_solve(mats, EngineParams(lookahead=1, **base), seeds=seeds)
-    for look in (2, 4, 8, 16):
+    for look in (2, 4, 8, 16, 32):
         say(f"batch lookahead={look}")
         got, _ = eng.batch_solve(mats, EngineParams(lookahead=look, **base), seeds=seeds)
         for q in range(len(mats)):

@@ -56,7 +56,7 @@ def test_l1_random_config(eng, k):
     n = A0.shape[0]
     if np.count_nonzero(np.triu(A0, 1)) == 0:
         pytest.skip("diagonal start")
-    look = [0, 1, 2, 4, 8, 16][k % 6]  # proposals evaluated at once per replica (0: the rule)
+    look = [0, 1, 2, 4, 8, 16, 32][k % 7]  # proposals evaluated at once per replica (0: the rule)
     print("config", k, "n", n, "lookahead", look, kw, flush=True)
     sys.stdout.flush()
     res = eng.run_annealer(A0, None, EngineParams(max_epochs=3, lookahead=look, **kw),

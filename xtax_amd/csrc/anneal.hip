@@ -618,12 +618,12 @@ static_assert(sizeof(Proposal) == 64, "two proposals per 128 bytes");
 // sweeps and reseeds write the same way. Every wave thus takes the same
 // branches and meets the same barriers.
 template <int G, int C, bool TRI, int K>
-__global__ void __launch_bounds__(256, K == 1 ? 4 : 2)
+__global__ void __launch_bounds__(G * K > 256 ? 512 : 256, K == 1 ? 4 : 2)
     k_anneal(const RepRef* __restrict__ table, int count, Inst* __restrict__ insts, int nmax,
              int rep_bytes, const volatile int* __restrict__ stop) {
     constexpr int W = K == 1 ? 1 : G * K / 64;  // waves per replica
     static_assert(K == 1 || (G * K) % 64 == 0, "lookahead groups fill whole waves");
-    static_assert(W <= 4, "a workgroup of at most 256 lanes");
+    static_assert(W <= 8, "a workgroup of at most 512 lanes");
     extern __shared__ __align__(16) unsigned char smem[];
     using Gp = Grp<G>;
     constexpr int U = G * K;  // lanes per replica
@@ -1792,6 +1792,7 @@ static const AnnealVariant kVariants[] = {
     {32, 1, true, 2, k_anneal<32, 1, true, 2>},   {32, 2, true, 2, k_anneal<32, 2, true, 2>},
     {16, 1, true, 16, k_anneal<16, 1, true, 16>}, {16, 2, true, 16, k_anneal<16, 2, true, 16>},
     {32, 2, true, 8, k_anneal<32, 2, true, 8>},
+    {16, 1, true, 32, k_anneal<16, 1, true, 32>}, {16, 2, true, 32, k_anneal<16, 2, true, 32>},
 };
 
 static const AnnealVariant* find_variant(int G, int C, bool tri, int K) {
@@ -1827,7 +1828,8 @@ hipError_t launch_anneal(hipStream_t st, int G, int C, bool tri, int K, int unit
                          const volatile int* stop) {
     if (count <= 0) return hipSuccess;
     const AnnealVariant* v = find_variant(G, C, tri, K);
-    if (!v || units_per_block < 1 || units_per_block * G * K > 256) return hipErrorInvalidValue;
+    if (!v || units_per_block < 1 || units_per_block * G * K > (G * K > 256 ? 512 : 256))
+        return hipErrorInvalidValue;
     const int groups_per_block = units_per_block;
     const int threads = units_per_block * G * K;
     const int blocks = (count + groups_per_block - 1) / groups_per_block;

@@ -80,8 +80,8 @@ void check_params(const xtx_params& p) {
     if (!(p.cooling > 0.0)) bad("cooling must be > 0");
     if (p.move_log && p.move_log_cap < 1) bad("move_log_cap must be >= 1 when move_log is set");
     if (p.tempering && p.replicas > 8000) bad("tempering supports at most 8000 replicas");
-    if (p.lookahead < 0 || p.lookahead > 16 || (p.lookahead & (p.lookahead - 1)) != 0)
-        bad("lookahead must be 0, 1, 2, 4, 8 or 16");
+    if (p.lookahead < 0 || p.lookahead > 32 || (p.lookahead & (p.lookahead - 1)) != 0)
+        bad("lookahead must be 0, 1, 2, 4, 8, 16 or 32");
 }
 
 // Builds the start state exactly: A_start = X0^T A0 X0 in __int128, rejecting

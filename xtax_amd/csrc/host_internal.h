@@ -89,9 +89,11 @@ inline Layout layout_for(int n) {
 
 // The lookahead layout of order nmax: K groups of G lanes per replica (one
 // wave for K G = 64, a workgroup of K G / 64 waves above), K the largest of
-// 16, 8, 4 and 2 that is at most `want` and has a kernel for the order (C <= 2);
+// 32, 16, 8, 4 and 2 that is at most `want` and has a kernel for the order (C <= 2);
 // K = 1 (the order's default layout) otherwise.
 inline Layout lookahead_layout(int nmax, int want) {
+    if (want >= 32 && nmax <= 16) return make_layout(nmax, 16, 1, true, 0, 32);
+    if (want >= 32 && nmax <= 32) return make_layout(nmax, 16, 2, true, 0, 32);
     if (want >= 16 && nmax <= 16) return make_layout(nmax, 16, 1, true, 0, 16);
     if (want >= 16 && nmax <= 32) return make_layout(nmax, 16, 2, true, 0, 16);
     if (want >= 8 && nmax <= 64) return make_layout(nmax, 32, 2, true, 0, 8);
