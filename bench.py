@@ -369,6 +369,22 @@ def bench_random(args, eng):
                        f"T in [0.5,50], greedy 0.5, cap {args.cap_epochs} epochs"}
     out = base_line(args, moves / el, "moves/s", el, cfg)
     out.update({"solved": solved, "instances": args.count, "cpu_baseline": None})
+    if not args.no_cpu and os.path.exists(REF_BIN):
+        # the reference on the first seeds, 8 threads = 8 replicas as here
+        extra = ["-t", "8", "--t-init", "50", "--t-min", "0.5", "--greedy-fraction", "0.5"]
+        cmv = csolved = 0
+        t1 = time.perf_counter()
+        for s in list(seeds)[:4]:
+            A0 = workloads.random_symmetric(10, -20, 20, seed=s)
+            ok, mv, _ = run_ref(A0, extra + ["--seed", str(s), "--max-seconds", str(args.cpu_cap)],
+                                timeout=args.cpu_cap + 30)
+            cmv += mv
+            csolved += int(ok)
+        cel = time.perf_counter() - t1
+        out["cpu_baseline"] = {"value": cmv / cel, "unit": "moves/s", "cores": 8,
+                               "kind": "reference", "solved": csolved,
+                               "sample": f"first 4 seeds: xtax_ref {' '.join(extra)} --seed s "
+                                         f"--max-seconds {args.cpu_cap}"}
     return out
 
 
