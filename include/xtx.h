@@ -80,10 +80,15 @@ typedef struct {
                                  takes a whole wave (16 or 8: a workgroup of
                                  four waves) and resolves them in order, the
                                  same trajectory at up to that many moves per
-                                 step. 0 = auto: xtx_solve_plain uses 16 when
-                                 replicas <= compute units, 4 when <= 4 per
-                                 compute unit, else 1; the batch and serve
-                                 forms use 1 */
+                                 step. 0 = auto, by the replicas resident
+                                 together (the call's, the batch's, or at a
+                                 serve admission those resident with it): up
+                                 to 32 per compute unit, 16 (or the order's
+                                 lookahead configuration); more, the order's
+                                 configuration (xtx_launch_set /
+                                 xtx_autotune) or the default layout. 1: one
+                                 group (the order's configuration if it has
+                                 one group) */
     /* Plain forms (xtx_solve_plain, xtx_dual_solve): called on the calling
      * thread at an epoch boundary whenever the instance best improves -- the
      * reference's "new best" report (congruence_anneal.hpp:309-352). seconds
@@ -212,7 +217,9 @@ typedef struct {
     int32_t waves_per_block;  /* 1, 2 or 4; 0 = the params' waves_per_block */
     int32_t tuned;            /* out: 1 if set (by hand or autotuner), 0 = rule */
     int32_t lds_bytes;        /* out: LDS per replica */
-    int32_t reserved;
+    int32_t lookahead;        /* lane groups per replica (xtx_params.lookahead):
+                                 0 or 1 = one; 2, 4 (one wave), 8, 16, 32 (a
+                                 workgroup; waves_per_block must be 0) */
     double moves_per_sec;     /* out of xtx_autotune: measured rate */
 } xtx_launch;
 

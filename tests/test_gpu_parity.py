@@ -244,6 +244,13 @@ def test_alone_vs_batch_vs_serve(eng):
         if d["solved"]:
             assert d["solve_move"] == alone[q][3]
         assert np.array_equal(A, alone[q][0]) and np.array_equal(X, alone[q][1]), q
+    # the rule gives these few replicas lookahead; one group per replica agrees
+    say("batch, one group per replica")
+    b1, _ = eng.batch_solve(mats, EngineParams(**{**ep.__dict__, "lookahead": 1}), seeds=seeds)
+    for q in range(len(mats)):
+        A, X, d = b1[q]
+        assert d["solved"] == alone[q][2] and d["verify_status"] == 0
+        assert np.array_equal(A, alone[q][0]) and np.array_equal(X, alone[q][1]), q
     say("serve")
     eng.serve_open(ep)
     try:
@@ -272,40 +279,52 @@ def test_alone_vs_batch_vs_serve(eng):
 
 # ------------------------------------------------------- launch configurations
 def _configs(order):
+    """(lanes, cols, triangular, waves per workgroup, lookahead)"""
     for G in (16, 32, 64):
         C = -(-order // G)
         if C > 2 or (G, C) == (16, 2):
             continue
         for tri in (1, 0):
             for waves in (1, 2, 4):
-                yield G, C, tri, waves
+                yield G, C, tri, waves, 1
+    for K in (2, 4, 8, 16, 32):
+        for G in (16, 32):
+            C = -(-order // G)
+            if C <= 2:
+                yield G, C, 1, 0, K
 
 
 @pytest.mark.parametrize("n", [6, 12, 20, 33, 47, 70])
 def test_every_launch_configuration_gives_the_same_result(eng, n):
-    """Lanes per replica, columns per lane, LDS layout and waves per workgroup
-    change speed only: the batch result of every configuration of an order
+    """Lanes per replica, columns per lane, LDS layout, waves per workgroup and
+    lookahead change speed only: the batch result of every configuration of an order
     equals the rule's, bit for bit (A, X, solve move, best score, and the
     proposal count of an unsolved instance; after a solve the other replicas
     stop at their next poll of the solve key, so that count is timing's)."""
     order = -(-n // 8) * 8
     mats = [workloads.planted(n, 20, 2 * n, seed=300 + q)[0] for q in range(3)]
     mats.append(workloads.random_symmetric(n, -20, 20, seed=7))
-    ep = EngineParams(replicas=6, t_init=30, t_min=0.5, exchange_interval=300, max_epochs=6)
+    base = dict(replicas=6, t_init=30, t_min=0.5, exchange_interval=300, max_epochs=6)
+    ep = EngineParams(**base)
     seeds = [77 + q for q in range(len(mats))]
     eng.launch_reset(order)
     ref, _ = eng.batch_solve(mats, ep, seeds=seeds)
     tried = 0
     try:
-        for G, C, tri, waves in _configs(order):
+        for G, C, tri, waves, K in _configs(order):
             try:
-                eng.launch_set(order, G, C, tri, waves)
+                eng.launch_set(order, G, C, tri, waves, K)
             except RuntimeError:
                 continue  # no kernel with that shape, or its LDS does not fit
             cfg = eng.launch_get(n)
-            assert (cfg["lanes"], cfg["cols"], cfg["triangular"], cfg["tuned"]) == (G, C, tri, 1)
-            say(f"n={n} G={G} C={C} tri={tri} waves={waves}")
-            got, _ = eng.batch_solve(mats, ep, seeds=seeds)
+            assert (cfg["lanes"], cfg["cols"], cfg["triangular"], cfg["tuned"],
+                    cfg["lookahead"]) == (G, C, tri, 1, K)
+            say(f"n={n} G={G} C={C} tri={tri} waves={waves} lookahead={K}")
+            # these few replicas get a lookahead layout by the rule: a one-group
+            # configuration applies where one group is requested, a lookahead
+            # one under the rule
+            got, _ = eng.batch_solve(mats, EngineParams(**base, lookahead=1 if K == 1 else 0),
+                                     seeds=seeds)
             tried += 1
             for q in range(len(mats)):
                 A, X, d = got[q]
@@ -313,7 +332,7 @@ def test_every_launch_configuration_gives_the_same_result(eng, n):
                 assert np.array_equal(A, ref[q][0]) and np.array_equal(X, ref[q][1]), (q, G, C)
                 keys = ("solved", "solve_move", "best_score") + (() if d["solved"] else ("moves",))
                 for k in keys:
-                    assert d[k] == ref[q][2][k], (k, q, G, C, tri, waves)
+                    assert d[k] == ref[q][2][k], (k, q, G, C, tri, waves, K)
     finally:
         eng.launch_reset(order)
     assert tried >= 2
@@ -363,9 +382,10 @@ def test_autotune_picks_a_measured_configuration(eng):
             assert chosen[0]["moves_per_sec"] == max(r["moves_per_sec"] for r in rows)
             cfg = eng.launch_get(n)
             assert cfg["tuned"] == 1
-            assert (cfg["lanes"], cfg["cols"], cfg["triangular"], cfg["waves_per_block"]) == (
+            assert (cfg["lanes"], cfg["cols"], cfg["triangular"], cfg["waves_per_block"],
+                    cfg["lookahead"]) == (
                 chosen[0]["lanes"], chosen[0]["cols"], chosen[0]["triangular"],
-                chosen[0]["waves_per_block"])
+                chosen[0]["waves_per_block"], chosen[0]["lookahead"])
         got, _ = eng.batch_solve(mats, ep, seeds=seeds)
         for q in range(len(mats)):
             assert np.array_equal(got[q][0], ref[q][0]) and np.array_equal(got[q][1], ref[q][1])

@@ -395,16 +395,40 @@ static std::string g_open_error;
 
 namespace {
 
-Layout layout_in(const xtx_ctx* ctx, int n) {
-    const auto it = ctx->tuned.find(order_of(n));
-    return it != ctx->tuned.end() ? it->second : layout_for(n);
+// The layout instances of order n run in. `requested`: the params' lookahead
+// (0 = automatic); `rule`: the lookahead rule's value for the replicas resident
+// together. A requested lookahead gives its layout (1: the order's
+// configuration from xtx_launch_set / xtx_autotune if it has one lane group,
+// else the default layout). Automatic with the rule at 1 (a full device): the
+// order's configuration, else the default layout; with the rule above 1 (few
+// replicas): the order's configuration if it is a lookahead one, else the
+// rule's lookahead layout.
+Layout choose_layout(const xtx_ctx* ctx, int n, int requested, int rule) {
+    const int nmax = order_of(n);
+    const auto it = ctx->tuned.find(nmax);
+    const bool has = it != ctx->tuned.end();
+    if (requested > 1) return lookahead_layout(nmax, requested);
+    if (requested == 1) return has && it->second.K == 1 ? it->second : layout_for(n);
+    if (rule > 1) return has && it->second.K > 1 ? it->second : lookahead_layout(nmax, rule);
+    return has ? it->second : layout_for(n);
 }
 
-// lookahead: the most groups per replica wanted (1: the tuned or default layout)
-void apply_layout(const xtx_ctx* ctx, HostInst& h, int lookahead) {
-    h.lay = lookahead > 1 ? lookahead_layout(order_of(h.n), lookahead) : layout_in(ctx, h.n);
+void apply_layout(const xtx_ctx* ctx, HostInst& h, int requested, int rule) {
+    h.lay = choose_layout(ctx, h.n, requested, rule);
     if (h.lay.rep_bytes > g_lds_block)
         throw Err{XTX_E_ARG, "n too large for an LDS-resident replica"};
+}
+
+// The lookahead rule for `replicas` resident together (a plain call's, a
+// batch's, or a serve admission's with those already resident): 16 while they
+// number at most 32 per compute unit, else 1. Measured: 256 replicas of order
+// 24 run 5.4x the moves/s of one group per replica; serving at 200 requests/s
+// (about 4000 replicas resident) halves the latency; an overloaded serve
+// (400/s, where a lookahead replica's larger LDS share admits fewer requests)
+// runs faster with one group per replica, and a full batch (32768) the same.
+constexpr int kLookahead = 16;
+int lookahead_rule(const xtx_ctx* ctx, int64_t replicas) {
+    return replicas <= 32 * (int64_t)ctx->prop.multiProcessorCount ? kLookahead : 1;
 }
 
 int groups_per_block_for(const Layout& L, int waves) {
@@ -587,13 +611,7 @@ int xtx_solve_plain(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t 
         HostInst& h = ctx->plain;
         h = HostInst{};
         prepare(h, A0, X0, n, p.replicas, p.seed, active, n_active, p);
-        // Few replicas leave most of the device idle: each then gets a wave
-        // (at most one replica per compute unit: a workgroup of four waves)
-        // whose lane groups evaluate its next proposals at once.
-        const int cus = ctx->prop.multiProcessorCount;
-        const int look = p.lookahead != 0 ? p.lookahead
-                         : (h.R <= cus ? 16 : (h.R <= 4 * cus ? 4 : 1));
-        apply_layout(ctx, h, look);
+        apply_layout(ctx, h, p.lookahead, lookahead_rule(ctx, h.R));
         xtx_stats st{};
         st.verify_status = -1;
         if (n < 2 || h.N < 2 || h.offnz0 == 0) {
@@ -898,6 +916,8 @@ struct Server {
     std::map<BucketKey, Bucket> buckets;
     bool dirty = true;
     int64_t lds_in_use = 0;
+    int64_t resident_reps = 0;
+    bool auto_look = false;  // serve: the lookahead rule at each admission
     int max_r = 1;
     clk::time_point t0 = clk::now();
     double kernel_ms = 0;
@@ -932,6 +952,8 @@ struct Server {
     bool try_admit(std::unique_ptr<HostInst>& h, std::vector<Staging>* keep = nullptr) {
         if (cfg.max_resident > 0 && (int)(slots.size() - free_slots.size()) >= cfg.max_resident)
             return false;
+        if (auto_look && p.lookahead == 0)
+            apply_layout(ctx, *h, 0, lookahead_rule(ctx, resident_reps + h->R));
         if (lds_in_use + lds_cost(*h) > lds_budget() && lds_in_use > 0) return false;
         char* dm = arena.alloc(h->bytes);
         if (!dm) return false;
@@ -946,6 +968,7 @@ struct Server {
             HIPCHK(hipStreamSynchronize(ctx->stream));
         h->admit_s = now();
         lds_in_use += lds_cost(*h);
+        resident_reps += h->R;
         int slot;
         if (!free_slots.empty()) {
             slot = free_slots.back();
@@ -1088,6 +1111,7 @@ struct Server {
             moves_retired += o.moves;
             accepts_retired += o.accepts;
             lds_in_use -= lds_cost(h);
+            resident_reps -= h.R;
             arena.free(h.dmem, h.bytes);
             slots[s].reset();
             free_slots.push_back((int)s);
@@ -1116,7 +1140,7 @@ std::unique_ptr<HostInst> make_inst(const xtx_ctx* ctx, const xtx_instance& in,
     q.replicas = R;
     check_params(q);
     prepare(*h, in.A0, in.X0, in.n, R, in.seed, in.active, in.n_active, q);
-    apply_layout(ctx, *h, p.lookahead);
+    apply_layout(ctx, *h, p.lookahead, 1);  // batch and serve apply their rule
     if (h->n < 2 || h->N < 2)
         throw Err{XTX_E_ARG, "instance needs n >= 2 and at least two active indices"};
     h->bytes = device_bytes(*h);
@@ -1139,33 +1163,41 @@ void fill_launch(const Layout& L, bool tuned, xtx_launch* out) {
     out->waves_per_block = L.waves;
     out->tuned = tuned ? 1 : 0;
     out->lds_bytes = L.rep_bytes;
+    out->lookahead = L.K;
 }
 
 // A configuration the kernels can run for this order, or an error message.
-const char* launch_invalid(int nmax, int G, int C, bool tri, int waves) {
+const char* launch_invalid(int nmax, int G, int C, bool tri, int waves, int K) {
     if (nmax < 8 || nmax > kMaxN || nmax % 8) return "order must be a multiple of 8 in [8, 128]";
     if (G * C < nmax) return "lanes * cols must cover the order";
-    if (!anneal_variant_exists(G, C, tri, 1)) return "no kernel with these lanes/cols/layout";
+    if (!anneal_variant_exists(G, C, tri, K)) return "no kernel with these lanes/cols/layout/lookahead";
     if (waves < 0 || waves > 4 || waves == 3) return "waves_per_block must be 0, 1, 2 or 4";
-    const int rb = rep_bytes_of(nmax, G, tri);
-    if ((int64_t)(64 / G) * rb > g_lds_block) return "replica LDS exceeds a workgroup";
+    if (K > 1 && waves != 0) return "a lookahead replica takes its own wave or workgroup (waves 0)";
+    const int rb = rep_bytes_of(nmax, G, tri, K);
+    if ((int64_t)std::max(1, 64 / (G * K)) * rb > g_lds_block) return "replica LDS exceeds a workgroup";
     return nullptr;
 }
 
 // Every valid configuration of an order: G with C = ceil(nmax / G) <= 2,
-// both layouts, 1, 2 or 4 waves per workgroup (those the LDS allows).
+// both layouts, 1, 2 or 4 waves per workgroup (those the LDS allows); and the
+// lookahead configurations (K = 2 .. 32 groups per replica, triangular A).
 std::vector<Layout> candidates(int nmax) {
     std::vector<Layout> v;
     for (int G : {16, 32, 64}) {
         const int C = (nmax + G - 1) / G;
         for (bool tri : {true, false})
             for (int w : {1, 2, 4}) {
-                if (launch_invalid(nmax, G, C, tri, w)) continue;
+                if (launch_invalid(nmax, G, C, tri, w, 1)) continue;
                 const Layout L = make_layout(nmax, G, C, tri, w);
                 if ((int64_t)w * (64 / G) * L.rep_bytes > g_lds_block) continue;
                 v.push_back(L);
             }
     }
+    for (int K : {2, 4, 8, 16, 32})
+        for (int G : {16, 32}) {
+            const int C = (nmax + G - 1) / G;
+            if (!launch_invalid(nmax, G, C, true, 0, K)) v.push_back(make_layout(nmax, G, C, true, 0, K));
+        }
     return v;
 }
 
@@ -1281,7 +1313,7 @@ void autotune_orders(xtx_ctx* ctx, const xtx_instance* sample, int count,
             for (size_t q = first; q < out->size(); ++q) {
                 const xtx_launch& r = (*out)[q];
                 if (r.lanes == best.G && r.cols == best.C && r.triangular == (best.tri ? 1 : 0) &&
-                    r.waves_per_block == best.waves)
+                    r.waves_per_block == best.waves && r.lookahead == best.K)
                     (*out)[q].tuned = 1;
             }
         }
@@ -1295,7 +1327,7 @@ extern "C" {
 int xtx_launch_get(xtx_ctx* ctx, int32_t n, xtx_launch* out) {
     if (!ctx || !out || n < 2 || n > kMaxN) return XTX_E_ARG;
     const bool tuned = ctx->tuned.count(order_of(n)) != 0;
-    fill_launch(layout_in(ctx, n), tuned, out);
+    fill_launch(choose_layout(ctx, n, 0, 1), tuned, out);
     return XTX_OK;
 }
 
@@ -1305,11 +1337,12 @@ int xtx_launch_set(xtx_ctx* ctx, const xtx_launch* cfg) {
         ctx->tuned.erase(cfg->order);
         return XTX_OK;
     }
+    const int K = cfg->lookahead > 1 ? cfg->lookahead : 1;
     if (const char* why = launch_invalid(cfg->order, cfg->lanes, cfg->cols, cfg->triangular != 0,
-                                         cfg->waves_per_block))
+                                         cfg->waves_per_block, K))
         return fail(ctx, Err{XTX_E_ARG, why});
     ctx->tuned[cfg->order] = make_layout(cfg->order, cfg->lanes, cfg->cols,
-                                         cfg->triangular != 0, cfg->waves_per_block);
+                                         cfg->triangular != 0, cfg->waves_per_block, K);
     return XTX_OK;
 }
 
@@ -1356,6 +1389,10 @@ int xtx_batch_solve(xtx_ctx* ctx, const xtx_instance* instances, int32_t count,
             hs.back()->tag = (uint64_t)q;
             total += hs.back()->bytes;
         }
+        int64_t reps = 0;
+        for (const auto& h : hs) reps += h->R;
+        const int rule = lookahead_rule(ctx, reps);
+        for (auto& h : hs) apply_layout(ctx, *h, params->lookahead, rule);
         S.arena.reserve(total + 4096);
         S.t0 = clk::now();
         std::vector<Staging> keep;
@@ -1423,6 +1460,7 @@ int xtx_serve_open(xtx_ctx* ctx, const xtx_params* params, const xtx_serve_confi
         auto S = std::make_unique<Server>();
         S->ctx = ctx;
         S->p = *params;
+        S->auto_look = true;
         if (cfg) S->cfg = *cfg;
         const int64_t hbm = S->cfg.hbm_budget_bytes > 0 ? S->cfg.hbm_budget_bytes
                                                          : (int64_t)8 << 30;
