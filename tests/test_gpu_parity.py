@@ -213,6 +213,45 @@ def test_trajectory_matches_model(eng, orc, case, look):
     assert st_o & ~2 == 0 and sign in (1, -1)
 
 
+@pytest.mark.parametrize("look", [0, 1, 4], ids=["auto", "K1", "K4"])
+def test_trajectory_active_rows_start_transform_and_cap(eng, orc, look):
+    """Moves restricted to an active list (as --gram-rows and the deflation
+    loop use it), a start transform X0 and a per-replica move cap that falls
+    inside an epoch (and inside a lookahead step): every replica ends in the
+    oracle model's state, with its accepted-move log, bit for bit."""
+    n = 20
+    A0 = workloads.random_symmetric(n, -20, 20, seed=21)
+    X0 = np.eye(n, dtype=np.int64)
+    X0[0, 3], X0[5, 1], X0[7, 12] = 2, -1, 1
+    act = [0, 1, 3, 5, 6, 9, 12, 14, 17, 19]
+    A_start = (X0.T.astype(object) @ A0.astype(object) @ X0.astype(object)).astype(np.int64)
+    cap = 100000
+    ep, op = _both(replicas=5, seed=23, exchange_interval=300, t_init=40, t_min=0.5,
+                   max_moves=1237, sweep_fraction=0.3, stuck_threshold=150, max_epochs=6,
+                   move_log=True, move_log_cap=cap)
+    ep.lookahead = look
+    say(f"active rows, X0, cap: lookahead={look}")
+    res = eng.run_annealer(A0, X0, ep, active=act)
+    model = orc.model_run(A_start, X0, op, 6, active=act, log_cap=cap)
+    st = res.stats
+    gA, gXt, gsc = eng.last_replicas(ep.replicas, n)
+    assert st["solved"] == model["solved"] == 0
+    assert st["epochs"] == model["epochs"]
+    assert np.array_equal(gA, model["A"]) and np.array_equal(gXt, model["Xt"])
+    assert np.array_equal(gsc, model["scal"]), (gsc, model["scal"])
+    assert (gsc[:, 2] == 1237).all()  # every replica stopped at the cap
+    b = model["best_replica"]
+    assert np.array_equal(res.A, model["best_A"][b]) and np.array_equal(res.X, model["best_Xt"][b].T)
+    for r in range(ep.replicas):
+        glog, trunc = eng.movelog(r)
+        assert not trunc and np.array_equal(glog, model["logs"][r]), r
+        assert set(glog["i"]) <= set(act) and set(glog["j"]) <= set(act)
+    # columns outside the active list keep X0's
+    for k in range(n):
+        if k not in act:
+            assert np.array_equal(res.X[:, k], X0[:, k])
+
+
 # --------------------------------------------------------------- result forms
 def test_plain_solves_reference_sample(eng, orc):
     A0 = _matrix("ten")
