@@ -1830,10 +1830,9 @@ hipError_t launch_anneal(hipStream_t st, int G, int C, bool tri, int K, int unit
     const AnnealVariant* v = find_variant(G, C, tri, K);
     if (!v || units_per_block < 1 || units_per_block * G * K > (G * K > 256 ? 512 : 256))
         return hipErrorInvalidValue;
-    const int groups_per_block = units_per_block;
     const int threads = units_per_block * G * K;
-    const int blocks = (count + groups_per_block - 1) / groups_per_block;
-    const size_t lds = (size_t)groups_per_block * rep_bytes;
+    const int blocks = (count + units_per_block - 1) / units_per_block;
+    const size_t lds = (size_t)units_per_block * rep_bytes;
     hipLaunchKernelGGL(v->fn, dim3(blocks), dim3(threads), lds, st, table, count, insts, nmax,
                        rep_bytes, stop);
     return hipGetLastError();

@@ -386,7 +386,7 @@ struct Bucket {
     std::vector<RepRef> refs;
     RepRef* d_refs = nullptr;
     int cap = 0;
-    int groups_per_block = 1;  // replicas (table entries) per block
+    int units_per_block = 1;  // replicas (table entries) per block
 };
 
 }  // namespace
@@ -431,7 +431,7 @@ int lookahead_rule(const xtx_ctx* ctx, int64_t replicas) {
     return replicas <= 32 * (int64_t)ctx->prop.multiProcessorCount ? kLookahead : 1;
 }
 
-int groups_per_block_for(const Layout& L, int waves) {
+int units_per_block_for(const Layout& L, int waves) {
     if (L.G * L.K > 64) return 1;  // a replica over several waves: one per block
     const int per_wave = 64 / (L.G * L.K);
     int w = L.waves > 0 ? L.waves : (waves > 0 ? waves : 1);
@@ -461,7 +461,7 @@ void launch_epoch(xtx_ctx* ctx, std::vector<Bucket>& buckets, Inst* d_insts, int
         const Bucket& b = *order[q];
         const int lane = (int)(q % (size_t)(ns + 1));
         hipStream_t st = lane == 0 ? ctx->stream : ctx->side[lane - 1];
-        HIPCHK(launch_anneal(st, b.lay.G, b.lay.C, b.lay.tri, b.lay.K, b.groups_per_block, b.d_refs,
+        HIPCHK(launch_anneal(st, b.lay.G, b.lay.C, b.lay.tri, b.lay.K, b.units_per_block, b.d_refs,
                              (int)b.refs.size(), d_insts, b.lay.nmax, b.lay.rep_bytes,
                              ctx->stop_dev));
     }
@@ -646,7 +646,7 @@ int xtx_solve_plain(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t 
                               ctx->stream));
         std::vector<Bucket> buckets(1);
         buckets[0].lay = h.lay;
-        buckets[0].groups_per_block = groups_per_block_for(h.lay, p.waves_per_block);
+        buckets[0].units_per_block = units_per_block_for(h.lay, p.waves_per_block);
         for (int r = 0; r < h.R; ++r) buckets[0].refs.push_back(RepRef{0, r});
         pad_to_wave(buckets[0].refs, h.lay.G * h.lay.K, 0);
         Inst* pin = nullptr;
@@ -687,7 +687,7 @@ int xtx_solve_plain(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t 
                 HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
                 for (auto& b : buckets)
                     HIPCHK(launch_anneal(ctx->stream, b.lay.G, b.lay.C, b.lay.tri, b.lay.K,
-                                         b.groups_per_block,
+                                         b.units_per_block,
                                          b.d_refs, (int)b.refs.size(), ctx->d_plain,
                                          b.lay.nmax, b.lay.rep_bytes, ctx->stop_dev));
                 HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
@@ -1013,7 +1013,7 @@ struct Server {
             max_r = std::max(max_r, h.R);
             Bucket& b = buckets[key_of(h.lay)];
             b.lay = h.lay;
-            b.groups_per_block = groups_per_block_for(h.lay, p.waves_per_block);
+            b.units_per_block = units_per_block_for(h.lay, p.waves_per_block);
             for (int r = 0; r < h.R; ++r) b.refs.push_back(RepRef{s, r});
             pad_to_wave(b.refs, h.lay.G * h.lay.K, s);
         }
@@ -1219,7 +1219,7 @@ double time_layout(xtx_ctx* ctx, const std::vector<const HostInst*>& proto, cons
     std::vector<Inst> recs(hs.size());
     Bucket b;
     b.lay = L;
-    b.groups_per_block = groups_per_block_for(L, p.waves_per_block);
+    b.units_per_block = units_per_block_for(L, p.waves_per_block);
     for (size_t q = 0; q < hs.size(); ++q) {
         HostInst& h = hs[q];
         upload(h, arena.alloc(h.bytes), ctx->stream, keep[q]);
@@ -1240,7 +1240,7 @@ double time_layout(xtx_ctx* ctx, const std::vector<const HostInst*>& proto, cons
         HIPCHK(hipMemcpyAsync(b.d_refs, b.refs.data(), b.refs.size() * sizeof(RepRef),
                               hipMemcpyHostToDevice, ctx->stream));
         HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-        HIPCHK(launch_anneal(ctx->stream, L.G, L.C, L.tri, L.K, b.groups_per_block, b.d_refs,
+        HIPCHK(launch_anneal(ctx->stream, L.G, L.C, L.tri, L.K, b.units_per_block, b.d_refs,
                              (int)b.refs.size(), d_insts, L.nmax, L.rep_bytes, ctx->stop_dev));
         HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
