@@ -255,3 +255,26 @@ def test_publish_reorder(tmp_path, flag, bits):
     assert np.array_equal(A, Ae) and np.array_equal(X, Xe)
     assert np.array_equal(A, Ap[np.ix_(perm, perm)]) and np.array_equal(X, Xp[:, perm])
     assert (X.T @ A0 @ X == A).all()
+
+
+@pytest.mark.gpu
+def test_lookahead_flag_changes_speed_only(tmp_path):
+    """--lookahead: one lane group per replica, or the rule's 16 proposals at
+    once, write the same best_A.csv and best_X.csv for a fixed epoch budget;
+    an unsupported value is an input error."""
+    from xtax_amd import workloads
+    A0 = workloads.random_symmetric(12, -20, 20, seed=31)
+    outs = []
+    for look in ("1", "0", "4"):
+        d = tmp_path / look
+        d.mkdir()
+        write_csv(d / "a.csv", A0)
+        r = run(["-A", "a.csv", "-t", "6", "--seed", "7", "--max-epochs", "4",
+                 "--exchange-interval", "400", "--lookahead", look], d)
+        assert r.returncode == 0, r.stderr
+        outs.append((read_csv(d / "best_A.csv"), read_csv(d / "best_X.csv")))
+    for A, X in outs[1:]:
+        assert np.array_equal(A, outs[0][0]) and np.array_equal(X, outs[0][1])
+    write_csv(tmp_path / "a.csv", A0)
+    r = run(["-A", "a.csv", "-t", "6", "--max-epochs", "1", "--lookahead", "3"], tmp_path)
+    assert r.returncode == 1 and "lookahead" in r.stderr

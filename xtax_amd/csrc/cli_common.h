@@ -284,7 +284,8 @@ inline int parse_args(int argc, char** argv, const std::vector<Opt>& opts, const
 
 // The engine options both drop-ins share (congruence_anneal.hpp:122-153 via
 // main.cpp:583-640 and xdual_main.cpp:465-500), plus the GPU-only --device,
-// --max-moves and --max-epochs.
+// --max-moves, --max-epochs and --lookahead (xtx_params.lookahead; the L1
+// engine only).
 inline std::vector<Opt> engine_options(xtx_params& p, int& device) {
     return {
         {{"-t", "--threads", "--replicas"}, false, set_int(p.replicas)},
@@ -312,6 +313,7 @@ inline std::vector<Opt> engine_options(xtx_params& p, int& device) {
         {{"--device"}, false, [&device](const std::string& s) { device = (int)std::stol(s); }},
         {{"--max-moves"}, false, [&p](const std::string& s) { p.max_moves = std::stoull(s); }},
         {{"--max-epochs"}, false, set_int(p.max_epochs)},
+        {{"--lookahead"}, false, set_int(p.lookahead)},
     };
 }
 

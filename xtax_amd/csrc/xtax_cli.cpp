@@ -10,7 +10,8 @@
 // reference's deflation outer loop over the engine; --rcm / --centroid reorder
 // the published best. GPU-only additions: --device, --max-moves (proposals
 // per replica), --max-epochs (epochs of exchange-interval moves; a run with
-// it and no time limit is deterministic).
+// it and no time limit is deterministic), --lookahead (proposals of a replica
+// evaluated at once; speed only).
 #include <algorithm>
 #include <atomic>
 #include <chrono>
