@@ -114,17 +114,20 @@ TRAJ = [  # n, R, tempering, extra engine settings, dual_refresh, dual_check
 ]
 
 
+# lookahead 0: the rule (here 16, 8 or 4 lane groups per replica over a
+# four-wave workgroup for n <= 64), 1: one group, 2/4: one wave's groups
+@pytest.mark.parametrize("look", [0, 1, 4], ids=["auto", "K1", "K4"])
 @pytest.mark.parametrize("n,R,tempering,extra,refresh,check", TRAJ)
-def test_trajectory_equals_model(eng, po, n, R, tempering, extra, refresh, check):
+def test_trajectory_equals_model(eng, po, n, R, tempering, extra, refresh, check, look):
     seed = 100 + n
     A0 = gram(n, seed)
     kw = dict(replicas=R, seed=seed, tempering=tempering, exchange_interval=300,
               stuck_threshold=150, sweep_fraction=0.5, greedy_fraction=0.6)
     kw.update(extra)
     ep = EngineParams(max_epochs=6 if extra else 4, move_log=True, move_log_cap=4096,
-                      verify=True, **kw)
+                      verify=True, lookahead=look, **kw)
     dpar = DualParams(dual_refresh=refresh, dual_check=check, dual_tol=1e-9)
-    say("dual trajectory n", n, "R", R, "tempering", tempering, extra)
+    say("dual trajectory n", n, "R", R, "tempering", tempering, extra, "lookahead", look)
     res = eng.run_xdual(A0, None, ep, dpar, allow_unverified=False)
     st = res.stats
     P, Q, Xt, dsc, sc = eng.dual_last_replicas(R, n)

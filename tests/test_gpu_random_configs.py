@@ -89,8 +89,9 @@ def test_dual_random_config(eng, k):
     rng = np.random.default_rng(2000 + k)
     refresh = int(rng.choice([0, 150, 600])), int(rng.choice([0, 100, 400]))
     lam = float(rng.choice([0.5, 1.0, 2.0]))
-    print("dual config", k, "n", n, kw, refresh, lam, flush=True)
-    res = eng.run_xdual(A0, None, EngineParams(max_epochs=3, **kw),
+    look = [0, 1, 2, 4, 8, 16][k % 6]  # lane groups per replica (0: the rule)
+    print("dual config", k, "n", n, "lookahead", look, kw, refresh, lam, flush=True)
+    res = eng.run_xdual(A0, None, EngineParams(max_epochs=3, lookahead=look, **kw),
                         DualParams(lambda_=lam, dual_refresh=refresh[0], dual_check=refresh[1],
                                    dual_tol=1e-9), allow_unverified=False)
     st = res.stats

@@ -569,16 +569,6 @@ struct Replica {
     }
 };
 
-// int64 value of lane `src` (runtime, per lane) of the wave, in every lane
-__device__ __forceinline__ int64_t bperm64(int64_t v, int src) {
-    const int lo = __builtin_amdgcn_ds_bpermute(src << 2, (int)(uint32_t)(uint64_t)v);
-    const int hi = __builtin_amdgcn_ds_bpermute(src << 2, (int)(uint32_t)((uint64_t)v >> 32));
-    return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
-}
-__device__ __forceinline__ double readlane_f64(double v, int lane) {
-    return __builtin_bit_cast(double, readlane64(__builtin_bit_cast(int64_t, v), lane));
-}
-
 // One lookahead proposal as the other waves of its replica read it (LDS).
 struct alignas(16) Proposal {
     int64_t delta;
@@ -640,7 +630,7 @@ __global__ void __launch_bounds__(G * K > 256 ? 512 : 256, K == 1 ? 4 : 2)
                                        (int)(threadIdx.x >> 6) * (64 / U));
     Inst* I = insts + __builtin_amdgcn_readfirstlane(table[wave_gid0].inst);
     if (I->status != ST_RUNNING) return;
-    const int r = U == 64 ? __builtin_amdgcn_readfirstlane(table[gid].rep) : table[gid].rep;
+    const int r = U >= 64 ? __builtin_amdgcn_readfirstlane(table[gid].rep) : table[gid].rep;
     if (r < 0) return;  // padding group
     GLOBAL RepState* S = (GLOBAL RepState*)I->rs + r;
     if (S->flags & RF_STOPPED) return;
@@ -1307,9 +1297,6 @@ __global__ void __launch_bounds__(G * K > 256 ? 512 : 256, K == 1 ? 4 : 2)
     } else {
         // lookahead over W waves: group kk proposes move m0 + kk and publishes it
         const int ln = (int)(threadIdx.x & 63u);
-        auto uni_f64 = [](double v) {
-            return __builtin_bit_cast(double, uni<64>(__builtin_bit_cast(int64_t, v)));
-        };
         for (uint32_t it = 0; moves < stop_at; ++it) {
             const uint64_t m0 = moves + 1;
             const uint64_t mg = m0 + (uint64_t)kk;

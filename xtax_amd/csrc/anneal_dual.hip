@@ -13,7 +13,8 @@
 // (nv - old)(nv + old) with both factors exact; no fused multiply-add.
 //
 // Kernels:
-//   k_anneal_dual<G,C>  one epoch of many replicas
+//   k_anneal_dual<G,C,K> one epoch of many replicas (K > 1: lookahead, as in
+//                       anneal.hip's k_anneal)
 //   k_dual_maint        per replica, before an epoch: dual re-inversion
 //                       (Gauss-Jordan, invert_to, src/mat_io.hpp:297-337) and
 //                       residual check when their periods were crossed, and
@@ -300,22 +301,45 @@ __device__ __forceinline__ int64_t dual_best_shear(const int64_t (&pi)[C], const
 // one epoch
 // ---------------------------------------------------------------------------
 
-template <int G, int C>
+// One lookahead proposal as the other waves of its replica read it (LDS).
+struct alignas(16) DualProposal {
+    double d;
+    int64_t s;
+    double u;           // the Metropolis draw
+    int32_t i, j;
+    uint32_t w4z, w4w;  // the stall words
+    uint32_t flags;     // 1: over the bound, 2: passes at the step's temperature, 4: greedy
+    uint32_t pad[3];
+};
+static_assert(sizeof(DualProposal) == 64, "two proposals per 128 bytes");
+
+// K > 1: lookahead exactly as in k_anneal (anneal.hip): the replica's K lane
+// groups evaluate its next K proposals on its current state and resolve them
+// in move order; K G = 64 in one wave (values through v_readlane, a committed
+// move's rows through ds_bpermute), K G > 64 over a workgroup of K G / 64 waves
+// (proposals through LDS, a committed move evaluated again by every group,
+// workgroup barriers before every write of the shared state).
+template <int G, int C, int K>
 __global__ void __launch_bounds__(256, 2)
     k_anneal_dual(const RepRef* __restrict__ table, int count, Inst* __restrict__ insts, int nmax,
                   int rep_bytes, const volatile int* __restrict__ stop) {
+    constexpr int W = K == 1 ? 1 : G * K / 64;  // waves per replica
+    static_assert(K == 1 || (G * K) % 64 == 0, "lookahead groups fill whole waves");
+    static_assert(W <= 4, "a workgroup of at most 256 lanes");
     extern __shared__ __align__(16) unsigned char smem[];
     using Gp = Grp<G>;
+    constexpr int U = G * K;  // lanes per replica
     const int g = (int)threadIdx.x & (G - 1);
-    const int slot = (int)threadIdx.x / G;
-    const int gid = (int)blockIdx.x * ((int)blockDim.x / G) + slot;
+    const int kk = K == 1 ? 0 : (int)(threadIdx.x % (unsigned)U) / G;  // lookahead index
+    const int slot = (int)threadIdx.x / U;
+    const int gid = (int)blockIdx.x * ((int)blockDim.x / U) + slot;
     if (gid >= count) return;
     const int wave_gid0 =
-        __builtin_amdgcn_readfirstlane((int)blockIdx.x * ((int)blockDim.x / G) +
-                                       (int)(threadIdx.x >> 6) * (64 / G));
+        __builtin_amdgcn_readfirstlane((int)blockIdx.x * ((int)blockDim.x / U) +
+                                       (int)(threadIdx.x >> 6) * (64 / U));
     Inst* I = insts + __builtin_amdgcn_readfirstlane(table[wave_gid0].inst);
     if (I->status != ST_RUNNING) return;
-    const int r = G == 64 ? __builtin_amdgcn_readfirstlane(table[gid].rep) : table[gid].rep;
+    const int r = U >= 64 ? __builtin_amdgcn_readfirstlane(table[gid].rep) : table[gid].rep;
     if (r < 0) return;
     GLOBAL RepState* S = (GLOBAL RepState*)I->rs + r;
     if (S->flags & RF_STOPPED) return;
@@ -324,8 +348,14 @@ __global__ void __launch_bounds__(256, 2)
     DualReplica<G, C> rp;
     rp.sP = (int64_t*)(smem + (size_t)slot * rep_bytes);
     rp.sQ = (double*)(rp.sP + tri);
-    uint4* swords = (uint4*)(rp.sQ + tri);
-    uint8_t* sact = (uint8_t*)(swords + 5);
+    // then five staged Philox blocks per group, the active list, and for
+    // W > 1 two buffers of K proposals (by step parity) and the poll word
+    uint4* const words0 = (uint4*)(rp.sQ + tri);
+    uint4* const swords = words0 + 5 * kk;
+    uint8_t* sact = (uint8_t*)(words0 + 5 * K);
+    DualProposal* const props =
+        (DualProposal*)((((uintptr_t)(sact + nmax)) + 15) & ~(uintptr_t)15);
+    volatile uint32_t* const spoll = (volatile uint32_t*)(props + 2 * K);
     rp.n = I->n;
     rp.g = g;
     rp.init_lanes();
@@ -358,17 +388,17 @@ __global__ void __launch_bounds__(256, 2)
     const double cw = I->c;
     const double gb = I->gb_dscore;  // instance best at the epoch start
 
-    uint64_t moves = (uint64_t)uni<G>((int64_t)S->moves);
+    uint64_t moves = (uint64_t)uni<U>((int64_t)S->moves);
     double score = S->dscore, local_best = S->dlocal_best, best_score = S->dbest_score;
-    int64_t offnz = uni<G>(S->offnz);
-    uint32_t msi = (uint32_t)uni<G>((int32_t)S->msi);
+    int64_t offnz = uni<U>(S->offnz);
+    uint32_t msi = (uint32_t)uni<U>((int32_t)S->msi);
     double T = tempering ? ((const GLOBAL double*)I->temps)[S->rung] : S->T;
     double nbeta = -1.0 / T;
     double cooling = S->cooling;
-    int32_t cool_counter = uni<G>(S->cool_counter), win_p = uni<G>(S->win_p);
-    int32_t win_a = uni<G>(S->win_a);
-    int64_t best_move = uni<G>(S->best_move);
-    int32_t flags = uni<G>(S->flags);
+    int32_t cool_counter = uni<U>(S->cool_counter), win_p = uni<U>(S->win_p);
+    int32_t win_a = uni<U>(S->win_a);
+    int64_t best_move = uni<U>(S->best_move);
+    int32_t flags = uni<U>(S->flags);
     const Coin greedy_coin(S->gf), target_coin(S->tf);
     uint32_t accepts = 0, greedy_n = 0, infeasible = 0, sweeps = 0, logged = 0, recomputes = 0;
 
@@ -377,8 +407,10 @@ __global__ void __launch_bounds__(256, 2)
             const Inst* Iq = opaque(I);
             if (best_score <= Iq->gb_dscore) {
                 const int64_t off = (int64_t)r * n * n;
-                rp.store((GLOBAL int64_t*)Iq->bA + off, (GLOBAL double*)Iq->bQ + off);
-                rp.copy_cols((GLOBAL int64_t*)Iq->bXt + off, rp.gX);
+                if (kk == 0) {  // lookahead groups hold the same state
+                    rp.store((GLOBAL int64_t*)Iq->bA + off, (GLOBAL double*)Iq->bQ + off);
+                    rp.copy_cols((GLOBAL int64_t*)Iq->bXt + off, rp.gX);
+                }
                 flags &= ~RF_BEST_STALE;
             } else {
                 flags |= RF_BEST_STALE;
@@ -390,7 +422,7 @@ __global__ void __launch_bounds__(256, 2)
         if (logging) {
             const Inst* Iq = opaque(I);
             const int64_t at = opaque_v(S)->log_count + logged;
-            if (at < Iq->log_cap && g == 0) {
+            if (at < Iq->log_cap && g == 0 && kk == 0) {
                 GLOBAL LogEnt* e = (GLOBAL LogEnt*)Iq->log + (int64_t)r * Iq->log_cap + at;
                 e->move = (int64_t)m;
                 e->i = (int16_t)i;
@@ -434,7 +466,10 @@ __global__ void __launch_bounds__(256, 2)
             int64_t pi[C], pj[C];
             double qi[C], qj[C];
             rp.load_rows(pv, jj, pi, pj, qi, qj);
-            const int64_t s = dual_best_shear<G, C>(pi, pj, qi, qj, g, n, pv, jj, cw);
+            int64_t s = dual_best_shear<G, C>(pi, pj, qi, qj, g, n, pv, jj, cw);
+            // lookahead groups all run the same sweep: their decisions are
+            // wave-uniform
+            if constexpr (K > 1) s = uni<64>(s);
             ++greedy_n;
             if (s == 0) continue;
             DualEval<C> e;
@@ -442,11 +477,19 @@ __global__ void __launch_bounds__(256, 2)
                                 rp.sP[rp.at(jj, pv)], rp.sP[rp.at(jj, jj)],
                                 rp.sQ[rp.at(pv, pv)], rp.sQ[rp.at(pv, jj)],
                                 rp.sQ[rp.at(jj, jj)], cw, e);
+            if constexpr (K > 1) {
+                e.bad = uni<64>((int32_t)e.bad) != 0;
+                e.d = uni_f64(e.d);
+                e.dnz = uni<64>(e.dnz);
+            }
             if (e.bad || e.d > 0.0) continue;
             int64_t xi[C], xj[C], nx[C];
             rp.prefetch_xt(pv, jj, xi, xj);
-            if (!rp.xt_row(s, xi, xj, nx)) continue;
+            bool fits = rp.xt_row(s, xi, xj, nx);
+            if constexpr (K > 1) fits = uni<64>((int32_t)fits) != 0;
+            if (!fits) continue;
             leave_best();
+            if constexpr (W > 1) __syncthreads();  // every wave has read what changes
             write(pv, jj, pj, e, nx);
             score += e.d;
             offnz += e.dnz;
@@ -464,72 +507,12 @@ __global__ void __launch_bounds__(256, 2)
         }
     };
 
-    uint4 w = make_uint4(0u, 0u, 0u, 0u);
-    bool have_w = false;
-    bool halt = false;
-    for (uint32_t it = 0; moves < stop_at; ++it) {
-        if ((it & 255u) == 0u) {
-            const unsigned long long key =
-                __hip_atomic_load(&I->solve_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (key != kNoKey && moves >= (key >> kRepBits)) break;
-            if (*stop) {
-                halt = true;
-                break;
-            }
-        }
-        const uint64_t m = ++moves;
-        // Philox slots of move m as in anneal.hip: counter (q, r, 16 t + c)
-        constexpr int SUB = G / 16;
-        const uint32_t t4 = (uint32_t)((m - 1) & 3u);
-        const int sub = (int)(t4 & (uint32_t)(SUB - 1));
-        if (sub == 0 || !have_w) {
-            const uint64_t q = (m - 1) >> 2;
-            uint32_t k0 = key0, k1 = key1;
-            opaque_u32(k0);
-            opaque_u32(k1);
-            w = philox10((uint32_t)q, (uint32_t)(q >> 32), (uint32_t)r,
-                         16u * (t4 & ~(uint32_t)(SUB - 1)) + (uint32_t)g, k0, k1);
-            have_w = true;
-        }
-        const int base = 16 * sub;
-        const int cs = g - base;
-        uint32_t w0z, w0w, w1z, w2w, w3z;
-        uint32_t l1w = 0, l2z = 0, l3w = 0, l4z = 0, l4w = 0;
-        if constexpr (G == 64) {
-            w0z = readlane32(w.z, base + 0);
-            w0w = readlane32(w.w, base + 0);
-            w1z = readlane32(w.z, base + 1);
-            w2w = readlane32(w.w, base + 2);
-            w3z = readlane32(w.z, base + 3);
-        } else {
-            if (cs >= 0 && cs < 5) swords[cs] = w;
-            __builtin_amdgcn_wave_barrier();
-            const uint4 b0 = swords[0], b1 = swords[1], b2 = swords[2], b3 = swords[3];
-            const uint4 b4 = swords[4];
-            __builtin_amdgcn_wave_barrier();
-            w0z = b0.z;
-            w0w = b0.w;
-            w1z = b1.z;
-            l1w = b1.w;
-            l2z = b2.z;
-            w2w = b2.w;
-            w3z = b3.z;
-            l3w = b3.w;
-            l4z = b4.z;
-            l4w = b4.w;
-        }
-        auto word = [&](int slot_, bool zw, uint32_t staged) -> uint32_t {
-            if constexpr (G == 64) return readlane32(zw ? w.w : w.z, base + slot_);
-            return staged;
-        };
-        auto w1w = [&]() { return word(1, true, l1w); };
-        auto w2z = [&]() { return word(2, false, l2z); };
-        auto w3w = [&]() { return word(3, true, l3w); };
-        auto w4z = [&]() { return word(4, false, l4z); };
-        auto w4w = [&]() { return word(4, true, l4w); };
-
-        // ---- pair (congruence_anneal.hpp:283-288, :211-228) ----
-        int i, j;
+    // ---- the lookahead path's steps (the one-group loop below spells the
+    // same steps out inline) ----
+    // pair (congruence_anneal.hpp:283-288, :211-228)
+    auto choose_pair = [&](const uint4& w, int base, int cs, int sub, uint32_t w0z,
+                           uint32_t w2w, uint32_t w3z, uint32_t w3w, int& i,
+                           int& j) __attribute__((always_inline)) {
         if (target_coin(w0z)) {
             const bool in = cs >= 0 && cs < TS;
             const uint32_t p = in ? mulhi32(w.x, (uint32_t)N) : 0u;
@@ -547,7 +530,7 @@ __global__ void __launch_bounds__(256, 2)
                 Gp::slot_max(valid ? (((uint64_t)v << 5) | 16u | (uint32_t)(15 - cs)) : 0ull, sub);
             int ibest;
             if (vmax == 0) {
-                ibest = rowat((tpos + 1u + mulhi32(w3w(), (uint32_t)(N - 1))) % (uint32_t)N);
+                ibest = rowat((tpos + 1u + mulhi32(w3w, (uint32_t)(N - 1))) % (uint32_t)N);
             } else {
                 ibest = Gp::bcast(cand, base + 15 - (int)(vmax & 15));
             }
@@ -559,60 +542,42 @@ __global__ void __launch_bounds__(256, 2)
             i = rowat(ip);
             j = rowat(jp);
         }
-
-        int64_t pi[C], pj[C], xi[C], xj[C];
-        double qi[C], qj[C];
-        rp.load_rows(i, j, pi, pj, qi, qj);
-        rp.prefetch_xt(i, j, xi, xj);
-
-        // ---- shear (congruence_anneal.hpp:290-296) ----
-        const int64_t pm = (w1z & 1u) ? 1 : -1;
-        int64_t s = pm;
-        if (greedy_coin(w0w)) {
-            s = dual_best_shear<G, C>(pi, pj, qi, qj, g, n, i, j, cw);
-            ++greedy_n;
-            if (s == 0) s = pm;
+    };
+    // the commit of an accepted move (i, j, s, e) whose transform row fits:
+    // the reference's recompute before a new global best decides a new best
+    // on the recomputed score, the state it leaves flushed first if it is the
+    // replica's best; `before_write` runs between the flush and the writes
+    auto commit = [&](uint64_t m, int i, int j, int64_t s, const int64_t (&pj)[C],
+                      const DualEval<C>& e, const int64_t (&nx)[C],
+                      auto before_write) __attribute__((always_inline)) {
+        const double ns = score + e.d;
+        const bool rec = e.d < 0.0 && ns < best_score && ns < gb;
+        bool newbest = ns < best_score;
+        if (rec || !newbest) leave_best();
+        before_write();
+        write(i, j, pj, e, nx);
+        score = ns;
+        offnz += e.dnz;
+        if (rec) {
+            score = rp.score(cw);
+            if constexpr (K > 1) score = uni_f64(score);
+            ++recomputes;
+            newbest = score < best_score;
         }
-
-        // ---- attempt_add (congruence_anneal.hpp:232-240) ----
-        DualEval<C> e;
-        dual_evaluate<G, C>(pi, pj, qi, qj, g, n, i, j, s, rp.sP[rp.at(i, i)],
-                            rp.sP[rp.at(j, i)], rp.sP[rp.at(j, j)], rp.sQ[rp.at(i, i)],
-                            rp.sQ[rp.at(i, j)], rp.sQ[rp.at(j, j)], cw, e);
-        int res = -1;
-        if (e.bad) {
-            ++infeasible;
-        } else if (e.d <= 0.0 || u53(w1w(), w2z()) < xexp(e.d * nbeta)) {
-            int64_t nx[C];
-            if (rp.xt_row(s, xi, xj, nx)) {
-                // the reference recomputes the running score before it
-                // publishes a new global best (congruence_anneal.hpp:310-315);
-                // whether the move is a new best of the replica is then
-                // decided on the recomputed score, so the state it leaves is
-                // flushed first if it is the replica's best
-                const double ns = score + e.d;
-                const bool rec = e.d < 0.0 && ns < best_score && ns < gb;
-                bool newbest = ns < best_score;
-                if (rec || !newbest) leave_best();
-                write(i, j, pj, e, nx);
-                score = ns;
-                offnz += e.dnz;
-                if (rec) {
-                    score = rp.score(cw);
-                    ++recomputes;
-                    newbest = score < best_score;
-                }
-                if (newbest) {
-                    best_score = score;
-                    best_move = (int64_t)m;
-                    flags |= RF_CUR_BEST;
-                    flags &= ~RF_BEST_STALE;
-                }
-                ++accepts;
-                log_move(m, i, j, s, 0);
-                res = e.d < 0.0 ? 1 : 0;
-            }
+        if (newbest) {
+            best_score = score;
+            best_move = (int64_t)m;
+            flags |= RF_CUR_BEST;
+            flags &= ~RF_BEST_STALE;
         }
+        ++accepts;
+        log_move(m, i, j, s, 0);
+    };
+    // bookkeeping after move m and the stall handling that follows it
+    // (congruence_anneal.hpp:363-412); true when the replica is solved,
+    // `changed` set when a sweep commit or a reseed changed the state
+    auto finish = [&](uint64_t m, int res, uint32_t w4z, uint32_t w4w,
+                      bool& changed) __attribute__((always_inline)) -> bool {
         ++win_p;
         if (res >= 0) ++win_a;
         if (score < local_best) {
@@ -623,15 +588,14 @@ __global__ void __launch_bounds__(256, 2)
         }
         if (offnz == 0) {
             flags |= RF_SOLVED | RF_STOPPED;
-            break;
+            return true;
         }
-
-        // ---- stall handling (congruence_anneal.hpp:363-412) ----
+        const uint32_t accepts0 = accepts;
         if (tempering) {
             if ((int64_t)msi > stuck) {
                 const InstParams& Qp = opaque(I)->P;
-                if (Qp.sweep_fraction > 0.0 && u01(w4z()) < Qp.sweep_fraction) {
-                    sweep(sact[mulhi32(w4w(), (uint32_t)N)], m);
+                if (Qp.sweep_fraction > 0.0 && u01(w4z) < Qp.sweep_fraction) {
+                    sweep(sact[mulhi32(w4w, (uint32_t)N)], m);
                     track_best(m);
                 }
                 local_best = score;
@@ -640,8 +604,8 @@ __global__ void __launch_bounds__(256, 2)
         } else {
             if ((int64_t)msi > stuck) {
                 const Inst* Iq = opaque(I);
-                if (Iq->P.sweep_fraction > 0.0 && u01(w4z()) < Iq->P.sweep_fraction) {
-                    if (sweep(sact[mulhi32(w4w(), (uint32_t)N)], m)) {
+                if (Iq->P.sweep_fraction > 0.0 && u01(w4z) < Iq->P.sweep_fraction) {
+                    if (sweep(sact[mulhi32(w4w, (uint32_t)N)], m)) {
                         local_best = score;
                         msi = 0;
                     }
@@ -653,12 +617,14 @@ __global__ void __launch_bounds__(256, 2)
                 if (score > gb * Iv->P.reseed_factor) {
                     leave_best();
                     log_move(m, -1, -1, 0, 2);
+                    if constexpr (W > 1) __syncthreads();
                     rp.load((const GLOBAL int64_t*)Iv->gA, (const GLOBAL double*)Iv->gQ);
                     rp.copy_cols(rp.gX, (const GLOBAL int64_t*)Iv->gXt);
                     rp.compute_row_mass();
                     score = gb;
                     offnz = Iv->gb_offnz;
                     track_best(m);
+                    changed = true;
                 }
                 local_best = score;
                 msi = 0;
@@ -681,13 +647,468 @@ __global__ void __launch_bounds__(256, 2)
                 nbeta = -1.0 / T;
             }
         }
+        changed |= accepts != accepts0;
         if (offnz == 0) {
             flags |= RF_SOLVED | RF_STOPPED;
-            break;
+            return true;
+        }
+        return false;
+    };
+    bool halt = false;
+    constexpr int SB = G / 16;  // moves per Philox block of a group
+    // this group's proposal of move mg: pair, rows, shear, evaluation, the
+    // Metropolis draw and the stall words
+    struct Prop {
+        int i, j;
+        int64_t s;
+        int64_t pi[C], pj[C], xi[C], xj[C];
+        DualEval<C> e;
+        double u;
+        bool gr;
+        uint32_t w4z, w4w;
+    };
+    auto propose = [&](uint64_t mg, bool fetch_xt, Prop& P) __attribute__((always_inline)) {
+        const uint32_t t4 = (uint32_t)((mg - 1) & 3u);
+        const int sub = (int)(t4 & (uint32_t)(SB - 1));
+        const uint64_t q = (mg - 1) >> 2;
+        // the key as scalars (its load may have landed in vector registers)
+        uint32_t k0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)key0);
+        uint32_t k1 = (uint32_t)__builtin_amdgcn_readfirstlane((int)key1);
+        opaque_u32(k0);
+        opaque_u32(k1);
+        const uint4 w = philox10((uint32_t)q, (uint32_t)(q >> 32), (uint32_t)r,
+                                 16u * (t4 & ~(uint32_t)(SB - 1)) + (uint32_t)g, k0, k1);
+        const int base = 16 * sub;
+        const int cs = g - base;
+        if (cs >= 0 && cs < 5) swords[cs] = w;
+        __builtin_amdgcn_wave_barrier();
+        const uint4 b0 = swords[0], b1 = swords[1], b2 = swords[2], b3 = swords[3];
+        const uint4 b4 = swords[4];
+        __builtin_amdgcn_wave_barrier();
+        choose_pair(w, base, cs, sub, b0.z, b2.w, b3.z, b3.w, P.i, P.j);
+        double qi[C], qj[C];
+        rp.load_rows(P.i, P.j, P.pi, P.pj, qi, qj);
+        if (fetch_xt) rp.prefetch_xt(P.i, P.j, P.xi, P.xj);
+        const int64_t pm = (b1.z & 1u) ? 1 : -1;
+        P.s = pm;
+        P.gr = greedy_coin(b0.w);
+        if (P.gr) {
+            P.s = dual_best_shear<G, C>(P.pi, P.pj, qi, qj, g, n, P.i, P.j, cw);
+            if (P.s == 0) P.s = pm;
+        }
+        dual_evaluate<G, C>(P.pi, P.pj, qi, qj, g, n, P.i, P.j, P.s, rp.sP[rp.at(P.i, P.i)],
+                            rp.sP[rp.at(P.j, P.i)], rp.sP[rp.at(P.j, P.j)],
+                            rp.sQ[rp.at(P.i, P.i)], rp.sQ[rp.at(P.i, P.j)],
+                            rp.sQ[rp.at(P.j, P.j)], cw, P.e);
+        P.u = u53(b1.w, b2.z);
+        P.w4z = b4.z;
+        P.w4w = b4.w;
+    };
+
+    if constexpr (K == 1) {
+        uint4 w = make_uint4(0u, 0u, 0u, 0u);
+        bool have_w = false;
+        for (uint32_t it = 0; moves < stop_at; ++it) {
+            if ((it & 255u) == 0u) {
+                const unsigned long long key =
+                    __hip_atomic_load(&I->solve_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (key != kNoKey && moves >= (key >> kRepBits)) break;
+                if (*stop) {
+                    halt = true;
+                    break;
+                }
+            }
+            const uint64_t m = ++moves;
+            // Philox slots of move m as in anneal.hip: counter (q, r, 16 t + c)
+            constexpr int SUB = G / 16;
+            const uint32_t t4 = (uint32_t)((m - 1) & 3u);
+            const int sub = (int)(t4 & (uint32_t)(SUB - 1));
+            if (sub == 0 || !have_w) {
+                const uint64_t q = (m - 1) >> 2;
+                uint32_t k0 = key0, k1 = key1;
+                opaque_u32(k0);
+                opaque_u32(k1);
+                w = philox10((uint32_t)q, (uint32_t)(q >> 32), (uint32_t)r,
+                             16u * (t4 & ~(uint32_t)(SUB - 1)) + (uint32_t)g, k0, k1);
+                have_w = true;
+            }
+            const int base = 16 * sub;
+            const int cs = g - base;
+            uint32_t w0z, w0w, w1z, w2w, w3z;
+            uint32_t l1w = 0, l2z = 0, l3w = 0, l4z = 0, l4w = 0;
+            if constexpr (G == 64) {
+                w0z = readlane32(w.z, base + 0);
+                w0w = readlane32(w.w, base + 0);
+                w1z = readlane32(w.z, base + 1);
+                w2w = readlane32(w.w, base + 2);
+                w3z = readlane32(w.z, base + 3);
+            } else {
+                if (cs >= 0 && cs < 5) swords[cs] = w;
+                __builtin_amdgcn_wave_barrier();
+                const uint4 b0 = swords[0], b1 = swords[1], b2 = swords[2], b3 = swords[3];
+                const uint4 b4 = swords[4];
+                __builtin_amdgcn_wave_barrier();
+                w0z = b0.z;
+                w0w = b0.w;
+                w1z = b1.z;
+                l1w = b1.w;
+                l2z = b2.z;
+                w2w = b2.w;
+                w3z = b3.z;
+                l3w = b3.w;
+                l4z = b4.z;
+                l4w = b4.w;
+            }
+            auto word = [&](int slot_, bool zw, uint32_t staged) -> uint32_t {
+                if constexpr (G == 64) return readlane32(zw ? w.w : w.z, base + slot_);
+                return staged;
+            };
+            auto w1w = [&]() { return word(1, true, l1w); };
+            auto w2z = [&]() { return word(2, false, l2z); };
+            auto w3w = [&]() { return word(3, true, l3w); };
+            auto w4z = [&]() { return word(4, false, l4z); };
+            auto w4w = [&]() { return word(4, true, l4w); };
+
+            // ---- pair (congruence_anneal.hpp:283-288, :211-228) ----
+            int i, j;
+            if (target_coin(w0z)) {
+                const bool in = cs >= 0 && cs < TS;
+                const uint32_t p = in ? mulhi32(w.x, (uint32_t)N) : 0u;
+                const int row = rowat(p);
+                const int64_t wt = rp.mass_of(row);
+                const uint64_t mx =
+                    Gp::slot_max(in ? (((uint64_t)wt << 5) | 16u | (uint32_t)(15 - cs)) : 0ull, sub);
+                const uint32_t tpos = Gp::bcast(p, base + 15 - (int)(mx & 15));
+                const int target = rowat(tpos);
+                const uint32_t pp = in ? mulhi32(w.y, (uint32_t)N) : 0u;
+                const int cand = rowat(pp);
+                const bool valid = in && cand != target;
+                const int64_t v = valid ? iabs64(rp.sP[rp.at(target, cand)]) : 0;
+                const uint64_t vmax =
+                    Gp::slot_max(valid ? (((uint64_t)v << 5) | 16u | (uint32_t)(15 - cs)) : 0ull, sub);
+                int ibest;
+                if (vmax == 0) {
+                    ibest = rowat((tpos + 1u + mulhi32(w3w(), (uint32_t)(N - 1))) % (uint32_t)N);
+                } else {
+                    ibest = Gp::bcast(cand, base + 15 - (int)(vmax & 15));
+                }
+                i = ibest;
+                j = target;
+            } else {
+                const uint32_t ip = mulhi32(w2w, (uint32_t)N);
+                const uint32_t jp = (ip + 1u + mulhi32(w3z, (uint32_t)(N - 1))) % (uint32_t)N;
+                i = rowat(ip);
+                j = rowat(jp);
+            }
+
+            int64_t pi[C], pj[C], xi[C], xj[C];
+            double qi[C], qj[C];
+            rp.load_rows(i, j, pi, pj, qi, qj);
+            rp.prefetch_xt(i, j, xi, xj);
+
+            // ---- shear (congruence_anneal.hpp:290-296) ----
+            const int64_t pm = (w1z & 1u) ? 1 : -1;
+            int64_t s = pm;
+            if (greedy_coin(w0w)) {
+                s = dual_best_shear<G, C>(pi, pj, qi, qj, g, n, i, j, cw);
+                ++greedy_n;
+                if (s == 0) s = pm;
+            }
+
+            // ---- attempt_add (congruence_anneal.hpp:232-240) ----
+            DualEval<C> e;
+            dual_evaluate<G, C>(pi, pj, qi, qj, g, n, i, j, s, rp.sP[rp.at(i, i)],
+                                rp.sP[rp.at(j, i)], rp.sP[rp.at(j, j)], rp.sQ[rp.at(i, i)],
+                                rp.sQ[rp.at(i, j)], rp.sQ[rp.at(j, j)], cw, e);
+            int res = -1;
+            if (e.bad) {
+                ++infeasible;
+            } else if (e.d <= 0.0 || u53(w1w(), w2z()) < xexp(e.d * nbeta)) {
+                int64_t nx[C];
+                if (rp.xt_row(s, xi, xj, nx)) {
+                    // the reference recomputes the running score before it
+                    // publishes a new global best (congruence_anneal.hpp:310-315);
+                    // whether the move is a new best of the replica is then
+                    // decided on the recomputed score, so the state it leaves is
+                    // flushed first if it is the replica's best
+                    const double ns = score + e.d;
+                    const bool rec = e.d < 0.0 && ns < best_score && ns < gb;
+                    bool newbest = ns < best_score;
+                    if (rec || !newbest) leave_best();
+                    write(i, j, pj, e, nx);
+                    score = ns;
+                    offnz += e.dnz;
+                    if (rec) {
+                        score = rp.score(cw);
+                        ++recomputes;
+                        newbest = score < best_score;
+                    }
+                    if (newbest) {
+                        best_score = score;
+                        best_move = (int64_t)m;
+                        flags |= RF_CUR_BEST;
+                        flags &= ~RF_BEST_STALE;
+                    }
+                    ++accepts;
+                    log_move(m, i, j, s, 0);
+                    res = e.d < 0.0 ? 1 : 0;
+                }
+            }
+            ++win_p;
+            if (res >= 0) ++win_a;
+            if (score < local_best) {
+                local_best = score;
+                msi = 0;
+            } else {
+                ++msi;
+            }
+            if (offnz == 0) {
+                flags |= RF_SOLVED | RF_STOPPED;
+                break;
+            }
+
+            // ---- stall handling (congruence_anneal.hpp:363-412) ----
+            if (tempering) {
+                if ((int64_t)msi > stuck) {
+                    const InstParams& Qp = opaque(I)->P;
+                    if (Qp.sweep_fraction > 0.0 && u01(w4z()) < Qp.sweep_fraction) {
+                        sweep(sact[mulhi32(w4w(), (uint32_t)N)], m);
+                        track_best(m);
+                    }
+                    local_best = score;
+                    msi = 0;
+                }
+            } else {
+                if ((int64_t)msi > stuck) {
+                    const Inst* Iq = opaque(I);
+                    if (Iq->P.sweep_fraction > 0.0 && u01(w4z()) < Iq->P.sweep_fraction) {
+                        if (sweep(sact[mulhi32(w4w(), (uint32_t)N)], m)) {
+                            local_best = score;
+                            msi = 0;
+                        }
+                        track_best(m);
+                    }
+                    const Inst* Iv = opaque(I);
+                    T = Iv->P.reheat * Iv->P.t_init;
+                    nbeta = -1.0 / T;
+                    if (score > gb * Iv->P.reseed_factor) {
+                        leave_best();
+                        log_move(m, -1, -1, 0, 2);
+                        if constexpr (W > 1) __syncthreads();
+                        rp.load((const GLOBAL int64_t*)Iv->gA, (const GLOBAL double*)Iv->gQ);
+                        rp.copy_cols(rp.gX, (const GLOBAL int64_t*)Iv->gXt);
+                        rp.compute_row_mass();
+                        score = gb;
+                        offnz = Iv->gb_offnz;
+                        track_best(m);
+                    }
+                    local_best = score;
+                    msi = 0;
+                    cool_counter = 0;
+                } else if (++cool_counter >= per_cool) {
+                    const InstParams& Qp = opaque(I)->P;
+                    cool_counter = 0;
+                    if (Qp.adaptive && win_p > 0) {
+                        const double acc = (double)win_a / (double)win_p;
+                        if (acc > Qp.target_accept)
+                            cooling *= 0.999;
+                        else
+                            cooling /= 0.999;
+                        cooling = cooling < 0.90 ? 0.90 : (cooling > 0.99999 ? 0.99999 : cooling);
+                    }
+                    win_p = 0;
+                    win_a = 0;
+                    T *= cooling;
+                    if (T < Qp.t_min) T = Qp.t_min;
+                    nbeta = -1.0 / T;
+                }
+            }
+            if (offnz == 0) {
+                flags |= RF_SOLVED | RF_STOPPED;
+                break;
+            }
+        }
+    } else if constexpr (W == 1) {
+        // lookahead in one wave: group kk proposes move m0 + kk
+        for (uint32_t it = 0; moves < stop_at; ++it) {
+            if ((it & 255u) == 0u) {
+                const unsigned long long key =
+                    __hip_atomic_load(&I->solve_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (key != kNoKey && moves >= (key >> kRepBits)) break;
+                if (*stop) {
+                    halt = true;
+                    break;
+                }
+            }
+            const uint64_t m0 = moves + 1;
+            Prop P;
+            propose(m0 + (uint64_t)kk, true, P);
+            // Metropolis at the step's temperature; a move after a cooling
+            // step inside this step is decided again at the new one
+            const double nb0 = nbeta;
+            const bool pass = !P.e.bad && (P.e.d <= 0.0 || P.u < xexp(P.e.d * nb0));
+            const int kn = (stop_at - moves) < (uint64_t)K ? (int)(stop_at - moves) : K;
+            const uint64_t live = kn == K ? ~0ull : ((1ull << (kn * G)) - 1ull);
+            if ((__ballot(pass) & live) == 0ull && score >= local_best &&
+                (int64_t)msi + kn <= (int64_t)stuck && (tempering || cool_counter + kn < per_cool)) {
+                // every resolvable move is rejected and none reaches a stall or
+                // a cooling step: the sequential bookkeeping, kn times
+                const uint64_t lead = __ballot(g == 0) & live;
+                greedy_n += (uint32_t)__popcll(__ballot(P.gr) & lead);
+                infeasible += (uint32_t)__popcll(__ballot(P.e.bad) & lead);
+                moves += (uint64_t)kn;
+                win_p += kn;
+                msi += (uint32_t)kn;
+                if (!tempering) cool_counter += kn;
+                continue;
+            }
+            int64_t nx[C];
+            bool feas = false, have_nx = false, solved = false;
+            for (int k = 0; k < kn; ++k) {
+                const int L = k * G;  // first lane of group k
+                const uint64_t m = m0 + (uint64_t)k;
+                moves = m;
+                if (readlane32((uint32_t)P.gr, L)) ++greedy_n;
+                int res = -1;
+                bool changed = false;
+                if (readlane32((uint32_t)P.e.bad, L)) {
+                    ++infeasible;
+                } else {
+                    const double d = readlane_f64(P.e.d, L);
+                    bool acc = readlane32((uint32_t)pass, L) != 0;
+                    if (nbeta != nb0) acc = d <= 0.0 || readlane_f64(P.u, L) < xexp(d * nbeta);
+                    if (acc && !have_nx) {
+                        feas = rp.xt_row(P.s, P.xi, P.xj, nx);  // every group's, once
+                        have_nx = true;
+                    }
+                    if (acc && readlane32((uint32_t)feas, L)) {
+                        // group k's move, written by every group
+                        const int src = L + g;
+                        int64_t cj[C], cx[C];
+                        DualEval<C> ce;
+#pragma unroll
+                        for (int c = 0; c < C; ++c) {
+                            cj[c] = bperm64(P.pj[c], src);
+                            ce.np[c] = bperm64(P.e.np[c], src);
+                            ce.nq[c] = __builtin_bit_cast(
+                                double, bperm64(__builtin_bit_cast(int64_t, P.e.nq[c]), src));
+                            cx[c] = bperm64(nx[c], src);
+                        }
+                        ce.d = d;
+                        ce.dnz = (int32_t)readlane32((uint32_t)P.e.dnz, L);
+                        ce.bad = false;
+                        commit(m, (int)readlane32((uint32_t)P.i, L), (int)readlane32((uint32_t)P.j, L),
+                               readlane64(P.s, L), cj, ce, cx, [] {});
+                        res = d < 0.0 ? 1 : 0;
+                        changed = true;
+                    }
+                }
+                if (finish(m, res, readlane32(P.w4z, L), readlane32(P.w4w, L), changed)) {
+                    solved = true;
+                    break;
+                }
+                if (changed) break;
+            }
+            if (solved) break;
+        }
+    } else {
+        // lookahead over W waves: group kk proposes move m0 + kk and publishes it
+        const int ln = (int)(threadIdx.x & 63u);
+        for (uint32_t it = 0; moves < stop_at; ++it) {
+            const uint64_t m0 = moves + 1;
+            Prop P;
+            propose(m0 + (uint64_t)kk, false, P);
+            const double nb0 = nbeta;
+            const bool pass = !P.e.bad && (P.e.d <= 0.0 || P.u < xexp(P.e.d * nb0));
+            DualProposal* const pb = props + (it & 1u) * K;
+            if (g == 0) {
+                DualProposal Q;
+                Q.d = P.e.d;
+                Q.s = P.s;
+                Q.u = P.u;
+                Q.i = P.i;
+                Q.j = P.j;
+                Q.w4z = P.w4z;
+                Q.w4w = P.w4w;
+                Q.flags = (P.e.bad ? 1u : 0u) | (pass ? 2u : 0u) | (P.gr ? 4u : 0u);
+                pb[kk] = Q;
+            }
+            // the solve key and the stop flag, read once for every wave
+            if ((it & 255u) == 0u && threadIdx.x == 0) {
+                const unsigned long long key =
+                    __hip_atomic_load(&I->solve_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                *spoll = (key != kNoKey && moves >= (key >> kRepBits)) ? 1u : (*stop ? 2u : 0u);
+            }
+            __syncthreads();
+            if ((it & 255u) == 0u) {
+                const uint32_t pr = (uint32_t)uni<64>((int32_t)*spoll);
+                if (pr == 2u) halt = true;
+                if (pr != 0u) break;
+            }
+            const int kn = (stop_at - moves) < (uint64_t)K ? (int)(stop_at - moves) : K;
+            const uint32_t fl = ln < kn ? pb[ln].flags : 0u;  // proposal ln's flags in lane ln
+            if (__ballot((fl & 2u) != 0u) == 0ull && score >= local_best &&
+                (int64_t)msi + kn <= (int64_t)stuck && (tempering || cool_counter + kn < per_cool)) {
+                greedy_n += (uint32_t)__popcll(__ballot((fl & 4u) != 0u));
+                infeasible += (uint32_t)__popcll(__ballot((fl & 1u) != 0u));
+                moves += (uint64_t)kn;
+                win_p += kn;
+                msi += (uint32_t)kn;
+                if (!tempering) cool_counter += kn;
+                continue;
+            }
+            bool solved = false;
+            for (int k = 0; k < kn; ++k) {
+                const DualProposal* const Q = pb + k;
+                const uint32_t f = (uint32_t)uni<64>((int32_t)Q->flags);
+                const uint64_t m = m0 + (uint64_t)k;
+                moves = m;
+                if (f & 4u) ++greedy_n;
+                int res = -1;
+                bool changed = false;
+                if (f & 1u) {
+                    ++infeasible;
+                } else {
+                    const double d = uni_f64(Q->d);
+                    bool acc = (f & 2u) != 0u;
+                    if (nbeta != nb0) acc = d <= 0.0 || uni_f64(Q->u) < xexp(d * nbeta);
+                    if (acc) {
+                        // every group evaluates move k again from the shared
+                        // state: its rows, its X^T columns
+                        const int ik = uni<64>(Q->i), jk = uni<64>(Q->j);
+                        const int64_t sk = uni<64>(Q->s);
+                        int64_t ci[C], cj[C], xi[C], xj[C], nx[C];
+                        double qi[C], qj[C];
+                        rp.load_rows(ik, jk, ci, cj, qi, qj);
+                        rp.prefetch_xt(ik, jk, xi, xj);
+                        DualEval<C> ce;
+                        dual_evaluate<G, C>(ci, cj, qi, qj, g, n, ik, jk, sk, rp.sP[rp.at(ik, ik)],
+                                            rp.sP[rp.at(jk, ik)], rp.sP[rp.at(jk, jk)],
+                                            rp.sQ[rp.at(ik, ik)], rp.sQ[rp.at(ik, jk)],
+                                            rp.sQ[rp.at(jk, jk)], cw, ce);
+                        ce.d = d;
+                        ce.dnz = uni<64>(ce.dnz);
+                        if (uni<64>((int32_t)rp.xt_row(sk, xi, xj, nx)) != 0) {
+                            // every wave has read what changes before any writes
+                            commit(m, ik, jk, sk, cj, ce, nx, [] { __syncthreads(); });
+                            res = d < 0.0 ? 1 : 0;
+                            changed = true;
+                        }
+                    }
+                }
+                if (finish(m, res, (uint32_t)uni<64>((int32_t)Q->w4z),
+                           (uint32_t)uni<64>((int32_t)Q->w4w), changed)) {
+                    solved = true;
+                    break;
+                }
+                if (changed) break;
+            }
+            if (solved) break;
         }
     }
     if (!(flags & RF_STOPPED) && !halt && cap != 0 && moves >= cap) flags |= RF_STOPPED | RF_CAPPED;
 
+    if (kk != 0) return;  // lookahead: group 0 writes the replica back
     rp.store(gPcur, gQcur);
     if (g == 0) {
         S->moves = moves;
@@ -971,22 +1392,33 @@ __global__ void __launch_bounds__(64)
 // host entry points
 // ---------------------------------------------------------------------------
 
+using DualKernel = void (*)(const RepRef*, int, Inst*, int, int, const volatile int*);
 struct DualVariant {
-    int G, C;
-    const void* fn;
+    int G, C, K;
+    DualKernel fn;
 };
 
+// The group width is fixed by the order (the column sums' bits depend on it);
+// lookahead adds K groups per replica: K G = 64 in one wave, above in a
+// workgroup of K G / 64 waves.
 static const DualVariant kDualVariants[] = {
-    {16, 1, (const void*)&k_anneal_dual<16, 1>},
-    {32, 1, (const void*)&k_anneal_dual<32, 1>},
-    {64, 1, (const void*)&k_anneal_dual<64, 1>},
-    {64, 2, (const void*)&k_anneal_dual<64, 2>},
+    {16, 1, 1, k_anneal_dual<16, 1, 1>},   {32, 1, 1, k_anneal_dual<32, 1, 1>},
+    {64, 1, 1, k_anneal_dual<64, 1, 1>},   {64, 2, 1, k_anneal_dual<64, 2, 1>},
+    {16, 1, 4, k_anneal_dual<16, 1, 4>},   {32, 1, 2, k_anneal_dual<32, 1, 2>},
+    {16, 1, 16, k_anneal_dual<16, 1, 16>}, {32, 1, 8, k_anneal_dual<32, 1, 8>},
+    {64, 1, 4, k_anneal_dual<64, 1, 4>},
 };
+
+bool dual_variant_exists(int G, int C, int K) {
+    for (const auto& v : kDualVariants)
+        if (v.G == G && v.C == C && v.K == K) return true;
+    return false;
+}
 
 hipError_t init_dual_kernel_attrs(int max_lds_per_block) {
     for (const auto& v : kDualVariants) {
         const hipError_t e = hipFuncSetAttribute(
-            v.fn, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds_per_block);
+            (const void*)v.fn, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds_per_block);
         if (e != hipSuccess) return e;
     }
     const void* probes[] = {(const void*)&k_dual_probe<16, 1>, (const void*)&k_dual_probe<32, 1>,
@@ -999,23 +1431,20 @@ hipError_t init_dual_kernel_attrs(int max_lds_per_block) {
     return hipSuccess;
 }
 
-hipError_t launch_anneal_dual(hipStream_t st, int G, int C, int groups_per_block,
+// units_per_block: table entries (replicas, or padding) per block, G K lanes each
+hipError_t launch_anneal_dual(hipStream_t st, int G, int C, int K, int units_per_block,
                               const RepRef* table, int count, Inst* insts, int nmax,
                               int rep_bytes, const volatile int* stop) {
     if (count <= 0) return hipSuccess;
-    const int threads = groups_per_block * G;
-    const int blocks = (count + groups_per_block - 1) / groups_per_block;
-    const size_t lds = (size_t)groups_per_block * rep_bytes;
-    if (G == 16 && C == 1)
-        k_anneal_dual<16, 1><<<blocks, threads, lds, st>>>(table, count, insts, nmax, rep_bytes, stop);
-    else if (G == 32 && C == 1)
-        k_anneal_dual<32, 1><<<blocks, threads, lds, st>>>(table, count, insts, nmax, rep_bytes, stop);
-    else if (G == 64 && C == 1)
-        k_anneal_dual<64, 1><<<blocks, threads, lds, st>>>(table, count, insts, nmax, rep_bytes, stop);
-    else if (G == 64 && C == 2)
-        k_anneal_dual<64, 2><<<blocks, threads, lds, st>>>(table, count, insts, nmax, rep_bytes, stop);
-    else
-        return hipErrorInvalidValue;
+    const DualVariant* v = nullptr;
+    for (const auto& x : kDualVariants)
+        if (x.G == G && x.C == C && x.K == K) v = &x;
+    if (!v || units_per_block < 1 || units_per_block * G * K > 256) return hipErrorInvalidValue;
+    const int threads = units_per_block * G * K;
+    const int blocks = (count + units_per_block - 1) / units_per_block;
+    const size_t lds = (size_t)units_per_block * rep_bytes;
+    hipLaunchKernelGGL(v->fn, dim3(blocks), dim3(threads), lds, st, table, count, insts, nmax,
+                       rep_bytes, stop);
     return hipGetLastError();
 }
 

@@ -113,20 +113,24 @@ __device__ __forceinline__ T* opaque_v(T* p) {
     return p;
 }
 
-// A wave-uniform value in scalar registers (whole-wave groups only).
+// A wave-uniform value in scalar registers (groups of a whole wave or more).
 template <int G>
 __device__ __forceinline__ int32_t uni(int32_t v) {
-    if constexpr (G == 64) return __builtin_amdgcn_readfirstlane(v);
+    if constexpr (G >= 64) return __builtin_amdgcn_readfirstlane(v);
     return v;
 }
 template <int G>
 __device__ __forceinline__ int64_t uni(int64_t v) {
-    if constexpr (G == 64) {
+    if constexpr (G >= 64) {
         const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(uint64_t)v);
         const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
         return (int64_t)(((uint64_t)hi << 32) | lo);
     }
     return v;
+}
+// a double every lane of the wave holds alike, in scalar registers
+__device__ __forceinline__ double uni_f64(double v) {
+    return __builtin_bit_cast(double, uni<64>(__builtin_bit_cast(int64_t, v)));
 }
 
 // Coin u01(w) < f as an integer test: w 2^-32 and f 2^32 are exact doubles,
@@ -178,6 +182,16 @@ __device__ __forceinline__ int64_t readlane64(int64_t v, int lane) {
     const uint32_t lo = readlane32((uint32_t)(uint64_t)v, lane);
     const uint32_t hi = readlane32((uint32_t)((uint64_t)v >> 32), lane);
     return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// int64 value of lane `src` (runtime, per lane) of the wave, for every lane
+__device__ __forceinline__ int64_t bperm64(int64_t v, int src) {
+    const int lo = __builtin_amdgcn_ds_bpermute(src << 2, (int)(uint32_t)(uint64_t)v);
+    const int hi = __builtin_amdgcn_ds_bpermute(src << 2, (int)(uint32_t)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+    return __builtin_bit_cast(double, readlane64(__builtin_bit_cast(int64_t, v), lane));
 }
 
 struct OpAdd {

@@ -17,7 +17,7 @@
 #include "xtx.h"
 
 namespace xtx {
-hipError_t launch_anneal_dual(hipStream_t st, int G, int C, int groups_per_block,
+hipError_t launch_anneal_dual(hipStream_t st, int G, int C, int K, int units_per_block,
                               const RepRef* table, int count, Inst* insts, int nmax,
                               int rep_bytes, const volatile int* stop);
 hipError_t launch_dual_maint(hipStream_t st, Inst* insts, int count, int max_r);
@@ -34,20 +34,31 @@ using namespace xtx::host;
 namespace {
 
 // Lane group and LDS bytes of a replica: P and Q upper triangles of the
-// padded order, 5 staged Philox blocks, the active list. The group width is
-// fixed by n (G = 16, 32, 64 lanes; two columns per lane above 64), which the
-// fold-then-pair column sum needs to give the same bits for every n.
+// padded order, 5 staged Philox blocks per lane group, the active list, and
+// for a replica over several waves two buffers of K 64-byte proposals and a
+// poll word. The group width is fixed by n (G = 16, 32, 64 lanes; two columns
+// per lane above 64), which the fold-then-pair column sum needs to give the
+// same bits for every n; lookahead K (the most up to `want` that the order
+// has a kernel for: 16 or 4 with 16 lanes, 8 or 2 with 32, 4 with 64) adds
+// lane groups per replica.
 struct DualLayout {
-    int G = 0, C = 0, nmax = 0, rep_bytes = 0;
+    int G = 0, C = 0, K = 1, nmax = 0, rep_bytes = 0;
 };
 
-DualLayout dual_layout(int n) {
+DualLayout dual_layout(int n, int want) {
     DualLayout L;
     L.nmax = order_of(n);
     L.G = L.nmax <= 16 ? 16 : (L.nmax <= 32 ? 32 : 64);
     L.C = L.nmax <= 64 ? 1 : 2;
+    if (L.C == 1) {
+        if (L.G == 16) L.K = want >= 16 ? 16 : (want >= 4 ? 4 : 1);
+        if (L.G == 32) L.K = want >= 8 ? 8 : (want >= 2 ? 2 : 1);
+        if (L.G == 64) L.K = want >= 4 ? 4 : 1;
+    }
     const int64_t tri = (int64_t)L.nmax * (L.nmax + 1) / 2;
-    L.rep_bytes = (int)((tri * 16 + 5 * 16 + L.nmax + 15) / 16 * 16);
+    int64_t bytes = tri * 16 + L.K * 5 * 16 + L.nmax;
+    if (L.G * L.K > 64) bytes = (bytes + 15) / 16 * 16 + 2 * L.K * 64 + 16;
+    L.rep_bytes = (int)((bytes + 15) / 16 * 16);
     return L;
 }
 
@@ -206,7 +217,8 @@ int xtx_dual_solve(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t n
         h.t_init = p.t_init > 0.0 ? p.t_init : std::max(1.0, 2.0 * avg);
         h.P.t_init = h.t_init;
         if (h.P.tempering) h.temps = ladder(h.t_init, p.t_min, h.R);
-        const DualLayout L = dual_layout(n);
+        const DualLayout L = dual_layout(
+            n, p.lookahead != 0 ? p.lookahead : lookahead_rule(ctx->prop.multiProcessorCount, h.R));
         if (L.rep_bytes > g_lds_block) throw Err{XTX_E_ARG, "n too large for an LDS-resident replica"};
 
         xtx_dual_stats st{};
@@ -326,7 +338,8 @@ int xtx_dual_solve(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t n
         // replica table: whole waves of one instance
         std::vector<RepRef> refs;
         for (int r = 0; r < R; ++r) refs.push_back(RepRef{0, r});
-        while (refs.size() % (size_t)(64 / L.G)) refs.push_back(RepRef{0, -1});
+        const int per_wave = std::max(1, 64 / (L.G * L.K));  // replicas per wave
+        while (refs.size() % (size_t)per_wave) refs.push_back(RepRef{0, -1});
         RepRef* d_refs = nullptr;
         Inst* pin = nullptr;
         // released on every exit, including an error thrown below
@@ -341,10 +354,9 @@ int xtx_dual_solve(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t n
         HIPCHK(hipMalloc(&d_refs, refs.size() * sizeof(RepRef)));
         HIPCHK(hipMemcpyAsync(d_refs, refs.data(), refs.size() * sizeof(RepRef),
                               hipMemcpyHostToDevice, sm));
-        const int per_wave = 64 / L.G;
-        int waves = p.waves_per_block > 0 ? p.waves_per_block : 1;
+        int waves = p.waves_per_block > 0 && L.G * L.K <= 64 ? p.waves_per_block : 1;
         while (waves > 1 && (int64_t)waves * per_wave * L.rep_bytes > g_lds_block) --waves;
-        const int gpb = waves * per_wave;
+        const int gpb = waves * per_wave;  // replicas per block
         HIPCHK(hipHostMalloc((void**)&pin, sizeof(Inst), hipHostMallocDefault));
         HIPCHK(hipStreamSynchronize(sm));
 
@@ -415,8 +427,8 @@ int xtx_dual_solve(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t n
                 }
                 HIPCHK(launch_dual_maint(sm, ctx->d_plain, 1, R));
                 HIPCHK(hipEventRecord(ctx->ev0, sm));
-                HIPCHK(launch_anneal_dual(sm, L.G, L.C, gpb, d_refs, (int)refs.size(), ctx->d_plain,
-                                          L.nmax, L.rep_bytes, ctx->stop_dev));
+                HIPCHK(launch_anneal_dual(sm, L.G, L.C, L.K, gpb, d_refs, (int)refs.size(),
+                                          ctx->d_plain, L.nmax, L.rep_bytes, ctx->stop_dev));
                 HIPCHK(hipEventRecord(ctx->ev1, sm));
                 HIPCHK(launch_epoch_end(sm, ctx->d_plain, 1, R));
                 HIPCHK(hipMemcpyAsync(pin, ctx->d_plain, sizeof(Inst), hipMemcpyDeviceToHost, sm));
@@ -565,7 +577,7 @@ int xtx_dual_probe(xtx_ctx* ctx, const int64_t* P, const double* Q, int32_t n, i
                 x[1] == x[2] || x[3] < -kShearCap || x[3] > kShearCap)
                 throw Err{XTX_E_ARG, "bad probe query"};
         }
-        const DualLayout L = dual_layout(n);
+        const DualLayout L = dual_layout(n, 1);
         if ((size_t)(64 / L.G) * (size_t)L.nmax * (L.nmax + 1) / 2 * 16 > (size_t)g_lds_block)
             throw Err{XTX_E_ARG, "n too large for the probe"};
         HIPCHK(hipSetDevice(ctx->device));

@@ -419,16 +419,8 @@ void apply_layout(const xtx_ctx* ctx, HostInst& h, int requested, int rule) {
         throw Err{XTX_E_ARG, "n too large for an LDS-resident replica"};
 }
 
-// The lookahead rule for `replicas` resident together (a plain call's, a
-// batch's, or a serve admission's with those already resident): 16 while they
-// number at most 32 per compute unit, else 1. Measured: 256 replicas of order
-// 24 run 5.4x the moves/s of one group per replica; serving at 200 requests/s
-// (about 4000 replicas resident) halves the latency; an overloaded serve
-// (400/s, where a lookahead replica's larger LDS share admits fewer requests)
-// runs faster with one group per replica, and a full batch (32768) the same.
-constexpr int kLookahead = 16;
 int lookahead_rule(const xtx_ctx* ctx, int64_t replicas) {
-    return replicas <= 32 * (int64_t)ctx->prop.multiProcessorCount ? kLookahead : 1;
+    return xtx::host::lookahead_rule(ctx->prop.multiProcessorCount, replicas);
 }
 
 int units_per_block_for(const Layout& L, int waves) {
