@@ -284,8 +284,8 @@ inline int parse_args(int argc, char** argv, const std::vector<Opt>& opts, const
 
 // The engine options both drop-ins share (congruence_anneal.hpp:122-153 via
 // main.cpp:583-640 and xdual_main.cpp:465-500), plus the GPU-only --device,
-// --max-moves, --max-epochs and --lookahead (xtx_params.lookahead; the L1
-// engine only).
+// --max-moves, --max-epochs and --lookahead (xtx_params.lookahead: proposals
+// of a replica evaluated at once; speed only).
 inline std::vector<Opt> engine_options(xtx_params& p, int& device) {
     return {
         {{"-t", "--threads", "--replicas"}, false, set_int(p.replicas)},
