@@ -75,12 +75,13 @@ typedef struct {
                                  batch that has no configuration yet (on the
                                  batch itself, before the run) */
     int32_t lookahead;        /* proposals of one replica evaluated at once
-                                 (1, 2, 4, 8 or 16; orders up to 32 allow 16,
-                                 up to 64 allow 8, larger ones 1): the replica
-                                 takes a whole wave (16 or 8: a workgroup of
-                                 four waves) and resolves them in order, the
-                                 same trajectory at up to that many moves per
-                                 step. 0 = auto, by the replicas resident
+                                 (1, 2, 4, 8, 16 or 32; the L1 engine allows
+                                 up to 32 to order 32, 8 to 64, 4 above; the
+                                 xdual engine 16 to order 16, 8 to 32, 4 to
+                                 64): the replica takes a whole wave (2, 4) or
+                                 a workgroup of four or eight waves and
+                                 resolves them in order, the same trajectory
+                                 at up to that many moves per step. 0 = auto, by the replicas resident
                                  together (the call's, the batch's, or at a
                                  serve admission those resident with it): up
                                  to 32 per compute unit, 16 (or the order's

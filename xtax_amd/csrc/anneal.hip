@@ -1780,6 +1780,7 @@ static const AnnealVariant kVariants[] = {
     {16, 1, true, 16, k_anneal<16, 1, true, 16>}, {16, 2, true, 16, k_anneal<16, 2, true, 16>},
     {32, 2, true, 8, k_anneal<32, 2, true, 8>},
     {16, 1, true, 32, k_anneal<16, 1, true, 32>}, {16, 2, true, 32, k_anneal<16, 2, true, 32>},
+    {64, 2, true, 4, k_anneal<64, 2, true, 4>},
 };
 
 static const AnnealVariant* find_variant(int G, int C, bool tri, int K) {

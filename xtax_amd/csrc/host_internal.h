@@ -89,8 +89,9 @@ inline Layout layout_for(int n) {
 
 // The lookahead layout of order nmax: K groups of G lanes per replica (one
 // wave for K G = 64, a workgroup of K G / 64 waves above), K the largest of
-// 32, 16, 8, 4 and 2 that is at most `want` and has a kernel for the order (C <= 2);
-// K = 1 (the order's default layout) otherwise.
+// 32, 16, 8, 4 and 2 that is at most `want` and has a kernel for the order (C <= 2;
+// above order 64 four whole-wave groups); K = 1 (the order's default layout)
+// otherwise.
 inline Layout lookahead_layout(int nmax, int want) {
     if (want >= 32 && nmax <= 16) return make_layout(nmax, 16, 1, true, 0, 32);
     if (want >= 32 && nmax <= 32) return make_layout(nmax, 16, 2, true, 0, 32);
@@ -101,6 +102,7 @@ inline Layout lookahead_layout(int nmax, int want) {
     if (want >= 4 && nmax <= 32) return make_layout(nmax, 16, 2, true, 0, 4);
     if (want >= 2 && nmax <= 32) return make_layout(nmax, 32, 1, true, 0, 2);
     if (want >= 2 && nmax <= 64) return make_layout(nmax, 32, 2, true, 0, 2);
+    if (want >= 4 && nmax <= 128) return make_layout(nmax, 64, 2, true, 0, 4);
     return layout_for(nmax);
 }
 
