@@ -168,11 +168,12 @@ def bench_batch(args, eng):
         rows = eng.autotune(mats, p, moves=512, seeds=seeds)
         tune = {"seconds": time.perf_counter() - t1,
                 "chosen": [{k: r[k] for k in ("order", "lanes", "cols", "triangular",
-                                              "waves_per_block", "moves_per_sec")}
+                                              "waves_per_block", "lookahead", "moves_per_sec")}
                            for r in rows if r["tuned"]]}
         log(f"autotune {tune['seconds']:.2f}s: " + ", ".join(
             f"{c['order']}:{c['lanes']}x{c['cols']}{'t' if c['triangular'] else 's'}"
-            f"w{c['waves_per_block']}" for c in tune["chosen"]))
+            f"w{c['waves_per_block']}" + (f"k{c['lookahead']}" if c["lookahead"] > 1 else "")
+            for c in tune["chosen"]))
     for w in range(args.warmup):
         eng.batch_solve(mats, p, seeds=seeds)
     solved = verified = moves = 0
