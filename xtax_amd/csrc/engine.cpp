@@ -307,12 +307,17 @@ struct Outcome {
     std::vector<RepState> rs;
 };
 
-void fetch_outcome(const HostInst& h, const Inst& rec, hipStream_t st, Outcome& out) {
-    const size_t nn = (size_t)h.n * h.n;
+// An instance's outcome in three stages, so that many instances can share one
+// wait between them: the replica records' copy; the result replica (the
+// solving one, else the instance best) and its A and X^T copied; the score.
+void outcome_records(const HostInst& h, const Inst& rec, hipStream_t st, Outcome& out) {
     out.rs.resize((size_t)h.R);
     HIPCHK(hipMemcpyAsync(out.rs.data(), rec.rs, (size_t)h.R * sizeof(RepState),
                           hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+}
+
+void outcome_state(const HostInst& h, const Inst& rec, hipStream_t st, Outcome& out) {
+    const size_t nn = (size_t)h.n * h.n;
     out.moves = out.accepts = out.greedy = out.infeasible = out.sweeps = 0;
     for (const auto& s : out.rs) {
         out.moves += (int64_t)s.moves;
@@ -351,8 +356,31 @@ void fetch_outcome(const HostInst& h, const Inst& rec, hipStream_t st, Outcome& 
     }
     HIPCHK(hipMemcpyAsync(out.A.data(), srcA, nn * 8, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(out.Xt.data(), srcX, nn * 8, hipMemcpyDeviceToHost, st));
+}
+
+void fetch_outcome(const HostInst& h, const Inst& rec, hipStream_t st, Outcome& out) {
+    outcome_records(h, rec, st, out);
+    HIPCHK(hipStreamSynchronize(st));
+    outcome_state(h, rec, st, out);
     HIPCHK(hipStreamSynchronize(st));
     out.best_score = score_of(out.A.data(), h.n);
+}
+
+// fn(i) for i < count on up to 16 host threads (results are independent)
+template <class F>
+void parallel_for(size_t count, F fn) {
+    const size_t nt = std::min<size_t>(count, std::min(16u, std::max(1u, std::thread::hardware_concurrency())));
+    if (nt <= 1) {
+        for (size_t i = 0; i < count; ++i) fn(i);
+        return;
+    }
+    std::vector<std::thread> pool;
+    std::atomic<size_t> next{0};
+    for (size_t t = 0; t < nt; ++t)
+        pool.emplace_back([&]() {
+            for (size_t i = next++; i < count; i = next++) fn(i);
+        });
+    for (auto& th : pool) th.join();
 }
 
 void write_outputs(const HostInst& h, const Outcome& o, int64_t* A_out, int64_t* X_out) {
@@ -919,12 +947,22 @@ struct Server {
     // graph of several epochs, rebuilt when residency changes
     hipGraphExec_t gexec = nullptr;
     int graph_epochs = 0;
+    // the overlapped batch loop: a pinned snapshot of the instance records
+    // copied after the epochs in flight, and a stream for retirement copies
+    // while they run
+    Inst* pin_insts = nullptr;
+    int pin_cap = 0;
+    int inflight_ns = 0, inflight_k = 0;
+    float last_ms = 0;
+    hipStream_t fetch_stream = nullptr;
 
     ~Server() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         for (auto& kv : buckets)
             if (kv.second.d_refs) (void)hipFree(kv.second.d_refs);
         if (d_insts) (void)hipFree(d_insts);
+        if (pin_insts) (void)hipHostFree(pin_insts);
+        if (fetch_stream) (void)hipStreamDestroy(fetch_stream);
     }
 
     double now() const { return std::chrono::duration<double>(clk::now() - t0).count(); }
@@ -1040,11 +1078,61 @@ struct Server {
     }
 
     // Runs k epochs: captured once into a graph per residency, replayed.
-    void run_epochs(int k) {
+    // Launches k epochs as one captured graph (recaptured when k or the
+    // launch tables change) and the copy of the instance records into pinned
+    // memory, without waiting; finish_epochs waits.
+    void launch_epochs(int k) {
+        const int ns = (int)slots.size();
+        if (ns == 0) return;
+        if (pin_cap < ns) {
+            if (pin_insts) (void)hipHostFree(pin_insts);
+            pin_insts = nullptr;
+            pin_cap = std::max(ns, 2 * pin_cap);
+            HIPCHK(hipHostMalloc((void**)&pin_insts, (size_t)pin_cap * sizeof(Inst),
+                                 hipHostMallocDefault));
+        }
+        if (!gexec || graph_epochs != k) {
+            if (gexec) (void)hipGraphExecDestroy(gexec);
+            gexec = nullptr;
+            std::vector<Bucket> bl = bucket_list();
+            hipGraph_t graph;
+            HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            for (int e = 0; e < k; ++e) launch_epoch(ctx, bl, d_insts, ns, max_r);
+            HIPCHK(hipStreamEndCapture(ctx->stream, &graph));
+            HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+            (void)hipGraphDestroy(graph);
+            graph_epochs = k;
+        }
+        HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+        HIPCHK(hipGraphLaunch(gexec, ctx->stream));
+        HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+        HIPCHK(hipMemcpyAsync(pin_insts, d_insts, (size_t)ns * sizeof(Inst),
+                              hipMemcpyDeviceToHost, ctx->stream));
+        inflight_ns = ns;
+        inflight_k = k;
+    }
+
+    void finish_epochs() {
+        if (inflight_ns == 0) return;
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        const int ns = std::min(inflight_ns, (int)std::min(slots.size(), h_insts.size()));
+        for (int s = 0; s < ns; ++s) {
+            h_insts[(size_t)s] = pin_insts[s];
+            if (slots[(size_t)s]) slots[(size_t)s]->rec = pin_insts[s];
+        }
+        HIPCHK(hipEventElapsedTime(&last_ms, ctx->ev0, ctx->ev1));
+        kernel_ms += last_ms;
+        epochs += inflight_k;
+        inflight_ns = 0;
+    }
+
+    // graph: replay the k epochs as one captured graph (recaptured when k or
+    // the launch tables change), also for k = 1
+    void run_epochs(int k, bool graph = false) {
         const int ns = (int)slots.size();
         if (ns == 0) return;
         std::vector<Bucket> bl = bucket_list();
-        if (k > 1) {
+        if (k > 1 || graph) {
             if (!gexec || graph_epochs != k) {
                 if (gexec) (void)hipGraphExecDestroy(gexec);
                 hipGraph_t graph;
@@ -1077,21 +1165,41 @@ struct Server {
 
     // Retires every resident instance whose status left RUNNING (or all, when
     // force); writes outputs and queues results.
-    void retire(bool force, uint64_t only_tag = 0, bool by_tag = false) {
+    // st: the stream for the outcome copies (default: the engine's). The
+    // copies of all retiring instances share two waits; outputs and exact
+    // verification run on host threads.
+    void retire(bool force, uint64_t only_tag = 0, bool by_tag = false, hipStream_t st = nullptr) {
+        if (!st) st = ctx->stream;
+        std::vector<size_t> fin;
         for (size_t s = 0; s < slots.size(); ++s) {
             if (!slots[s]) continue;
             HostInst& h = *slots[s];
-            Inst& rec = h_insts[s];
             if (by_tag && h.tag != only_tag) continue;
-            const bool fin = rec.status != ST_RUNNING;
-            if (!fin && !force && !by_tag) continue;
-            h.rec = rec;
-            Outcome o;
-            fetch_outcome(h, rec, ctx->stream, o);
-            write_outputs(h, o, h.A_out, h.X_out);
+            if (h_insts[s].status == ST_RUNNING && !force && !by_tag) continue;
+            h.rec = h_insts[s];
+            fin.push_back(s);
+        }
+        std::vector<Outcome> outs(fin.size());
+        for (size_t i = 0; i < fin.size(); ++i)
+            outcome_records(*slots[fin[i]], h_insts[fin[i]], st, outs[i]);
+        if (!fin.empty()) HIPCHK(hipStreamSynchronize(st));
+        for (size_t i = 0; i < fin.size(); ++i)
+            outcome_state(*slots[fin[i]], h_insts[fin[i]], st, outs[i]);
+        if (!fin.empty()) HIPCHK(hipStreamSynchronize(st));
+        std::vector<int> vstat(fin.size(), -1);
+        parallel_for(fin.size(), [&](size_t i) {
+            const HostInst& h = *slots[fin[i]];
+            outs[i].best_score = score_of(outs[i].A.data(), h.n);
+            write_outputs(h, outs[i], h.A_out, h.X_out);
+            if (p.verify) vstat[i] = verify_outcome(h, outs[i]);
+        });
+        for (size_t i = 0; i < fin.size(); ++i) {
+            const size_t s = fin[i];
+            HostInst& h = *slots[s];
+            const Outcome& o = outs[i];
             xtx_result res{};
             res.solved = o.solved;
-            res.verify_status = p.verify ? verify_outcome(h, o) : -1;
+            res.verify_status = vstat[i];
             res.solve_move = o.solve_move;
             res.best_score = o.best_score;
             res.moves = o.moves;
@@ -1397,23 +1505,50 @@ int xtx_batch_solve(xtx_ctx* ctx, const xtx_instance* instances, int32_t count,
         keep.clear();
         S.rebuild();
         const double upload_s = S.now();
-        // epochs per graph launch: about 8 ms of work between polls
-        const int k = 4;
+        // Epochs per poll: about 8 ms of work between polls (one epoch at the
+        // full batch), so a solved instance is collected within an epoch or
+        // 8 ms of its solve. The next epochs are launched before a poll's
+        // retirements (finished instances' waves exit at once), so the device
+        // stays busy while the host copies out and verifies results; the
+        // launch tables are rebuilt, with nothing in flight, once a quarter
+        // of the residents have gone (each rebuild costs a graph capture).
+        HIPCHK(hipStreamCreate(&S.fetch_stream));
+        int k = 1;
         Deadline dl(ctx->stop_host, params->max_seconds);
         int reason = 0;
-        while (S.resident() > 0) {
-            S.run_epochs(k);
-            S.retire(false);
-            if (*(volatile int*)ctx->stop_host) {
+        S.launch_epochs(k);
+        for (;;) {
+            S.finish_epochs();
+            const int kn = std::clamp((int)(8.0 / std::max((double)S.last_ms / k, 1e-3)), 1, 8);
+            bool running = false;
+            int finished = 0;
+            for (size_t s = 0; s < S.slots.size(); ++s) {
+                if (!S.slots[s]) continue;
+                if (S.h_insts[s].status == ST_RUNNING)
+                    running = true;
+                else
+                    ++finished;
+            }
+            const bool stop = *(volatile int*)ctx->stop_host != 0;
+            const bool capped = params->max_epochs > 0 && S.epochs >= params->max_epochs;
+            const bool rebuild = 4 * (S.retired_since_rebuild + finished) > S.resident();
+            const bool overlap = running && !stop && !capped && !rebuild;
+            if (overlap) {
+                k = kn;
+                S.launch_epochs(k);
+            }
+            S.retire(false, 0, false, overlap ? S.fetch_stream : nullptr);
+            if (overlap) continue;
+            if (stop) {
                 reason = 1;
                 break;
             }
-            if (params->max_epochs > 0 && S.epochs >= params->max_epochs) break;
-            // retired instances stay in the launch tables (their waves exit at
-            // once) until a quarter of the residents have gone: each rebuild
-            // costs a graph capture
-            if (4 * S.retired_since_rebuild > S.resident()) S.rebuild();
+            if (!running || capped || S.resident() == 0) break;
+            S.rebuild();
+            k = kn;
+            S.launch_epochs(k);
         }
+        S.finish_epochs();
         S.retire(true);
         if (reason && dl.fired()) *(volatile int*)ctx->stop_host = 0;
         int solved = 0, verified = 0;
