@@ -81,15 +81,14 @@ typedef struct {
                                  64): the replica takes a whole wave (2, 4) or
                                  a workgroup of four or eight waves and
                                  resolves them in order, the same trajectory
-                                 at up to that many moves per step. 0 = auto, by the replicas resident
-                                 together (the call's, the batch's, or at a
-                                 serve admission those resident with it): up
-                                 to 32 per compute unit, 16 (or the order's
-                                 lookahead configuration); more, the order's
-                                 configuration (xtx_launch_set /
-                                 xtx_autotune) or the default layout. 1: one
-                                 group (the order's configuration if it has
-                                 one group) */
+                                 at up to that many moves per step. 0 = auto:
+                                 plain and batch calls 16 (or the order's
+                                 lookahead configuration from xtx_launch_set
+                                 / xtx_autotune); a serve admission 16 while
+                                 at most 32 replicas per compute unit are
+                                 resident with it, else 1 (the order's
+                                 one-group configuration or the default
+                                 layout). 1: one group per replica. */
     /* Plain forms (xtx_solve_plain, xtx_dual_solve): called on the calling
      * thread at an epoch boundary whenever the instance best improves -- the
      * reference's "new best" report (congruence_anneal.hpp:309-352). seconds
@@ -224,15 +223,20 @@ typedef struct {
     double moves_per_sec;     /* out of xtx_autotune: measured rate */
 } xtx_launch;
 
-/* Configuration used for instances of order n. */
+/* The configuration set for order n (tuned = 1), else the one a plain or
+ * batch call with automatic lookahead uses (tuned = 0). A set configuration
+ * applies where its kind is chosen: a lookahead one under automatic
+ * lookahead, a one-group one at lookahead 1 or a loaded server. */
 int xtx_launch_get(xtx_ctx* ctx, int32_t n, xtx_launch* out);
 /* Sets the configuration of cfg->order (lanes = 0 restores the rule). Affects
  * instances admitted afterwards. XTX_E_ARG if no kernel has that shape or its
  * LDS does not fit a workgroup. */
 int xtx_launch_set(xtx_ctx* ctx, const xtx_launch* cfg);
 /* Autotuner: for every order present in sample[0..count), times each valid
- * configuration on those instances (repeated until they fill the device) for
- * `moves` proposals per replica and keeps the fastest for the context.
+ * configuration at the lookahead depth plain and batch calls use (16 where
+ * the order allows it, else its largest) on those instances (repeated until
+ * they fill the device) for `moves` proposals per replica and keeps the
+ * fastest for the context.
  * Writes up to max_results measurements (every candidate, the chosen ones
  * with tuned = 1) to results; *n_results is the number written. */
 int xtx_autotune(xtx_ctx* ctx, const xtx_instance* sample, int32_t count,

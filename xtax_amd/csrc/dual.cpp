@@ -218,7 +218,7 @@ int xtx_dual_solve(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t n
         h.P.t_init = h.t_init;
         if (h.P.tempering) h.temps = ladder(h.t_init, p.t_min, h.R);
         const DualLayout L = dual_layout(
-            n, p.lookahead != 0 ? p.lookahead : lookahead_rule(ctx->prop.multiProcessorCount, h.R));
+            n, p.lookahead != 0 ? p.lookahead : kLookahead);
         if (L.rep_bytes > g_lds_block) throw Err{XTX_E_ARG, "n too large for an LDS-resident replica"};
 
         xtx_dual_stats st{};

@@ -424,21 +424,20 @@ static std::string g_open_error;
 namespace {
 
 // The layout instances of order n run in. `requested`: the params' lookahead
-// (0 = automatic); `rule`: the lookahead rule's value for the replicas resident
-// together. A requested lookahead gives its layout (1: the order's
-// configuration from xtx_launch_set / xtx_autotune if it has one lane group,
-// else the default layout). Automatic with the rule at 1 (a full device): the
-// order's configuration, else the default layout; with the rule above 1 (few
-// replicas): the order's configuration if it is a lookahead one, else the
-// rule's lookahead layout.
+// (0 = automatic); `rule`: the form's lookahead rule (16 for plain and batch
+// calls; by load for a serve admission). A requested lookahead gives its
+// layout (1: the order's configuration from xtx_launch_set / xtx_autotune if
+// it has one lane group, else the default layout). Automatic: the order's
+// configuration if it has lookahead (rule above 1) or one group (rule 1),
+// else the rule's layout.
 Layout choose_layout(const xtx_ctx* ctx, int n, int requested, int rule) {
     const int nmax = order_of(n);
     const auto it = ctx->tuned.find(nmax);
     const bool has = it != ctx->tuned.end();
     if (requested > 1) return lookahead_layout(nmax, requested);
-    if (requested == 1) return has && it->second.K == 1 ? it->second : layout_for(n);
-    if (rule > 1) return has && it->second.K > 1 ? it->second : lookahead_layout(nmax, rule);
-    return has ? it->second : layout_for(n);
+    const int K = requested == 1 ? 1 : rule;
+    if (has && (it->second.K > 1) == (K > 1)) return it->second;
+    return K > 1 ? lookahead_layout(nmax, K) : layout_for(n);
 }
 
 void apply_layout(const xtx_ctx* ctx, HostInst& h, int requested, int rule) {
@@ -631,7 +630,7 @@ int xtx_solve_plain(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t 
         HostInst& h = ctx->plain;
         h = HostInst{};
         prepare(h, A0, X0, n, p.replicas, p.seed, active, n_active, p);
-        apply_layout(ctx, h, p.lookahead, lookahead_rule(ctx, h.R));
+        apply_layout(ctx, h, p.lookahead, kLookahead);
         xtx_stats st{};
         st.verify_status = -1;
         if (n < 2 || h.N < 2 || h.offnz0 == 0) {
@@ -1294,7 +1293,7 @@ std::vector<Layout> candidates(int nmax) {
             }
     }
     for (int K : {2, 4, 8, 16, 32})
-        for (int G : {16, 32}) {
+        for (int G : {16, 32, 64}) {
             const int C = (nmax + G - 1) / G;
             if (!launch_invalid(nmax, G, C, true, 0, K)) v.push_back(make_layout(nmax, G, C, true, 0, K));
         }
@@ -1396,7 +1395,14 @@ void autotune_orders(xtx_ctx* ctx, const xtx_instance* sample, int count,
         double best_rate = -1;
         Layout best;
         size_t first = out->size();
+        // The batch and plain forms run the rule's lookahead depth; the tuner
+        // chooses the lane shape at that depth only. A deeper or shallower
+        // depth timed here, on a device filled twice over, would rank
+        // differently from a batch whose resident set thins out as matrices
+        // retire.
+        const int rule_K = lookahead_layout(nmax, kLookahead).K;
         for (const Layout& L : candidates(nmax)) {
+            if (L.K != rule_K) continue;
             const double ms = time_layout(ctx, proto, L, p);
             const double rate = ms > 0 ? (double)reps * moves / (ms * 1e-3) : 0.0;
             xtx_launch r;
@@ -1426,8 +1432,11 @@ extern "C" {
 
 int xtx_launch_get(xtx_ctx* ctx, int32_t n, xtx_launch* out) {
     if (!ctx || !out || n < 2 || n > kMaxN) return XTX_E_ARG;
-    const bool tuned = ctx->tuned.count(order_of(n)) != 0;
-    fill_launch(choose_layout(ctx, n, 0, 1), tuned, out);
+    // the order's configuration if one is set, else what a plain or batch
+    // call with automatic lookahead runs
+    const auto it = ctx->tuned.find(order_of(n));
+    const bool tuned = it != ctx->tuned.end();
+    fill_launch(tuned ? it->second : choose_layout(ctx, n, 0, kLookahead), tuned, out);
     return XTX_OK;
 }
 
@@ -1489,10 +1498,7 @@ int xtx_batch_solve(xtx_ctx* ctx, const xtx_instance* instances, int32_t count,
             hs.back()->tag = (uint64_t)q;
             total += hs.back()->bytes;
         }
-        int64_t reps = 0;
-        for (const auto& h : hs) reps += h->R;
-        const int rule = lookahead_rule(ctx, reps);
-        for (auto& h : hs) apply_layout(ctx, *h, params->lookahead, rule);
+        for (auto& h : hs) apply_layout(ctx, *h, params->lookahead, kLookahead);
         S.arena.reserve(total + 4096);
         S.t0 = clk::now();
         std::vector<Staging> keep;

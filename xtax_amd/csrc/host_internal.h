@@ -106,14 +106,14 @@ inline Layout lookahead_layout(int nmax, int want) {
     return layout_for(nmax);
 }
 
-// The lookahead rule for `replicas` resident together (a plain call's, a
-// batch's, or a serve admission's with those already resident) on a device
-// of `cus` compute units: 16 while they number at most 32 per compute unit,
-// else 1. Measured: 256 replicas of order 24 run 5.4x the moves/s of one
-// group per replica; serving at 200 requests/s (about 4000 replicas resident)
-// halves the latency; an overloaded serve (400/s, where a lookahead replica's
-// larger LDS share admits fewer requests) runs faster with one group per
-// replica, and a full batch (32768) the same.
+// The lookahead rule. Plain and batch calls: 16 (measured: 256 replicas of
+// order 24 run 5.4x the moves/s of one group per replica, the full batch of
+// 1024 instances x 32 replicas 4.5% faster). Serve admissions, by the
+// replicas resident with the request on a device of `cus` compute units: 16
+// up to 32 per compute unit, else 1 (at 200 requests/s, about 4000 resident,
+// lookahead halves the latency; overloaded at 400/s a lookahead replica's
+// larger LDS share admits fewer requests and one group per replica serves
+// more).
 constexpr int kLookahead = 16;
 inline int lookahead_rule(int cus, int64_t replicas) {
     return replicas <= 32 * (int64_t)cus ? kLookahead : 1;
