@@ -35,7 +35,8 @@ Other configurations (`--workload ...`), same line format:
            4096 replicas, greedy 0.7, T in [50, 2000]
   lattice  24x24 lattice Gram (entries U[-9, 9]), 256 replicas, greedy 0.5,
            sweep 0.05
-  random   n = 10 random symmetric (U[-20, 20]), 8 replicas, T in [0.5, 50]
+  random   n = 10 random symmetric (U[-20, 20]), 8 replicas, T in [0.5, 50],
+           one seed per launch; "batched" = the same seeds as one batch
   dual     xdual's primal/dual objective (xtx_dual_solve) on the lattice
            workload's Gram, 4096 replicas, greedy 0.5; CPU baseline the
            reference's xdual binary (oracle/_ref/xdual_ref)
@@ -355,14 +356,20 @@ def bench_lattice(args, eng):
 
 def bench_random(args, eng):
     seeds = range(1, args.count + 1)
+    mats = [workloads.random_symmetric(10, -20, 20, seed=s) for s in seeds]
+
+    def params(seed):
+        return EngineParams(replicas=8, t_init=50.0, t_min=0.5, greedy_fraction=0.5,
+                            exchange_interval=args.exchange_interval,
+                            max_epochs=args.cap_epochs, seed=seed, verify=True,
+                            lookahead=args.lookahead)
+
     solved = moves = 0
+    plain = []
     t0 = time.perf_counter()
-    for s in seeds:
-        A0 = workloads.random_symmetric(10, -20, 20, seed=s)
-        p = EngineParams(replicas=8, t_init=50.0, t_min=0.5, greedy_fraction=0.5,
-                         exchange_interval=args.exchange_interval, max_epochs=args.cap_epochs,
-                         seed=s, verify=True, lookahead=args.lookahead)
-        r = eng.run_annealer(A0, None, p)
+    for s, A0 in zip(seeds, mats):
+        r = eng.run_annealer(A0, None, params(s))
+        plain.append((r.A, r.X))
         solved += r.stats["solved"]
         moves += r.stats["moves"]
     el = time.perf_counter() - t0
@@ -370,6 +377,16 @@ def bench_random(args, eng):
                        f"T in [0.5,50], greedy 0.5, cap {args.cap_epochs} epochs"}
     out = base_line(args, moves / el, "moves/s", el, cfg)
     out.update({"solved": solved, "instances": args.count, "cpu_baseline": None})
+    # The same seeds as one batch: the same trajectories run concurrently
+    # (each instance's A and X equal its plain run's, checked here).
+    res, bst = eng.batch_solve(mats, params(1), seeds=list(seeds))
+    same = all(np.array_equal(res[q][0], plain[q][0]) and np.array_equal(res[q][1], plain[q][1])
+               for q in range(len(mats)))
+    resident = bst["seconds"] - bst["upload_seconds"]
+    out["batched"] = {"value": bst["moves"] / resident, "unit": "moves/s",
+                      "seconds": resident, "solved": bst["solved"], "equal_to_plain": same}
+    log(f"batched: {bst['moves'] / resident:.3e} moves/s, solved {bst['solved']}, "
+        f"results equal to the plain runs: {same}")
     if not args.no_cpu and os.path.exists(REF_BIN):
         # the reference on the first seeds, 8 threads = 8 replicas as here
         extra = ["-t", "8", "--t-init", "50", "--t-min", "0.5", "--greedy-fraction", "0.5"]
