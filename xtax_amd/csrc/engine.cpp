@@ -1040,9 +1040,11 @@ struct Server {
             HostInst& h = *slots[(size_t)s];
             h_insts[(size_t)s] = h.rec;
             max_r = std::max(max_r, h.R);
-            Bucket& b = buckets[key_of(h.lay)];
-            b.lay = h.lay;
-            b.units_per_block = units_per_block_for(h.lay, p.waves_per_block);
+            Bucket& b = buckets[launch_key(h.lay)];
+            if (b.refs.empty() || h.lay.nmax > b.lay.nmax) {
+                b.lay = h.lay;
+                b.units_per_block = units_per_block_for(h.lay, p.waves_per_block);
+            }
             for (int r = 0; r < h.R; ++r) b.refs.push_back(RepRef{s, r});
             pad_to_wave(b.refs, h.lay.G * h.lay.K, s);
         }
@@ -1396,10 +1398,10 @@ void autotune_orders(xtx_ctx* ctx, const xtx_instance* sample, int count,
         Layout best;
         size_t first = out->size();
         // The batch and plain forms run the rule's lookahead depth; the tuner
-        // chooses the lane shape at that depth only. A deeper or shallower
-        // depth timed here, on a device filled twice over, would rank
-        // differently from a batch whose resident set thins out as matrices
-        // retire.
+        // chooses the lane shape at that depth only. Depths do not rank here
+        // as they do in a run: one epoch from the start matrices commits
+        // often, and lookahead makes most of its proposals again after a
+        // commit, while later epochs commit rarely.
         const int rule_K = lookahead_layout(nmax, kLookahead).K;
         for (const Layout& L : candidates(nmax)) {
             if (L.K != rule_K) continue;

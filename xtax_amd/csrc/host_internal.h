@@ -120,7 +120,16 @@ inline int lookahead_rule(int cus, int64_t replicas) {
 }
 
 using BucketKey = std::tuple<int, int, int, bool, int, int>;
-inline BucketKey key_of(const Layout& L) { return {L.nmax, L.G, L.C, L.tri, L.waves, L.K}; }
+// The launch a resident instance joins. A kernel runs any order up to the
+// launch's padded order (A's rows in LDS are laid out by the instance's own
+// n; only the offsets of the replica's other LDS arrays follow the launch's
+// order). Lookahead layouts give each replica its own workgroup, so orders
+// that share a kernel share one launch sized for the largest: fewer launches
+// than streams, all of an epoch's kernels concurrent. One-group layouts keep
+// one launch per order (a larger order's LDS would cost them occupancy).
+inline BucketKey launch_key(const Layout& L) {
+    return {L.K > 1 ? 0 : L.nmax, L.G, L.C, L.tri, L.waves, L.K};
+}
 
 constexpr int kLdsPerCu = 160 * 1024;
 // Largest LDS one block may use on the open device (set by xtx_open).
