@@ -168,7 +168,8 @@ typedef struct {
     int32_t epochs;
     double seconds;
     double kernel_ms;
-    int32_t buckets;
+    int32_t buckets;          /* kernel launches per epoch (orders whose
+                                 lookahead configurations match share one) */
     int32_t graph_epochs;     /* epochs per captured graph launch */
     double upload_seconds;    /* admission: host inputs to device-resident
                                  state; seconds - upload_seconds is the run
@@ -207,7 +208,8 @@ void xtx_clear_stop(xtx_ctx* ctx);
  * No counterpart in the reference (its CPU threads have no launch shape).
  * Instances are bucketed by padded order (n rounded up to a multiple of 8);
  * each bucket runs with one configuration: lanes per replica, columns per
- * lane, LDS layout of A and waves per workgroup. The configuration changes
+ * lane, LDS layout of A, waves per workgroup and lookahead. Orders whose
+ * lookahead configurations match run in one launch. The configuration changes
  * speed only: every configuration computes the same moves and results. */
 typedef struct {
     int32_t order;            /* padded order: a multiple of 8 in [8, 128] */
