@@ -273,8 +273,12 @@ def bench_serve(args, eng):
     nxt = 0
     t0 = time.perf_counter()
     arrivals = {}
+    next_report = 20.0
     while done < count:
         now = time.perf_counter() - t0
+        if now >= next_report:
+            log(f"serve: {now:.0f}s, {done}/{count} retired, {solved} solved")
+            next_report += 20.0
         while nxt < count and reqs[nxt][0] <= now:
             eng.serve_submit(reqs[nxt][1], nxt, args.seed * 1_000_003 + nxt)
             arrivals[nxt] = t0 + reqs[nxt][0]
