@@ -127,6 +127,8 @@ using BucketKey = std::tuple<int, int, int, bool, int, int>;
 // that share a kernel share one launch sized for the largest: fewer launches
 // than streams, all of an epoch's kernels concurrent. One-group layouts keep
 // one launch per order (a larger order's LDS would cost them occupancy).
+// A server's admission budget still charges each instance its own order's
+// LDS share: the budget bounds oversubscription, not what one launch holds.
 inline BucketKey launch_key(const Layout& L) {
     return {L.K > 1 ? 0 : L.nmax, L.G, L.C, L.tri, L.waves, L.K};
 }
