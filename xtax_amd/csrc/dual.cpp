@@ -452,7 +452,7 @@ int xtx_dual_solve(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t n
                     break;
                 }
                 if (*(volatile int*)ctx->stop_host) {
-                    reason = dl.fired() ? 1 : 2;
+                    reason = stop_reason_of(ctx->stop_host);
                     break;
                 }
                 if (p.max_epochs > 0 && epochs >= p.max_epochs) {
@@ -461,7 +461,6 @@ int xtx_dual_solve(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t n
                 }
             }
         }
-        if (reason == 1) *(volatile int*)ctx->stop_host = 0;
         I = *pin;
 
         // outcome: the solving replica's state, else the instance best

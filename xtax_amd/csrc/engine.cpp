@@ -592,11 +592,12 @@ const char* xtx_last_error(const xtx_ctx* ctx) {
 }
 
 void xtx_request_stop(xtx_ctx* ctx) {
-    if (ctx && ctx->stop_host) *(volatile int*)ctx->stop_host = 1;
+    // a lock-free atomic OR: async-signal-safe
+    if (ctx && ctx->stop_host) __atomic_fetch_or(ctx->stop_host, kStopRequest, __ATOMIC_SEQ_CST);
 }
 
 void xtx_clear_stop(xtx_ctx* ctx) {
-    if (ctx && ctx->stop_host) *(volatile int*)ctx->stop_host = 0;
+    if (ctx && ctx->stop_host) __atomic_fetch_and(ctx->stop_host, ~kStopRequest, __ATOMIC_SEQ_CST);
 }
 
 int xtx_query_limits(xtx_ctx* ctx, xtx_limits* out) {
@@ -733,7 +734,7 @@ int xtx_solve_plain(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t 
                     break;
                 }
                 if (*(volatile int*)ctx->stop_host) {
-                    reason = dl.fired() ? 1 : 2;
+                    reason = stop_reason_of(ctx->stop_host);
                     break;
                 }
                 if (p.max_epochs > 0 && epochs >= p.max_epochs) {
@@ -742,7 +743,6 @@ int xtx_solve_plain(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t 
                 }
             }
         }
-        if (reason == 1) *(volatile int*)ctx->stop_host = 0;  // deadline flag is ours
         h.rec = *pin;
         Outcome o;
         fetch_outcome(h, h.rec, ctx->stream, o);
@@ -966,7 +966,52 @@ struct Server {
 
     double now() const { return std::chrono::duration<double>(clk::now() - t0).count(); }
 
-    int64_t lds_cost(const HostInst& h) const { return (int64_t)h.R * h.lay.rep_bytes; }
+    // LDS the resident set occupies, charged by launch: every replica of a
+    // launch holds the LDS share of that launch's largest order (orders that
+    // share a lookahead kernel run in one launch sized for the largest).
+    // launch_reps[key][order] = resident replicas of that order in that launch.
+    std::map<BucketKey, std::map<int, int64_t>> launch_reps;
+
+    static int64_t launch_lds(const BucketKey& k, const std::map<int, int64_t>& orders) {
+        int64_t reps = 0;
+        for (const auto& kv : orders) reps += kv.second;
+        if (orders.empty()) return 0;
+        const int nmax = orders.rbegin()->first;
+        return reps * rep_bytes_of(nmax, std::get<1>(k), std::get<3>(k), std::get<5>(k));
+    }
+
+    // LDS in use after adding (sign +1) or removing (-1) h's replicas
+    int64_t lds_after(const HostInst& h, int sign) const {
+        const BucketKey key = launch_key(h.lay);
+        int64_t total = 0;
+        bool seen = false;
+        for (const auto& kv : launch_reps) {
+            if (kv.first != key) {
+                total += launch_lds(kv.first, kv.second);
+                continue;
+            }
+            seen = true;
+            auto m = kv.second;
+            m[h.lay.nmax] += sign * (int64_t)h.R;
+            if (m[h.lay.nmax] <= 0) m.erase(h.lay.nmax);
+            total += launch_lds(kv.first, m);
+        }
+        if (!seen && sign > 0) {
+            std::map<int, int64_t> m{{h.lay.nmax, (int64_t)h.R}};
+            total += launch_lds(key, m);
+        }
+        return total;
+    }
+
+    void charge(const HostInst& h, int sign) {
+        const BucketKey key = launch_key(h.lay);
+        auto& m = launch_reps[key];
+        m[h.lay.nmax] += sign * (int64_t)h.R;
+        if (m[h.lay.nmax] <= 0) m.erase(h.lay.nmax);
+        if (m.empty()) launch_reps.erase(key);
+        lds_in_use = 0;
+        for (const auto& kv : launch_reps) lds_in_use += launch_lds(kv.first, kv.second);
+    }
 
     int64_t lds_budget() const {
         if (cfg.lds_budget_bytes > 0) return cfg.lds_budget_bytes;
@@ -983,7 +1028,7 @@ struct Server {
             return false;
         if (auto_look && p.lookahead == 0)
             apply_layout(ctx, *h, 0, lookahead_rule(ctx, resident_reps + h->R));
-        if (lds_in_use + lds_cost(*h) > lds_budget() && lds_in_use > 0) return false;
+        if (lds_in_use > 0 && lds_after(*h, +1) > lds_budget()) return false;
         char* dm = arena.alloc(h->bytes);
         if (!dm) return false;
         Staging stg;
@@ -996,7 +1041,7 @@ struct Server {
         else
             HIPCHK(hipStreamSynchronize(ctx->stream));
         h->admit_s = now();
-        lds_in_use += lds_cost(*h);
+        charge(*h, +1);
         resident_reps += h->R;
         int slot;
         if (!free_slots.empty()) {
@@ -1211,7 +1256,7 @@ struct Server {
             done.push_back(res);
             moves_retired += o.moves;
             accepts_retired += o.accepts;
-            lds_in_use -= lds_cost(h);
+            charge(h, -1);
             resident_reps -= h.R;
             arena.free(h.dmem, h.bytes);
             slots[s].reset();
@@ -1523,7 +1568,6 @@ int xtx_batch_solve(xtx_ctx* ctx, const xtx_instance* instances, int32_t count,
         HIPCHK(hipStreamCreate(&S.fetch_stream));
         int k = 1;
         Deadline dl(ctx->stop_host, params->max_seconds);
-        int reason = 0;
         S.launch_epochs(k);
         for (;;) {
             S.finish_epochs();
@@ -1547,10 +1591,7 @@ int xtx_batch_solve(xtx_ctx* ctx, const xtx_instance* instances, int32_t count,
             }
             S.retire(false, 0, false, overlap ? S.fetch_stream : nullptr);
             if (overlap) continue;
-            if (stop) {
-                reason = 1;
-                break;
-            }
+            if (stop) break;
             if (!running || capped || S.resident() == 0) break;
             S.rebuild();
             k = kn;
@@ -1558,7 +1599,6 @@ int xtx_batch_solve(xtx_ctx* ctx, const xtx_instance* instances, int32_t count,
         }
         S.finish_epochs();
         S.retire(true);
-        if (reason && dl.fired()) *(volatile int*)ctx->stop_host = 0;
         int solved = 0, verified = 0;
         for (const auto& r : S.done) {
             if (results) results[r.tag] = r;
@@ -1689,14 +1729,17 @@ int xtx_serve_retire(xtx_ctx* ctx, uint64_t tag) {
             }
         }
         const size_t before = S.done.size();
-        if (!S.h_insts.empty() || S.resident()) {
-            // make sure h_insts reflects the device
-            const int ns = (int)S.slots.size();
-            S.h_insts.resize((size_t)ns);
-            if (ns)
-                HIPCHK(hipMemcpy(S.h_insts.data(), S.d_insts, (size_t)ns * sizeof(Inst),
-                                 hipMemcpyDeviceToHost));
-        }
+        // Requests admitted since the last epoch (a step admits after its
+        // retirements) have records only on the host: rebuild writes every
+        // slot's record to the device instance array (growing it as needed),
+        // then the array is read back so each slot's record is its own.
+        HIPCHK(hipSetDevice(ctx->device));
+        S.rebuild();
+        const int ns = (int)S.slots.size();
+        S.h_insts.resize((size_t)ns);
+        if (ns)
+            HIPCHK(hipMemcpy(S.h_insts.data(), S.d_insts, (size_t)ns * sizeof(Inst),
+                             hipMemcpyDeviceToHost));
         S.retire(false, tag, true);
         if (S.done.size() == before) throw Err{XTX_E_ARG, "unknown tag"};
         return XTX_OK;

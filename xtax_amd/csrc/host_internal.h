@@ -127,8 +127,8 @@ using BucketKey = std::tuple<int, int, int, bool, int, int>;
 // that share a kernel share one launch sized for the largest: fewer launches
 // than streams, all of an epoch's kernels concurrent. One-group layouts keep
 // one launch per order (a larger order's LDS would cost them occupancy).
-// A server's admission budget still charges each instance its own order's
-// LDS share: the budget bounds oversubscription, not what one launch holds.
+// A server's admission budget charges every replica of a launch the LDS share
+// of the launch's largest order.
 inline BucketKey launch_key(const Layout& L) {
     return {L.K > 1 ? 0 : L.nmax, L.G, L.C, L.tri, L.waves, L.K};
 }
@@ -234,17 +234,26 @@ std::vector<double> ladder(double top, double t_min, int R);
 void fractions(const xtx_params& p, int r, int R, double* greedy, double* target);
 size_t align256(size_t b);
 
-// Watchdog: raises the stop flag at a wall-clock deadline (the reference's
-// monitor-thread max_seconds check, congruence_anneal.hpp:512-519).
+// The stop word the kernels poll (nonzero = stop at the next poll point) holds
+// two bits: a caller's stop request (xtx_request_stop, the reference's g_stop,
+// stop_signal.hpp:40-64) and a solve's own max_seconds deadline. Each owner
+// sets and clears only its bit, so a deadline that fires while a solve winds
+// down cannot stop the next solve, and a deadline never clears a request.
+constexpr int kStopRequest = 1;
+constexpr int kStopDeadline = 2;
+
+// Watchdog: raises the deadline bit at a wall-clock deadline (the reference's
+// monitor-thread max_seconds check, congruence_anneal.hpp:512-519) and clears
+// it again when the solve ends (the destructor joins the watchdog first).
 class Deadline {
   public:
-    Deadline(int* flag, double seconds) {
+    Deadline(int* flag, double seconds) : flag_(flag) {
         if (!(seconds > 0.0)) return;
-        th_ = std::thread([this, flag, seconds] {
+        th_ = std::thread([this, seconds] {
             std::unique_lock<std::mutex> lk(m_);
             if (!cv_.wait_for(lk, std::chrono::duration<double>(seconds), [this] { return done_; })) {
                 fired_ = true;
-                *(volatile int*)flag = 1;
+                __atomic_fetch_or(flag_, kStopDeadline, __ATOMIC_SEQ_CST);
             }
         });
     }
@@ -257,16 +266,24 @@ class Deadline {
             cv_.notify_all();
             th_.join();
         }
+        if (fired_) __atomic_fetch_and(flag_, ~kStopDeadline, __ATOMIC_SEQ_CST);
     }
     bool fired() const { return fired_; }
 
   private:
+    int* flag_;
     std::thread th_;
     std::mutex m_;
     std::condition_variable cv_;
     bool done_ = false;
     std::atomic<bool> fired_{false};
 };
+
+// Why a solve loop saw a nonzero stop word: 2 = a caller's request (it wins
+// when both bits are set), 1 = the solve's deadline.
+inline int stop_reason_of(const int* flag) {
+    return (__atomic_load_n(flag, __ATOMIC_SEQ_CST) & kStopRequest) ? 2 : 1;
+}
 
 // Host copy of the last primal/dual solve (dual.cpp).
 struct DualLast {

@@ -288,7 +288,8 @@ int main(int argc, char** argv) {
     } else {
         std::cout << "[t=0s] start score=" << score_of(A) << " n=" << n << " threads="
                   << p.replicas << ((p.tempering && p.replicas >= 2) ? " (parallel tempering)" : "")
-                  << "\n";
+                  << "\n"
+                  << std::flush;  // the run has started, also when stdout is a pipe
         xtx_stats st{};
         // mid-run saves of the best (publish_files, main.cpp:413-416), reordered
         // like every published best under --rcm/--centroid
