@@ -254,6 +254,23 @@ int xtx_solve_plain(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t 
                     const xtx_params* params, const int32_t* active, int32_t n_active,
                     int64_t* A_out, int64_t* X_out, xtx_stats* stats);
 
+/* The plain form without allocation. No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * arguments, results and errors as xtx_solve_plain, whose kernels it runs;
+ * every device buffer of the solve (instance record, replica table, each
+ * replica's A, X^T, best snapshots, scalars, ladder, active list and move log)
+ * is carved out of one fixed 32 MiB device array per GPU, held until the
+ * process ends, and the call makes no hipMalloc, hipFree, hipHostMalloc or
+ * other allocating HIP call. Sizes whose state does not fit (about
+ * 4 R n^2 8 bytes, plus R move_log_cap 16 with a move log) are XTX_E_ARG with
+ * a message naming the bytes needed. All calls on one GPU share its array:
+ * concurrent calls on one GPU (from any contexts) race and must not overlap;
+ * xtx_last_replicas and the move log read it until the next call. */
+int xtx_solve_plain_noalloc(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t n,
+                            const xtx_params* params, const int32_t* active, int32_t n_active,
+                            int64_t* A_out, int64_t* X_out, xtx_stats* stats);
+
 /* The instance best as of the last epoch boundary of the running plain solve
  * (L1: A and X; xdual: P, Q and X; each pointer nullable). Valid only from
  * inside an on_best report (the reference's throttled best_*.csv writes,

@@ -134,6 +134,9 @@ SIGNATURES = {
                                C.c_int32, C.POINTER(Launch), C.c_int32, I32P]),
     "xtx_solve_plain": (C.c_int, [C.c_void_p, I64P, I64P, C.c_int32, C.POINTER(Params), I32P,
                                   C.c_int32, I64P, I64P, C.POINTER(Stats)]),
+    "xtx_solve_plain_noalloc": (C.c_int, [C.c_void_p, I64P, I64P, C.c_int32,
+                                          C.POINTER(Params), I32P, C.c_int32, I64P, I64P,
+                                          C.POINTER(Stats)]),
     "xtx_last_replicas": (C.c_int, [C.c_void_p, I64P, I64P, I64P]),
     "xtx_best_so_far": (C.c_int, [C.c_void_p, I64P, C.POINTER(C.c_double), I64P]),
     "xtx_movelog_get": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.POINTER(LogEntry)),

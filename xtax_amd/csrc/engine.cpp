@@ -45,6 +45,10 @@ hipError_t launch_probe_rng(hipStream_t st, const uint32_t* ck, int count, uint3
 hipError_t launch_replicate(hipStream_t st, int64_t* dst, const int64_t* src, int64_t nn, int R);
 hipError_t init_kernel_attrs(int max_lds_per_block);
 hipError_t init_dual_kernel_attrs(int max_lds_per_block);
+// the no-alloc solve's device scratch (scratch.hip)
+hipError_t scratch_init(int device);
+void* scratch_address(int device);
+size_t scratch_bytes();
 }  // namespace xtx
 
 using namespace xtx;
@@ -545,6 +549,7 @@ int xtx_open(int device, xtx_ctx** out) {
         g_lds_block = (int)ctx->prop.sharedMemPerBlock;
         HIPCHK(init_kernel_attrs(g_lds_block));
         HIPCHK(init_dual_kernel_attrs(g_lds_block));
+        HIPCHK(scratch_init(device));
         HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         HIPCHK(hipEventCreate(&ctx->ev0));
         HIPCHK(hipEventCreate(&ctx->ev1));
@@ -617,10 +622,27 @@ int xtx_query_limits(xtx_ctx* ctx, xtx_limits* out) {
     return XTX_OK;
 }
 
+}  // extern "C"
+
 // ---------------------------------------------------------------- plain form
-int xtx_solve_plain(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t n,
-                    const xtx_params* params, const int32_t* active, int32_t n_active,
-                    int64_t* A_out, int64_t* X_out, xtx_stats* stats) {
+namespace {
+
+// Device memory a plain solve may use instead of allocating: the fixed
+// scratch array of the no-alloc entry point (its device address and size).
+struct ScratchRegion {
+    char* base;
+    size_t bytes;
+};
+
+// One instance per launch (xtx_solve_plain). With `scr` null the solve
+// allocates its device buffers (the instance arena, the record and the
+// replica table) and a pinned host record; otherwise every device buffer is
+// carved out of scr (checked against its size before any device work) and the
+// record is read back into host memory of the call's own frame, so the solve
+// calls no allocating HIP routine.
+int solve_plain(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t n,
+                const xtx_params* params, const int32_t* active, int32_t n_active,
+                int64_t* A_out, int64_t* X_out, xtx_stats* stats, const ScratchRegion* scr) {
     if (!ctx || !params) return XTX_E_ARG;
     const auto t0 = clk::now();
     try {
@@ -653,41 +675,59 @@ int xtx_solve_plain(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t 
             ctx->plain_rs.clear();
             return XTX_OK;
         }
-        ctx->plain_arena.reserve(device_bytes(h));
-        Staging stg;
-        char* dm = ctx->plain_arena.alloc(device_bytes(h));
-        if (!dm) throw Err{XTX_E_NOMEM, "arena"};
-        upload(h, dm, ctx->stream, stg);
-        set_fractions(h, stg, p);
-        HIPCHK(hipMemcpyAsync(h.rec.rs, stg.rs.data(), (size_t)h.R * sizeof(RepState),
-                              hipMemcpyHostToDevice, ctx->stream));
-        if (!ctx->d_plain) HIPCHK(hipMalloc(&ctx->d_plain, sizeof(Inst)));
-        HIPCHK(hipMemcpyAsync(ctx->d_plain, &h.rec, sizeof(Inst), hipMemcpyHostToDevice,
-                              ctx->stream));
         std::vector<Bucket> buckets(1);
         buckets[0].lay = h.lay;
         buckets[0].units_per_block = units_per_block_for(h.lay, p.waves_per_block);
         for (int r = 0; r < h.R; ++r) buckets[0].refs.push_back(RepRef{0, r});
         pad_to_wave(buckets[0].refs, h.lay.G * h.lay.K, 0);
+        const size_t inst_bytes = device_bytes(h);
+        const size_t refs_bytes = buckets[0].refs.size() * sizeof(RepRef);
+        char* dm = nullptr;
+        Inst* d_rec = nullptr;
+        Inst frame_rec{};
         Inst* pin = nullptr;
-        // released on every exit, including an error thrown below
+        // allocated buffers are released on every exit, including an error
+        // thrown below
         struct Release {
+            bool owned;
             RepRef*& refs;
             Inst*& pin;
             ~Release() {
+                if (!owned) return;
                 if (refs) (void)hipFree(refs);
                 if (pin) (void)hipHostFree(pin);
             }
-        } release{buckets[0].d_refs, pin};
-        HIPCHK(hipMalloc(&buckets[0].d_refs, buckets[0].refs.size() * sizeof(RepRef)));
-        HIPCHK(hipMemcpyAsync(buckets[0].d_refs, buckets[0].refs.data(),
-                              buckets[0].refs.size() * sizeof(RepRef), hipMemcpyHostToDevice,
-                              ctx->stream));
+        } release{scr == nullptr, buckets[0].d_refs, pin};
+        if (scr) {
+            const size_t need = align256(sizeof(Inst)) + align256(refs_bytes) + inst_bytes;
+            if (need > scr->bytes)
+                throw Err{XTX_E_ARG, "the no-alloc solve needs " + std::to_string(need) +
+                                         " bytes of device scratch for this order, replica"
+                                         " count and move log; it has " +
+                                         std::to_string(scr->bytes)};
+            d_rec = reinterpret_cast<Inst*>(scr->base);
+            buckets[0].d_refs = reinterpret_cast<RepRef*>(scr->base + align256(sizeof(Inst)));
+            dm = scr->base + align256(sizeof(Inst)) + align256(refs_bytes);
+            pin = &frame_rec;
+        } else {
+            ctx->plain_arena.reserve(inst_bytes);
+            dm = ctx->plain_arena.alloc(inst_bytes);
+            if (!dm) throw Err{XTX_E_NOMEM, "arena"};
+            if (!ctx->d_plain) HIPCHK(hipMalloc(&ctx->d_plain, sizeof(Inst)));
+            d_rec = ctx->d_plain;
+            HIPCHK(hipMalloc(&buckets[0].d_refs, refs_bytes));
+            HIPCHK(hipHostMalloc((void**)&pin, sizeof(Inst), hipHostMallocDefault));
+        }
+        Staging stg;
+        upload(h, dm, ctx->stream, stg);
+        set_fractions(h, stg, p);
+        HIPCHK(hipMemcpyAsync(h.rec.rs, stg.rs.data(), (size_t)h.R * sizeof(RepState),
+                              hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(d_rec, &h.rec, sizeof(Inst), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(buckets[0].d_refs, buckets[0].refs.data(), refs_bytes,
+                              hipMemcpyHostToDevice, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
 
-        HIPCHK(hipHostMalloc((void**)&pin, sizeof(Inst), hipHostMallocDefault));
-        const int stop_before = *(volatile int*)ctx->stop_host;
-        (void)stop_before;
         double kernel_ms = 0;
         int epochs = 0;
         int reason = 3;
@@ -708,11 +748,11 @@ int xtx_solve_plain(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t 
                 for (auto& b : buckets)
                     HIPCHK(launch_anneal(ctx->stream, b.lay.G, b.lay.C, b.lay.tri, b.lay.K,
                                          b.units_per_block,
-                                         b.d_refs, (int)b.refs.size(), ctx->d_plain,
+                                         b.d_refs, (int)b.refs.size(), d_rec,
                                          b.lay.nmax, b.lay.rep_bytes, ctx->stop_dev));
                 HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
-                HIPCHK(launch_epoch_end(ctx->stream, ctx->d_plain, 1, h.R));
-                HIPCHK(hipMemcpyAsync(pin, ctx->d_plain, sizeof(Inst), hipMemcpyDeviceToHost,
+                HIPCHK(launch_epoch_end(ctx->stream, d_rec, 1, h.R));
+                HIPCHK(hipMemcpyAsync(pin, d_rec, sizeof(Inst), hipMemcpyDeviceToHost,
                                       ctx->stream));
                 HIPCHK(hipStreamSynchronize(ctx->stream));
                 float ms = 0;
@@ -782,6 +822,29 @@ int xtx_solve_plain(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t 
     } catch (const Err& e) {
         return fail(ctx, e);
     }
+}
+
+}  // namespace
+
+extern "C" {
+
+int xtx_solve_plain(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t n,
+                    const xtx_params* params, const int32_t* active, int32_t n_active,
+                    int64_t* A_out, int64_t* X_out, xtx_stats* stats) {
+    return solve_plain(ctx, A0, X0, n, params, active, n_active, A_out, X_out, stats, nullptr);
+}
+
+int xtx_solve_plain_noalloc(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t n,
+                            const xtx_params* params, const int32_t* active, int32_t n_active,
+                            int64_t* A_out, int64_t* X_out, xtx_stats* stats) {
+    if (!ctx || !params) return XTX_E_ARG;
+    void* base = scratch_address(ctx->device);
+    if (!base) {
+        ctx->err = "no device scratch address for this device (xtx_open resolves it)";
+        return XTX_E_STATE;
+    }
+    const ScratchRegion scr{static_cast<char*>(base), scratch_bytes()};
+    return solve_plain(ctx, A0, X0, n, params, active, n_active, A_out, X_out, stats, &scr);
 }
 
 int xtx_best_so_far(xtx_ctx* ctx, int64_t* A_out, double* Q_out, int64_t* X_out) {

@@ -146,8 +146,12 @@ class Engine:
         self._lib.xtx_clear_stop(self._h)
 
     # -- plain form -------------------------------------------------------
-    def run_annealer(self, A0, X0=None, params=None, active=None, allow_unverified=False):
-        """Anneal X0^T A0 X0 toward a diagonal; returns SolveResult(A, X, stats)."""
+    def run_annealer(self, A0, X0=None, params=None, active=None, allow_unverified=False,
+                     noalloc=False):
+        """Anneal X0^T A0 X0 toward a diagonal; returns SolveResult(A, X, stats).
+
+        noalloc: run through xtx_solve_plain_noalloc (the same solve with its
+        device state in the library's fixed scratch array)."""
         params = params or EngineParams()
         A0 = _i64(A0)
         n = A0.shape[0]
@@ -157,7 +161,8 @@ class Engine:
         X = np.zeros((n, n), np.int64)
         st = N.Stats()
         p = params.to_c()
-        rc = self._lib.xtx_solve_plain(
+        fn = self._lib.xtx_solve_plain_noalloc if noalloc else self._lib.xtx_solve_plain
+        rc = fn(
             self._h, _p64(A0), _p64(X0), n, C.byref(p),
             act.ctypes.data_as(N.I32P) if act is not None else None,
             len(act) if act is not None else 0, _p64(A), _p64(X), C.byref(st))
