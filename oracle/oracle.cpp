@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <random>
 #include <utility>
 #include <vector>
 
@@ -543,6 +544,73 @@ inline double u53(uint32_t hi, uint32_t lo) {
     return (double)(((uint64_t)(hi >> 5) << 26) | (lo >> 6)) * 0x1p-53;
 }
 
+// Planted mistakes, for the tests that show the pins catch them (0 = none):
+//   1 auto-accept only strictly downhill moves (d < 0 instead of d <= 0)
+//   2 the hot-row tournament keeps the lighter row (< instead of >)
+//   3 the worker-diversity offset with the wrong sign
+//   4 the exchange argument with the wrong sign
+//   5 the Metropolis exponent with the wrong sign (uphill moves as likely
+//     as their inverse)
+int g_mutation = 0;
+
+// The engine's draws for one move: Philox words of slots 0 .. 15 (above).
+struct PhiloxDraw {
+    uint32_t W[32][4];
+    uint32_t N = 0;
+    int row = 0, piv = 0;  // tournament candidates drawn so far
+    bool coin(double f) const { return u01(W[0][2]) < f; }
+    uint32_t index() { return mulhi32(W[row++][0], N); }
+    uint32_t pivot_index() { return mulhi32(W[piv++][1], N); }
+    uint32_t fallback(uint32_t tpos) const {
+        return (tpos + 1 + mulhi32(W[3][3], N - 1)) % N;
+    }
+    void pair(uint32_t& ip, uint32_t& jp) const {
+        ip = mulhi32(W[2][3], N);
+        jp = (ip + 1 + mulhi32(W[3][2], N - 1)) % N;
+    }
+    bool greedy_coin(double f) const { return u01(W[0][3]) < f; }
+    int64_t sign() const { return (W[1][2] & 1u) ? 1 : -1; }
+    // exp argument: d times -1/T (the reciprocal formed once per
+    // temperature, as the device keeps it)
+    bool metropolis(int64_t d, double T) const {
+        return u53(W[1][3], W[2][2]) < orc_xexp((double)d * (-1.0 / T));
+    }
+    bool sweep_coin(double f) const { return u01(W[4][2]) < f; }
+    uint32_t sweep_index() const { return mulhi32(W[4][3], N); }
+};
+
+// The reference's draws (congruence_anneal.hpp:189-192): one mt19937 stream
+// per worker, libstdc++'s uniform_int_distribution over active positions and
+// uniform_real_distribution(0, 1), consumed in the loop's order, with the
+// redraw loops for j != i (:287) and for the fallback pivot (:226), the
+// sign from the low bit of a raw draw (:293, :295) and std::exp(-d / T) in
+// the Metropolis test (:236).
+struct RefDraw {
+    std::mt19937 rng;
+    std::uniform_int_distribution<int> aidx;
+    std::uniform_real_distribution<double> unit{0.0, 1.0};
+    RefDraw(uint32_t seed, int N) : rng(seed), aidx(0, N - 1) {}
+    bool coin(double f) { return unit(rng) < f; }
+    uint32_t index() { return (uint32_t)aidx(rng); }
+    uint32_t pivot_index() { return (uint32_t)aidx(rng); }
+    uint32_t fallback(uint32_t tpos) {
+        uint32_t q;
+        do q = (uint32_t)aidx(rng);
+        while (q == tpos);
+        return q;
+    }
+    void pair(uint32_t& ip, uint32_t& jp) {
+        ip = (uint32_t)aidx(rng);
+        do jp = (uint32_t)aidx(rng);
+        while (jp == ip);
+    }
+    bool greedy_coin(double f) { return unit(rng) < f; }
+    int64_t sign() { return (rng() & 1u) ? 1 : -1; }
+    bool metropolis(int64_t d, double T) { return unit(rng) < std::exp(-(double)d / T); }
+    bool sweep_coin(double f) { return unit(rng) < f; }
+    uint32_t sweep_index() { return (uint32_t)aidx(rng); }
+};
+
 struct Model {
     int n = 0, N = 0;
     std::vector<int> act;
@@ -555,6 +623,9 @@ struct Model {
     int64_t gb_score = 0;  // instance best at epoch start
     std::vector<int64_t> gbA, gbXt;
     int64_t log_cap = 0;
+    // reference-draw mode: one worker, the reference's draws and publish rule
+    bool ref = false;
+    double rung_T = 0;  // tempering: the worker's (fixed) rung temperature
 
     // Slot c of move m: Philox block at counter (q, r, 16 t + c) with
     // q = (m - 1) / 4 and t = (m - 1) mod 4, so one block call can serve
@@ -588,6 +659,10 @@ struct Model {
     // Greedy reduction sweep from a pivot (congruence_anneal.hpp:246-260).
     bool sweep(ModelRep& R, int pivot, uint64_t m) {
         const int64_t before = R.score;
+        // The reference's sweep ends as soon as the run is done (:250): in the
+        // reference-draw mode the move cap stands for that stop, raised before
+        // the capped move's stall handling.
+        if (ref && p.max_moves && R.moves >= p.max_moves) return false;
         std::vector<int64_t> row((size_t)n);
         for (int q = 0; q < N; ++q) {
             const int j = act[(size_t)q];
@@ -604,6 +679,7 @@ struct Model {
     }
 
     void track_best(ModelRep& R, uint64_t m) {
+        if (ref) return;  // the reference keeps no per-worker best state
         if (R.score < R.best_score) {
             R.best_score = R.score;
             R.best_move = (int64_t)m;
@@ -612,26 +688,39 @@ struct Model {
         }
     }
 
-    // One iteration of the worker loop (congruence_anneal.hpp:276-413).
+    // One iteration of the worker loop (congruence_anneal.hpp:276-413) with
+    // the engine's draws: Philox slots of (replica, move).
     void step(ModelRep& R, uint32_t r) {
         const uint64_t m = ++R.moves;
-        if (tempering) R.T = temps[(size_t)R.rung];
+        PhiloxDraw D;
+        D.N = (uint32_t)N;
         const int G = std::max(p.target_samples, 5);
-        uint32_t W[32][4];
-        for (int g = 0; g < G; ++g) block(m, r, (uint32_t)g, W[g]);
+        for (int g = 0; g < G; ++g) block(m, r, (uint32_t)g, D.W[g]);
+        step_with(R, D, m);
+    }
 
+    // The loop body, shared by both draw sources. `ref` selects the
+    // reference's publish and solve rule (a strict new global best on an
+    // accepted downhill move, congruence_anneal.hpp:309-352) in place of the
+    // engine's (any replica at a diagonal; per-replica bests).
+    template <class D>
+    void step_with(ModelRep& R, D& draw, uint64_t m) {
+        if (tempering) R.T = ref ? rung_T : temps[(size_t)R.rung];
+
+        // pair: hot-row and pivot tournaments, else uniform (:211-228, :283-288)
         int i, j;
-        if (u01(W[0][2]) < R.tf) {
-            uint32_t tpos = mulhi32(W[0][0], (uint32_t)N);
+        if (draw.coin(R.tf)) {
+            uint32_t tpos = draw.index();
             for (int c = 1; c < p.target_samples; ++c) {
-                const uint32_t q = mulhi32(W[c][0], (uint32_t)N);
-                if (R.rm[(size_t)act[q]] > R.rm[(size_t)act[tpos]]) tpos = q;
+                const uint32_t q = draw.index();
+                const int64_t wq = R.rm[(size_t)act[q]], wt = R.rm[(size_t)act[tpos]];
+                if (g_mutation == 2 ? wq < wt : wq > wt) tpos = q;
             }
             const int target = act[tpos];
             int ibest = -1;
             int64_t vbest = -1;
             for (int c = 0; c < p.target_samples; ++c) {
-                const int cand = act[mulhi32(W[c][1], (uint32_t)N)];
+                const int cand = act[draw.pivot_index()];
                 if (cand == target) continue;
                 const int64_t v = iabs(R.A[(size_t)target * n + cand]);
                 if (v > vbest) {
@@ -639,33 +728,29 @@ struct Model {
                     ibest = cand;
                 }
             }
-            if (ibest < 0)
-                ibest = act[(tpos + 1 + mulhi32(W[3][3], (uint32_t)(N - 1))) % (uint32_t)N];
+            if (ibest < 0) ibest = act[draw.fallback(tpos)];
             i = ibest;
             j = target;
         } else {
-            const uint32_t ip = mulhi32(W[2][3], (uint32_t)N);
-            const uint32_t jp = (ip + 1 + mulhi32(W[3][2], (uint32_t)(N - 1))) % (uint32_t)N;
+            uint32_t ip, jp;
+            draw.pair(ip, jp);
             i = act[ip];
             j = act[jp];
         }
-        const int64_t pm = (W[1][2] & 1u) ? 1 : -1;
-        int64_t s;
-        if (u01(W[0][3]) < R.gf) {
-            s = orc_best_shear_l1(R.A.data(), n, i, j);
-            if (s == 0) s = pm;
-        } else {
-            s = pm;
-        }
+        // shear: the exact best with probability greedy, +-1 otherwise or
+        // when the best is 0 (:290-296)
+        int64_t s = 0;
+        if (draw.greedy_coin(R.gf)) s = orc_best_shear_l1(R.A.data(), n, i, j);
+        if (s == 0) s = draw.sign();
 
+        // attempt_add (:232-240): evaluate, Metropolis, commit
         int res = -1;
         {
             std::vector<int64_t> row((size_t)n);
             int64_t d = 0, dnz = 0;
             if (orc_try_add(R.A.data(), nullptr, n, i, j, s, row.data(), nullptr, &d, &dnz)) {
-                // exp argument: d times -1/T (the reciprocal formed once per
-                // temperature, as the device keeps it)
-                const bool ok = d <= 0 || u53(W[1][3], W[2][2]) < orc_xexp((double)d * (-1.0 / R.T));
+                const bool ok = (g_mutation == 1 ? d < 0 : d <= 0) ||
+                                draw.metropolis(g_mutation == 5 ? -d : d, R.T);
                 if (ok && commit_move(R, i, j, s, d, dnz, row, m, 0)) res = d < 0 ? 1 : 0;
             }
         }
@@ -677,27 +762,43 @@ struct Model {
         } else {
             ++R.msi;
         }
-        track_best(R, m);
-        if (R.offnz == 0) {
-            R.solved = true;
-            R.stopped = true;
-            R.solve_move = (int64_t)m;
-            return;
+        if (ref) {
+            if (res == 1 && R.score < gb_score) {
+                gb_score = R.score;
+                gbA = R.A;
+                gbXt = R.Xt;
+                if (R.offnz == 0) {
+                    R.solved = true;
+                    R.stopped = true;
+                    R.solve_move = (int64_t)m;
+                    return;
+                }
+            }
+        } else {
+            track_best(R, m);
+            if (R.offnz == 0) {
+                R.solved = true;
+                R.stopped = true;
+                R.solve_move = (int64_t)m;
+                return;
+            }
         }
 
         if (tempering) {
+            // stall: the optional reduction sweep only (:363-371)
             if (R.msi > p.stuck_threshold) {
-                if (p.sweep_fraction > 0.0 && u01(W[4][2]) < p.sweep_fraction) {
-                    sweep(R, act[mulhi32(W[4][3], (uint32_t)N)], m);
+                if (p.sweep_fraction > 0.0 && draw.sweep_coin(p.sweep_fraction)) {
+                    sweep(R, act[draw.sweep_index()], m);
                     track_best(R, m);
                 }
                 R.local_best = R.score;
                 R.msi = 0;
             }
         } else {
+            // stall: sweep, reheat, reseed when far behind (:372-394); else cool (:396-411)
             if (R.msi > p.stuck_threshold) {
-                if (p.sweep_fraction > 0.0 && u01(W[4][2]) < p.sweep_fraction) {
-                    if (sweep(R, act[mulhi32(W[4][3], (uint32_t)N)], m)) {
+                if (p.sweep_fraction > 0.0 && draw.sweep_coin(p.sweep_fraction)) {
+                    if (sweep(R, act[draw.sweep_index()], m)) {
                         R.local_best = R.score;
                         R.msi = 0;
                     }
@@ -732,7 +833,7 @@ struct Model {
                 if (R.T < p.t_min) R.T = p.t_min;
             }
         }
-        if (R.offnz == 0) {
+        if (!ref && R.offnz == 0) {
             R.solved = true;
             R.stopped = true;
             R.solve_move = (int64_t)m;
@@ -757,6 +858,46 @@ struct Model {
 
 }  // namespace
 
+int64_t orc_exchange_sweep(const double* temps, const double* score, int32_t* rung, int32_t R,
+                           uint64_t seed, uint64_t epoch) {
+    // congruence_anneal.hpp:525-550: owner of each rung, then adjacent rungs
+    // bottom up; the configuration carried up a rung is compared again at the
+    // next pair. The uniform of pair q after epoch e is Philox (key = seed,
+    // counter = {e, q, 0x40000000}).{x, y} as 53 bits.
+    std::vector<int> owner((size_t)R);
+    for (int r = 0; r < R; ++r) owner[(size_t)rung[r]] = r;
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    int64_t acc = 0;
+    for (int q = 0; q + 1 < R; ++q) {
+        const int wa = owner[(size_t)q], wb = owner[(size_t)q + 1];
+        double arg = (1.0 / temps[q] - 1.0 / temps[q + 1]) * (score[wa] - score[wb]);
+        if (g_mutation == 4) arg = -arg;
+        uint32_t w[4];
+        const uint32_t ctr[4] = {(uint32_t)epoch, (uint32_t)(epoch >> 32), (uint32_t)q,
+                                 0x40000000u};
+        orc_philox4x32_10(ctr, key, w);
+        if (arg >= 0.0 || u53(w[0], w[1]) < orc_xexp(arg)) {
+            rung[wa] = q + 1;
+            rung[wb] = q;
+            std::swap(owner[(size_t)q], owner[(size_t)q + 1]);
+            ++acc;
+        }
+    }
+    return acc;
+}
+
+void orc_set_mutation(int32_t which) { g_mutation = which; }
+
+int32_t orc_metropolis(int64_t d, double T, uint32_t u_hi, uint32_t u_lo) {
+    // the step's test with the engine's uniform (Model::step_with, PhiloxDraw)
+    PhiloxDraw D;
+    D.W[1][3] = u_hi;
+    D.W[2][2] = u_lo;
+    return ((g_mutation == 1 ? d < 0 : d <= 0) || D.metropolis(g_mutation == 5 ? -d : d, T))
+               ? 1
+               : 0;
+}
+
 void orc_ladder(double top, double t_min, int32_t R, double* temps) {
     // congruence_anneal.hpp:454-468
     const double bot = std::min(t_min, top);
@@ -777,7 +918,8 @@ void orc_fractions(const orc_params* p, int32_t r, double* greedy, double* targe
     // worker id.
     double g = p->greedy_fraction, t = p->target_fraction;
     if (p->worker_diversity > 0.0 && p->replicas > 1) {
-        const double off = p->worker_diversity * ((double)r / (double)(p->replicas - 1) - 0.5);
+        double off = p->worker_diversity * ((double)r / (double)(p->replicas - 1) - 0.5);
+        if (g_mutation == 3) off = -off;
         g = std::clamp(g + off, 0.0, 1.0);
         t = std::clamp(t + off, 0.0, 1.0);
     }
@@ -874,25 +1016,15 @@ int32_t orc_model_run(const int64_t* A_start, const int64_t* Xt_start, int32_t n
             break;
         }
         if (M.tempering) {
-            std::vector<int> owner((size_t)R);
-            for (int r = 0; r < R; ++r) owner[(size_t)M.reps[(size_t)r].rung] = r;
-            for (int q = 0; q + 1 < R; ++q) {
-                const int wa = owner[(size_t)q], wb = owner[(size_t)q + 1];
-                const double Ta = M.temps[(size_t)q], Tb = M.temps[(size_t)q + 1];
-                const double Sa = (double)M.reps[(size_t)wa].score;
-                const double Sb = (double)M.reps[(size_t)wb].score;
-                const double arg = (1.0 / Ta - 1.0 / Tb) * (Sa - Sb);
-                uint32_t w[4];
-                const uint32_t ctr[4] = {(uint32_t)e, (uint32_t)((uint64_t)e >> 32), (uint32_t)q,
-                                         0x40000000u};
-                orc_philox4x32_10(ctr, M.key, w);
-                if (arg >= 0.0 || u53(w[0], w[1]) < orc_xexp(arg)) {
-                    M.reps[(size_t)wa].rung = q + 1;
-                    M.reps[(size_t)wb].rung = q;
-                    std::swap(owner[(size_t)q], owner[(size_t)q + 1]);
-                    ++exchanges;
-                }
+            std::vector<int32_t> rung((size_t)R);
+            std::vector<double> score((size_t)R);
+            for (int r = 0; r < R; ++r) {
+                rung[(size_t)r] = M.reps[(size_t)r].rung;
+                score[(size_t)r] = (double)M.reps[(size_t)r].score;
             }
+            exchanges += orc_exchange_sweep(M.temps.data(), score.data(), rung.data(), R,
+                                            pp->seed, (uint64_t)e);
+            for (int r = 0; r < R; ++r) M.reps[(size_t)r].rung = rung[(size_t)r];
         }
         const int b = M.best_rep();
         if (M.reps[(size_t)b].best_score < M.gb_score) {
@@ -930,6 +1062,89 @@ int32_t orc_model_run(const int64_t* A_start, const int64_t* Xt_start, int32_t n
             std::copy(x.log.begin(), x.log.end(), log_out + (int64_t)r * log_cap);
             log_count[r] = (int64_t)x.log.size();
         }
+    }
+    if (res) *res = out;
+    return out.solved;
+}
+
+// ---------------------------------------------------------------------------
+// Reference-draw mode of the model: one worker of the reference's annealer
+// ---------------------------------------------------------------------------
+// The same loop body as the engine's model (Model::step_with), driven by the
+// reference's draws (RefDraw) and its publish rule, for one worker `t` of a
+// pool of `threads`: the worker's seed, its diversity offsets, and either its
+// fixed tempering rung (no exchange) or the cooling schedule. With threads = 1
+// this is run_annealer's whole single-threaded run (tempering is off below two
+// workers, congruence_anneal.hpp:427), which is deterministic.
+
+uint32_t orc_ref_worker_seed(uint64_t seed, int32_t t) {
+    // congruence_anneal.hpp:484-488 (seed 0 draws from random_device there)
+    std::mt19937 seed_rng((uint32_t)seed);
+    uint32_t w = 0;
+    for (int k = 0; k <= t; ++k) w = seed_rng();
+    return w ^ ((uint32_t)t * 0x9e3779b9u);
+}
+
+int32_t orc_ref_worker_run(const int64_t* A_start, const int64_t* Xt_start, int32_t n,
+                           const int32_t* active, int32_t n_active, const orc_ref_params* rp,
+                           int64_t* A_out, int64_t* Xt_out, int64_t* gb_A, int64_t* gb_Xt,
+                           orc_log_entry* log_out, int64_t* log_count, int64_t log_cap,
+                           orc_ref_result* res) {
+    Model M;
+    M.n = n;
+    M.p = rp->base;
+    M.ref = true;
+    M.log_cap = log_cap;
+    if (active && n_active > 0) {
+        M.act.assign(active, active + n_active);
+    } else {
+        M.act.resize((size_t)n);
+        for (int k = 0; k < n; ++k) M.act[(size_t)k] = k;
+    }
+    M.N = (int)M.act.size();
+    const int threads = std::max(1, rp->base.replicas);
+    M.tempering = rp->base.tempering && threads >= 2;
+    M.rung_T = rp->rung_T;
+    const int64_t nn = (int64_t)n * n;
+    std::vector<int64_t> A0(A_start, A_start + nn), X0(Xt_start, Xt_start + nn);
+    const int64_t score0 = orc_score(A0.data(), n);
+    M.t_init = rp->base.t_init > 0.0 ? rp->base.t_init : orc_suggest_t_init(A0.data(), n);
+    M.reps.resize(1);
+    ModelRep& x = M.reps[0];
+    x.A = A0;
+    x.Xt = X0;
+    x.rm.resize((size_t)n);
+    orc_compute_row_mass(x.A.data(), n, x.rm.data());
+    x.score = score0;
+    x.offnz = orc_offdiag_nonzero(A0.data(), n);
+    x.local_best = score0;
+    x.T = M.t_init;
+    x.cooling = rp->base.cooling;
+    x.best_score = score0;
+    orc_fractions(&rp->base, rp->thread_id, &x.gf, &x.tf);
+    M.gb_score = score0;
+    M.gbA = A0;
+    M.gbXt = X0;
+    orc_ref_result out{};
+    if (n >= 2 && M.N >= 2) {  // run_annealer returns the start otherwise (:435, :444)
+        RefDraw D(rp->worker_seed, M.N);
+        while (!x.stopped) {
+            const uint64_t m = ++x.moves;
+            M.step_with(x, D, m);
+        }
+    }
+    out.solved = x.solved;
+    out.moves = (int64_t)x.moves;
+    out.accepts = x.accepts;
+    out.score = x.score;
+    out.gb_score = M.gb_score;
+    if (A_out) std::copy(x.A.begin(), x.A.end(), A_out);
+    if (Xt_out) std::copy(x.Xt.begin(), x.Xt.end(), Xt_out);
+    if (gb_A) std::copy(M.gbA.begin(), M.gbA.end(), gb_A);
+    if (gb_Xt) std::copy(M.gbXt.begin(), M.gbXt.end(), gb_Xt);
+    if (log_out && log_count) {
+        std::copy(x.log.begin(), x.log.end(), log_out);
+        *log_count = (int64_t)x.log.size();
     }
     if (res) *res = out;
     return out.solved;

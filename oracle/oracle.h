@@ -21,7 +21,11 @@
  *   - a trajectory model of the engine's epoch-synchronous algorithm (the
  *     reference's anneal_worker loop, congruence_anneal.hpp:177-416, with the
  *     reference's mt19937 replaced by counter-based Philox and its 5 ms monitor
- *     exchange replaced by an exchange every `exchange_interval` moves)
+ *     exchange replaced by an exchange every `exchange_interval` moves), whose
+ *     loop body also runs on the reference's own draws (std::mt19937 and
+ *     libstdc++'s distributions) and publish rule: in that mode it reproduces
+ *     the reference's single-threaded runs and its worker loop exactly, which
+ *     pins the loop body the engine is tested against
  *
  * Each function is pinned by tests/test_oracle_golden.py (golden vectors from
  * the reference's own code, closed forms, known answers).
@@ -186,6 +190,57 @@ int32_t orc_dual_model_run(const int64_t* P_start, const double* Q_start,
                            double* dscal_out, int64_t* scal_out, int64_t* best_P,
                            double* best_Q, int64_t* best_Xt, orc_log_entry* log_out,
                            int64_t* log_count, int64_t log_cap, orc_dual_result* res);
+
+/* ---- reference-draw mode of the trajectory model ----
+ * One worker of the reference's annealer (anneal_worker,
+ * congruence_anneal.hpp:177-416) run through the model's own loop body, with
+ * the reference's draws in place of Philox: std::mt19937 seeded with the
+ * worker seed, libstdc++'s uniform_int_distribution / uniform_real_distribution,
+ * the redraw loops, the raw-draw sign bit and std::exp(-d / T); and with the
+ * reference's publish rule (the global best is replaced by a strictly better
+ * state after an accepted downhill move; the run stops when that state is
+ * diagonal). base.replicas is the pool size (threads: it sets the diversity
+ * offsets and, below 2, turns tempering off), thread_id the worker, rung_T its
+ * fixed tempering rung (no exchange). base.max_moves caps the run (0: until
+ * solved). Outputs: the worker's final state, the global best, the accepted
+ * moves (flag bit 0 sweep commit, bit 1 reseed). */
+typedef struct {
+    orc_params base;
+    uint32_t worker_seed;
+    int32_t thread_id;
+    double rung_T;
+} orc_ref_params;
+
+typedef struct {
+    int32_t solved;
+    int32_t reserved;
+    int64_t moves;       /* proposals, the reference's "done moves=" */
+    int64_t accepts;
+    int64_t score;       /* final state */
+    int64_t gb_score;    /* global best */
+} orc_ref_result;
+
+/* seed_rng() ^ t * 0x9e3779b9 for worker t (congruence_anneal.hpp:484-488) */
+uint32_t orc_ref_worker_seed(uint64_t seed, int32_t t);
+int32_t orc_ref_worker_run(const int64_t* A_start, const int64_t* Xt_start, int32_t n,
+                           const int32_t* active, int32_t n_active, const orc_ref_params* p,
+                           int64_t* A_out, int64_t* Xt_out, int64_t* gb_A, int64_t* gb_Xt,
+                           orc_log_entry* log_out, int64_t* log_count, int64_t log_cap,
+                           orc_ref_result* res);
+
+/* The model's replica-exchange sweep after epoch `epoch` (rung[r] = rung of
+ * replica r, updated in place; score[r] its current score); returns the
+ * number of swaps. */
+int64_t orc_exchange_sweep(const double* temps, const double* score, int32_t* rung, int32_t R,
+                           uint64_t seed, uint64_t epoch);
+/* Test hook: plants one of a few plausible mistakes in the model (0 = none,
+ * 1 strict auto-accept, 2 reversed hot-row tournament, 3 diversity offset
+ * sign, 4 exchange argument sign, 5 Metropolis exponent sign), so tests can
+ * show that the checks pinning the model reject each. */
+void orc_set_mutation(int32_t which);
+/* The model's Metropolis decision for a proposal of score change d at
+ * temperature T, given the move's two uniform words: 1 = accept. */
+int32_t orc_metropolis(int64_t d, double T, uint32_t u_hi, uint32_t u_lo);
 
 /* Ladder and per-replica fractions exactly as the model derives them (exposed
  * so tests can check the engine's host setup). */

@@ -56,6 +56,16 @@ class DualResult(C.Structure):
     ]
 
 
+class RefParams(C.Structure):
+    _fields_ = [("base", Params), ("worker_seed", C.c_uint32), ("thread_id", C.c_int32),
+                ("rung_T", C.c_double)]
+
+
+class RefResult(C.Structure):
+    _fields_ = [("solved", C.c_int32), ("reserved", C.c_int32), ("moves", C.c_int64),
+                ("accepts", C.c_int64), ("score", C.c_int64), ("gb_score", C.c_int64)]
+
+
 LOG_DTYPE = np.dtype([("move", "<i8"), ("i", "<i2"), ("j", "<i2"), ("s", "<i2"),
                       ("flags", "<u2")])
 
@@ -142,6 +152,18 @@ def lib():
         L.orc_dual_model_run.argtypes = [I64P, DP, I64P, C.c_int32, VP, C.c_int32, VP, VP,
                                          C.c_int32, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP,
                                          C.c_int64, VP]
+        L.orc_exchange_sweep.restype = C.c_int64
+        L.orc_exchange_sweep.argtypes = [DP, DP, I32P, C.c_int32, C.c_uint64, C.c_uint64]
+        L.orc_set_mutation.restype = None
+        L.orc_set_mutation.argtypes = [C.c_int32]
+        L.orc_metropolis.restype = C.c_int32
+        L.orc_metropolis.argtypes = [C.c_int64, C.c_double, C.c_uint32, C.c_uint32]
+        L.orc_ref_worker_seed.restype = C.c_uint32
+        L.orc_ref_worker_seed.argtypes = [C.c_uint64, C.c_int32]
+        L.orc_ref_worker_run.restype = C.c_int32
+        L.orc_ref_worker_run.argtypes = [I64P, I64P, C.c_int32, VP, C.c_int32,
+                                         C.POINTER(RefParams), VP, VP, VP, VP, VP, VP,
+                                         C.c_int64, C.POINTER(RefResult)]
         L.orc_dual_residual.restype = C.c_double
         L.orc_dual_residual.argtypes = [I64P, DP, C.c_int32, C.c_int32]
         _LIB = L
@@ -329,6 +351,64 @@ def model_run(A0, X0, params, max_epochs, active=None, log_cap=0):
     if log_cap:
         out["logs"] = [logs[r * log_cap: r * log_cap + counts[r]].copy() for r in range(R)]
     return out
+
+
+def exchange_sweep(temps, scores, rungs, seed, epoch):
+    """The model's exchange sweep; returns (new rungs, swaps)."""
+    temps = np.ascontiguousarray(temps, dtype=np.float64)
+    scores = np.ascontiguousarray(scores, dtype=np.float64)
+    r = np.ascontiguousarray(rungs, dtype=np.int32).copy()
+    DP = C.POINTER(C.c_double)
+    acc = lib().orc_exchange_sweep(temps.ctypes.data_as(DP), scores.ctypes.data_as(DP),
+                                   r.ctypes.data_as(I32P), len(r), int(seed), int(epoch))
+    return r, acc
+
+
+def set_mutation(which):
+    """Plants a mistake in the model (0 = none); see oracle.h."""
+    lib().orc_set_mutation(int(which))
+
+
+def metropolis(d, T, u_hi, u_lo):
+    return lib().orc_metropolis(int(d), float(T), int(u_hi), int(u_lo))
+
+
+def ref_worker_seed(seed, t):
+    """Worker t's seed in a reference run with --seed `seed`
+    (congruence_anneal.hpp:484-488)."""
+    return lib().orc_ref_worker_seed(int(seed), int(t))
+
+
+def ref_worker_run(A0, X0, worker_seed, threads=1, thread_id=0, rung_T=0.0, active=None,
+                   log_cap=0, **kw):
+    """The model's loop body with the reference's draws and publish rule for
+    one worker (see oracle.h). kw: engine parameters (tempering, t_init, ...,
+    max_moves). Returns a dict: A, Xt (final), gbA, gbXt (global best), log,
+    solved, moves, accepts, score, gb_score."""
+    A0 = _i64(A0)
+    n = A0.shape[0]
+    Xt0 = _i64(np.asarray(X0).T)
+    rp = RefParams()
+    rp.base = make_params(**{**kw, "replicas": threads})
+    rp.worker_seed = int(worker_seed) & 0xFFFFFFFF
+    rp.thread_id = thread_id
+    rp.rung_T = rung_T
+    A = np.zeros((n, n), np.int64)
+    Xt = np.zeros((n, n), np.int64)
+    gA = np.zeros((n, n), np.int64)
+    gXt = np.zeros((n, n), np.int64)
+    log = np.zeros(max(1, log_cap), LOG_DTYPE)
+    cnt = np.zeros(1, np.int64)
+    act = np.ascontiguousarray(active, dtype=np.int32) if active is not None else None
+    res = RefResult()
+    lib().orc_ref_worker_run(_p(A0), _p(Xt0), n, act.ctypes.data if act is not None else None,
+                             len(act) if act is not None else 0, C.byref(rp), A.ctypes.data,
+                             Xt.ctypes.data, gA.ctypes.data, gXt.ctypes.data,
+                             log.ctypes.data if log_cap else None,
+                             cnt.ctypes.data if log_cap else None, log_cap, C.byref(res))
+    return dict(A=A, Xt=Xt, gbA=gA, gbXt=gXt, log=log[:cnt[0]].copy() if log_cap else None,
+                solved=res.solved, moves=res.moves, accepts=res.accepts, score=res.score,
+                gb_score=res.gb_score)
 
 
 # ---- xdual primal/dual objective (oracle_dual.cpp) ----

@@ -27,6 +27,14 @@ regenerates only that file.
 G4 (xdual): eval_add / eval_add_d, DualObjective::best_shear, invert_to and
 the off-diagonal norms (src/xdual_main.cpp:67-364, src/mat_io.hpp:297-337)
 through oracle/_ref/libref_dual.so into g4_xdual.npz (`gen_golden.py g4`).
+
+G5 (the worker loop): 64 runs of the reference's own canneal::anneal_worker
+(src/congruence_anneal.hpp:177-416) on the calling thread, recorded by the
+harness's objective wrapper (ref_worker_trace: every accepted move, sweep
+commit and reseed, the final state and the global best, capped at a move
+count), over random configurations of both temperature modes; and
+single-threaded runs of the reference binary with non-default flags that stop
+at a diagonal (deterministic), into g5_worker.npz (`gen_golden.py g5`).
 """
 import ctypes as C
 import os
@@ -552,9 +560,153 @@ def g3(L):
     np.savez_compressed(os.path.join(OUT, "g3_publish_deflate.npz"), **out)
 
 
+def g5_configs(rng, count):
+    """Random worker configurations for the trace fixtures: matrix families
+    (random symmetric, planted, bound-edge magnitudes, lattice Grams), both
+    temperature modes, every stall setting, active lists and start
+    transforms."""
+    from oracle import refharness
+    sys.path.insert(0, ROOT)
+    from xtax_amd import workloads
+    out = []
+    while len(out) < count:
+        n = int(rng.integers(2, 33))
+        fam = int(rng.integers(0, 4))
+        if fam == 0:
+            A = workloads.random_symmetric(n, -20, 20, seed=int(rng.integers(1 << 30)))
+        elif fam == 1:
+            A = workloads.planted(n, 100, 3 * n, seed=int(rng.integers(1 << 30)))[0]
+        elif fam == 2:
+            sc = int(rng.choice([3, 1000, 1 << 40, LIM - 1]))
+            M = rng.integers(-sc, sc + 1, (n, n)) * (rng.random((n, n)) < 0.7)
+            A = (np.triu(M) + np.triu(M, 1).T).astype(np.int64)
+        else:
+            A = workloads.lattice(min(n, 24), min(n, 24), -9, 9,
+                                  seed=int(rng.integers(1 << 30)))[1]
+            n = A.shape[0]
+        threads = int(rng.choice([1, 2, 4, 8, 16]))
+        kw = dict(tempering=int(rng.integers(0, 2)),
+                  greedy_fraction=float(rng.choice([0, 0.3, 0.5, 1.0])),
+                  target_fraction=float(rng.choice([0, 0.5, 1.0])),
+                  target_samples=int(rng.integers(1, 17)),
+                  worker_diversity=float(rng.choice([0, 0.4, 1.0])),
+                  sweep_fraction=float(rng.choice([0, 0.05, 0.5, 1.0])),
+                  stuck_threshold=int(rng.choice([20, 100, 1000, 20000])),
+                  t_init=float(rng.choice([0, 1, 30, 500])),
+                  t_min=float(rng.choice([1e-3, 0.5, 5])),
+                  cooling=float(rng.choice([0.999, 0.95])),
+                  moves_per_cool=int(rng.choice([1, 20, 200])),
+                  reheat=float(rng.choice([1.0, 0.5])),
+                  reseed_factor=float(rng.choice([1.0, 1.25, 3.0])),
+                  adaptive_cooling=int(rng.integers(0, 2)))
+        act = np.zeros(0, np.int32)
+        if n >= 3 and rng.random() < 0.3:
+            act = np.sort(rng.choice(n, int(rng.integers(2, n + 1)), replace=False)).astype(np.int32)
+        X0 = np.eye(n, dtype=np.int64)
+        if fam != 2 and rng.random() < 0.25:
+            for _ in range(n):
+                a, b = rng.choice(n, 2, replace=False)
+                X0[:, b] += int(rng.choice([-1, 1])) * X0[:, a]
+        A_start = (X0.T.astype(object) @ A.astype(object) @ X0.astype(object)).astype(np.int64)
+        dp, ip = refharness.pack(kw, threads, int(rng.integers(0, threads)),
+                                 int(rng.integers(0, 1 << 32)),
+                                 int(rng.choice([200, 2000, 6000])),
+                                 float(rng.choice([0.5, 5, 50, 1000])))
+        out.append((A_start, X0, act, dp, ip))
+    return out
+
+
+def g5():
+    """Worker traces of the reference's own anneal_worker (recorded by
+    oracle/_ref/libref_harness.so) and single-threaded reference runs with
+    non-default flags, for the model's reference-draw mode."""
+    from oracle import refharness
+    rng = np.random.default_rng(2025)
+    out = {}
+    for t, (A, X0, act, dp, ip) in enumerate(g5_configs(rng, 64)):
+        r = refharness.trace(A, X0, dp, ip, act)
+        out.update({f"t{t}_A": A, f"t{t}_X0": X0, f"t{t}_act": act, f"t{t}_dp": dp,
+                    f"t{t}_ip": ip, f"t{t}_Af": r["A"], f"t{t}_Xtf": r["Xt"],
+                    f"t{t}_gbA": r["gbA"], f"t{t}_gbXt": r["gbXt"],
+                    f"t{t}_log": np.stack([r["log"]["move"], r["log"]["i"], r["log"]["j"],
+                                           r["log"]["s"], r["log"]["flags"]], 1).astype(np.int64),
+                    f"t{t}_sc": np.array([r["moves"], r["score"], r["gb_score"], r["solved"]])})
+    out["traces"] = np.array(64)
+
+    # the reference binary, one thread (deterministic until it stops at a
+    # diagonal), with non-default flags
+    A10 = read_csv("/root/reference/10x10.csv")
+    P5 = planted(31, 5, 9, 10)
+    P6 = planted(32, 6, 9, 12)
+    P7 = planted(33, 7, 20, 14)
+    # a planted form on rows {0, 1, 2, 4} of order 6, rows 3 and 5 already split off
+    G = np.zeros((6, 6), np.int64)
+    sub = planted(34, 4, 9, 8)
+    idx = [0, 1, 2, 4]
+    for a in range(4):
+        for b in range(4):
+            G[idx[a], idx[b]] = sub[a, b]
+    G[3, 3], G[5, 5] = 7, -3
+    # a start transform for P6 (unimodular: a product of shears)
+    Xs = np.eye(6, dtype=np.int64)
+    for (a, b, sgn) in [(0, 3, 1), (2, 5, -1), (4, 1, 1), (3, 2, -1)]:
+        Xs[:, b] += sgn * Xs[:, a]
+    cases = [
+        ("10x10_sweep", A10, ["--seed", "21", "--sweep-fraction", "0.5", "--stuck-threshold", "300"]),
+        ("10x10_adaptive", A10, ["--seed", "22", "--adaptive-cooling", "--moves-per-cool", "50"]),
+        ("10x10_samples3", A10, ["--seed", "23", "--target-samples", "3", "--target-fraction", "1.0"]),
+        ("10x10_greedy1", A10, ["--seed", "24", "--greedy-fraction", "1.0"]),
+        ("p7_greedy0", P7, ["--seed", "25", "--greedy-fraction", "0.0", "--t-init", "10"]),
+        ("p7_reseed", P7, ["--seed", "26", "--reheat", "0.5", "--reseed-factor", "1.0",
+                           "--stuck-threshold", "200"]),
+        ("p6_cool", P6, ["--seed", "27", "--cooling", "0.99", "--t-min", "0.5"]),
+        ("p5_target0", P5, ["--seed", "28", "--target-fraction", "0.0"]),
+        ("gram_rows", G, ["--seed", "29", "--gram-rows", "0..2,4"]),
+        ("p6_start_X", P6, ["--seed", "30", "-X", "X.csv"]),
+        ("10x10_no_tempering_t1", A10, ["--seed", "31", "--no-tempering", "--t-init", "5"]),
+    ]
+    k = 0
+    for name, A0, args in cases:
+        X0 = Xs if "-X" in args else None
+        solved, moves, bA, bX = run_ref_x(A0, X0, ["-t", "1", "--max-seconds", "20"] + args)
+        print(f"G5 cli {name}: solved={solved} moves={moves}", file=sys.stderr)
+        if not solved:
+            continue
+        out.update({f"c{k}_A0": A0, f"c{k}_X0": X0 if X0 is not None else np.zeros((0, 0), np.int64),
+                    f"c{k}_A": bA, f"c{k}_X": bX, f"c{k}_moves": np.array(moves)})
+        out[f"c{k}_args"] = np.array(" ".join(args))
+        out[f"c{k}_name"] = np.array(name)
+        k += 1
+    out["cli_runs"] = np.array(k)
+    np.savez_compressed(os.path.join(OUT, "g5_worker.npz"), **out)
+
+
+def run_ref_x(A, X0, args, timeout=60):
+    """run_ref with an optional -X start transform written as X.csv."""
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "A.csv")
+        with open(path, "w") as f:
+            for row in A:
+                f.write(",".join(str(int(v)) for v in row) + "\n")
+        if X0 is not None:
+            with open(os.path.join(td, "X.csv"), "w") as f:
+                for row in X0:
+                    f.write(",".join(str(int(v)) for v in row) + "\n")
+        cp = subprocess.run([os.path.join(REF_DIR, "xtax_ref"), "-A", path] + args, cwd=td,
+                            capture_output=True, text=True, timeout=timeout)
+        moves = None
+        for line in cp.stdout.splitlines():
+            if line.startswith("done moves="):
+                moves = int(line.split()[1].split("=")[1])
+        return ("matrix is diagonal" in cp.stdout, moves, read_csv(os.path.join(td, "best_A.csv")),
+                read_csv(os.path.join(td, "best_X.csv")))
+
+
 if __name__ == "__main__":
+    if ROOT not in sys.path:
+        sys.path.insert(0, ROOT)
     L = load_harness()
-    which = sys.argv[1:] or ["g1", "g2", "g3", "g4"]
+    which = sys.argv[1:] or ["g1", "g2", "g3", "g4", "g5"]
     if "g1" in which:
         g1(L)
     if "g2" in which:
@@ -563,6 +715,8 @@ if __name__ == "__main__":
         g3(L)
     if "g4" in which:
         g4(load_dual_harness())
+    if "g5" in which:
+        g5()
     for f in sorted(os.listdir(OUT)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(OUT, f)), file=sys.stderr)
