@@ -15,10 +15,14 @@ order bucket is autotuned once before the timed steps (`--no-tune`: the rule).
 host-to-device upload is excluded and reported under `with_upload`).
 
 Prints one JSON line: exact-diagonalized matrices per second, ms_per_step,
-solve-latency percentiles, shear moves per second, the roofline of the dominant
-kernel (k_anneal) and a CPU baseline: the reference's own xtax binary
-(oracle/_ref/xtax_ref, compiled from the reference sources) on a bounded sample
-of the same instances on this host's cores.
+solve-latency percentiles (overall and per order bucket, with the solved
+fraction), shear moves per second, the plain form's matrices/s on the first
+`--plain-count` instances with the same settings (one instance per launch) and
+the batch/plain ratio, the roofline of the dominant kernel (k_anneal) and a
+CPU baseline: the reference's own xtax binary (oracle/_ref/xtax_ref, compiled
+from the reference sources) on bounded samples of the same instances on this
+host's cores, one instance at a time on every core and as concurrent
+single-thread runs, each the median of three runs.
 
 The ladders and the target fraction are per workload (--t-init, --t-min and
 --target-fraction override them), chosen by solve rate on the workloads' own
@@ -117,12 +121,94 @@ def run_ref(A, args_list, timeout, binary=REF_BIN):
     return "matrix is diagonal" in cp.stdout, moves, best
 
 
+def host_cores():
+    """Cores this process may run on (the box's CPU share, not the machine's)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_model():
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return None
+
+
+def cpu_concurrent(mats, extra, budget_s, cap_s, workers):
+    """Mode (ii): `workers` concurrent single-thread reference runs (-t 1,
+    cooling schedule, as the reference runs one worker) over the instances in
+    order, each capped at cap_s seconds, until budget_s is spent."""
+    import concurrent.futures as cf
+    t0 = time.perf_counter()
+    nxt = 0
+    solved = moves = done = 0
+
+    def one(q):
+        return run_ref(mats[q], ["-t", "1", "--seed", str(q + 1), "--max-seconds", str(cap_s)]
+                       + extra, timeout=cap_s + 30)
+
+    with cf.ThreadPoolExecutor(workers) as ex:
+        running = set()
+        while nxt < len(mats) and len(running) < workers:
+            running.add(ex.submit(one, nxt))
+            nxt += 1
+        while running:
+            fin, running = cf.wait(running, return_when=cf.FIRST_COMPLETED)
+            for f in fin:
+                ok, mv, _ = f.result()
+                done += 1
+                moves += mv
+                solved += ok
+            while (nxt < len(mats) and len(running) < workers
+                   and time.perf_counter() - t0 < budget_s):
+                running.add(ex.submit(one, nxt))
+                nxt += 1
+    el = time.perf_counter() - t0
+    return {"value": solved / el, "solved": solved, "instances": done, "seconds": el,
+            "moves_per_sec": moves / el}
+
+
+def cpu_baseline_modes(mats, extra, budget_s, cap_s, cap_single_s, repeats=3):
+    """The reference binary on this host, both ways BASELINE.md asks, each the
+    median of `repeats` runs: (i) one instance at a time with every core
+    (-t cores), (ii) `cores` concurrent single-thread instances."""
+    if not os.path.exists(REF_BIN):
+        return None
+    cores = host_cores()
+    runs_i = [cpu_sample(mats, extra, budget_s, cap_s, cores) for _ in range(repeats)]
+    runs_ii = [cpu_concurrent(mats, extra, budget_s, cap_single_s, cores)
+               for _ in range(repeats)]
+    med_i = sorted(runs_i, key=lambda r: r["value"])[repeats // 2]
+    med_ii = sorted(runs_ii, key=lambda r: r["value"])[repeats // 2]
+    best = med_i if med_i["value"] >= med_ii["value"] else med_ii
+    return {"value": best["value"], "unit": "matrices/s", "cores": cores, "kind": "reference",
+            "cpu_model": cpu_model(),
+            "sample": (f"median of {repeats} runs of {budget_s:.0f} s each way over the batch "
+                       f"instances in order: (i) xtax_ref -t {cores} {' '.join(extra)} "
+                       f"--max-seconds {cap_s} one at a time, {med_i['value']:.3f} matrices/s "
+                       f"({med_i['solved']} of {med_i['instances']}); (ii) {cores} concurrent "
+                       f"xtax_ref -t 1 --max-seconds {cap_single_s}, {med_ii['value']:.3f} "
+                       f"matrices/s ({med_ii['solved']} of {med_ii['instances']}); value = "
+                       f"the faster mode"),
+            "mode_i": {"runs": [r["value"] for r in runs_i], "median": med_i["value"],
+                       "moves_per_sec": med_i["moves_per_sec"],
+                       "p50_latency_s": med_i["p50_latency_s"]},
+            "mode_ii": {"runs": [r["value"] for r in runs_ii], "median": med_ii["value"],
+                        "moves_per_sec": med_ii["moves_per_sec"]}}
+
+
 def cpu_sample(mats, extra, budget_s, cap_s, threads=None):
     """The reference binary on the first instances, one at a time with every
     core, each capped at cap_s seconds, until budget_s is spent."""
     if not os.path.exists(REF_BIN):
         return None
-    cores = threads or os.cpu_count() or 1
+    cores = threads or host_cores()
     t0 = time.perf_counter()
     solved = moves = done = 0
     lat = []
@@ -181,15 +267,20 @@ def bench_batch(args, eng):
     lat = []
     kernel_ms = 0.0
     resident_s = 0.0  # each step less its upload of the host inputs
+    buckets = {}  # n bucket -> [instances, solved, latencies]
     t0 = time.perf_counter()
     for k in range(args.steps):
         res, bst = eng.batch_solve(mats, p, seeds=seeds)
-        for A, X, d in res:
+        for q, (A, X, d) in enumerate(res):
+            b = buckets.setdefault(bucket_of(mats[q].shape[0]), [0, 0, []])
+            b[0] += 1
             if d["solved"]:
                 solved += 1
                 if d["verify_status"] == 0:
                     verified += 1
                     lat.append(d["retire_s"] - d["submit_s"])
+                    b[1] += 1
+                    b[2].append(d["retire_s"] - d["submit_s"])
         moves += bst["moves"]
         kernel_ms += bst["kernel_ms"]
         resident_s += bst["seconds"] - bst["upload_seconds"]
@@ -216,13 +307,51 @@ def bench_batch(args, eng):
                 "p99_latency_s": pct(lat, 99), "moves_per_sec": moves / resident_s,
                 "kernel_moves_per_sec": moves / (kernel_ms / 1e3) if kernel_ms else None,
                 "roofline": roofline(kernel_ms, moves, mean_n, 0.5), "cpu_baseline": None})
+    out["by_order"] = {
+        name: {"instances": b[0], "solved_fraction": b[1] / b[0],
+               "p50_latency_s": pct(b[2], 50), "p99_latency_s": pct(b[2], 99)}
+        for name, b in sorted(buckets.items(), key=lambda kv: int(kv[0].split("-")[0]))}
+    if args.plain_count > 0:
+        # the plain form (one instance per launch, xtx_solve_plain) on the first
+        # instances of the same batch with the same settings and seeds
+        out["plain_form"] = plain_leg(eng, mats[:args.plain_count], seeds, p)
+        out["batch_over_plain"] = out["value"] / max(out["plain_form"]["value"], 1e-12)
+        log(f"plain form: {out['plain_form']['value']:.3f} matrices/s "
+            f"({out['plain_form']['solved']} of {args.plain_count}); batch/plain "
+            f"{out['batch_over_plain']:.1f}x")
     if not args.no_cpu:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_sample(
+        out["cpu_baseline"] = cpu_baseline_modes(
             mats, ["--t-init", str(args.t_init), "--t-min", str(args.t_min),
                    "--greedy-fraction", "0.5", "--target-fraction", str(args.target_fraction)],
-            args.cpu_seconds, args.cpu_cap)
+            args.cpu_seconds, args.cpu_cap, args.cpu_cap_single)
     return out
+
+
+def bucket_of(n):
+    lo = (n // 8) * 8
+    return f"{lo}-{lo + 7}"
+
+
+def plain_leg(eng, mats, seeds, p):
+    """matrices/s of the plain form: each instance alone, one per launch."""
+    solved = moves = 0
+    lat = []
+    t0 = time.perf_counter()
+    for q, A in enumerate(mats):
+        pq = EngineParams(**{**p.__dict__, "seed": seeds[q]})
+        t1 = time.perf_counter()
+        r = eng.run_annealer(A, None, pq)
+        moves += r.stats["moves"]
+        if r.stats["solved"] and r.stats["verify_status"] == 0:
+            solved += 1
+            lat.append(time.perf_counter() - t1)
+    el = time.perf_counter() - t0
+    return {"value": solved / el, "unit": "matrices/s", "instances": len(mats),
+            "solved": solved, "seconds": el, "moves_per_sec": moves / el,
+            "p50_latency_s": pct(lat, 50), "p99_latency_s": pct(lat, 99),
+            "form": "xtx_solve_plain per instance (inputs uploaded per call), same settings "
+                    "and seeds as the batch"}
 
 
 # --------------------------------------------------------------------- plain
@@ -505,8 +634,13 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--lookahead", type=int, default=0,
                     help="proposals of a replica evaluated at once (0: the engine's rule)")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-seconds", type=float, default=None,
+                    help="seconds per CPU baseline run (batch: 10, three runs each way)")
     ap.add_argument("--cpu-cap", type=float, default=None)
+    ap.add_argument("--cpu-cap-single", type=float, default=1.0,
+                    help="batch, CPU mode (ii): seconds per single-thread instance")
+    ap.add_argument("--plain-count", type=int, default=64,
+                    help="batch: instances also run one per launch (the plain form)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="batch: launch configurations by rule instead of the autotuner")
@@ -520,6 +654,8 @@ def main():
         # (replicas x cap moves = 3.2M at the defaults, about 0.05 s on 16 cores),
         # doubled; single-instance runs get a fixed 2 s
         args.cpu_cap = 0.1 if args.workload == "batch" else 2.0
+    if args.cpu_seconds is None:
+        args.cpu_seconds = 10.0 if args.workload == "batch" else 20.0
     ladder = {"batch": (18.0, 14.0), "plain": (18.0, 14.0), "serve": (50.0, 36.0),
               "large": (2000.0, 50.0)}.get(args.workload, (200.0, 5.0))
     if args.target_fraction is None:

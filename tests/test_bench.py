@@ -24,7 +24,8 @@ def test_more_than_one_gpu_is_refused():
 
 @pytest.mark.gpu
 def test_batch_line_contract():
-    r = run(["--count", "48", "--cap-epochs", "4", "--no-tune", "--cpu-seconds", "1"])
+    r = run(["--count", "48", "--cap-epochs", "4", "--no-tune", "--cpu-seconds", "1",
+             "--plain-count", "8"])
     assert r.returncode == 0, r.stderr[-2000:]
     lines = [l for l in r.stdout.splitlines() if l.strip()]
     assert len(lines) == 1
@@ -37,7 +38,12 @@ def test_batch_line_contract():
     cb = d["cpu_baseline"]
     if cb is not None:  # the reference binary is built only where its sources are
         assert {"value", "unit", "cores", "kind", "sample"} <= set(cb)
+        assert len(cb["mode_i"]["runs"]) == len(cb["mode_ii"]["runs"]) == 3
     assert d["verified"] == d["solved"]
+    pf = d["plain_form"]
+    assert pf["instances"] == 8 and d["batch_over_plain"] == pytest.approx(
+        d["value"] / max(pf["value"], 1e-12))
+    assert sum(b["instances"] for b in d["by_order"].values()) == 48
 
 
 @pytest.mark.gpu

@@ -152,6 +152,33 @@ def test_lattice_mode(tmp_path):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("spec,sel", [("0..4,6", [0, 1, 2, 3, 4, 6]), ("1,3..5", [1, 3, 4, 5])])
+def test_lattice_dims_restricts_the_metric(tmp_path, spec, sel):
+    """--lattice-dims (main.cpp:765-800): the Gram is built from the selected
+    lattice columns only; the transform acts on the full lattice, and the
+    annealed Gram equals the restricted Gram of final_L (main.cpp:849-857)."""
+    from xtax_amd import workloads
+    L = workloads.lattice(6, 8, seed=4)[0]
+    write_csv(tmp_path / "l.csv", L)
+    r = run(["-L", "l.csv", "--lattice-dims", spec, "-t", "16", "--seed", "3",
+             "--max-seconds", "2"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert f"[lattice-dims] Gram built from {len(sel)} of 8 lattice dimensions" in r.stdout
+    assert ("fewer selected dimensions" in r.stdout) == (len(sel) < 6)
+    assert "[warn]" not in r.stdout
+    Lf = read_csv(tmp_path / "final_L.csv")
+    A = read_csv(tmp_path / "best_A.csv")
+    X = read_csv(tmp_path / "best_X.csv")
+    assert Lf.shape == L.shape and (X.T @ L == Lf).all()
+    assert (Lf[:, sel] @ Lf[:, sel].T == A).all()
+    assert round(abs(np.linalg.det(X))) == 1
+    # the restricted metric is what was annealed: its score did not grow
+    G0 = L[:, sel] @ L[:, sel].T
+    score = lambda M: 2 * np.abs(M).sum() - np.abs(np.diag(M)).sum()
+    assert score(A) <= score(G0)
+
+
+@pytest.mark.gpu
 def test_gram_rows_restricts_moves(tmp_path, orc):
     A0 = sample_matrix()
     write_csv(tmp_path / "a.csv", A0)

@@ -237,6 +237,21 @@ int main(int argc, char** argv) {
             A = gram(Lat, dims);
             std::cout << "[lattice] m=" << Lat.rows << " d=" << Lat.cols
                       << " score=" << score_of(A) << "\n";
+            if (!dims.empty()) {
+                // the restricted metric's report (main.cpp:797-809)
+                std::cout << "[lattice-dims] Gram built from " << dims.size() << " of "
+                          << Lat.cols << " lattice dimensions\n";
+                if ((int)dims.size() < Lat.rows)
+                    std::cout << "[lattice-dims] note: fewer selected dimensions ("
+                              << dims.size() << ") than rows (" << Lat.rows
+                              << "), so the restricted Gram is singular\n";
+                int zero_diag = 0;
+                for (int i = 0; i < Lat.rows; ++i) zero_diag += A.at(i, i) == 0;
+                if (zero_diag > 0)
+                    std::cout << "[lattice-dims] warning: " << zero_diag
+                              << " row(s) are zero on the selected dimensions; the restricted "
+                                 "Gram is singular\n";
+            }
         }
         if (!gram_rows.empty()) {
             rows = parse_index_spec(gram_rows, A.rows, "--gram-rows");
