@@ -210,25 +210,32 @@ void xtx_clear_stop(xtx_ctx* ctx);
  * each bucket runs with one configuration: lanes per replica, columns per
  * lane, LDS layout of A, waves per workgroup and lookahead. Orders whose
  * lookahead configurations match run in one launch. The configuration changes
- * speed only: every configuration computes the same moves and results. */
+ * speed only: every configuration computes the same moves and results.
+ * lanes = 1 is the lane kernel: one lane per proposal, the whole row pair in
+ * its registers, 16 proposals per replica and step (cols = order, triangular,
+ * lookahead 16, orders 8 to 48); plain and batch calls with automatic
+ * lookahead choose it when their replicas fill the device (16 per CU). */
 typedef struct {
     int32_t order;            /* padded order: a multiple of 8 in [8, 128] */
-    int32_t lanes;            /* 16, 32 or 64 lanes per replica */
-    int32_t cols;             /* 1 or 2 columns per lane; lanes * cols >= order */
+    int32_t lanes;            /* 16, 32 or 64 lanes per replica; 1 = lane kernel */
+    int32_t cols;             /* 1 or 2 columns per lane; lanes * cols >= order
+                                 (the order itself for the lane kernel) */
     int32_t triangular;       /* 1: A as its upper triangle in LDS, 0: square */
     int32_t waves_per_block;  /* 1, 2 or 4; 0 = the params' waves_per_block */
     int32_t tuned;            /* out: 1 if set (by hand or autotuner), 0 = rule */
     int32_t lds_bytes;        /* out: LDS per replica */
     int32_t lookahead;        /* lane groups per replica (xtx_params.lookahead):
                                  0 or 1 = one; 2, 4 (one wave), 8, 16, 32 (a
-                                 workgroup; waves_per_block must be 0) */
+                                 workgroup; waves_per_block must be 0); for
+                                 the lane kernel, proposals per step: 16 */
     double moves_per_sec;     /* out of xtx_autotune: measured rate */
 } xtx_launch;
 
 /* The configuration set for order n (tuned = 1), else the one a plain or
  * batch call with automatic lookahead uses (tuned = 0). A set configuration
  * applies where its kind is chosen: a lookahead one under automatic
- * lookahead, a one-group one at lookahead 1 or a loaded server. */
+ * lookahead, a one-group one at lookahead 1 or a loaded server; a lane-kernel
+ * one the autotuner chose, only to calls whose replicas fill the device. */
 int xtx_launch_get(xtx_ctx* ctx, int32_t n, xtx_launch* out);
 /* Sets the configuration of cfg->order (lanes = 0 restores the rule). Affects
  * instances admitted afterwards. XTX_E_ARG if no kernel has that shape or its

@@ -331,6 +331,9 @@ def _configs(order):
             C = -(-order // G)
             if C <= 2:
                 yield G, C, 1, 0, K
+    if order <= 48:  # the lane kernel: one lane per proposal
+        for waves in (1, 2, 4):
+            yield 1, order, 1, waves, 16
 
 
 @pytest.mark.parametrize("n", [6, 12, 20, 33, 47, 70])

@@ -45,6 +45,11 @@ hipError_t launch_probe_rng(hipStream_t st, const uint32_t* ck, int count, uint3
 hipError_t launch_replicate(hipStream_t st, int64_t* dst, const int64_t* src, int64_t nn, int R);
 hipError_t init_kernel_attrs(int max_lds_per_block);
 hipError_t init_dual_kernel_attrs(int max_lds_per_block);
+// the lane kernel (anneal_lane.hip)
+hipError_t launch_anneal_lane(hipStream_t st, int nmax, int Q, int units_per_block,
+                              const RepRef* table, int count, Inst* insts, int rep_bytes,
+                              const volatile int* stop);
+hipError_t init_lane_kernel_attrs(int max_lds_per_block);
 // the no-alloc solve's device scratch (scratch.hip)
 hipError_t scratch_init(int device);
 void* scratch_address(int device);
@@ -433,19 +438,32 @@ namespace {
 // layout (1: the order's configuration from xtx_launch_set / xtx_autotune if
 // it has one lane group, else the default layout). Automatic: the order's
 // configuration if it has lookahead (rule above 1) or one group (rule 1),
-// else the rule's layout.
-Layout choose_layout(const xtx_ctx* ctx, int n, int requested, int rule) {
+// else the rule's layout. `lane`: the form has replicas enough to fill the
+// device with the lane kernel (one lane per proposal, four replicas per
+// wave), which then runs the orders it covers under automatic lookahead,
+// unless the order has a lookahead configuration of its own. A lane-kernel
+// configuration the autotuner chose (timed with the device full) applies only
+// to such forms; one set by hand applies wherever its kind is chosen.
+Layout choose_layout(const xtx_ctx* ctx, int n, int requested, int rule, bool lane = false) {
     const int nmax = order_of(n);
     const auto it = ctx->tuned.find(nmax);
     const bool has = it != ctx->tuned.end();
     if (requested > 1) return lookahead_layout(nmax, requested);
     const int K = requested == 1 ? 1 : rule;
-    if (has && (it->second.K > 1) == (K > 1)) return it->second;
+    if (has && (it->second.K > 1) == (K > 1) && (lane || !it->second.tuner || it->second.G > 1))
+        return it->second;
+    if (requested == 0 && lane && lane_layout_ok(nmax)) return lane_layout(nmax);
     return K > 1 ? lookahead_layout(nmax, K) : layout_for(n);
 }
 
-void apply_layout(const xtx_ctx* ctx, HostInst& h, int requested, int rule) {
-    h.lay = choose_layout(ctx, h.n, requested, rule);
+// Replicas that fill the device with the lane kernel: four waves of four
+// replicas per compute unit.
+bool lane_load(const xtx_ctx* ctx, int64_t replicas) {
+    return replicas >= (int64_t)16 * ctx->prop.multiProcessorCount;
+}
+
+void apply_layout(const xtx_ctx* ctx, HostInst& h, int requested, int rule, bool lane = false) {
+    h.lay = choose_layout(ctx, h.n, requested, rule, lane);
     if (h.lay.rep_bytes > g_lds_block)
         throw Err{XTX_E_ARG, "n too large for an LDS-resident replica"};
 }
@@ -461,6 +479,16 @@ int units_per_block_for(const Layout& L, int waves) {
     // keep a block within the LDS of one CU
     while (w > 1 && (int64_t)w * per_wave * L.rep_bytes > g_lds_block) --w;
     return w * per_wave;
+}
+
+// One launch of layout L's kernel over a replica table.
+hipError_t launch_layout(hipStream_t st, const Layout& L, int units_per_block, const RepRef* table,
+                         int count, Inst* insts, const volatile int* stop) {
+    if (L.G == 1)
+        return launch_anneal_lane(st, L.nmax, L.K, units_per_block, table, count, insts,
+                                  L.rep_bytes, stop);
+    return launch_anneal(st, L.G, L.C, L.tri, L.K, units_per_block, table, count, insts, L.nmax,
+                         L.rep_bytes, stop);
 }
 
 // Launches one epoch on ctx->stream: the bucket kernels, independent of one
@@ -484,8 +512,7 @@ void launch_epoch(xtx_ctx* ctx, std::vector<Bucket>& buckets, Inst* d_insts, int
         const Bucket& b = *order[q];
         const int lane = (int)(q % (size_t)(ns + 1));
         hipStream_t st = lane == 0 ? ctx->stream : ctx->side[lane - 1];
-        HIPCHK(launch_anneal(st, b.lay.G, b.lay.C, b.lay.tri, b.lay.K, b.units_per_block, b.d_refs,
-                             (int)b.refs.size(), d_insts, b.lay.nmax, b.lay.rep_bytes,
+        HIPCHK(launch_layout(st, b.lay, b.units_per_block, b.d_refs, (int)b.refs.size(), d_insts,
                              ctx->stop_dev));
     }
     for (int q = 0; q < ns; ++q) {
@@ -549,6 +576,7 @@ int xtx_open(int device, xtx_ctx** out) {
         g_lds_block = (int)ctx->prop.sharedMemPerBlock;
         HIPCHK(init_kernel_attrs(g_lds_block));
         HIPCHK(init_dual_kernel_attrs(g_lds_block));
+        HIPCHK(init_lane_kernel_attrs(g_lds_block));
         HIPCHK(scratch_init(device));
         HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         HIPCHK(hipEventCreate(&ctx->ev0));
@@ -746,10 +774,8 @@ int solve_plain(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t n,
             for (;;) {
                 HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
                 for (auto& b : buckets)
-                    HIPCHK(launch_anneal(ctx->stream, b.lay.G, b.lay.C, b.lay.tri, b.lay.K,
-                                         b.units_per_block,
-                                         b.d_refs, (int)b.refs.size(), d_rec,
-                                         b.lay.nmax, b.lay.rep_bytes, ctx->stop_dev));
+                    HIPCHK(launch_layout(ctx->stream, b.lay, b.units_per_block, b.d_refs,
+                                         (int)b.refs.size(), d_rec, ctx->stop_dev));
                 HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
                 HIPCHK(launch_epoch_end(ctx->stream, d_rec, 1, h.R));
                 HIPCHK(hipMemcpyAsync(pin, d_rec, sizeof(Inst), hipMemcpyDeviceToHost,
@@ -1090,7 +1116,8 @@ struct Server {
         if (cfg.max_resident > 0 && (int)(slots.size() - free_slots.size()) >= cfg.max_resident)
             return false;
         if (auto_look && p.lookahead == 0)
-            apply_layout(ctx, *h, 0, lookahead_rule(ctx, resident_reps + h->R));
+            apply_layout(ctx, *h, 0, lookahead_rule(ctx, resident_reps + h->R),
+                         lane_load(ctx, resident_reps + h->R));
         if (lds_in_use > 0 && lds_after(*h, +1) > lds_budget()) return false;
         char* dm = arena.alloc(h->bytes);
         if (!dm) return false;
@@ -1379,6 +1406,16 @@ void fill_launch(const Layout& L, bool tuned, xtx_launch* out) {
 const char* launch_invalid(int nmax, int G, int C, bool tri, int waves, int K) {
     if (nmax < 8 || nmax > kMaxN || nmax % 8) return "order must be a multiple of 8 in [8, 128]";
     if (G * C < nmax) return "lanes * cols must cover the order";
+    if (G == 1) {
+        // the lane kernel: one lane per proposal, the whole row per lane
+        if (C != nmax || !tri || !lane_variant_exists(nmax, K))
+            return "no lane kernel with this order/lookahead (lanes 1, cols = order, "
+                   "triangular, lookahead 16, orders to 48)";
+        if (waves < 0 || waves > 4 || waves == 3) return "waves_per_block must be 0, 1, 2 or 4";
+        if ((int64_t)std::max(1, waves) * (64 / K) * rep_bytes_of(nmax, G, tri, K) > g_lds_block)
+            return "replica LDS exceeds a workgroup";
+        return nullptr;
+    }
     if (!anneal_variant_exists(G, C, tri, K)) return "no kernel with these lanes/cols/layout/lookahead";
     if (waves < 0 || waves > 4 || waves == 3) return "waves_per_block must be 0, 1, 2 or 4";
     if (K > 1 && waves != 0) return "a lookahead replica takes its own wave or workgroup (waves 0)";
@@ -1407,6 +1444,9 @@ std::vector<Layout> candidates(int nmax) {
             const int C = (nmax + G - 1) / G;
             if (!launch_invalid(nmax, G, C, true, 0, K)) v.push_back(make_layout(nmax, G, C, true, 0, K));
         }
+    for (int w : {1, 2, 4})
+        if (!launch_invalid(nmax, 1, nmax, true, w, kLaneQ))
+            v.push_back(make_layout(nmax, 1, nmax, true, w, kLaneQ));
     return v;
 }
 
@@ -1449,8 +1489,8 @@ double time_layout(xtx_ctx* ctx, const std::vector<const HostInst*>& proto, cons
         HIPCHK(hipMemcpyAsync(b.d_refs, b.refs.data(), b.refs.size() * sizeof(RepRef),
                               hipMemcpyHostToDevice, ctx->stream));
         HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-        HIPCHK(launch_anneal(ctx->stream, L.G, L.C, L.tri, L.K, b.units_per_block, b.d_refs,
-                             (int)b.refs.size(), d_insts, L.nmax, L.rep_bytes, ctx->stop_dev));
+        HIPCHK(launch_layout(ctx->stream, L, b.units_per_block, b.d_refs, (int)b.refs.size(),
+                             d_insts, ctx->stop_dev));
         HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
         float ms = 0;
@@ -1509,10 +1549,11 @@ void autotune_orders(xtx_ctx* ctx, const xtx_instance* sample, int count,
         // chooses the lane shape at that depth only. Depths do not rank here
         // as they do in a run: one epoch from the start matrices commits
         // often, and lookahead makes most of its proposals again after a
-        // commit, while later epochs commit rarely.
+        // commit, while later epochs commit rarely. The lane kernel (one
+        // lane per proposal, 16 per step) competes at every order it covers.
         const int rule_K = lookahead_layout(nmax, kLookahead).K;
         for (const Layout& L : candidates(nmax)) {
-            if (L.K != rule_K) continue;
+            if (L.K != rule_K && L.G != 1) continue;
             const double ms = time_layout(ctx, proto, L, p);
             const double rate = ms > 0 ? (double)reps * moves / (ms * 1e-3) : 0.0;
             xtx_launch r;
@@ -1525,6 +1566,7 @@ void autotune_orders(xtx_ctx* ctx, const xtx_instance* sample, int count,
             }
         }
         if (best_rate > 0) {
+            best.tuner = true;
             ctx->tuned[nmax] = best;
             for (size_t q = first; q < out->size(); ++q) {
                 const xtx_launch& r = (*out)[q];
@@ -1608,7 +1650,10 @@ int xtx_batch_solve(xtx_ctx* ctx, const xtx_instance* instances, int32_t count,
             hs.back()->tag = (uint64_t)q;
             total += hs.back()->bytes;
         }
-        for (auto& h : hs) apply_layout(ctx, *h, params->lookahead, kLookahead);
+        int64_t batch_reps = 0;
+        for (auto& h : hs) batch_reps += h->R;
+        const bool lane = lane_load(ctx, batch_reps);
+        for (auto& h : hs) apply_layout(ctx, *h, params->lookahead, kLookahead, lane);
         S.arena.reserve(total + 4096);
         S.t0 = clk::now();
         std::vector<Staging> keep;

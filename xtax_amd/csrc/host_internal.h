@@ -52,14 +52,28 @@ struct Layout {
     bool tri = true;
     int waves = 0;
     int K = 1;  // lookahead: lane groups per replica (G K == 64 when K > 1)
+    // chosen by the autotuner, which times the device full of replicas: a
+    // lane-kernel choice then applies only to calls that fill the device
+    bool tuner = false;
 };
 
 inline int order_of(int n) { return std::max(8, (n + 7) / 8 * 8); }
 
-// A, then per lane group 5 staged Philox blocks and a median histogram, and the
-// active list; a replica over several waves adds two buffers of K 64-byte
-// proposals and a poll word (16-byte aligned). X^T stays in HBM.
+}  // namespace host
+// LDS of one replica of the lane kernel (anneal_lane.hip) at order nmax
+int lane_rep_bytes(int nmax);
+bool lane_variant_exists(int nmax, int Q);
+namespace host {
+
+// The lane kernel (G = 1: one lane per proposal, C = nmax columns per lane,
+// K = Q proposals per replica and step): A's upper triangle at the order, the
+// row masses and the active list.
+// Otherwise: A, then per lane group 5 staged Philox blocks and a median
+// histogram, and the active list; a replica over several waves adds two
+// buffers of K 64-byte proposals and a poll word (16-byte aligned). X^T stays
+// in HBM.
 inline int rep_bytes_of(int nmax, int G, bool tri, int K = 1) {
+    if (G == 1) return lane_rep_bytes(nmax);
     const int64_t a = tri ? (int64_t)nmax * (nmax + 1) / 2 : (int64_t)nmax * (nmax + 1);
     int64_t bytes = a * 8 + K * (5 * 16 + 4 * med_slots(G)) + nmax;
     if (G * K > 64) bytes = (bytes + 15) / 16 * 16 + 2 * K * 64 + 16;
@@ -106,6 +120,13 @@ inline Layout lookahead_layout(int nmax, int want) {
     return layout_for(nmax);
 }
 
+// The lane kernel's layout at order nmax (orders to 48): 16 proposals of a
+// replica per step, one per lane, four replicas per wave.
+constexpr int kLaneQ = 16;
+constexpr int kLaneMaxOrder = 48;
+inline Layout lane_layout(int nmax) { return make_layout(nmax, 1, nmax, true, 0, kLaneQ); }
+inline bool lane_layout_ok(int nmax) { return nmax <= kLaneMaxOrder; }
+
 // The lookahead rule. Plain and batch calls: 16 (measured: 256 replicas of
 // order 24 run 5.4x the moves/s of one group per replica, the full batch of
 // 1024 instances x 32 replicas 4.5% faster). Serve admissions, by the
@@ -130,7 +151,8 @@ using BucketKey = std::tuple<int, int, int, bool, int, int>;
 // A server's admission budget charges every replica of a launch the LDS share
 // of the launch's largest order.
 inline BucketKey launch_key(const Layout& L) {
-    return {L.K > 1 ? 0 : L.nmax, L.G, L.C, L.tri, L.waves, L.K};
+    // the lane kernel is compiled per order: one launch per order
+    return {(L.K > 1 && L.G > 1) ? 0 : L.nmax, L.G, L.C, L.tri, L.waves, L.K};
 }
 
 constexpr int kLdsPerCu = 160 * 1024;
