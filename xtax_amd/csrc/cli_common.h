@@ -11,6 +11,7 @@
 #include <filesystem>
 #include <fstream>
 #include <functional>
+#include <random>
 #include <iostream>
 #include <sstream>
 #include <stdexcept>
@@ -280,6 +281,15 @@ inline int parse_args(int argc, char** argv, const std::vector<Opt>& opts, const
         return 109;
     }
     return -1;
+}
+
+// --seed 0 (the default) seeds from std::random_device, as the reference's
+// run_annealer does (congruence_anneal.hpp:484-485); the engine's own key 0
+// is an ordinary key.
+inline void seed_from_device(xtx_params& p) {
+    if (p.seed != 0) return;
+    std::random_device rd;
+    while (p.seed == 0) p.seed = ((uint64_t)rd() << 32) | (uint64_t)rd();
 }
 
 // The engine options both drop-ins share (congruence_anneal.hpp:122-153 via

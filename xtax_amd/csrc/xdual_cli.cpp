@@ -88,6 +88,7 @@ int main(int argc, char** argv) {
         argc, argv, opts, "xdual (gfx950 engine): see the reference README for the options\n");
     if (rc_args >= 0) return rc_args;
     if (p.replicas < 1) p.replicas = 1;
+    seed_from_device(p);
 
     const bool have_A = !a_csv.empty(), have_L = !l_csv.empty();
     if (have_A == have_L) {
