@@ -18,7 +18,7 @@ Prints one JSON line: exact-diagonalized matrices per second, ms_per_step,
 solve-latency percentiles (overall and per order bucket, with the solved
 fraction), shear moves per second, the plain form's matrices/s on the first
 `--plain-count` instances with the same settings (one instance per launch) and
-the batch/plain ratio, the roofline of the dominant kernel (k_anneal) and a
+the batch/plain ratio, the roofline of the dominant kernel (k_anneal_lane) and a
 CPU baseline: the reference's own xtax binary (oracle/_ref/xtax_ref, compiled
 from the reference sources) on bounded samples of the same instances on this
 host's cores, one instance at a time on every core and as concurrent
@@ -82,8 +82,8 @@ def base_line(args, value, unit, el, config, higher=True):
             "dtype": "int64", "data": "synthetic", "config": config}
 
 
-def roofline(kernel_ms, moves, mean_n, greedy):
-    """Algorithmic integer work of k_anneal over its measured time.
+def roofline(kernel_ms, moves, mean_n, greedy, kernel="k_anneal"):
+    """Algorithmic integer work of the annealing kernel over its measured time.
 
     Per proposal the method evaluates one Add (6n int64 ops over the n
     columns: multiply, add, bound check, two absolute values, accumulate) and,
@@ -97,7 +97,7 @@ def roofline(kernel_ms, moves, mean_n, greedy):
     peak = MI355X_INT_LANE_OPS / 1e12
     return {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TOP/s",
             "frac": achieved / peak, "traffic": None,
-            "kernel": "k_anneal", "kernel_ms": kernel_ms, "moves": moves,
+            "kernel": kernel, "kernel_ms": kernel_ms, "moves": moves,
             "note": "integer VALU issue bound: no dense contraction for MFMA, and HBM is "
                     "touched only for X^T rows and epoch boundaries"}
 
@@ -266,6 +266,7 @@ def bench_batch(args, eng):
     solved = verified = moves = 0
     lat = []
     kernel_ms = 0.0
+    lane_instances = 0
     resident_s = 0.0  # each step less its upload of the host inputs
     buckets = {}  # n bucket -> [instances, solved, latencies]
     t0 = time.perf_counter()
@@ -283,6 +284,7 @@ def bench_batch(args, eng):
                     b[2].append(d["retire_s"] - d["submit_s"])
         moves += bst["moves"]
         kernel_ms += bst["kernel_ms"]
+        lane_instances += bst["lane_instances"]
         resident_s += bst["seconds"] - bst["upload_seconds"]
         log(f"step {k}: solved {bst['solved']}/{bst['instances']} verified {bst['verified']} "
             f"epochs {bst['epochs']} buckets {bst['buckets']} {bst['seconds']:.2f}s "
@@ -306,7 +308,8 @@ def bench_batch(args, eng):
                 "p50_latency_s": pct(lat, 50), "p95_latency_s": pct(lat, 95),
                 "p99_latency_s": pct(lat, 99), "moves_per_sec": moves / resident_s,
                 "kernel_moves_per_sec": moves / (kernel_ms / 1e3) if kernel_ms else None,
-                "roofline": roofline(kernel_ms, moves, mean_n, 0.5), "cpu_baseline": None})
+                "roofline": roofline(kernel_ms, moves, mean_n, 0.5, kernel_name(
+                    lane_instances, args.count * args.steps)), "cpu_baseline": None})
     out["by_order"] = {
         name: {"instances": b[0], "solved_fraction": b[1] / b[0],
                "p50_latency_s": pct(b[2], 50), "p99_latency_s": pct(b[2], 99)}
@@ -326,6 +329,16 @@ def bench_batch(args, eng):
                    "--greedy-fraction", "0.5", "--target-fraction", str(args.target_fraction)],
             args.cpu_seconds, args.cpu_cap, args.cpu_cap_single)
     return out
+
+
+def kernel_name(lane_instances, instances):
+    """The kernel family the batch ran: the lane kernel (one lane per
+    proposal) when the replicas fill the device, else the lane-group one."""
+    if lane_instances == instances:
+        return "k_anneal_lane"
+    if lane_instances == 0:
+        return "k_anneal"
+    return f"k_anneal_lane ({lane_instances} of {instances} instances) + k_anneal"
 
 
 def bucket_of(n):
@@ -383,7 +396,8 @@ def bench_plain(args, eng):
     out = base_line(args, solved / el, "matrices/s", el, cfg)
     out.update({"solved": solved, "instances": len(mats), "p50_latency_s": pct(lat, 50),
                 "p99_latency_s": pct(lat, 99), "moves_per_sec": moves / el,
-                "roofline": roofline(kernel_ms, moves, mean_n, 0.5), "cpu_baseline": None})
+                "roofline": roofline(kernel_ms, moves, mean_n, 0.5, kernel_name(
+                    lane_instances, args.count * args.steps)), "cpu_baseline": None})
     return out
 
 
@@ -471,10 +485,43 @@ def bench_large(args, eng):
                      t_min=args.t_min, exchange_interval=args.exchange_interval,
                      max_seconds=args.seconds, seed=args.seed, verify=True,
                      lookahead=args.lookahead)
-    return bench_single(args, eng, "large: planted n=80, D=+-U{1..1000}, 4n shears, 4096 "
-                        "replicas, greedy 0.7", A0, p,
-                        ["--greedy-fraction", "0.7", "--t-init", str(args.t_init),
-                         "--t-min", str(args.t_min)])
+    name = "large: planted n=80, D=+-U{1..1000}, 4n shears, 4096 replicas, greedy 0.7"
+    extra = ["--greedy-fraction", "0.7", "--t-init", str(args.t_init), "--t-min", str(args.t_min)]
+    if (args.count or 1) <= 1:
+        return bench_single(args, eng, name, A0, p, extra)
+    return large_to_diagonal(args, eng, name, p)
+
+
+def large_to_diagonal(args, eng, name, p):
+    """Time to an exact diagonal on `--count` planted n = 80 instances
+    (planted seeds 5, 6, ...), each run until a replica reaches a diagonal or
+    `--seconds` pass; every solved result verified exactly."""
+    runs = []
+    moves = 0
+    el_total = 0.0
+    for q in range(args.count):
+        A0 = workloads.planted(80, 1000, 320, seed=5 + q)[0]
+        t0 = time.perf_counter()
+        r = eng.run_annealer(A0, None, p)
+        el = time.perf_counter() - t0
+        el_total += el
+        moves += r.stats["moves"]
+        ok = bool(r.stats["solved"]) and r.stats["verify_status"] == 0
+        start = int(2 * np.abs(A0).sum() - np.abs(np.diag(A0)).sum())
+        runs.append({"planted_seed": 5 + q, "solved": ok, "seconds": el,
+                     "best_score": r.stats["best_score"], "start_score": start,
+                     "moves": r.stats["moves"]})
+        log(f"large instance {q}: solved {ok} in {el:.1f} s, best {r.stats['best_score']} "
+            f"from {start}, {r.stats['moves'] / el:.3e} moves/s")
+    solved = [x for x in runs if x["solved"]]
+    cfg = {"workload": name + f", {args.count} instances, cap {args.seconds:.0f} s each",
+           "n": 80, "replicas": p.replicas, "instances": args.count}
+    out = base_line(args, moves / el_total, "moves/s", el_total, cfg)
+    out.update({"solved": len(solved), "solved_fraction": len(solved) / args.count,
+                "time_to_diagonal_s": [x["seconds"] for x in solved],
+                "p50_time_to_diagonal_s": pct([x["seconds"] for x in solved], 50),
+                "cap_seconds": args.seconds, "runs": runs, "cpu_baseline": None})
+    return out
 
 
 def bench_lattice(args, eng):

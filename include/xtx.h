@@ -174,6 +174,7 @@ typedef struct {
     double upload_seconds;    /* admission: host inputs to device-resident
                                  state; seconds - upload_seconds is the run
                                  with inputs resident in HBM */
+    int32_t lane_instances;   /* instances run by the lane kernel (lanes = 1) */
 } xtx_batch_stats;
 
 typedef struct {

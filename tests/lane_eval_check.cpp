@@ -1,9 +1,13 @@
 // lane_eval_check.cpp -- the one-lane proposal arithmetic of
 // xtax_amd/csrc/lane_eval.h (host build) against the oracle's restatement of
 // the reference's best_shear_l1 and try_add (oracle/oracle.cpp, pinned by the
-// golden vectors), on seeded random matrices: small entries (the int32 path),
-// entries up to and beyond the 2^48 bound (the int64 path), sparse rows, every
-// order to 64 and every ordered pair (i, j) of each matrix.
+// golden vectors), on seeded random matrices with entries below 2^16 (the
+// rows the lane arithmetic takes): dense and sparse rows, entries up to
+// 2^16 - 1, small entries under a heavy diagonal or beside a small one, every
+// order to 48 padded to its kernel order (columns past n are zero, as in the
+// kernel's LDS) and at 64, every ordered pair (i, j) of each
+// matrix, the best shear and the score change of it, of +-1, of the cap and of
+// a random shear.
 #include <cstdint>
 #include <cstdio>
 #include <random>
@@ -30,80 +34,68 @@ template <int NMAX>
 void check_pair(const std::vector<int64_t>& A, int n, int i, int j, std::mt19937_64& rng) {
     const int64_t* ri = A.data() + (size_t)i * n;
     const int64_t* rj = A.data() + (size_t)j * n;
-    const int64_t a_ii = ri[i], a_ji = rj[i], a_jj = rj[j];
-    auto row = [&](int k, int64_t& ai, int64_t& aj) {
-        ai = ri[k];
-        aj = rj[k];
-    };
-    const int64_t want = orc_best_shear_l1(A.data(), n, i, j);
-    int64_t boff = 0;
-    const int64_t got_big = shear_big(row, n, j, a_ii, a_ji, a_jj, &boff);
-    expect(got_big == want, "int64 best shear", n, i, j, want);
-
-    bool small = true;
-    for (int k = 0; k < n; ++k)
-        small &= abs64(ri[k]) < kSmallEntry && abs64(rj[k]) < kSmallEntry;
-    int32_t p[NMAX], q[NMAX];
-    int32_t sumq = 0;
+    Pair<NMAX> r;
     for (int k = 0; k < NMAX; ++k) {
-        p[k] = q[k] = 0;
-        if (k < n && k != j && small) {
-            p[k] = (int32_t)abs64(ri[k]);
-            q[k] = (int32_t)(ri[k] >= 0 ? rj[k] : -rj[k]);
-            sumq += abs32(q[k]);
+        const int32_t ai = k < n ? (int32_t)ri[k] : 0, aj = k < n ? (int32_t)rj[k] : 0;
+        to_pq(ai, aj, r.p[k], r.qb[k]);
+    }
+    to_pq((int32_t)ri[j], (int32_t)rj[j], r.pj, r.qbj);
+    r.a_ii = (int32_t)ri[i];
+    r.a_ji = (int32_t)rj[i];
+    r.a_jj = (int32_t)rj[j];
+    int32_t tot = 0, sumq = 0;
+    totals(r, tot, sumq);
+    int64_t want_tot = 0, want_sumq = 0;
+    for (int k = 0; k < n; ++k)
+        if (k != j) {
+            want_tot += ri[k] < 0 ? -ri[k] : ri[k];
+            want_sumq += rj[k] < 0 ? -rj[k] : rj[k];
         }
-    }
-    int32_t soff = 0;
-    if (small) {
-        const int64_t got_small = shear_small<NMAX>(p, q, sumq, (int32_t)a_ii, (int32_t)a_ji,
-                                                    (int32_t)a_jj, &soff);
-        expect(got_small == want, "int32 best shear", n, i, j, want);
-    }
-    // evaluations: the best shear (with its known off sum), +-1, the cap, a random s
+    expect(tot == want_tot && sumq == want_sumq, "row totals", n, i, j, 0);
+
+    const int64_t want = orc_best_shear_l1(A.data(), n, i, j);
+    int32_t off = 0;
+    const int32_t got = best_shear(r, tot, sumq, &off);
+    expect(got == want, "best shear", n, i, j, want);
+    // score changes: the best shear (through its off sum), +-1, the cap, a random s
     std::uniform_int_distribution<int> rs(-kCap, kCap);
-    const int64_t ss[6] = {want, 1, -1, kCap, -kCap, (int64_t)rs(rng)};
+    const int32_t ss[6] = {(int32_t)want, 1, -1, kCap, -kCap, rs(rng)};
     for (int t = 0; t < 6; ++t) {
-        const int64_t s = ss[t];
+        const int32_t s = ss[t];
         if (s == 0) continue;
         std::vector<int64_t> nrow((size_t)n);
         int64_t d = 0, dnz = 0;
         const bool ok = orc_try_add(A.data(), nullptr, n, i, j, s, nrow.data(), nullptr, &d, &dnz);
-        int64_t bd = 0, bo = 0;
-        int32_t bz = 0;
-        const bool bok = eval_big(row, n, j, s, a_ii, a_ji, a_jj, &bd, &bz, &bo);
-        expect(bok == ok, "int64 feasibility", n, i, j, s);
-        if (ok) expect(bd == d && bz == dnz, "int64 delta / nonzero change", n, i, j, s);
-        if (t == 0 && ok && want != 0) expect(bo == boff, "int64 off sum of the best", n, i, j, s);
-        if (small) {
-            expect(ok, "small rows stay in bound", n, i, j, s);
-            int64_t sd = 0;
-            int32_t sz = 0, so = 0;
-            const bool know = t == 0 && want != 0;
-            eval_small<NMAX>(p, q, sumq, (int32_t)s, (int32_t)a_ii, (int32_t)a_ji, (int32_t)a_jj,
-                             know, soff, &sd, &sz, &so);
-            expect(sd == d && sz == dnz, "int32 delta / nonzero change", n, i, j, s);
-        }
+        expect(ok, "small rows stay in bound", n, i, j, s);
+        const int32_t o = t == 0 ? off : abs_sum(r, s) - sumq;
+        expect(delta_of(r, s, o) == d, "score change", n, i, j, s);
     }
 }
 
 template <int NMAX>
 void run_order(int n, std::mt19937_64& rng, int mats) {
     for (int m = 0; m < mats; ++m) {
-        const int fam = m % 6;
+        const int fam = mats > 6 ? 6 : m % 6;
         int64_t scale = 20;
         double density = 1.0;
         if (fam == 1) scale = 3000;
         if (fam == 2) scale = kSmallEntry - 1;
-        if (fam == 3) scale = (int64_t)1 << 30;
-        if (fam == 4) scale = kLim + 1;  // some entries past the bound
-        if (fam == 5) density = 0.2;
+        if (fam == 3) scale = 1;
+        if (fam == 4) density = 0.2;
+        if (fam == 5) scale = 3;
+        if (fam == 6) scale = 6;
         std::uniform_int_distribution<int64_t> d(-scale, scale);
+        std::uniform_int_distribution<int64_t> dd(fam == 6 ? -3 : -200, fam == 6 ? 3 : 200);
         std::uniform_real_distribution<double> u(0.0, 1.0);
         std::vector<int64_t> A((size_t)n * n, 0);
         for (int i = 0; i < n; ++i)
             for (int j = i; j < n; ++j) {
                 int64_t v = u(rng) < density ? d(rng) : 0;
-                if (fam == 4 && u(rng) < 0.5) v = (v % 1000);
+                // small off-diagonal entries under a heavy diagonal (5), or
+                // entries to 6 under a diagonal to 3 (6): integer medians far
+                // from -round(a_ji / a_ii), where the floor of the median is a
+                // candidate of its own and wins now and then
+                if (fam >= 5 && i == j) v = dd(rng);
                 A[(size_t)i * n + j] = A[(size_t)j * n + i] = v;
             }
         for (int i = 0; i < n; ++i)
@@ -115,10 +107,14 @@ void run_order(int n, std::mt19937_64& rng, int mats) {
 
 int main() {
     std::mt19937_64 rng(2026);
-    for (int n = 2; n <= 16; ++n) run_order<16>(n, rng, 6);
-    for (int n = 17; n <= 32; n += 3) run_order<32>(n, rng, 6);
-    for (int n = 33; n <= 48; n += 5) run_order<48>(n, rng, 6);
+    for (int n = 2; n <= 8; ++n) run_order<8>(n, rng, 6);
+    for (int n = 9; n <= 16; ++n) run_order<16>(n, rng, 6);
+    for (int n = 17; n <= 24; n += 2) run_order<24>(n, rng, 6);
+    for (int n = 25; n <= 32; n += 3) run_order<32>(n, rng, 6);
+    for (int n = 33; n <= 40; n += 3) run_order<40>(n, rng, 6);
+    for (int n = 41; n <= 48; n += 3) run_order<48>(n, rng, 6);
     run_order<64>(64, rng, 6);
+    for (int n = 3; n <= 8; ++n) run_order<8>(n, rng, 4000);
     std::printf("lane_eval_check: %ld checks, %d failures\n", checks, failures);
     return failures ? 1 : 0;
 }

@@ -75,7 +75,8 @@ class BatchStats(C.Structure):
     _fields_ = [("moves", C.c_int64), ("accepts", C.c_int64), ("instances", C.c_int32),
                 ("solved", C.c_int32), ("verified", C.c_int32), ("epochs", C.c_int32),
                 ("seconds", C.c_double), ("kernel_ms", C.c_double), ("buckets", C.c_int32),
-                ("graph_epochs", C.c_int32), ("upload_seconds", C.c_double)]
+                ("graph_epochs", C.c_int32), ("upload_seconds", C.c_double),
+                ("lane_instances", C.c_int32)]
 
 
 class ServeConfig(C.Structure):

@@ -3,7 +3,7 @@
 // A replica takes Q lanes (Q = 16: four replicas per wave). Each step, lane q
 // of the replica makes and evaluates the replica's move m0 + q on its current
 // state, serially over the columns -- the pair tournaments, the exact best
-// shear and the evaluation (lane_eval.h), the Metropolis test -- with no
+// shear and the score change (lane_eval.h), the Metropolis test -- with no
 // reduction across lanes: at the batch's orders a proposal is a few thousand
 // integer instructions of one lane instead of a chain of group reductions.
 // The moves are then resolved in order exactly as the reference's worker loop
@@ -16,140 +16,44 @@
 // temperature, of the sequential loop: the trajectory is the one-group
 // kernel's and the model's, bit for bit.
 //
-// LDS per replica: A as its upper triangle laid out by the kernel's order NMAX
-// (row r holds columns r .. NMAX-1, A[a][b] at T(min) + max with
-// T(r) = r (2 NMAX - r - 1) / 2), the row masses (NMAX int64) and the active
-// list (NMAX bytes). X^T stays in HBM, read and written on a commit only, in
-// columns owned by fixed lanes (lane q of the replica owns columns q, q + Q,
-// ...), so every lane reads back only global data it wrote itself.
+// State: the replica's A stays exact (int64) in HBM, and its low 32 bits sit
+// in LDS as the square NMAX x NMAX (row r at r NMAX, zero past n), with the
+// row masses (int64), the number of entries of each row at or above 2^16
+// (diagonal included) and the active list. A lane reads a row pair of small
+// entries from LDS, four columns per 16-byte read, with no address arithmetic
+// per column; a proposal on a row with an entry of 2^16 or more (never seen
+// on the batch's instances) is evaluated by the whole group from HBM
+// (group_eval.h), and so is every pair of a stall sweep. In HBM, entry (r, c)
+// of A and of X^T is read and written by lane c % Q of the replica only (a
+// commit's column-j writes go through that lane), so every lane reads back
+// only global data it wrote itself.
 #include <hip/hip_runtime.h>
 
 #include "device_common.h"
 #include "engine_types.h"
+#include "group_eval.h"
 #include "lane_eval.h"
 
 namespace xtx {
 
+// LDS byte offsets of one replica at kernel order NMAX.
 template <int NMAX>
-__device__ __forceinline__ constexpr int ltri(int r) {
-    return r * (2 * NMAX - r - 1) / 2;
-}
-
-// The replica's LDS: A (upper triangle at order NMAX), row masses, active list.
-template <int NMAX, int Q>
-struct LaneRep {
-    int64_t* sA;
-    int64_t* sM;
-    uint8_t* sAct;
-    int n, N, qd;
-    bool ident;
-
-    __device__ __forceinline__ int at(int a, int b) const {
-        return a <= b ? ltri<NMAX>(a) + b : ltri<NMAX>(b) + a;
-    }
-    __device__ __forceinline__ int64_t A(int a, int b) const { return sA[at(a, b)]; }
-    __device__ __forceinline__ int act(uint32_t p) const { return ident ? (int)p : (int)sAct[p]; }
-
-    // rows i and j as the small path's p, q (lane_eval.h); false when some
-    // entry of the two rows reaches 2^16 (p, q are then not used)
-    __device__ __forceinline__ bool load_pq(int i, int j, int32_t (&p)[NMAX], int32_t (&q)[NMAX],
-                                            int32_t& sumq) const {
-        const int Ti = ltri<NMAX>(i), Tj = ltri<NMAX>(j);
-        bool big = false;
-        sumq = 0;
-#pragma unroll
-        for (int k = 0; k < NMAX; ++k) {
-            // eight columns' loads in flight at a time: all 2 NMAX 64-bit loads
-            // at once would hold 4 NMAX registers
-            if (k % 8 == 0) XTX_PHASE();
-            int64_t ai = 0, aj = 0;
-            if (k < n) {
-                ai = sA[k >= i ? Ti + k : ltri<NMAX>(k) + i];
-                aj = sA[k >= j ? Tj + k : ltri<NMAX>(k) + j];
-            }
-            big |= ai >= lane::kSmallEntry || ai <= -lane::kSmallEntry ||
-                   aj >= lane::kSmallEntry || aj <= -lane::kSmallEntry;
-            const bool use = k < n && k != j;
-            p[k] = use ? (int32_t)(ai < 0 ? -ai : ai) : 0;
-            q[k] = use ? (int32_t)(ai >= 0 ? aj : -aj) : 0;
-            sumq += q[k] < 0 ? -q[k] : q[k];
-        }
-        XTX_PHASE();
-        return !big;
-    }
-
-    // group-parallel (lane qd: columns qd, qd + Q, ...) moves between HBM
-    // (row-major n x n) and the LDS layout
-    __device__ __forceinline__ void load_A(const GLOBAL int64_t* src) {
-        for (int row = 0; row < n; ++row)
-            for (int k = row + qd; k < n; k += Q) sA[ltri<NMAX>(row) + k] = src[row * n + k];
-    }
-    __device__ __forceinline__ void store_A(GLOBAL int64_t* dst) const {
-        for (int row = 0; row < n; ++row)
-            for (int k = qd; k < n; k += Q) dst[row * n + k] = A(row, k);
-    }
-    __device__ __forceinline__ void copy_cols(GLOBAL int64_t* dst, const GLOBAL int64_t* src) const {
-        for (int row = 0; row < n; ++row)
-            for (int k = qd; k < n; k += Q) dst[row * n + k] = src[row * n + k];
-    }
-    __device__ __forceinline__ void row_masses() {
-        for (int k = qd; k < n; k += Q) {
-            int64_t m = 0;
-            for (int t = 0; t < n; ++t)
-                if (t != k) m += lane::abs64(A(k, t));
-            sM[k] = m;
-        }
-    }
+struct LaneLds {
+    static constexpr int kSquare = 0;              // int32 [NMAX][NMAX]
+    static constexpr int kMass = 4 * NMAX * NMAX;  // int64 [NMAX]
+    static constexpr int kBig = kMass + 8 * NMAX;  // int32 [NMAX]
+    static constexpr int kAct = kBig + 4 * NMAX;   // uint8 [NMAX]
+    static constexpr int kBytes = (kAct + NMAX + 15) / 16 * 16;
 };
 
-// A proposal's evaluation: the shear (the greedy best, or the given +-1 when
-// that is 0 or not asked for), its score change, nonzero change and
-// off-diagonal change; ok = false when it leaves the bound.
+// A proposal's outcome: the shear (the greedy best, or the given +-1 when that
+// is 0 or not asked for), its score change and off-diagonal change; ok = false
+// when it leaves the bound.
 struct EvalOut {
-    int64_t s, delta, off;
-    int32_t dnz;
+    int32_t s;
+    int64_t delta, off;
     bool ok;
 };
-
-// The pair's evaluation by one lane, small or int64 path, called where it is
-// rare (rows with an entry of 2^16 or more; the stall sweep), so its code and
-// registers stay out of the move loop.
-template <int NMAX, int Q>
-__device__ __noinline__ EvalOut eval_pair(const int64_t* sA, int n, int i, int j, bool greedy,
-                                          int64_t pm) {
-    LaneRep<NMAX, Q> rp;
-    rp.sA = (int64_t*)sA;
-    rp.n = n;
-    EvalOut o;
-    int32_t p[NMAX], q[NMAX], sumq;
-    const int64_t a_ii = rp.A(i, i), a_ji = rp.A(j, i), a_jj = rp.A(j, j);
-    if (rp.load_pq(i, j, p, q, sumq)) {
-        int32_t soff = 0, o32 = 0;
-        o.s = 0;
-        if (greedy)
-            o.s = lane::shear_small<NMAX>(p, q, sumq, (int32_t)a_ii, (int32_t)a_ji, (int32_t)a_jj,
-                                          &soff);
-        const bool know = o.s != 0;
-        if (o.s == 0) o.s = pm;
-        lane::eval_small<NMAX>(p, q, sumq, (int32_t)o.s, (int32_t)a_ii, (int32_t)a_ji,
-                               (int32_t)a_jj, know, soff, &o.delta, &o.dnz, &o32);
-        o.off = o32;
-        o.ok = true;
-        return o;
-    }
-    auto row = [&](int k, int64_t& ai, int64_t& aj) {
-        ai = rp.A(i, k);
-        aj = rp.A(j, k);
-    };
-    o.s = 0;
-    if (greedy) {
-        int64_t boff = 0;
-        o.s = lane::shear_big(row, n, j, a_ii, a_ji, a_jj, &boff);
-    }
-    if (o.s == 0) o.s = pm;
-    o.ok = lane::eval_big(row, n, j, o.s, a_ii, a_ji, a_jj, &o.delta, &o.dnz, &o.off);
-    return o;
-}
 
 template <int Q>
 __device__ __forceinline__ uint32_t group_bits(bool pred, int gbase) {
@@ -168,10 +72,83 @@ __device__ __forceinline__ int64_t from_lane64(int64_t v, int src) {
     return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
 }
 
-// Waves per SIMD the kernel is compiled for. A lane's proposal holds both rows
-// in registers (2 NMAX) besides the replica's state, so above order 16 one
-// wave per SIMD keeps everything in registers (the LDS of four replicas per
-// wave would allow two at order 32: 4.5 KB per replica, one at 48: 9.9 KB).
+// The replica: its LDS and HBM state and the lane's place in its group.
+template <int NMAX, int Q>
+struct LaneRep {
+    static constexpr int CQ = (NMAX + Q - 1) / Q;  // columns per lane: qd + c Q
+    int32_t* sS;
+    int64_t* sM;
+    int32_t* sBig;
+    uint8_t* sAct;
+    GLOBAL int64_t* gA;  // A, exact, row-major n x n
+    int n, N, qd, gbase;
+    bool ident;
+
+    __device__ __forceinline__ int act(uint32_t p) const { return ident ? (int)p : (int)sAct[p]; }
+    __device__ __forceinline__ int32_t s32(int a, int b) const { return sS[a * NMAX + b]; }
+    __device__ __forceinline__ bool small(int r) const { return sBig[r] == 0; }
+
+    // LDS from gA: the low words (zero past n), row masses and big counts;
+    // lane qd reads columns qd + c Q. Group-uniform.
+    __device__ void load_lds() {
+        for (int r = 0; r < NMAX; ++r) {
+            int64_t mass = 0;
+            int32_t big = 0;
+#pragma unroll
+            for (int c = 0; c < CQ; ++c) {
+                const int k = qd + c * Q;
+                const int64_t v = (r < n && k < n) ? gA[r * n + k] : 0;
+                if (k < NMAX) sS[r * NMAX + k] = (int32_t)v;
+                mass += k != r ? iabs64(v) : 0;
+                big += iabs64(v) >= lane::kSmallEntry;
+            }
+            mass = Grp<Q>::sum(mass);
+            big = Grp<Q>::sum32(big);
+            if (qd == 0) {
+                sM[r] = mass;
+                sBig[r] = big;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+
+    // rows i, j of small entries for one lane (lane_eval.h)
+    __device__ __forceinline__ void pair_small(int i, int j, lane::Pair<NMAX>& pr) const {
+        const int4* ri = (const int4*)(sS + i * NMAX);
+        const int4* rj = (const int4*)(sS + j * NMAX);
+#pragma unroll
+        for (int v = 0; v < NMAX / 4; ++v) {
+            const int4 a = ri[v], b = rj[v];
+            lane::to_pq(a.x, b.x, pr.p[4 * v], pr.qb[4 * v]);
+            lane::to_pq(a.y, b.y, pr.p[4 * v + 1], pr.qb[4 * v + 1]);
+            lane::to_pq(a.z, b.z, pr.p[4 * v + 2], pr.qb[4 * v + 2]);
+            lane::to_pq(a.w, b.w, pr.p[4 * v + 3], pr.qb[4 * v + 3]);
+        }
+        pr.a_ii = s32(i, i);
+        pr.a_ji = s32(j, i);
+        pr.a_jj = s32(j, j);
+        lane::to_pq(pr.a_ji, pr.a_jj, pr.pj, pr.qbj);
+    }
+
+    // entry (r, c) of gA, read by its lane and given to the whole group
+    __device__ __forceinline__ int64_t entry64(int r, int c) const {
+        const int64_t v = qd == (c & (Q - 1)) ? gA[r * n + c] : 0;
+        return from_lane64(v, gbase + (c & (Q - 1)));
+    }
+    // the lane's columns of rows i, j of gA (0 past n)
+    __device__ __forceinline__ void rows64(int i, int j, int64_t (&ai)[CQ], int64_t (&aj)[CQ]) const {
+#pragma unroll
+        for (int c = 0; c < CQ; ++c) {
+            const int k = qd + c * Q;
+            ai[c] = k < n ? gA[i * n + k] : 0;
+            aj[c] = k < n ? gA[j * n + k] : 0;
+        }
+    }
+};
+
+// Waves per SIMD the kernel is compiled for: two where both waves' registers
+// fit without spilling.
 constexpr int lane_waves_per_simd(int nmax) { return nmax <= 16 ? 2 : 1; }
 
 // One epoch of many replicas, Q lanes (proposals) each.
@@ -179,7 +156,10 @@ template <int NMAX, int Q>
 __global__ void __launch_bounds__(256, lane_waves_per_simd(NMAX))
     k_anneal_lane(const RepRef* __restrict__ table, int count, Inst* __restrict__ insts,
                   int nmax, int rep_bytes, const volatile int* __restrict__ stop) {
-    static_assert(Q == 8 || Q == 16 || Q == 32, "8, 16 or 32 lanes per replica");
+    static_assert(Q == 16, "16 lanes per replica");
+    static_assert(NMAX % 8 == 0, "orders are multiples of 8");
+    using L = LaneLds<NMAX>;
+    constexpr int CQ = LaneRep<NMAX, Q>::CQ;
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = (int)(threadIdx.x & 63u);
     const int qd = (int)(threadIdx.x % (unsigned)Q);
@@ -198,19 +178,20 @@ __global__ void __launch_bounds__(256, lane_waves_per_simd(NMAX))
     GLOBAL RepState* S = (GLOBAL RepState*)I->rs + r;
     if (S->flags & RF_STOPPED) return;
 
-    constexpr int kAbytes = NMAX * (NMAX + 1) / 2 * 8;
+    unsigned char* base = smem + (size_t)slot * rep_bytes;
     LaneRep<NMAX, Q> rp;
-    rp.sA = (int64_t*)(smem + (size_t)slot * rep_bytes);
-    rp.sM = (int64_t*)(smem + (size_t)slot * rep_bytes + kAbytes);
-    rp.sAct = (uint8_t*)(rp.sM + NMAX);
+    rp.sS = (int32_t*)(base + L::kSquare);
+    rp.sM = (int64_t*)(base + L::kMass);
+    rp.sBig = (int32_t*)(base + L::kBig);
+    rp.sAct = (uint8_t*)(base + L::kAct);
     rp.n = I->n;
     rp.N = I->N;
     rp.qd = qd;
+    rp.gbase = gbase;
     const int n = rp.n, N = rp.N;
     const int64_t nn = (int64_t)n * n;
-    GLOBAL int64_t* const gAcur = (GLOBAL int64_t*)I->A + (int64_t)r * nn;
+    rp.gA = (GLOBAL int64_t*)I->A + (int64_t)r * nn;
     GLOBAL int64_t* const gX = (GLOBAL int64_t*)I->Xt + (int64_t)r * nn;
-    rp.load_A(gAcur);
     const GLOBAL uint8_t* gact = (const GLOBAL uint8_t*)I->act;
     bool nonid = false;
     for (int t = qd; t < N; t += Q) {
@@ -219,11 +200,7 @@ __global__ void __launch_bounds__(256, lane_waves_per_simd(NMAX))
         nonid |= a != t;
     }
     rp.ident = N == n && __ballot(nonid) == 0ull;  // the wave's instance: uniform
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    rp.row_masses();
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    rp.load_lds();
 
     const uint32_t k0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)I->key0);
     const uint32_t k1 = (uint32_t)__builtin_amdgcn_readfirstlane((int)I->key1);
@@ -253,8 +230,13 @@ __global__ void __launch_bounds__(256, lane_waves_per_simd(NMAX))
             const Inst* Iq = I;
             if (best_score <= Iq->gb_score) {
                 const int64_t off = (int64_t)r * nn;
-                rp.store_A((GLOBAL int64_t*)Iq->bA + off);
-                rp.copy_cols((GLOBAL int64_t*)Iq->bXt + off, gX);
+                GLOBAL int64_t* bA = (GLOBAL int64_t*)Iq->bA + off;
+                GLOBAL int64_t* bX = (GLOBAL int64_t*)Iq->bXt + off;
+                for (int row = 0; row < n; ++row)
+                    for (int k = qd; k < n; k += Q) {
+                        bA[row * n + k] = rp.gA[row * n + k];
+                        bX[row * n + k] = gX[row * n + k];
+                    }
                 flags &= ~RF_BEST_STALE;
             } else {
                 flags |= RF_BEST_STALE;
@@ -289,77 +271,115 @@ __global__ void __launch_bounds__(256, lane_waves_per_simd(NMAX))
     // group's columns; true when every new X^T entry is in bound
     auto xt_fits = [&](int i, int j, int64_t s) -> bool {
         bool bad = false;
-        for (int k = qd; k < n; k += Q) bad |= lane::over(gX[j * n + k] + s * gX[i * n + k]);
+        for (int k = qd; k < n; k += Q) bad |= over(gX[j * n + k] + s * gX[i * n + k]);
         return group_bits<Q>(bad, gbase) == 0u;
     };
-    // commit_add (main.cpp:112-133) of Add(i, j, s), group-parallel: row j of
-    // A (one LDS entry per column), the row masses, row j of X^T; `off` is
-    // the off-diagonal change (row j's new mass is its old one plus it)
-    auto commit = [&](int i, int j, int64_t s, int64_t off) {
-        const int64_t a_ii = rp.A(i, i), a_ji = rp.A(j, i), a_jj = rp.A(j, j);
-        int64_t nv[(NMAX + Q - 1) / Q], dm[(NMAX + Q - 1) / Q];
+    // commit_add (main.cpp:112-133) of Add(i, j, s), group-parallel: row and
+    // column j of A (LDS low words, HBM), the row masses and big counts, row j
+    // of X^T; `off` is the off-diagonal change (row j's new mass is its old one
+    // plus it). Returns the change in off-diagonal nonzeros.
+    auto commit = [&](int i, int j, int64_t s, int64_t off) -> int32_t {
+        const bool sm = rp.small(i) && rp.small(j);
+        int64_t a_ii, a_ji, a_jj;
+        if (sm) {
+            a_ii = rp.s32(i, i);
+            a_ji = rp.s32(j, i);
+            a_jj = rp.s32(j, j);
+        } else {
+            a_ii = rp.entry64(i, i);
+            a_ji = rp.entry64(j, i);
+            a_jj = rp.entry64(j, j);
+        }
+        int64_t nv[CQ], dm[CQ];
+        int32_t db[CQ];
+        int32_t dnz = 0, dbj = 0;
 #pragma unroll
-        for (int c = 0; c < (NMAX + Q - 1) / Q; ++c) {
+        for (int c = 0; c < CQ; ++c) {
             const int k = qd + c * Q;
             nv[c] = 0;
             dm[c] = 0;
+            db[c] = 0;
             if (k < n) {
-                const int64_t aj = rp.A(j, k);
-                nv[c] = k == j ? a_jj + 2 * s * a_ji + s * s * a_ii : aj + s * rp.A(i, k);
-                dm[c] = k == j ? 0 : lane::abs64(nv[c]) - lane::abs64(aj);
+                const int64_t aj = sm ? (int64_t)rp.s32(j, k) : rp.gA[j * n + k];
+                const int64_t ai = sm ? (int64_t)rp.s32(i, k) : rp.gA[i * n + k];
+                nv[c] = k == j ? a_jj + 2 * s * a_ji + s * s * a_ii : aj + s * ai;
+                db[c] = (int32_t)(iabs64(nv[c]) >= lane::kSmallEntry) -
+                        (int32_t)(iabs64(aj) >= lane::kSmallEntry);
+                dbj += db[c];
+                if (k != j) {
+                    dnz += (int32_t)(nv[c] != 0) - (int32_t)(aj != 0);
+                    dm[c] = iabs64(nv[c]) - iabs64(aj);
+                }
             }
         }
         __builtin_amdgcn_wave_barrier();  // every lane has read what changes
 #pragma unroll
-        for (int c = 0; c < (NMAX + Q - 1) / Q; ++c) {
+        for (int c = 0; c < CQ; ++c) {
             const int k = qd + c * Q;
             if (k < n) {
-                rp.sA[rp.at(j, k)] = nv[c];
-                rp.sM[k] += k == j ? off : dm[c];
+                rp.sS[j * NMAX + k] = (int32_t)nv[c];
+                rp.gA[j * n + k] = nv[c];
                 gX[j * n + k] = gX[j * n + k] + s * gX[i * n + k];
+                if (k != j) {
+                    rp.sS[k * NMAX + j] = (int32_t)nv[c];
+                    rp.sM[k] += dm[c];
+                    rp.sBig[k] += db[c];
+                }
             }
+        }
+        // column j of gA through its lane
+        const int jl = j & (Q - 1);
+#pragma unroll
+        for (int c = 0; c < CQ; ++c)
+            for (int qq = 0; qq < Q; ++qq) {
+                const int k = qq + c * Q;
+                if (k < n && k != j) {
+                    const int64_t v = from_lane64(nv[c], gbase + qq);
+                    if (qd == jl) rp.gA[k * n + j] = v;
+                }
+            }
+        dbj = Grp<Q>::sum32(dbj);
+        if (qd == jl) {
+            rp.sM[j] += off;
+            rp.sBig[j] += dbj;
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        return Grp<Q>::sum32(dnz);
     };
-    // evaluation of a move by this lane alone: the small path inline, the
-    // int64 path (some entry of the rows at 2^16 or more) through eval_pair
-    auto evaluate = [&](int i, int j, bool greedy, int64_t pm) -> EvalOut {
-        int32_t p[NMAX], q[NMAX], sumq;
-        if (rp.load_pq(i, j, p, q, sumq)) {
-            const int64_t a_ii = rp.A(i, i), a_ji = rp.A(j, i), a_jj = rp.A(j, j);
-            EvalOut o;
-            int32_t soff = 0, o32 = 0;
-            o.s = 0;
-            if (greedy)
-                o.s = lane::shear_small<NMAX>(p, q, sumq, (int32_t)a_ii, (int32_t)a_ji,
-                                              (int32_t)a_jj, &soff);
-            const bool know = o.s != 0;
-            if (o.s == 0) o.s = pm;
-            lane::eval_small<NMAX>(p, q, sumq, (int32_t)o.s, (int32_t)a_ii, (int32_t)a_ji,
-                                   (int32_t)a_jj, know, soff, &o.delta, &o.dnz, &o32);
-            o.off = o32;
-            o.ok = true;
-            return o;
-        }
-        return eval_pair<NMAX, Q>(rp.sA, n, i, j, greedy, pm);
+    // a pair evaluated by the whole group from gA (exact int64, bound-checked)
+    auto eval_group = [&](int i, int j, bool greedy, int32_t pm) -> EvalOut {
+        int64_t ai[CQ], aj[CQ];
+        rp.rows64(i, j, ai, aj);
+        const int64_t a_ii = rp.entry64(i, i), a_ji = rp.entry64(j, i), a_jj = rp.entry64(j, j);
+        int64_t s = 0, off = 0;
+        if (greedy) s = best_shear<Q, CQ>(ai, aj, qd, n, j, a_ii, a_ji, a_jj, &off);
+        const bool know = s != 0;
+        if (s == 0) s = pm;
+        MoveEval<CQ> e;
+        evaluate<Q, CQ>(ai, aj, qd, n, j, s, a_ii, a_ji, a_jj, e, know, off);
+        EvalOut o;
+        o.s = (int32_t)s;
+        o.delta = e.delta;
+        o.off = e.off;
+        o.ok = !e.bad;
+        return o;
     };
     // greedy reduction sweep from a pivot (congruence_anneal.hpp:246-260);
-    // every lane of the group computes the same values
+    // group-uniform
     auto sweep = [&](int pv, uint64_t m) -> bool {
         const int64_t before = score;
         ++sweeps;
         for (int t = 0; t < N; ++t) {
             const int jj = rp.act((uint32_t)t);
             if (jj == pv) continue;
-            const EvalOut e = eval_pair<NMAX, Q>(rp.sA, n, pv, jj, true, 0);
+            const EvalOut e = eval_group(pv, jj, true, 0);
             ++greedy_n;
             if (e.s == 0 || !e.ok || e.delta > 0) continue;
             if (!xt_fits(pv, jj, e.s)) continue;
             leave_best();
-            commit(pv, jj, e.s, e.off);
+            offnz += commit(pv, jj, e.s, e.off);
             score += e.delta;
-            offnz += e.dnz;
             ++accepts;
             log_move(m, pv, jj, e.s, 1);
         }
@@ -408,14 +428,15 @@ __global__ void __launch_bounds__(256, lane_waves_per_simd(NMAX))
                 if ((double)score > (double)gb_epoch * Iv->P.reseed_factor) {
                     leave_best();
                     log_move(m, -1, -1, 0, 2);
+                    const GLOBAL int64_t* srcA = (const GLOBAL int64_t*)Iv->gA;
+                    const GLOBAL int64_t* srcX = (const GLOBAL int64_t*)Iv->gXt;
+                    for (int row = 0; row < n; ++row)
+                        for (int k = qd; k < n; k += Q) {
+                            rp.gA[row * n + k] = srcA[row * n + k];
+                            gX[row * n + k] = srcX[row * n + k];
+                        }
                     __builtin_amdgcn_wave_barrier();
-                    rp.load_A((const GLOBAL int64_t*)Iv->gA);
-                    rp.copy_cols(gX, (const GLOBAL int64_t*)Iv->gXt);
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                    rp.row_masses();
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                    rp.load_lds();
                     score = gb_epoch;
                     offnz = Iv->gb_offnz;
                     track_best(m);
@@ -471,18 +492,22 @@ __global__ void __launch_bounds__(256, lane_waves_per_simd(NMAX))
         const uint64_t mg = m0 + (uint64_t)qd;
         const uint64_t qq = (mg - 1) >> 2;
         const uint32_t c3 = 16u * (uint32_t)((mg - 1) & 3u);
+        const int kn = (stop_at - moves) < (uint64_t)Q ? (int)(stop_at - moves) : Q;
+        const uint32_t live = kn >= 32 ? ~0u : ((1u << kn) - 1u);
         // the key in scalar registers, fresh every step (not carried across it)
         uint32_t key0 = k0, key1 = k1;
         opaque_u32(key0);
         opaque_u32(key1);
         // Philox slots 0..4 (the coins, sign, Metropolis uniform, uniform pair,
-        // fallback pivot, stall words) and the tournament draws of slots < TS
-        uint32_t wz[5], ww[5], ys[kMaxSamples];
+        // fallback pivot, stall words) and the tournament draws of slots < TS:
+        // the hot-row tournament as the draws come, the pivot tournament's
+        // positions kept a byte each
+        uint32_t wz[5], ww[5], pk[kMaxSamples / 4];
         uint32_t tpos = 0;
         int64_t tw = -1;
-#pragma unroll 16
+#pragma unroll
         for (int c = 0; c < kMaxSamples; ++c) {
-            ys[c] = 0u;
+            if (c % 4 == 0) pk[c / 4] = 0u;
             if (c < 5 || c < TS) {
                 const uint4 W = philox10((uint32_t)qq, (uint32_t)(qq >> 32), (uint32_t)r,
                                          c3 + (uint32_t)c, key0, key1);
@@ -490,7 +515,6 @@ __global__ void __launch_bounds__(256, lane_waves_per_simd(NMAX))
                     wz[c] = W.z;
                     ww[c] = W.w;
                 }
-                ys[c] = W.y;
                 if (c < TS) {
                     // hot-row tournament: first maximum of the row masses
                     const uint32_t pos = mulhi32(W.x, (uint32_t)N);
@@ -499,49 +523,110 @@ __global__ void __launch_bounds__(256, lane_waves_per_simd(NMAX))
                         tw = w;
                         tpos = pos;
                     }
+                    pk[c / 4] |= mulhi32(W.y, (uint32_t)N) << (8 * (c % 4));
                 }
             }
         }
-        int i, j;
-        if (target_coin(wz[0])) {
-            // pivot tournament: first maximum of |A[target][c]| over draws c != target
-            const int target = rp.act(tpos);
-            int ibest = -1;
-            int64_t vbest = -1;
-#pragma unroll 16
-            for (int c = 0; c < kMaxSamples; ++c) {
-                if (c < TS) {
-                    const int cand = rp.act(mulhi32(ys[c], (uint32_t)N));
-                    if (cand != target) {
-                        const int64_t v = lane::abs64(rp.A(target, cand));
-                        if (v > vbest) {
-                            vbest = v;
-                            ibest = cand;
+        const bool targeted = target_coin(wz[0]);
+        int i = -1, j;
+        bool defer = false;  // rows with an entry of 2^16 or more: the group evaluates
+        if (targeted) {
+            j = rp.act(tpos);
+            if (rp.small(j)) {
+                // pivot tournament: first maximum of |A[j][c]| over draws c != j
+                int32_t vbest = -1;
+#pragma unroll
+                for (int c = 0; c < kMaxSamples; ++c) {
+                    if (c < TS) {
+                        const int cand = rp.act((pk[c / 4] >> (8 * (c % 4))) & 0xffu);
+                        if (cand != j) {
+                            const int32_t v = lane::abs32(rp.s32(j, cand));
+                            if (v > vbest) {
+                                vbest = v;
+                                i = cand;
+                            }
                         }
                     }
                 }
+                if (i < 0) i = rp.act((tpos + 1u + mulhi32(ww[3], (uint32_t)(N - 1))) % (uint32_t)N);
+                defer = !rp.small(i);
+            } else {
+                defer = true;
             }
-            if (ibest < 0)
-                ibest = rp.act((tpos + 1u + mulhi32(ww[3], (uint32_t)(N - 1))) % (uint32_t)N);
-            i = ibest;
-            j = target;
         } else {
             const uint32_t ip = mulhi32(ww[2], (uint32_t)N);
             const uint32_t jp = (ip + 1u + mulhi32(wz[3], (uint32_t)(N - 1))) % (uint32_t)N;
             i = rp.act(ip);
             j = rp.act(jp);
+            defer = !rp.small(i) || !rp.small(j);
         }
         const bool gr = greedy_coin(ww[0]);
-        const EvalOut ev = evaluate(i, j, gr, (wz[1] & 1u) ? 1 : -1);
-        const int64_t s = ev.s, delta = ev.delta, off = ev.off;
-        const int32_t dnz = ev.dnz;
+        const int32_t pm = (wz[1] & 1u) ? 1 : -1;
+        EvalOut ev;
+        ev.s = 0;
+        ev.delta = 0;
+        ev.off = 0;
+        ev.ok = true;
+        if (!defer) {
+            lane::Pair<NMAX> pr;
+            rp.pair_small(i, j, pr);
+            int32_t tot, sumq, off = 0, s = 0;
+            lane::totals(pr, tot, sumq);
+            if (gr) s = lane::best_shear(pr, tot, sumq, &off);
+            if (s == 0) {
+                s = pm;
+                off = lane::abs_sum(pr, s) - sumq;
+            }
+            ev.s = s;
+            ev.off = off;
+            ev.delta = lane::delta_of(pr, s, off);
+        }
+        // the deferred proposals of the group, one at a time, by the group
+        uint32_t dmask = group_bits<Q>(defer, gbase) & live;
+        while (dmask) {
+            const int b = __builtin_ctz(dmask);
+            dmask &= dmask - 1u;
+            const int src = gbase + b;
+            const int bj = from_lane32(j, src);
+            int bi = from_lane32(i, src);
+            if (bi < 0) {
+                // the pivot tournament on a row with large entries
+                int64_t vbest = -1;
+#pragma unroll
+                for (int c = 0; c < kMaxSamples; ++c) {
+                    if (c < TS) {
+                        const uint32_t w = (uint32_t)from_lane32((int32_t)pk[c / 4], src);
+                        const int cand = rp.act((w >> (8 * (c % 4))) & 0xffu);
+                        if (cand != bj) {
+                            const int64_t v = iabs64(rp.entry64(bj, cand));
+                            if (v > vbest) {
+                                vbest = v;
+                                bi = cand;
+                            }
+                        }
+                    }
+                }
+                if (bi < 0) {
+                    const uint32_t btp = (uint32_t)from_lane32((int32_t)tpos, src);
+                    const uint32_t bw3 = (uint32_t)from_lane32((int32_t)ww[3], src);
+                    bi = rp.act((btp + 1u + mulhi32(bw3, (uint32_t)(N - 1))) % (uint32_t)N);
+                }
+            }
+            const bool bgr = from_lane32(gr ? 1 : 0, src) != 0;
+            const int32_t bpm = from_lane32(pm, src);
+            const EvalOut e = eval_group(bi, bj, bgr, bpm);
+            if (qd == b) {
+                i = bi;
+                ev = e;
+            }
+        }
+        const int32_t s = ev.s;
+        const int64_t delta = ev.delta, off = ev.off;
         const bool bad = !ev.ok;
         const double u = u53(ww[1], wz[2]);
         bool pass = !bad && (delta <= 0 || u < xexp((double)delta * nbeta));
 
         // ---- resolve moves m0 .. m0 + kn - 1 in order (per replica) ----
-        const int kn = (stop_at - moves) < (uint64_t)Q ? (int)(stop_at - moves) : Q;
-        const uint32_t live = kn >= 32 ? ~0u : ((1u << kn) - 1u);
         uint32_t pmask = group_bits<Q>(pass, gbase) & live;
         const uint32_t grm = group_bits<Q>(gr, gbase) & live;
         const uint32_t badm = group_bits<Q>(bad, gbase) & live;
@@ -579,9 +664,8 @@ __global__ void __launch_bounds__(256, lane_waves_per_simd(NMAX))
                 ++infeasible;
             } else if (pos == kp) {
                 const int ik = from_lane32(i, src), jk = from_lane32(j, src);
-                const int64_t sk = from_lane64(s, src), dk = from_lane64(delta, src);
-                const int64_t ok = from_lane64(off, src);
-                const int32_t zk = from_lane32(dnz, src);
+                const int32_t sk = from_lane32(s, src);
+                const int64_t dk = from_lane64(delta, src), ok = from_lane64(off, src);
                 if (xt_fits(ik, jk, sk)) {
                     const int64_t ns = score + dk;
                     if (ns < best_score) {
@@ -592,9 +676,8 @@ __global__ void __launch_bounds__(256, lane_waves_per_simd(NMAX))
                     } else {
                         leave_best();
                     }
-                    commit(ik, jk, sk, ok);
+                    offnz += commit(ik, jk, sk, ok);
                     score = ns;
-                    offnz += zk;
                     ++accepts;
                     log_move(m, ik, jk, sk, 0);
                     res = dk < 0 ? 1 : 0;
@@ -621,8 +704,6 @@ __global__ void __launch_bounds__(256, lane_waves_per_simd(NMAX))
     // the move cap ends the loop through stop_at
     if (!(flags & RF_STOPPED) && !halt && cap != 0 && moves >= cap) flags |= RF_STOPPED | RF_CAPPED;
 
-    __builtin_amdgcn_wave_barrier();
-    rp.store_A(gAcur);
     if (qd == 0) {
         S->moves = moves;
         S->score = score;
@@ -672,9 +753,9 @@ static const LaneVariant* find_lane_variant(int nmax, int Q) {
 
 bool lane_variant_exists(int nmax, int Q) { return find_lane_variant(nmax, Q) != nullptr; }
 
-// LDS of one replica of the lane kernel at order nmax
+// LDS of one replica of the lane kernel at order nmax (LaneLds)
 int lane_rep_bytes(int nmax) {
-    const int b = nmax * (nmax + 1) / 2 * 8 + nmax * 8 + nmax;
+    const int b = 4 * nmax * nmax + 8 * nmax + 4 * nmax + nmax;
     return (b + 15) / 16 * 16;
 }
 

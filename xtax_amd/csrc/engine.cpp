@@ -1653,7 +1653,11 @@ int xtx_batch_solve(xtx_ctx* ctx, const xtx_instance* instances, int32_t count,
         int64_t batch_reps = 0;
         for (auto& h : hs) batch_reps += h->R;
         const bool lane = lane_load(ctx, batch_reps);
-        for (auto& h : hs) apply_layout(ctx, *h, params->lookahead, kLookahead, lane);
+        int lane_instances = 0;
+        for (auto& h : hs) {
+            apply_layout(ctx, *h, params->lookahead, kLookahead, lane);
+            lane_instances += h->lay.G == 1;
+        }
         S.arena.reserve(total + 4096);
         S.t0 = clk::now();
         std::vector<Staging> keep;
@@ -1726,6 +1730,7 @@ int xtx_batch_solve(xtx_ctx* ctx, const xtx_instance* instances, int32_t count,
             bs.buckets = (int)S.buckets.size();
             bs.graph_epochs = k;
             bs.upload_seconds = upload_s;
+            bs.lane_instances = lane_instances;
             *stats = bs;
         }
         return XTX_OK;
