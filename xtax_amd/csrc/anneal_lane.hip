@@ -76,10 +76,10 @@ __device__ __forceinline__ int64_t from_lane64(int64_t v, int src) {
 template <int NMAX, int Q>
 struct LaneRep {
     static constexpr int CQ = (NMAX + Q - 1) / Q;  // columns per lane: qd + c Q
-    int32_t* sS;
-    int64_t* sM;
-    int32_t* sBig;
-    uint8_t* sAct;
+    SHARED int32_t* sS;
+    SHARED int64_t* sM;
+    SHARED int32_t* sBig;
+    SHARED uint8_t* sAct;
     GLOBAL int64_t* gA;  // A, exact, row-major n x n
     int n, N, qd, gbase;
     bool ident;
@@ -90,7 +90,7 @@ struct LaneRep {
 
     // LDS from gA: the low words (zero past n), row masses and big counts;
     // lane qd reads columns qd + c Q. Group-uniform.
-    __device__ void load_lds() {
+    __device__ __forceinline__ void load_lds() {
         for (int r = 0; r < NMAX; ++r) {
             int64_t mass = 0;
             int32_t big = 0;
@@ -115,11 +115,12 @@ struct LaneRep {
 
     // rows i, j of small entries for one lane (lane_eval.h)
     __device__ __forceinline__ void pair_small(int i, int j, lane::Pair<NMAX>& pr) const {
-        const int4* ri = (const int4*)(sS + i * NMAX);
-        const int4* rj = (const int4*)(sS + j * NMAX);
+        typedef int32_t v4i __attribute__((ext_vector_type(4)));
+        const SHARED v4i* ri = (const SHARED v4i*)(sS + i * NMAX);
+        const SHARED v4i* rj = (const SHARED v4i*)(sS + j * NMAX);
 #pragma unroll
         for (int v = 0; v < NMAX / 4; ++v) {
-            const int4 a = ri[v], b = rj[v];
+            const v4i a = ri[v], b = rj[v];
             lane::to_pq(a.x, b.x, pr.p[4 * v], pr.qb[4 * v]);
             lane::to_pq(a.y, b.y, pr.p[4 * v + 1], pr.qb[4 * v + 1]);
             lane::to_pq(a.z, b.z, pr.p[4 * v + 2], pr.qb[4 * v + 2]);
@@ -136,16 +137,42 @@ struct LaneRep {
         const int64_t v = qd == (c & (Q - 1)) ? gA[r * n + c] : 0;
         return from_lane64(v, gbase + (c & (Q - 1)));
     }
-    // the lane's columns of rows i, j of gA (0 past n)
-    __device__ __forceinline__ void rows64(int i, int j, int64_t (&ai)[CQ], int64_t (&aj)[CQ]) const {
-#pragma unroll
-        for (int c = 0; c < CQ; ++c) {
-            const int k = qd + c * Q;
-            ai[c] = k < n ? gA[i * n + k] : 0;
-            aj[c] = k < n ? gA[j * n + k] : 0;
-        }
-    }
 };
+
+// A pair evaluated by the lane group (lane qd: columns qd + c Q) from the
+// replica's A in HBM, exact int64 and bound-checked (group_eval.h): proposals
+// on rows with an entry of 2^16 or more, and the pairs of a stall sweep. Inline
+// with no call inside: a call would make the kernel keep its live values in
+// memory or in the registers the calling convention saves. Group-uniform.
+template <int NMAX, int Q>
+__device__ __forceinline__ EvalOut eval_group(const GLOBAL int64_t* gA, int n, int qd, int gbase,
+                                           int i, int j, bool greedy, int32_t pm) {
+    constexpr int CQ = LaneRep<NMAX, Q>::CQ;
+    int64_t ai[CQ], aj[CQ];
+#pragma unroll
+    for (int c = 0; c < CQ; ++c) {
+        const int k = qd + c * Q;
+        ai[c] = k < n ? gA[i * n + k] : 0;
+        aj[c] = k < n ? gA[j * n + k] : 0;
+    }
+    auto entry = [&](int r, int c) -> int64_t {
+        const int64_t v = qd == (c & (Q - 1)) ? gA[r * n + c] : 0;
+        return from_lane64(v, gbase + (c & (Q - 1)));
+    };
+    const int64_t a_ii = entry(i, i), a_ji = entry(j, i), a_jj = entry(j, j);
+    int64_t s = 0, off = 0;
+    if (greedy) s = best_shear_inl<Q, CQ>(ai, aj, qd, n, j, a_ii, a_ji, a_jj, &off);
+    const bool know = s != 0;
+    if (s == 0) s = pm;
+    MoveEval<CQ> e;
+    evaluate<Q, CQ>(ai, aj, qd, n, j, s, a_ii, a_ji, a_jj, e, know, off);
+    EvalOut o;
+    o.s = (int32_t)s;
+    o.delta = e.delta;
+    o.off = e.off;
+    o.ok = !e.bad;
+    return o;
+}
 
 // Waves per SIMD the kernel is compiled for: two where both waves' registers
 // fit without spilling.
@@ -171,19 +198,19 @@ __global__ void __launch_bounds__(256, lane_waves_per_simd(NMAX))
     // instance's replica list to whole waves)
     const int wave_gid0 = __builtin_amdgcn_readfirstlane((int)blockIdx.x * ((int)blockDim.x / Q) +
                                                          (int)(threadIdx.x >> 6) * (64 / Q));
-    Inst* I = insts + __builtin_amdgcn_readfirstlane(table[wave_gid0].inst);
+    GLOBAL Inst* I = (GLOBAL Inst*)insts + __builtin_amdgcn_readfirstlane(table[wave_gid0].inst);
     if (I->status != ST_RUNNING) return;
     const int r = table[gid].rep;
     if (r < 0) return;  // padding
     GLOBAL RepState* S = (GLOBAL RepState*)I->rs + r;
     if (S->flags & RF_STOPPED) return;
 
-    unsigned char* base = smem + (size_t)slot * rep_bytes;
+    SHARED unsigned char* base = (SHARED unsigned char*)smem + (size_t)slot * rep_bytes;
     LaneRep<NMAX, Q> rp;
-    rp.sS = (int32_t*)(base + L::kSquare);
-    rp.sM = (int64_t*)(base + L::kMass);
-    rp.sBig = (int32_t*)(base + L::kBig);
-    rp.sAct = (uint8_t*)(base + L::kAct);
+    rp.sS = (SHARED int32_t*)(base + L::kSquare);
+    rp.sM = (SHARED int64_t*)(base + L::kMass);
+    rp.sBig = (SHARED int32_t*)(base + L::kBig);
+    rp.sAct = (SHARED uint8_t*)(base + L::kAct);
     rp.n = I->n;
     rp.N = I->N;
     rp.qd = qd;
@@ -225,9 +252,9 @@ __global__ void __launch_bounds__(256, lane_waves_per_simd(NMAX))
     const Coin greedy_coin(S->gf), target_coin(S->tf);
     uint32_t accepts = 0, greedy_n = 0, infeasible = 0, sweeps = 0, logged = 0;
 
-    auto leave_best = [&]() {
+    auto leave_best = [&]() __attribute__((always_inline)) {
         if (flags & RF_CUR_BEST) {
-            const Inst* Iq = I;
+            const GLOBAL Inst* Iq = I;
             if (best_score <= Iq->gb_score) {
                 const int64_t off = (int64_t)r * nn;
                 GLOBAL int64_t* bA = (GLOBAL int64_t*)Iq->bA + off;
@@ -244,7 +271,7 @@ __global__ void __launch_bounds__(256, lane_waves_per_simd(NMAX))
             flags &= ~RF_CUR_BEST;
         }
     };
-    auto track_best = [&](uint64_t m) {
+    auto track_best = [&](uint64_t m) __attribute__((always_inline)) {
         if (score < best_score) {
             best_score = score;
             best_move = (int64_t)m;
@@ -252,9 +279,10 @@ __global__ void __launch_bounds__(256, lane_waves_per_simd(NMAX))
             flags &= ~RF_BEST_STALE;
         }
     };
-    auto log_move = [&](uint64_t m, int i, int j, int64_t s, uint16_t f) {
+    auto log_move = [&](uint64_t m, int i, int j, int64_t s, uint16_t f)
+                        __attribute__((always_inline)) {
         if (logging) {
-            const Inst* Iq = I;
+            const GLOBAL Inst* Iq = I;
             const int64_t at = opaque_v(S)->log_count + logged;
             if (at < Iq->log_cap && qd == 0) {
                 GLOBAL LogEnt* e = (GLOBAL LogEnt*)Iq->log + (int64_t)r * Iq->log_cap + at;
@@ -269,7 +297,7 @@ __global__ void __launch_bounds__(256, lane_waves_per_simd(NMAX))
     };
     // add_transform_row's bound (main.cpp:97-108) for Add(i, j, s) over the
     // group's columns; true when every new X^T entry is in bound
-    auto xt_fits = [&](int i, int j, int64_t s) -> bool {
+    auto xt_fits = [&](int i, int j, int64_t s) __attribute__((always_inline)) -> bool {
         bool bad = false;
         for (int k = qd; k < n; k += Q) bad |= over(gX[j * n + k] + s * gX[i * n + k]);
         return group_bits<Q>(bad, gbase) == 0u;
@@ -278,7 +306,8 @@ __global__ void __launch_bounds__(256, lane_waves_per_simd(NMAX))
     // column j of A (LDS low words, HBM), the row masses and big counts, row j
     // of X^T; `off` is the off-diagonal change (row j's new mass is its old one
     // plus it). Returns the change in off-diagonal nonzeros.
-    auto commit = [&](int i, int j, int64_t s, int64_t off) -> int32_t {
+    auto commit = [&](int i, int j, int64_t s, int64_t off)
+                      __attribute__((always_inline)) -> int32_t {
         const bool sm = rp.small(i) && rp.small(j);
         int64_t a_ii, a_ji, a_jj;
         if (sm) {
@@ -347,33 +376,15 @@ __global__ void __launch_bounds__(256, lane_waves_per_simd(NMAX))
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         return Grp<Q>::sum32(dnz);
     };
-    // a pair evaluated by the whole group from gA (exact int64, bound-checked)
-    auto eval_group = [&](int i, int j, bool greedy, int32_t pm) -> EvalOut {
-        int64_t ai[CQ], aj[CQ];
-        rp.rows64(i, j, ai, aj);
-        const int64_t a_ii = rp.entry64(i, i), a_ji = rp.entry64(j, i), a_jj = rp.entry64(j, j);
-        int64_t s = 0, off = 0;
-        if (greedy) s = best_shear<Q, CQ>(ai, aj, qd, n, j, a_ii, a_ji, a_jj, &off);
-        const bool know = s != 0;
-        if (s == 0) s = pm;
-        MoveEval<CQ> e;
-        evaluate<Q, CQ>(ai, aj, qd, n, j, s, a_ii, a_ji, a_jj, e, know, off);
-        EvalOut o;
-        o.s = (int32_t)s;
-        o.delta = e.delta;
-        o.off = e.off;
-        o.ok = !e.bad;
-        return o;
-    };
     // greedy reduction sweep from a pivot (congruence_anneal.hpp:246-260);
     // group-uniform
-    auto sweep = [&](int pv, uint64_t m) -> bool {
+    auto sweep = [&](int pv, uint64_t m) __attribute__((always_inline)) -> bool {
         const int64_t before = score;
         ++sweeps;
         for (int t = 0; t < N; ++t) {
             const int jj = rp.act((uint32_t)t);
             if (jj == pv) continue;
-            const EvalOut e = eval_group(pv, jj, true, 0);
+            const EvalOut e = eval_group<NMAX, Q>(rp.gA, n, qd, gbase, pv, jj, true, 0);
             ++greedy_n;
             if (e.s == 0 || !e.ok || e.delta > 0) continue;
             if (!xt_fits(pv, jj, e.s)) continue;
@@ -387,7 +398,8 @@ __global__ void __launch_bounds__(256, lane_waves_per_simd(NMAX))
     };
     // bookkeeping after move m and the stall handling that follows it
     // (congruence_anneal.hpp:298-412); true when the replica is solved
-    auto finish = [&](uint64_t m, int res, uint32_t w4z, uint32_t w4w, bool& changed) -> bool {
+    auto finish = [&](uint64_t m, int res, uint32_t w4z, uint32_t w4w, bool& changed)
+                      __attribute__((always_inline)) -> bool {
         ++win_p;
         if (res >= 0) ++win_a;
         if (score < local_best) {
@@ -403,7 +415,7 @@ __global__ void __launch_bounds__(256, lane_waves_per_simd(NMAX))
         const uint32_t accepts0 = accepts;
         if (tempering) {
             if ((int64_t)msi > stuck) {
-                const InstParams& P = I->P;
+                const auto& P = I->P;
                 if (P.sweep_fraction > 0.0 && u01(w4z) < P.sweep_fraction) {
                     sweep(rp.act(mulhi32(w4w, (uint32_t)N)), m);
                     track_best(m);
@@ -413,7 +425,7 @@ __global__ void __launch_bounds__(256, lane_waves_per_simd(NMAX))
             }
         } else {
             if ((int64_t)msi > stuck) {
-                const Inst* Iq = I;
+                const GLOBAL Inst* Iq = I;
                 if (Iq->P.sweep_fraction > 0.0 && u01(w4z) < Iq->P.sweep_fraction) {
                     if (sweep(rp.act(mulhi32(w4w, (uint32_t)N)), m)) {
                         local_best = score;
@@ -421,7 +433,7 @@ __global__ void __launch_bounds__(256, lane_waves_per_simd(NMAX))
                     }
                     track_best(m);
                 }
-                const Inst* Iv = I;
+                const GLOBAL Inst* Iv = I;
                 T = Iv->P.reheat * Iv->P.t_init;
                 nbeta = -1.0 / T;
                 const int64_t gb_epoch = Iv->gb_score;
@@ -446,7 +458,7 @@ __global__ void __launch_bounds__(256, lane_waves_per_simd(NMAX))
                 msi = 0;
                 cool_counter = 0;
             } else if (++cool_counter >= per_cool) {
-                const InstParams& P = I->P;
+                const auto& P = I->P;
                 cool_counter = 0;
                 if (P.adaptive && win_p > 0) {
                     const double acc = (double)win_a / (double)win_p;
@@ -614,7 +626,7 @@ __global__ void __launch_bounds__(256, lane_waves_per_simd(NMAX))
             }
             const bool bgr = from_lane32(gr ? 1 : 0, src) != 0;
             const int32_t bpm = from_lane32(pm, src);
-            const EvalOut e = eval_group(bi, bj, bgr, bpm);
+            const EvalOut e = eval_group<NMAX, Q>(rp.gA, n, qd, gbase, bi, bj, bgr, bpm);
             if (qd == b) {
                 i = bi;
                 ev = e;
@@ -725,7 +737,8 @@ __global__ void __launch_bounds__(256, lane_waves_per_simd(NMAX))
         S->log_count += logged;
         if (flags & RF_SOLVED) S->solve_move = (int64_t)moves;
         if (flags & RF_SOLVED)
-            atomicMin(&I->solve_key, ((unsigned long long)moves << kRepBits) | (unsigned)r);
+            atomicMin((unsigned long long*)&I->solve_key,
+                      ((unsigned long long)moves << kRepBits) | (unsigned)r);
     }
 }
 

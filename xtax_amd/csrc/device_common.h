@@ -15,6 +15,10 @@ namespace xtx {
 // them become global_* instructions (vector-memory counter only) instead of
 // flat_* ones, which would also hold up every LDS wait.
 #define GLOBAL __attribute__((address_space(1)))
+// LDS pointers the same way: ds_* instructions even where the pointer passes
+// through a struct or a call (a generic pointer there becomes flat_*, which
+// waits on both counters and takes the vector-memory path)
+#define SHARED __attribute__((address_space(3)))
 
 // ---------------------------------------------------------------------------
 // building blocks

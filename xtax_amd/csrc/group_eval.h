@@ -107,9 +107,12 @@ __device__ __forceinline__ void breakpoint64(int64_t ai, int64_t aj, int32_t& ce
 // moves it by at most 2^-5/|a_ik|).
 // Candidates, clamping, dedup order and the strict-improvement rule follow the
 // reference; returns 0 when no candidate lowers the score.
+// best_shear_inl is the body, for callers that need it inline (no call, no
+// stack arrays); best_shear is the same as a function of its own.
 template <int G, int C>
-__device__ int64_t best_shear(const int64_t (&ai)[C], const int64_t (&aj)[C], int g, int n, int j,
-                              int64_t a_ii, int64_t a_ji, int64_t a_jj, int64_t* off_out) {
+__device__ __forceinline__ int64_t best_shear_inl(const int64_t (&ai)[C], const int64_t (&aj)[C],
+                                                  int g, int n, int j, int64_t a_ii, int64_t a_ji,
+                                                  int64_t a_jj, int64_t* off_out) {
     int64_t w[C];
     int32_t ce[C], fl[C];
     int64_t wl = 0;
@@ -186,6 +189,12 @@ __device__ int64_t best_shear(const int64_t (&ai)[C], const int64_t (&aj)[C], in
     }
     *off_out = best_off;
     return best_s;
+}
+
+template <int G, int C>
+__device__ int64_t best_shear(const int64_t (&ai)[C], const int64_t (&aj)[C], int g, int n, int j,
+                              int64_t a_ii, int64_t a_ji, int64_t a_jj, int64_t* off_out) {
+    return best_shear_inl<G, C>(ai, aj, g, n, j, a_ii, a_ji, a_jj, off_out);
 }
 
 }  // namespace xtx
