@@ -213,9 +213,11 @@ void xtx_clear_stop(xtx_ctx* ctx);
  * lookahead configurations match run in one launch. The configuration changes
  * speed only: every configuration computes the same moves and results.
  * lanes = 1 is the lane kernel: one lane per proposal, the whole row pair in
- * its registers, 16 proposals per replica and step (cols = order, triangular,
- * lookahead 16, orders 8 to 48); plain and batch calls with automatic
- * lookahead choose it when their replicas fill the device (16 per CU). */
+ * its registers, 16 or 32 proposals per replica and step (cols = order,
+ * triangular, lookahead 16 or 32, orders 8 to 48; orders 8 and 16, 24 and 32,
+ * 40 and 48 share a launch); plain and batch calls with automatic lookahead
+ * choose it, at 32 proposals, when their replicas fill the device (16 per
+ * CU). */
 typedef struct {
     int32_t order;            /* padded order: a multiple of 8 in [8, 128] */
     int32_t lanes;            /* 16, 32 or 64 lanes per replica; 1 = lane kernel */

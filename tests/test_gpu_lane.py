@@ -1,9 +1,10 @@
-"""The lane kernel (one lane per proposal, 16 proposals per replica and step,
-four replicas per wave; anneal_lane.hip) against the oracle's trajectory model
-and against the lane-group kernels, bit for bit.
+"""The lane kernel (one lane per proposal, 16 or 32 proposals per replica and
+step, four or two replicas per wave; two orders per launch; anneal_lane.hip)
+against the oracle's trajectory model and against the lane-group kernels, bit
+for bit.
 
 1. Trajectories: plain solves of three epochs with the lane configuration set
-   for the order (xtx_launch_set: lanes 1, cols = order, lookahead 16) end in
+   for the order (xtx_launch_set: lanes 1, cols = order, lookahead 16 or 32) end in
    the model's state -- every replica's A, X^T and scalars, the returned best
    and the accepted-move log (which replays to the final state) -- over the
    matrix families, both temperature modes with stall sweeps, reseeds,
@@ -11,8 +12,10 @@ and against the lane-group kernels, bit for bit.
    transform and a move cap inside a step, rows past 2^16 (the int64 path),
    and waves per workgroup 0, 1, 2.
 2. 24 randomly drawn configurations at orders up to 48.
-3. Batches: the same instances under the lane kernel and under the lookahead
-   kernels give the same result for every instance.
+3. Batches: the same instances under the lane kernel (orders paired in one
+   launch, both orders present in every launch) and under the lookahead
+   kernels give the same result for every instance, at 16 and 32 proposals
+   per step.
 """
 import sys
 
@@ -37,11 +40,11 @@ def eng():
     e.close()
 
 
-def _lane(eng, n, waves=0):
+def _lane(eng, n, waves=0, q=16):
     order = max(8, -(-n // 8) * 8)
-    eng.launch_set(order, 1, order, 1, waves, 16)
+    eng.launch_set(order, 1, order, 1, waves, q)
     cfg = eng.launch_get(n)
-    assert (cfg["lanes"], cfg["cols"], cfg["lookahead"], cfg["tuned"]) == (1, order, 16, 1)
+    assert (cfg["lanes"], cfg["cols"], cfg["lookahead"], cfg["tuned"]) == (1, order, q, 1)
     return order
 
 
@@ -119,15 +122,16 @@ def _edge(n):
     return A.astype(np.int64)
 
 
+@pytest.mark.parametrize("q", [16, 32])
 @pytest.mark.parametrize("waves", [0, 2])
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
-def test_lane_trajectory_matches_model(eng, orc, case, waves):
+def test_lane_trajectory_matches_model(eng, orc, case, waves, q):
     name, mk, kw = case
     A0 = mk()
     n = A0.shape[0]
-    order = _lane(eng, n, waves)
+    order = _lane(eng, n, waves, q)
     try:
-        say(f"lane trajectory {name} n={n} R={kw['replicas']} waves={waves}")
+        say(f"lane trajectory {name} n={n} R={kw['replicas']} waves={waves} q={q}")
         _trajectory(eng, orc, A0, kw)
     finally:
         eng.launch_reset(order)
@@ -183,25 +187,40 @@ def test_lane_random_config(eng, orc, k):
     n = A0.shape[0]
     if np.count_nonzero(np.triu(A0, 1)) == 0:
         pytest.skip("diagonal start")
-    order = _lane(eng, n, k % 3)
+    q = 32 if k % 2 else 16
+    order = _lane(eng, n, k % 3, q)
     try:
-        say(f"lane random config {k} n={n} {kw}")
+        say(f"lane random config {k} n={n} q={q} {kw}")
         _trajectory(eng, orc, A0, kw)
     finally:
         eng.launch_reset(order)
 
 
-@pytest.mark.parametrize("count,reps", [(96, 32), (384, 16)])
-def test_lane_batch_equals_lookahead_batch(eng, count, reps):
-    """A batch large enough for the rule to choose the lane kernel gives the
-    result of the same batch under the lookahead kernels (explicit lookahead
-    16), instance by instance."""
+@pytest.mark.parametrize("count,reps,q", [(96, 32, 0), (384, 16, 0), (160, 32, 16)])
+def test_lane_batch_equals_lookahead_batch(eng, count, reps, q):
+    """A batch large enough for the rule to choose the lane kernel (32
+    proposals per step) gives the result of the same batch under the
+    lookahead kernels (explicit lookahead 16), instance by instance. q = 16:
+    the lane configuration at 16 proposals is set for every order."""
     mats = workloads.batch_config(count=count, seed=11)
+    orders = sorted({max(8, -(-a.shape[0] // 8) * 8) for a in mats})
+    assert orders == [8, 16, 24, 32, 40, 48]
     seeds = [3_000_017 + q for q in range(count)]
     base = dict(replicas=reps, t_init=18.0, t_min=14.0, target_fraction=1.0,
                 exchange_interval=1000, max_epochs=3)
-    say(f"lane batch {count} x {reps}")
-    lane, bl = eng.batch_solve(mats, EngineParams(**base), seeds=seeds)
+    say(f"lane batch {count} x {reps} q={q}")
+    if q:
+        for o in orders:
+            eng.launch_set(o, 1, o, 1, 0, q)
+    try:
+        lane, bl = eng.batch_solve(mats, EngineParams(**base), seeds=seeds)
+    finally:
+        for o in orders:
+            eng.launch_reset(o)
+    if q:
+        assert bl["lane_instances"] == count
+    else:
+        assert bl["lane_instances"] == (count if count * reps >= 4096 else 0)
     grp, bg = eng.batch_solve(mats, EngineParams(lookahead=16, **base), seeds=seeds)
     say(f"lane {bl['kernel_ms']:.1f} ms, lookahead {bg['kernel_ms']:.1f} ms, solved {bl['solved']}")
     assert bl["solved"] == bl["verified"]

@@ -176,14 +176,15 @@ __device__ __forceinline__ EvalOut eval_group(const GLOBAL int64_t* gA, int n, i
 
 // Waves per SIMD the kernel is compiled for: two where both waves' registers
 // fit without spilling.
-constexpr int lane_waves_per_simd(int nmax) { return nmax <= 16 ? 2 : 1; }
+constexpr int lane_waves_per_simd(int nmax) { return nmax <= 32 ? 2 : 1; }
 
-// One epoch of many replicas, Q lanes (proposals) each.
+// One epoch of the block's replicas at kernel order NMAX, Q lanes
+// (proposals) each. Inline with no call: the kernels below hold two orders.
 template <int NMAX, int Q>
-__global__ void __launch_bounds__(256, lane_waves_per_simd(NMAX))
-    k_anneal_lane(const RepRef* __restrict__ table, int count, Inst* __restrict__ insts,
-                  int nmax, int rep_bytes, const volatile int* __restrict__ stop) {
-    static_assert(Q == 16, "16 lanes per replica");
+__device__ __attribute__((always_inline)) inline void lane_epoch(
+    const RepRef* __restrict__ table, int count, Inst* __restrict__ insts, int rep_bytes,
+    const volatile int* __restrict__ stop) {
+    static_assert(Q == 16 || Q == 32, "16 or 32 lanes per replica");
     static_assert(NMAX % 8 == 0, "orders are multiples of 8");
     using L = LaneLds<NMAX>;
     constexpr int CQ = LaneRep<NMAX, Q>::CQ;
@@ -193,7 +194,7 @@ __global__ void __launch_bounds__(256, lane_waves_per_simd(NMAX))
     const int gbase = lane & ~(Q - 1);
     const int slot = (int)threadIdx.x / Q;
     const int gid = (int)blockIdx.x * ((int)blockDim.x / Q) + slot;
-    if (gid >= count || nmax != NMAX) return;
+    if (gid >= count) return;
     // every replica of a wave belongs to one instance (the host pads each
     // instance's replica list to whole waves)
     const int wave_gid0 = __builtin_amdgcn_readfirstlane((int)blockIdx.x * ((int)blockDim.x / Q) +
@@ -742,29 +743,54 @@ __global__ void __launch_bounds__(256, lane_waves_per_simd(NMAX))
     }
 }
 
+// One epoch of the replicas of two orders in one launch: table entries below
+// `split` (a whole number of blocks) are of order NHI, the rest of order NLO.
+// Every order of an epoch then runs concurrently on the device's few hardware
+// queues, the short lower order filling the compute units the higher one
+// leaves idle. `nmax`: the launch's largest order (NHI, or NLO when no entry
+// is of order NHI).
+template <int NLO, int NHI, int Q>
+__global__ void __launch_bounds__(256, lane_waves_per_simd(NHI))
+    k_anneal_lane(const RepRef* __restrict__ table, int count, int split,
+                  Inst* __restrict__ insts, int nmax, int rep_bytes,
+                  const volatile int* __restrict__ stop) {
+    if (nmax != NHI && nmax != NLO) return;
+    if ((int)blockIdx.x * ((int)blockDim.x / Q) < split)
+        lane_epoch<NHI, Q>(table, count, insts, rep_bytes, stop);
+    else
+        lane_epoch<NLO, Q>(table, count, insts, rep_bytes, stop);
+}
+
 // ---------------------------------------------------------------------------
 // host-side launch helpers
 // ---------------------------------------------------------------------------
 
-typedef void (*LaneKernel)(const RepRef*, int, Inst*, int, int, const volatile int*);
+typedef void (*LaneKernel)(const RepRef*, int, int, Inst*, int, int, const volatile int*);
 
 struct LaneVariant {
-    int nmax, Q;
+    int lo, hi, Q;
     LaneKernel fn;
 };
 static const LaneVariant kLaneVariants[] = {
-    {8, 16, k_anneal_lane<8, 16>},   {16, 16, k_anneal_lane<16, 16>},
-    {24, 16, k_anneal_lane<24, 16>}, {32, 16, k_anneal_lane<32, 16>},
-    {40, 16, k_anneal_lane<40, 16>}, {48, 16, k_anneal_lane<48, 16>},
+    {8, 16, 16, k_anneal_lane<8, 16, 16>},  {24, 32, 16, k_anneal_lane<24, 32, 16>},
+    {40, 48, 16, k_anneal_lane<40, 48, 16>}, {8, 16, 32, k_anneal_lane<8, 16, 32>},
+    {24, 32, 32, k_anneal_lane<24, 32, 32>}, {40, 48, 32, k_anneal_lane<40, 48, 32>},
 };
 
 static const LaneVariant* find_lane_variant(int nmax, int Q) {
     for (const auto& v : kLaneVariants)
-        if (v.nmax == nmax && v.Q == Q) return &v;
+        if ((v.lo == nmax || v.hi == nmax) && v.Q == Q) return &v;
     return nullptr;
 }
 
 bool lane_variant_exists(int nmax, int Q) { return find_lane_variant(nmax, Q) != nullptr; }
+
+// The higher order of the launch pair order nmax belongs to (0: none)
+int lane_pair_hi(int nmax) {
+    for (const auto& v : kLaneVariants)
+        if (v.lo == nmax || v.hi == nmax) return v.hi;
+    return 0;
+}
 
 // LDS of one replica of the lane kernel at order nmax (LaneLds)
 int lane_rep_bytes(int nmax) {
@@ -784,10 +810,13 @@ hipError_t init_lane_kernel_attrs(int max_lds_per_block) {
     return hipSuccess;
 }
 
-// units_per_block replicas (table entries, padding included) of Q lanes each
+// units_per_block replicas (table entries, padding included) of Q lanes each;
+// nmax: the largest order in the table; the first `split` entries (a multiple
+// of units_per_block) are of the pair's higher order, the rest of its lower
+// one.
 hipError_t launch_anneal_lane(hipStream_t st, int nmax, int Q, int units_per_block,
-                              const RepRef* table, int count, Inst* insts, int rep_bytes,
-                              const volatile int* stop) {
+                              const RepRef* table, int count, int split, Inst* insts,
+                              int rep_bytes, const volatile int* stop) {
     if (count <= 0) return hipSuccess;
     const LaneVariant* v = find_lane_variant(nmax, Q);
     if (!v || units_per_block < 1 || units_per_block * Q > 256 || (units_per_block * Q) % 64 != 0 ||
@@ -795,9 +824,12 @@ hipError_t launch_anneal_lane(hipStream_t st, int nmax, int Q, int units_per_blo
         return hipErrorInvalidValue;
     const int threads = units_per_block * Q;
     const int blocks = (count + units_per_block - 1) / units_per_block;
+    if (split < 0 || split > blocks * units_per_block || split % units_per_block != 0 ||
+        (split > 0 && nmax != v->hi))
+        return hipErrorInvalidValue;
     const size_t lds = (size_t)units_per_block * rep_bytes;
-    hipLaunchKernelGGL(v->fn, dim3(blocks), dim3(threads), lds, st, table, count, insts, nmax,
-                       rep_bytes, stop);
+    hipLaunchKernelGGL(v->fn, dim3(blocks), dim3(threads), lds, st, table, count, split, insts,
+                       nmax, rep_bytes, stop);
     return hipGetLastError();
 }
 

@@ -47,8 +47,8 @@ hipError_t init_kernel_attrs(int max_lds_per_block);
 hipError_t init_dual_kernel_attrs(int max_lds_per_block);
 // the lane kernel (anneal_lane.hip)
 hipError_t launch_anneal_lane(hipStream_t st, int nmax, int Q, int units_per_block,
-                              const RepRef* table, int count, Inst* insts, int rep_bytes,
-                              const volatile int* stop);
+                              const RepRef* table, int count, int split, Inst* insts,
+                              int rep_bytes, const volatile int* stop);
 hipError_t init_lane_kernel_attrs(int max_lds_per_block);
 // the no-alloc solve's device scratch (scratch.hip)
 hipError_t scratch_init(int device);
@@ -424,7 +424,34 @@ struct Bucket {
     RepRef* d_refs = nullptr;
     int cap = 0;
     int units_per_block = 1;  // replicas (table entries) per block
+    // lane kernel: entries of the launch pair's higher order come first, this
+    // many (whole blocks); -1: every entry is of the layout's order
+    int split = -1;
 };
+
+// The lane kernel's split for a table whose entries are all of order L.nmax.
+int whole_split(const Layout& L, int count, int units_per_block) {
+    if (L.nmax != lane_pair_hi(L.nmax)) return 0;
+    return (count + units_per_block - 1) / units_per_block * units_per_block;
+}
+
+// Orders a lane-kernel bucket's table: the pair's higher order first, padded
+// to whole blocks with idle entries, then the lower order. The kernel reads
+// which order a block runs from `split`.
+void split_lane_bucket(Bucket& b, const std::vector<std::unique_ptr<HostInst>>& slots) {
+    if (b.lay.G != 1 || b.refs.empty()) return;
+    const int hi = lane_pair_hi(b.lay.nmax);
+    std::vector<RepRef> top, rest;
+    for (const RepRef& e : b.refs) (slots[(size_t)e.inst]->lay.nmax == hi ? top : rest).push_back(e);
+    if (!top.empty() && !rest.empty()) {
+        const int inst = top.back().inst;
+        while (top.size() % (size_t)b.units_per_block) top.push_back(RepRef{inst, -1});
+    }
+    b.split = top.empty() ? 0 : (int)top.size();
+    if (!top.empty() && rest.empty()) b.split = whole_split(b.lay, (int)top.size(), b.units_per_block);
+    top.insert(top.end(), rest.begin(), rest.end());
+    b.refs.swap(top);
+}
 
 }  // namespace
 
@@ -439,7 +466,7 @@ namespace {
 // it has one lane group, else the default layout). Automatic: the order's
 // configuration if it has lookahead (rule above 1) or one group (rule 1),
 // else the rule's layout. `lane`: the form has replicas enough to fill the
-// device with the lane kernel (one lane per proposal, four replicas per
+// device with the lane kernel (one lane per proposal, two replicas per
 // wave), which then runs the orders it covers under automatic lookahead,
 // unless the order has a lookahead configuration of its own. A lane-kernel
 // configuration the autotuner chose (timed with the device full) applies only
@@ -456,8 +483,9 @@ Layout choose_layout(const xtx_ctx* ctx, int n, int requested, int rule, bool la
     return K > 1 ? lookahead_layout(nmax, K) : layout_for(n);
 }
 
-// Replicas that fill the device with the lane kernel: four waves of four
-// replicas per compute unit.
+// Replicas that fill the device with the lane kernel: 16 per compute unit
+// (eight waves of two replicas; the threshold was measured at four waves of
+// four, and the lookahead kernels serve fewer replicas better).
 bool lane_load(const xtx_ctx* ctx, int64_t replicas) {
     return replicas >= (int64_t)16 * ctx->prop.multiProcessorCount;
 }
@@ -481,12 +509,14 @@ int units_per_block_for(const Layout& L, int waves) {
     return w * per_wave;
 }
 
-// One launch of layout L's kernel over a replica table.
+// One launch of layout L's kernel over a replica table (split: the lane
+// kernel's, -1 when every entry is of order L.nmax).
 hipError_t launch_layout(hipStream_t st, const Layout& L, int units_per_block, const RepRef* table,
-                         int count, Inst* insts, const volatile int* stop) {
+                         int count, Inst* insts, const volatile int* stop, int split = -1) {
     if (L.G == 1)
-        return launch_anneal_lane(st, L.nmax, L.K, units_per_block, table, count, insts,
-                                  L.rep_bytes, stop);
+        return launch_anneal_lane(st, L.nmax, L.K, units_per_block, table, count,
+                                  split < 0 ? whole_split(L, count, units_per_block) : split,
+                                  insts, L.rep_bytes, stop);
     return launch_anneal(st, L.G, L.C, L.tri, L.K, units_per_block, table, count, insts, L.nmax,
                          L.rep_bytes, stop);
 }
@@ -513,7 +543,7 @@ void launch_epoch(xtx_ctx* ctx, std::vector<Bucket>& buckets, Inst* d_insts, int
         const int lane = (int)(q % (size_t)(ns + 1));
         hipStream_t st = lane == 0 ? ctx->stream : ctx->side[lane - 1];
         HIPCHK(launch_layout(st, b.lay, b.units_per_block, b.d_refs, (int)b.refs.size(), d_insts,
-                             ctx->stop_dev));
+                             ctx->stop_dev, b.split));
     }
     for (int q = 0; q < ns; ++q) {
         HIPCHK(hipEventRecord(ctx->join_ev[q], ctx->side[q]));
@@ -1185,6 +1215,7 @@ struct Server {
         }
         for (auto& kv : buckets) {
             Bucket& b = kv.second;
+            split_lane_bucket(b, slots);
             if ((int)b.refs.size() > b.cap) {
                 if (b.d_refs) (void)hipFree(b.d_refs);
                 b.cap = std::max((int)b.refs.size(), 2 * b.cap);
@@ -1410,7 +1441,7 @@ const char* launch_invalid(int nmax, int G, int C, bool tri, int waves, int K) {
         // the lane kernel: one lane per proposal, the whole row per lane
         if (C != nmax || !tri || !lane_variant_exists(nmax, K))
             return "no lane kernel with this order/lookahead (lanes 1, cols = order, "
-                   "triangular, lookahead 16, orders to 48)";
+                   "triangular, lookahead 16 or 32, orders to 48)";
         if (waves < 0 || waves > 4 || waves == 3) return "waves_per_block must be 0, 1, 2 or 4";
         if ((int64_t)std::max(1, waves) * (64 / K) * rep_bytes_of(nmax, G, tri, K) > g_lds_block)
             return "replica LDS exceeds a workgroup";
@@ -1444,9 +1475,10 @@ std::vector<Layout> candidates(int nmax) {
             const int C = (nmax + G - 1) / G;
             if (!launch_invalid(nmax, G, C, true, 0, K)) v.push_back(make_layout(nmax, G, C, true, 0, K));
         }
-    for (int w : {1, 2, 4})
-        if (!launch_invalid(nmax, 1, nmax, true, w, kLaneQ))
-            v.push_back(make_layout(nmax, 1, nmax, true, w, kLaneQ));
+    for (int Q : {16, 32})
+        for (int w : {1, 2, 4})
+            if (!launch_invalid(nmax, 1, nmax, true, w, Q))
+                v.push_back(make_layout(nmax, 1, nmax, true, w, Q));
     return v;
 }
 
@@ -1550,7 +1582,7 @@ void autotune_orders(xtx_ctx* ctx, const xtx_instance* sample, int count,
         // as they do in a run: one epoch from the start matrices commits
         // often, and lookahead makes most of its proposals again after a
         // commit, while later epochs commit rarely. The lane kernel (one
-        // lane per proposal, 16 per step) competes at every order it covers.
+        // lane per proposal, 16 or 32 per step) competes at every order it covers.
         const int rule_K = lookahead_layout(nmax, kLookahead).K;
         for (const Layout& L : candidates(nmax)) {
             if (L.K != rule_K && L.G != 1) continue;
