@@ -216,8 +216,8 @@ void xtx_clear_stop(xtx_ctx* ctx);
  * its registers, 16 or 32 proposals per replica and step (cols = order,
  * triangular, lookahead 16 or 32, orders 8 to 48; orders 8 and 16, 24 and 32,
  * 40 and 48 share a launch); plain and batch calls with automatic lookahead
- * choose it, at 32 proposals, when their replicas fill the device (16 per
- * CU). */
+ * choose it when their replicas fill the device (16 per CU), at 16 proposals
+ * up to order 32 and 32 above. */
 typedef struct {
     int32_t order;            /* padded order: a multiple of 8 in [8, 128] */
     int32_t lanes;            /* 16, 32 or 64 lanes per replica; 1 = lane kernel */

@@ -16,14 +16,17 @@
 // temperature, of the sequential loop: the trajectory is the one-group
 // kernel's and the model's, bit for bit.
 //
-// State: the replica's A stays exact (int64) in HBM, and its low 32 bits sit
+// State: the replica's A stays exact (int64) in HBM, and its low 16 bits sit
 // in LDS as the square NMAX x NMAX (row r at r NMAX, zero past n), with the
-// row masses (int64), the number of entries of each row at or above 2^16
+// row masses (int64), the number of entries of each row at or above 2^15
 // (diagonal included) and the active list. A lane reads a row pair of small
-// entries from LDS, four columns per 16-byte read, with no address arithmetic
-// per column; a proposal on a row with an entry of 2^16 or more (never seen
-// on the batch's instances) is evaluated by the whole group from HBM
-// (group_eval.h), and so is every pair of a stall sweep. In HBM, entry (r, c)
+// entries (below 2^15, so the 16-bit word is the entry) from LDS, eight
+// columns per 16-byte read, with no address arithmetic per column; a proposal
+// on a row with an entry of 2^15 or more (the batch's planted instances start
+// below 2^14) is evaluated by the whole group from HBM (group_eval.h), and so
+// is every pair of a stall sweep. 16-bit words halve the LDS of a replica
+// against 32-bit ones, and LDS is what bounds the waves resident per compute
+// unit. In HBM, entry (r, c)
 // of A and of X^T is read and written by lane c % Q of the replica only (a
 // commit's column-j writes go through that lane), so every lane reads back
 // only global data it wrote itself.
@@ -39,8 +42,8 @@ namespace xtx {
 // LDS byte offsets of one replica at kernel order NMAX.
 template <int NMAX>
 struct LaneLds {
-    static constexpr int kSquare = 0;              // int32 [NMAX][NMAX]
-    static constexpr int kMass = 4 * NMAX * NMAX;  // int64 [NMAX]
+    static constexpr int kSquare = 0;              // int16 [NMAX][NMAX]
+    static constexpr int kMass = 2 * NMAX * NMAX;  // int64 [NMAX]
     static constexpr int kBig = kMass + 8 * NMAX;  // int32 [NMAX]
     static constexpr int kAct = kBig + 4 * NMAX;   // uint8 [NMAX]
     static constexpr int kBytes = (kAct + NMAX + 15) / 16 * 16;
@@ -76,7 +79,7 @@ __device__ __forceinline__ int64_t from_lane64(int64_t v, int src) {
 template <int NMAX, int Q>
 struct LaneRep {
     static constexpr int CQ = (NMAX + Q - 1) / Q;  // columns per lane: qd + c Q
-    SHARED int32_t* sS;
+    SHARED int16_t* sS;
     SHARED int64_t* sM;
     SHARED int32_t* sBig;
     SHARED uint8_t* sAct;
@@ -86,6 +89,8 @@ struct LaneRep {
 
     __device__ __forceinline__ int act(uint32_t p) const { return ident ? (int)p : (int)sAct[p]; }
     __device__ __forceinline__ int32_t s32(int a, int b) const { return sS[a * NMAX + b]; }
+    // entries below this magnitude are exact in the LDS words
+    static constexpr int64_t kSmall = (int64_t)1 << 15;
     __device__ __forceinline__ bool small(int r) const { return sBig[r] == 0; }
 
     // LDS from gA: the low words (zero past n), row masses and big counts;
@@ -98,9 +103,9 @@ struct LaneRep {
             for (int c = 0; c < CQ; ++c) {
                 const int k = qd + c * Q;
                 const int64_t v = (r < n && k < n) ? gA[r * n + k] : 0;
-                if (k < NMAX) sS[r * NMAX + k] = (int32_t)v;
+                if (k < NMAX) sS[r * NMAX + k] = (int16_t)v;
                 mass += k != r ? iabs64(v) : 0;
-                big += iabs64(v) >= lane::kSmallEntry;
+                big += iabs64(v) >= kSmall;
             }
             mass = Grp<Q>::sum(mass);
             big = Grp<Q>::sum32(big);
@@ -113,18 +118,23 @@ struct LaneRep {
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
 
-    // rows i, j of small entries for one lane (lane_eval.h)
+    // rows i, j of small entries for one lane (lane_eval.h): eight columns
+    // per 16-byte read, two per 32-bit word (low half first)
     __device__ __forceinline__ void pair_small(int i, int j, lane::Pair<NMAX>& pr) const {
         typedef int32_t v4i __attribute__((ext_vector_type(4)));
         const SHARED v4i* ri = (const SHARED v4i*)(sS + i * NMAX);
         const SHARED v4i* rj = (const SHARED v4i*)(sS + j * NMAX);
+        auto two = [&](int k, int32_t wa, int32_t wb) __attribute__((always_inline)) {
+            lane::to_pq((int32_t)(int16_t)wa, (int32_t)(int16_t)wb, pr.p[k], pr.qb[k]);
+            lane::to_pq(wa >> 16, wb >> 16, pr.p[k + 1], pr.qb[k + 1]);
+        };
 #pragma unroll
-        for (int v = 0; v < NMAX / 4; ++v) {
+        for (int v = 0; v < NMAX / 8; ++v) {
             const v4i a = ri[v], b = rj[v];
-            lane::to_pq(a.x, b.x, pr.p[4 * v], pr.qb[4 * v]);
-            lane::to_pq(a.y, b.y, pr.p[4 * v + 1], pr.qb[4 * v + 1]);
-            lane::to_pq(a.z, b.z, pr.p[4 * v + 2], pr.qb[4 * v + 2]);
-            lane::to_pq(a.w, b.w, pr.p[4 * v + 3], pr.qb[4 * v + 3]);
+            two(8 * v, a.x, b.x);
+            two(8 * v + 2, a.y, b.y);
+            two(8 * v + 4, a.z, b.z);
+            two(8 * v + 6, a.w, b.w);
         }
         pr.a_ii = s32(i, i);
         pr.a_ji = s32(j, i);
@@ -174,9 +184,10 @@ __device__ __forceinline__ EvalOut eval_group(const GLOBAL int64_t* gA, int n, i
     return o;
 }
 
-// Waves per SIMD the kernel is compiled for: two where both waves' registers
-// fit without spilling.
-constexpr int lane_waves_per_simd(int nmax) { return nmax <= 32 ? 2 : 1; }
+// Waves per SIMD the kernel is compiled for: two, except orders 40 and 48 at
+// 16 proposals per step, which would spill; at 32 proposals those orders hold
+// one extra value in memory outside the step loop.
+constexpr int lane_waves_per_simd(int nmax, int Q) { return nmax <= 32 || Q == 32 ? 2 : 1; }
 
 // One epoch of the block's replicas at kernel order NMAX, Q lanes
 // (proposals) each. Inline with no call: the kernels below hold two orders.
@@ -208,7 +219,7 @@ __device__ __attribute__((always_inline)) inline void lane_epoch(
 
     SHARED unsigned char* base = (SHARED unsigned char*)smem + (size_t)slot * rep_bytes;
     LaneRep<NMAX, Q> rp;
-    rp.sS = (SHARED int32_t*)(base + L::kSquare);
+    rp.sS = (SHARED int16_t*)(base + L::kSquare);
     rp.sM = (SHARED int64_t*)(base + L::kMass);
     rp.sBig = (SHARED int32_t*)(base + L::kBig);
     rp.sAct = (SHARED uint8_t*)(base + L::kAct);
@@ -333,8 +344,8 @@ __device__ __attribute__((always_inline)) inline void lane_epoch(
                 const int64_t aj = sm ? (int64_t)rp.s32(j, k) : rp.gA[j * n + k];
                 const int64_t ai = sm ? (int64_t)rp.s32(i, k) : rp.gA[i * n + k];
                 nv[c] = k == j ? a_jj + 2 * s * a_ji + s * s * a_ii : aj + s * ai;
-                db[c] = (int32_t)(iabs64(nv[c]) >= lane::kSmallEntry) -
-                        (int32_t)(iabs64(aj) >= lane::kSmallEntry);
+                db[c] = (int32_t)(iabs64(nv[c]) >= LaneRep<NMAX, Q>::kSmall) -
+                        (int32_t)(iabs64(aj) >= LaneRep<NMAX, Q>::kSmall);
                 dbj += db[c];
                 if (k != j) {
                     dnz += (int32_t)(nv[c] != 0) - (int32_t)(aj != 0);
@@ -347,11 +358,11 @@ __device__ __attribute__((always_inline)) inline void lane_epoch(
         for (int c = 0; c < CQ; ++c) {
             const int k = qd + c * Q;
             if (k < n) {
-                rp.sS[j * NMAX + k] = (int32_t)nv[c];
+                rp.sS[j * NMAX + k] = (int16_t)nv[c];
                 rp.gA[j * n + k] = nv[c];
                 gX[j * n + k] = gX[j * n + k] + s * gX[i * n + k];
                 if (k != j) {
-                    rp.sS[k * NMAX + j] = (int32_t)nv[c];
+                    rp.sS[k * NMAX + j] = (int16_t)nv[c];
                     rp.sM[k] += dm[c];
                     rp.sBig[k] += db[c];
                 }
@@ -750,7 +761,7 @@ __device__ __attribute__((always_inline)) inline void lane_epoch(
 // leaves idle. `nmax`: the launch's largest order (NHI, or NLO when no entry
 // is of order NHI).
 template <int NLO, int NHI, int Q>
-__global__ void __launch_bounds__(256, lane_waves_per_simd(NHI))
+__global__ void __launch_bounds__(256, lane_waves_per_simd(NHI, Q))
     k_anneal_lane(const RepRef* __restrict__ table, int count, int split,
                   Inst* __restrict__ insts, int nmax, int rep_bytes,
                   const volatile int* __restrict__ stop) {
@@ -794,7 +805,7 @@ int lane_pair_hi(int nmax) {
 
 // LDS of one replica of the lane kernel at order nmax (LaneLds)
 int lane_rep_bytes(int nmax) {
-    const int b = 4 * nmax * nmax + 8 * nmax + 4 * nmax + nmax;
+    const int b = 2 * nmax * nmax + 8 * nmax + 4 * nmax + nmax;
     return (b + 15) / 16 * 16;
 }
 

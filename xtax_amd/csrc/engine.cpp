@@ -466,8 +466,8 @@ namespace {
 // it has one lane group, else the default layout). Automatic: the order's
 // configuration if it has lookahead (rule above 1) or one group (rule 1),
 // else the rule's layout. `lane`: the form has replicas enough to fill the
-// device with the lane kernel (one lane per proposal, two replicas per
-// wave), which then runs the orders it covers under automatic lookahead,
+// device with the lane kernel (one lane per proposal, four or two replicas
+// per wave), which then runs the orders it covers under automatic lookahead,
 // unless the order has a lookahead configuration of its own. A lane-kernel
 // configuration the autotuner chose (timed with the device full) applies only
 // to such forms; one set by hand applies wherever its kind is chosen.
@@ -484,8 +484,8 @@ Layout choose_layout(const xtx_ctx* ctx, int n, int requested, int rule, bool la
 }
 
 // Replicas that fill the device with the lane kernel: 16 per compute unit
-// (eight waves of two replicas; the threshold was measured at four waves of
-// four, and the lookahead kernels serve fewer replicas better).
+// (four waves of four replicas; the lookahead kernels serve fewer replicas
+// better).
 bool lane_load(const xtx_ctx* ctx, int64_t replicas) {
     return replicas >= (int64_t)16 * ctx->prop.multiProcessorCount;
 }

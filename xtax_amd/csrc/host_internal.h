@@ -122,15 +122,17 @@ inline Layout lookahead_layout(int nmax, int want) {
     return layout_for(nmax);
 }
 
-// The lane kernel's layout at order nmax (orders to 48): 32 proposals of a
-// replica per step, one per lane, two replicas per wave. Measured on the
-// 1024-instance batch (orders paired in three launches): kernel time 750 ms
-// against 792 ms at 16 proposals per step (four replicas per wave): a wave's
-// LDS holds half as many replicas, so twice as many waves are resident, which
-// outweighs the proposals made again after a commit.
-constexpr int kLaneQ = 32;
+// The lane kernel's layout at order nmax (orders to 48): 16 proposals of a
+// replica per step (four replicas per wave) up to order 32, 32 proposals (two
+// replicas per wave) at orders 40 and 48. Measured on the 1024-instance batch
+// (orders paired in three launches, 16-bit LDS words): kernel time 615 ms,
+// against 685 ms at 16 proposals for every order and 649 ms at 32: at the
+// high orders two replicas per wave let twice as many waves be resident,
+// which outweighs the proposals made again after a commit; at the low orders
+// residency is not the bound.
 constexpr int kLaneMaxOrder = 48;
-inline Layout lane_layout(int nmax) { return make_layout(nmax, 1, nmax, true, 0, kLaneQ); }
+inline int lane_q(int nmax) { return nmax >= 40 ? 32 : 16; }
+inline Layout lane_layout(int nmax) { return make_layout(nmax, 1, nmax, true, 0, lane_q(nmax)); }
 inline bool lane_layout_ok(int nmax) { return nmax <= kLaneMaxOrder; }
 
 // The lookahead rule. Plain and batch calls: 16 (measured: 256 replicas of
