@@ -1603,6 +1603,7 @@ static const AnnealVariant kVariants[] = {
     {16, 1, true, 4, k_anneal<16, 1, true, 4>},   {16, 2, true, 4, k_anneal<16, 2, true, 4>},
     {32, 1, true, 2, k_anneal<32, 1, true, 2>},   {32, 2, true, 2, k_anneal<32, 2, true, 2>},
     {16, 1, true, 16, k_anneal<16, 1, true, 16>}, {16, 2, true, 16, k_anneal<16, 2, true, 16>},
+    {16, 1, false, 16, k_anneal<16, 1, false, 16>}, {16, 2, false, 16, k_anneal<16, 2, false, 16>},
     {32, 2, true, 8, k_anneal<32, 2, true, 8>},
     {16, 1, true, 32, k_anneal<16, 1, true, 32>}, {16, 2, true, 32, k_anneal<16, 2, true, 32>},
     {64, 2, true, 4, k_anneal<64, 2, true, 4>},

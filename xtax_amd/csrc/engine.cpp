@@ -1473,7 +1473,9 @@ std::vector<Layout> candidates(int nmax) {
     for (int K : {2, 4, 8, 16, 32})
         for (int G : {16, 32, 64}) {
             const int C = (nmax + G - 1) / G;
-            if (!launch_invalid(nmax, G, C, true, 0, K)) v.push_back(make_layout(nmax, G, C, true, 0, K));
+            for (bool tri : {true, false})
+                if (!launch_invalid(nmax, G, C, tri, 0, K))
+                    v.push_back(make_layout(nmax, G, C, tri, 0, K));
         }
     for (int Q : {16, 32})
         for (int w : {1, 2, 4})
