@@ -36,7 +36,9 @@ Other configurations (`--workload ...`), same line format:
   serve    Poisson stream, n = round(exp(U[ln 6, ln 64])), D = +-U{1..200},
            64 replicas per request, continuous batching (xtx_serve_*)
   large    one planted n = 80 instance (D = +-U{1..1000}, 4n shears),
-           4096 replicas, greedy 0.7, T in [50, 2000]
+           4096 replicas, greedy 0.7, T in [50, 2000]; --count K: time to an
+           exact diagonal on K instances, each capped at --seconds;
+           --x-in-lds: X^T in LDS instead of HBM
   lattice  24x24 lattice Gram (entries U[-9, 9]), 256 replicas, greedy 0.5,
            sweep 0.05
   random   n = 10 random symmetric (U[-20, 20]), 8 replicas, T in [0.5, 50],
@@ -486,6 +488,9 @@ def bench_large(args, eng):
                      max_seconds=args.seconds, seed=args.seed, verify=True,
                      lookahead=args.lookahead)
     name = "large: planted n=80, D=+-U{1..1000}, 4n shears, 4096 replicas, greedy 0.7"
+    if args.x_in_lds:
+        eng.launch_set(80, 64, 2, 1, 0, 4, x_in_lds=1)
+        name += ", X^T in LDS"
     extra = ["--greedy-fraction", "0.7", "--t-init", str(args.t_init), "--t-min", str(args.t_min)]
     if (args.count or 1) <= 1:
         return bench_single(args, eng, name, A0, p, extra)
@@ -691,6 +696,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="batch: launch configurations by rule instead of the autotuner")
+    ap.add_argument("--x-in-lds", action="store_true",
+                    help="large: each replica's X^T in LDS for the epoch (64 lanes, 2 "
+                         "columns, lookahead 4) instead of in HBM")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this engine runs on one GPU; instances are independent, so N GPUs "

@@ -230,7 +230,12 @@ typedef struct {
     int32_t lookahead;        /* lane groups per replica (xtx_params.lookahead):
                                  0 or 1 = one; 2, 4 (one wave), 8, 16, 32 (a
                                  workgroup; waves_per_block must be 0); for
-                                 the lane kernel, proposals per step: 16 */
+                                 the lane kernel, proposals per step: 16 or 32 */
+    int32_t x_in_lds;         /* 1: each replica's X^T in LDS for the epoch
+                                 (n x n int64 beside A; 64 lanes, 2 cols,
+                                 triangular, lookahead 4, orders 72 to 112:
+                                 A and X^T in one workgroup's LDS), 0: in HBM */
+    int32_t reserved;         /* 0 */
     double moves_per_sec;     /* out of xtx_autotune: measured rate */
 } xtx_launch;
 

@@ -11,7 +11,11 @@ oracle's trajectory model and against the plain form.
   epochs, every replica's state, rung and the exchange count equal to the
   model's.
 * Large planted n = 80 (D = +-U{1..1000}, 4n shears), 4096 replicas, greedy
-  0.7: two epochs with a short exchange interval equal to the model.
+  0.7: two epochs with a short exchange interval equal to the model, with
+  each replica's X^T in HBM (the rule) and in LDS (xtx_launch x_in_lds);
+  the LDS form also on cooling runs with sweeps and reseeds, at orders 72 to
+  112 (the largest whose A and X^T fit one workgroup's LDS), and refused at
+  order 120.
 * The full batch: 1024 planted instances, n ~ U{8..48}, 32 replicas each, for
   three epochs: every solved result verified exactly, 32 sampled instances
   equal to their plain runs, four of them equal to the model.
@@ -108,11 +112,45 @@ def test_exchange_past_one_staging_chunk(eng, orc, R):
     assert model["exchanges"] > R  # swaps accepted on both sides of every chunk boundary
 
 
-def test_large_n80_4096_replicas(eng, orc):
+@pytest.mark.parametrize("xl", [0, 1])
+def test_large_n80_4096_replicas(eng, orc, xl):
     A0, _, _ = workloads.planted(80, 1000, 320, seed=1)
-    say("config large n=80, 4096 replicas, 2 epochs of 100 moves")
-    _trajectory(eng, orc, A0, dict(replicas=4096, seed=5, greedy_fraction=0.7, t_init=2000.0,
-                                   t_min=50.0, exchange_interval=100), epochs=2)
+    say(f"config large n=80, 4096 replicas, 2 epochs of 100 moves, X^T in LDS={xl}")
+    if xl:
+        eng.launch_set(80, 64, 2, 1, 0, 4, x_in_lds=1)
+        cfg = eng.launch_get(80)
+        assert (cfg["x_in_lds"], cfg["lookahead"], cfg["tuned"]) == (1, 4, 1)
+        assert cfg["lds_bytes"] >= 80 * 80 * 8 + 80 * 81 * 4
+    try:
+        _trajectory(eng, orc, A0, dict(replicas=4096, seed=5, greedy_fraction=0.7,
+                                       t_init=2000.0, t_min=50.0, exchange_interval=100),
+                    epochs=2)
+    finally:
+        eng.launch_reset(80)
+
+
+@pytest.mark.parametrize("n", [72, 80, 97, 112])
+def test_x_in_lds_sweeps_reseeds(eng, orc, n):
+    """X^T in LDS through the paths that read and write it outside a commit:
+    stall sweeps, reseeds from the instance best, best flushes, move logs."""
+    A0, _, _ = workloads.planted(n, 300, 2 * n, seed=n)
+    order = -(-n // 8) * 8
+    eng.launch_set(order, 64, 2, 1, 0, 4, x_in_lds=1)
+    say(f"X^T in LDS: n={n} cooling, sweeps, reseeds")
+    try:
+        _trajectory(eng, orc, A0, dict(replicas=5, seed=n, tempering=False, t_init=40.0,
+                                       exchange_interval=300, stuck_threshold=60,
+                                       sweep_fraction=0.5, reseed_factor=1.0,
+                                       greedy_fraction=0.7),
+                    epochs=3, log_cap=4000)
+    finally:
+        eng.launch_reset(order)
+
+
+def test_x_in_lds_refused_where_it_does_not_fit(eng):
+    from xtax_amd._native import XtxError
+    with pytest.raises(XtxError, match="LDS"):
+        eng.launch_set(120, 64, 2, 1, 0, 4, x_in_lds=1)
 
 
 def test_full_batch_1024(eng, orc):

@@ -93,7 +93,7 @@ class Launch(C.Structure):
     _fields_ = [("order", C.c_int32), ("lanes", C.c_int32), ("cols", C.c_int32),
                 ("triangular", C.c_int32), ("waves_per_block", C.c_int32),
                 ("tuned", C.c_int32), ("lds_bytes", C.c_int32), ("lookahead", C.c_int32),
-                ("moves_per_sec", C.c_double)]
+                ("x_in_lds", C.c_int32), ("reserved", C.c_int32), ("moves_per_sec", C.c_double)]
 
 
 class DualParams(C.Structure):

@@ -249,10 +249,13 @@ __device__ __forceinline__ void evaluate_any(const int64_t (&ai)[C], const int64
 //  * square: rows of odd stride P = nmax + 1 (column accesses hit distinct
 //    banks), both mirror entries written on commit -- fewer index operations
 //    where occupancy is bound by registers instead.
-template <int G, int C, bool TRI>
+// XL: X^T in LDS for the epoch (n x n row-major after the replica's other LDS
+// arrays; loaded at the epoch start, written back at its end), else in HBM.
+template <int G, int C, bool TRI, bool XL = false>
 struct Replica {
     int64_t* sT;
     GLOBAL int64_t* gX; // HBM: X^T of this replica, n x n row-major (touched on accept)
+    SHARED int64_t* sX; // XL: the LDS copy of X^T
     const uint8_t* act;
     int n, N, g, P;
     int Dk[C];          // TRI: D(k) of the lane's columns
@@ -290,6 +293,17 @@ struct Replica {
     }
     __device__ __forceinline__ int64_t entry(int a, int b) const { return sT[at(a, b)]; }
 
+    __device__ __forceinline__ int64_t xget(int r, int k) const {
+        if constexpr (XL) return sX[r * n + k];
+        return gX[r * n + k];
+    }
+    __device__ __forceinline__ void xset(int r, int k, int64_t v) {
+        if constexpr (XL)
+            sX[r * n + k] = v;
+        else
+            gX[r * n + k] = v;
+    }
+
     // Rows i and j of X^T, issued early so the HBM latency overlaps the
     // evaluation of the move (only an accepted move consumes them).
     __device__ __forceinline__ void prefetch_xt(int i, int j, int64_t (&xi)[C],
@@ -297,9 +311,27 @@ struct Replica {
 #pragma unroll
         for (int c = 0; c < C; ++c) {
             const int k = g + c * G;
-            xi[c] = k < n ? gX[i * n + k] : 0;
-            xj[c] = k < n ? gX[j * n + k] : 0;
+            xi[c] = k < n ? xget(i, k) : 0;
+            xj[c] = k < n ? xget(j, k) : 0;
         }
+    }
+
+    // X^T (column-owned) to an n x n matrix in HBM, and from one
+    __device__ __forceinline__ void x_to(GLOBAL int64_t* dst) const {
+        for (int row = 0; row < n; ++row)
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const int k = g + c * G;
+                if (k < n) dst[row * n + k] = xget(row, k);
+            }
+    }
+    __device__ __forceinline__ void x_from(const GLOBAL int64_t* src) {
+        for (int row = 0; row < n; ++row)
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const int k = g + c * G;
+                if (k < n) xset(row, k, src[row * n + k]);
+            }
     }
 
     __device__ __forceinline__ void compute_row_mass() {
@@ -347,17 +379,6 @@ struct Replica {
         }
     }
 
-    // Column-owned copy of an n x n matrix in HBM (each lane copies its columns).
-    __device__ __forceinline__ void copy_cols(GLOBAL int64_t* dst, const GLOBAL int64_t* src) const {
-        for (int row = 0; row < n; ++row) {
-#pragma unroll
-            for (int c = 0; c < C; ++c) {
-                const int k = g + c * G;
-                if (k < n) dst[row * n + k] = src[row * n + k];
-            }
-        }
-    }
-
     // Transform-row bound check of an evaluated Add(i, j, s)
     // (add_transform_row, main.cpp:97-108): false leaves the state untouched.
     __device__ __forceinline__ bool xt_row(int64_t s, const int64_t (&xi)[C],
@@ -388,7 +409,7 @@ struct Replica {
                 sT[at_lane(j, Dj, c)] = e.nv[c];
                 if (!TRI && k != j) sT[k * P + j] = e.nv[c];
                 rm[c] += (k != j) ? iabs64(e.nv[c]) - iabs64(aj[c]) : e.off;
-                gX[j * n + k] = nx[c];
+                xset(j, k, nx[c]);
             }
         }
     }
@@ -432,7 +453,11 @@ static_assert(sizeof(Proposal) == 64, "two proposals per 128 bytes");
 // barrier that lets every wave finish reading the state it changes. Stall
 // sweeps and reseeds write the same way. Every wave thus takes the same
 // branches and meets the same barriers.
-template <int G, int C, bool TRI, int K>
+//
+// XL: the replica's X^T sits in LDS for the epoch (the last nmax^2 int64 of
+// its LDS share) instead of being read and written in HBM on every proposal
+// and commit.
+template <int G, int C, bool TRI, int K, bool XL = false>
 __global__ void __launch_bounds__(G * K > 256 ? 512 : 256, K == 1 ? 4 : 2)
     k_anneal(const RepRef* __restrict__ table, int count, Inst* __restrict__ insts, int nmax,
              int rep_bytes, const volatile int* __restrict__ stop) {
@@ -460,8 +485,10 @@ __global__ void __launch_bounds__(G * K > 256 ? 512 : 256, K == 1 ? 4 : 2)
     GLOBAL RepState* S = (GLOBAL RepState*)I->rs + r;
     if (S->flags & RF_STOPPED) return;
 
-    Replica<G, C, TRI> rp;
+    Replica<G, C, TRI, XL> rp;
     rp.sT = (int64_t*)(smem + (size_t)slot * rep_bytes);
+    rp.sX = (SHARED int64_t*)((SHARED unsigned char*)smem + (size_t)slot * rep_bytes + rep_bytes -
+                              (XL ? (size_t)nmax * nmax * 8 : 0));
     rp.P = nmax + 1;
     // A: nmax (nmax + 1) / 2 or nmax (nmax + 1) int64, a multiple of 32 bytes
     // for nmax % 8 == 0; then five staged Philox blocks and a median
@@ -485,6 +512,7 @@ __global__ void __launch_bounds__(G * K > 256 ? 512 : 256, K == 1 ? 4 : 2)
     GLOBAL int64_t* const gAcur = (GLOBAL int64_t*)I->A + (int64_t)r * nn;
     rp.gX = (GLOBAL int64_t*)I->Xt + (int64_t)r * nn;
     rp.load_A(gAcur);
+    if constexpr (XL) rp.x_from(rp.gX);
     const GLOBAL uint8_t* gact = (const GLOBAL uint8_t*)I->act;
     bool nonid = false;
     for (int q = g; q < N; q += G) {
@@ -539,7 +567,7 @@ __global__ void __launch_bounds__(G * K > 256 ? 512 : 256, K == 1 ? 4 : 2)
                 const int64_t off = (int64_t)r * n * n;
                 if (kk == 0) {  // lookahead groups hold the same state
                     rp.store_A((GLOBAL int64_t*)Iq->bA + off);
-                    rp.copy_cols((GLOBAL int64_t*)Iq->bXt + off, rp.gX);
+                    rp.x_to((GLOBAL int64_t*)Iq->bXt + off);
                 }
                 flags &= ~RF_BEST_STALE;
             } else {
@@ -707,7 +735,7 @@ __global__ void __launch_bounds__(G * K > 256 ? 512 : 256, K == 1 ? 4 : 2)
                     log_move(m, -1, -1, 0, 2);
                     if constexpr (W > 1) __syncthreads();
                     rp.load_A((const GLOBAL int64_t*)Iv->gA);
-                    rp.copy_cols(rp.gX, (const GLOBAL int64_t*)Iv->gXt);
+                    rp.x_from((const GLOBAL int64_t*)Iv->gXt);
                     rp.compute_row_mass();
                     score = gb_epoch;
                     offnz = Iv->gb_offnz;
@@ -958,7 +986,7 @@ __global__ void __launch_bounds__(G * K > 256 ? 512 : 256, K == 1 ? 4 : 2)
                         leave_best();
                         log_move(m, -1, -1, 0, 2);
                         rp.load_A((const GLOBAL int64_t*)Iv->gA);
-                        rp.copy_cols(rp.gX, (const GLOBAL int64_t*)Iv->gXt);
+                        rp.x_from((const GLOBAL int64_t*)Iv->gXt);
                         rp.compute_row_mass();
                         score = gb_epoch;
                         offnz = Iv->gb_offnz;
@@ -1268,6 +1296,7 @@ __global__ void __launch_bounds__(G * K > 256 ? 512 : 256, K == 1 ? 4 : 2)
 
     if (kk != 0) return;  // lookahead: group 0 writes the replica back
     rp.store_A(gAcur);
+    if constexpr (XL) rp.x_to(rp.gX);
     if (g == 0) {
         S->moves = moves;
         S->score = score;
@@ -1593,6 +1622,7 @@ struct AnnealVariant {
     bool tri;
     int K;
     AnnealKernel fn;
+    bool xl = false;  // X^T in LDS
 };
 static const AnnealVariant kVariants[] = {
     {16, 1, true, 1, k_anneal<16, 1, true, 1>},   {16, 1, false, 1, k_anneal<16, 1, false, 1>},
@@ -1607,16 +1637,17 @@ static const AnnealVariant kVariants[] = {
     {32, 2, true, 8, k_anneal<32, 2, true, 8>},
     {16, 1, true, 32, k_anneal<16, 1, true, 32>}, {16, 2, true, 32, k_anneal<16, 2, true, 32>},
     {64, 2, true, 4, k_anneal<64, 2, true, 4>},
+    {64, 2, true, 4, k_anneal<64, 2, true, 4, true>, true},
 };
 
-static const AnnealVariant* find_variant(int G, int C, bool tri, int K) {
+static const AnnealVariant* find_variant(int G, int C, bool tri, int K, bool xl) {
     for (const auto& v : kVariants)
-        if (v.G == G && v.C == C && v.tri == tri && v.K == K) return &v;
+        if (v.G == G && v.C == C && v.tri == tri && v.K == K && v.xl == xl) return &v;
     return nullptr;
 }
 
-bool anneal_variant_exists(int G, int C, bool tri, int K) {
-    return find_variant(G, C, tri, K) != nullptr;
+bool anneal_variant_exists(int G, int C, bool tri, int K, bool xl) {
+    return find_variant(G, C, tri, K, xl) != nullptr;
 }
 
 // Dynamic LDS limits are raised once, outside any stream capture, to the
@@ -1637,11 +1668,11 @@ hipError_t init_kernel_attrs(int max_lds_per_block) {
 }
 
 // units_per_block: table entries (replicas, or padding) per block, G K lanes each
-hipError_t launch_anneal(hipStream_t st, int G, int C, bool tri, int K, int units_per_block,
-                         const RepRef* table, int count, Inst* insts, int nmax, int rep_bytes,
-                         const volatile int* stop) {
+hipError_t launch_anneal(hipStream_t st, int G, int C, bool tri, int K, bool xl,
+                         int units_per_block, const RepRef* table, int count, Inst* insts,
+                         int nmax, int rep_bytes, const volatile int* stop) {
     if (count <= 0) return hipSuccess;
-    const AnnealVariant* v = find_variant(G, C, tri, K);
+    const AnnealVariant* v = find_variant(G, C, tri, K, xl);
     if (!v || units_per_block < 1 || units_per_block * G * K > (G * K > 256 ? 512 : 256))
         return hipErrorInvalidValue;
     const int threads = units_per_block * G * K;

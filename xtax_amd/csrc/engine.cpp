@@ -33,10 +33,10 @@
 #include "xtx.h"
 
 namespace xtx {
-hipError_t launch_anneal(hipStream_t st, int G, int C, bool tri, int K, int units_per_block,
+hipError_t launch_anneal(hipStream_t st, int G, int C, bool tri, int K, bool xl, int units_per_block,
                          const RepRef* table, int count, Inst* insts, int nmax, int rep_bytes,
                          const volatile int* stop);
-bool anneal_variant_exists(int G, int C, bool tri, int K);
+bool anneal_variant_exists(int G, int C, bool tri, int K, bool xl = false);
 hipError_t launch_epoch_end(hipStream_t st, Inst* insts, int count, int max_r);
 hipError_t launch_probe(hipStream_t st, int G, int C, const int64_t* A, const int64_t* Xt, int n,
                         const int32_t* q, int nq, int64_t* out);
@@ -517,7 +517,7 @@ hipError_t launch_layout(hipStream_t st, const Layout& L, int units_per_block, c
         return launch_anneal_lane(st, L.nmax, L.K, units_per_block, table, count,
                                   split < 0 ? whole_split(L, count, units_per_block) : split,
                                   insts, L.rep_bytes, stop);
-    return launch_anneal(st, L.G, L.C, L.tri, L.K, units_per_block, table, count, insts, L.nmax,
+    return launch_anneal(st, L.G, L.C, L.tri, L.K, L.xl, units_per_block, table, count, insts, L.nmax,
                          L.rep_bytes, stop);
 }
 
@@ -1096,7 +1096,8 @@ struct Server {
         for (const auto& kv : orders) reps += kv.second;
         if (orders.empty()) return 0;
         const int nmax = orders.rbegin()->first;
-        return reps * rep_bytes_of(nmax, std::get<1>(k), std::get<3>(k), std::get<5>(k));
+        return reps * rep_bytes_of(nmax, std::get<1>(k), std::get<3>(k), std::get<5>(k),
+                                   std::get<6>(k));
     }
 
     // LDS in use after adding (sign +1) or removing (-1) h's replicas
@@ -1431,10 +1432,11 @@ void fill_launch(const Layout& L, bool tuned, xtx_launch* out) {
     out->tuned = tuned ? 1 : 0;
     out->lds_bytes = L.rep_bytes;
     out->lookahead = L.K;
+    out->x_in_lds = L.xl ? 1 : 0;
 }
 
 // A configuration the kernels can run for this order, or an error message.
-const char* launch_invalid(int nmax, int G, int C, bool tri, int waves, int K) {
+const char* launch_invalid(int nmax, int G, int C, bool tri, int waves, int K, bool xl) {
     if (nmax < 8 || nmax > kMaxN || nmax % 8) return "order must be a multiple of 8 in [8, 128]";
     if (G * C < nmax) return "lanes * cols must cover the order";
     if (G == 1) {
@@ -1447,10 +1449,12 @@ const char* launch_invalid(int nmax, int G, int C, bool tri, int waves, int K) {
             return "replica LDS exceeds a workgroup";
         return nullptr;
     }
-    if (!anneal_variant_exists(G, C, tri, K)) return "no kernel with these lanes/cols/layout/lookahead";
+    if (!anneal_variant_exists(G, C, tri, K, xl))
+        return xl ? "no kernel with X^T in LDS at these lanes/cols/layout/lookahead"
+                  : "no kernel with these lanes/cols/layout/lookahead";
     if (waves < 0 || waves > 4 || waves == 3) return "waves_per_block must be 0, 1, 2 or 4";
     if (K > 1 && waves != 0) return "a lookahead replica takes its own wave or workgroup (waves 0)";
-    const int rb = rep_bytes_of(nmax, G, tri, K);
+    const int rb = rep_bytes_of(nmax, G, tri, K, xl);
     if ((int64_t)std::max(1, 64 / (G * K)) * rb > g_lds_block) return "replica LDS exceeds a workgroup";
     return nullptr;
 }
@@ -1464,7 +1468,7 @@ std::vector<Layout> candidates(int nmax) {
         const int C = (nmax + G - 1) / G;
         for (bool tri : {true, false})
             for (int w : {1, 2, 4}) {
-                if (launch_invalid(nmax, G, C, tri, w, 1)) continue;
+                if (launch_invalid(nmax, G, C, tri, w, 1, false)) continue;
                 const Layout L = make_layout(nmax, G, C, tri, w);
                 if ((int64_t)w * (64 / G) * L.rep_bytes > g_lds_block) continue;
                 v.push_back(L);
@@ -1474,12 +1478,13 @@ std::vector<Layout> candidates(int nmax) {
         for (int G : {16, 32, 64}) {
             const int C = (nmax + G - 1) / G;
             for (bool tri : {true, false})
-                if (!launch_invalid(nmax, G, C, tri, 0, K))
-                    v.push_back(make_layout(nmax, G, C, tri, 0, K));
+                for (bool xl : {false, true})
+                    if (!launch_invalid(nmax, G, C, tri, 0, K, xl))
+                        v.push_back(make_layout(nmax, G, C, tri, 0, K, xl));
         }
     for (int Q : {16, 32})
         for (int w : {1, 2, 4})
-            if (!launch_invalid(nmax, 1, nmax, true, w, Q))
+            if (!launch_invalid(nmax, 1, nmax, true, w, Q, false))
                 v.push_back(make_layout(nmax, 1, nmax, true, w, Q));
     return v;
 }
@@ -1633,11 +1638,12 @@ int xtx_launch_set(xtx_ctx* ctx, const xtx_launch* cfg) {
         return XTX_OK;
     }
     const int K = cfg->lookahead > 1 ? cfg->lookahead : 1;
+    const bool xl = cfg->x_in_lds != 0;
     if (const char* why = launch_invalid(cfg->order, cfg->lanes, cfg->cols, cfg->triangular != 0,
-                                         cfg->waves_per_block, K))
+                                         cfg->waves_per_block, K, xl))
         return fail(ctx, Err{XTX_E_ARG, why});
     ctx->tuned[cfg->order] = make_layout(cfg->order, cfg->lanes, cfg->cols,
-                                         cfg->triangular != 0, cfg->waves_per_block, K);
+                                         cfg->triangular != 0, cfg->waves_per_block, K, xl);
     return XTX_OK;
 }
 

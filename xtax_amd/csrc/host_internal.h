@@ -52,6 +52,7 @@ struct Layout {
     bool tri = true;
     int waves = 0;
     int K = 1;  // lookahead: lane groups per replica (G K == 64 when K > 1)
+    bool xl = false;  // X^T in LDS for the epoch (else in HBM)
     // chosen by the autotuner, which times the device full of replicas: a
     // lane-kernel choice then applies only to calls that fill the device
     bool tuner = false;
@@ -74,15 +75,18 @@ namespace host {
 // histogram, and the active list; a replica over several waves adds two
 // buffers of K 64-byte proposals and a poll word (16-byte aligned). X^T stays
 // in HBM.
-inline int rep_bytes_of(int nmax, int G, bool tri, int K = 1) {
+inline int rep_bytes_of(int nmax, int G, bool tri, int K = 1, bool xl = false) {
     if (G == 1) return lane_rep_bytes(nmax);
     const int64_t a = tri ? (int64_t)nmax * (nmax + 1) / 2 : (int64_t)nmax * (nmax + 1);
     int64_t bytes = a * 8 + K * (5 * 16 + 4 * med_slots(G)) + nmax;
     if (G * K > 64) bytes = (bytes + 15) / 16 * 16 + 2 * K * 64 + 16;
-    return (int)((bytes + 15) / 16 * 16);
+    bytes = (bytes + 15) / 16 * 16;
+    if (xl) bytes += (int64_t)nmax * nmax * 8;  // X^T last
+    return (int)bytes;
 }
 
-inline Layout make_layout(int nmax, int G, int C, bool tri, int waves, int K = 1) {
+inline Layout make_layout(int nmax, int G, int C, bool tri, int waves, int K = 1,
+                          bool xl = false) {
     Layout L;
     L.nmax = nmax;
     L.G = G;
@@ -90,7 +94,8 @@ inline Layout make_layout(int nmax, int G, int C, bool tri, int waves, int K = 1
     L.tri = tri;
     L.waves = waves;
     L.K = K;
-    L.rep_bytes = rep_bytes_of(nmax, G, tri, K);
+    L.xl = xl;
+    L.rep_bytes = rep_bytes_of(nmax, G, tri, K, xl);
     return L;
 }
 
@@ -148,7 +153,7 @@ inline int lookahead_rule(int cus, int64_t replicas) {
     return replicas <= 32 * (int64_t)cus ? kLookahead : 1;
 }
 
-using BucketKey = std::tuple<int, int, int, bool, int, int>;
+using BucketKey = std::tuple<int, int, int, bool, int, int, bool>;
 // The launch a resident instance joins. A kernel runs any order up to the
 // launch's padded order (A's rows in LDS are laid out by the instance's own
 // n; only the offsets of the replica's other LDS arrays follow the launch's
@@ -161,8 +166,8 @@ using BucketKey = std::tuple<int, int, int, bool, int, int>;
 inline BucketKey launch_key(const Layout& L) {
     // the lane kernel runs orders in pairs (8 and 16, 24 and 32, 40 and 48):
     // one launch per pair
-    if (L.G == 1) return {-lane_pair_hi(L.nmax), 1, 0, L.tri, L.waves, L.K};
-    return {(L.K > 1 && L.G > 1) ? 0 : L.nmax, L.G, L.C, L.tri, L.waves, L.K};
+    if (L.G == 1) return {-lane_pair_hi(L.nmax), 1, 0, L.tri, L.waves, L.K, false};
+    return {(L.K > 1 && L.G > 1) ? 0 : L.nmax, L.G, L.C, L.tri, L.waves, L.K, L.xl};
 }
 
 constexpr int kLdsPerCu = 160 * 1024;

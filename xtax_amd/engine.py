@@ -285,12 +285,14 @@ class Engine:
         self._check(self._lib.xtx_launch_get(self._h, int(n), C.byref(out)))
         return _stats(out)
 
-    def launch_set(self, order, lanes, cols, triangular, waves_per_block=0, lookahead=1):
-        cfg = N.Launch(order, lanes, cols, int(triangular), waves_per_block, 0, 0, lookahead, 0.0)
+    def launch_set(self, order, lanes, cols, triangular, waves_per_block=0, lookahead=1,
+                   x_in_lds=0):
+        cfg = N.Launch(order, lanes, cols, int(triangular), waves_per_block, 0, 0, lookahead,
+                       int(x_in_lds), 0, 0.0)
         self._check(self._lib.xtx_launch_set(self._h, C.byref(cfg)))
 
     def launch_reset(self, order):
-        cfg = N.Launch(order, 0, 0, 0, 0, 0, 0, 0, 0.0)
+        cfg = N.Launch(order, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0.0)
         self._check(self._lib.xtx_launch_set(self._h, C.byref(cfg)))
 
     def autotune(self, mats, params=None, moves=512, seeds=None):
