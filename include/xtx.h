@@ -34,7 +34,8 @@ extern "C" {
 
 #define XTX_MAGNITUDE_LIMIT (((int64_t)1) << 48) /* mat_io.hpp:35 */
 #define XTX_SHEAR_CAP 64                          /* mat_io.hpp:38 */
-#define XTX_MAX_N 128                             /* largest order per replica */
+#define XTX_MAX_N 256                             /* largest order per replica (xdual:
+                                                     128) */
 #define XTX_MAX_TARGET_SAMPLES 16
 
 typedef struct xtx_ctx xtx_ctx;
@@ -219,7 +220,8 @@ void xtx_clear_stop(xtx_ctx* ctx);
  * choose it when their replicas fill the device (16 per CU), at 16 proposals
  * up to order 32 and 32 above. */
 typedef struct {
-    int32_t order;            /* padded order: a multiple of 8 in [8, 128] */
+    int32_t order;            /* padded order: a multiple of 8 in [8, 128]; 192 or
+                                 256: the wide lane kernel (orders above 128) */
     int32_t lanes;            /* 16, 32 or 64 lanes per replica; 1 = lane kernel */
     int32_t cols;             /* 1 or 2 columns per lane; lanes * cols >= order
                                  (the order itself for the lane kernel) */

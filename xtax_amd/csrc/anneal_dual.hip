@@ -1162,7 +1162,7 @@ __device__ double block_pairsum(double* buf, int n) {
 
 __global__ void __launch_bounds__(256) k_dual_maint(Inst* __restrict__ insts) {
     __shared__ double s_buf[256];
-    __shared__ double s_f[kMaxN];
+    __shared__ double s_f[kMaxGroupN];
     __shared__ double s_max[256];
     __shared__ int s_arg[256];
     Inst* I = insts + blockIdx.y;

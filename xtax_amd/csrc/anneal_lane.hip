@@ -39,11 +39,11 @@
 
 namespace xtx {
 
-// LDS byte offsets of one replica at kernel order NMAX.
-template <int NMAX>
+// LDS byte offsets of one replica at kernel order NMAX (WIDE: no square).
+template <int NMAX, bool WIDE = false>
 struct LaneLds {
-    static constexpr int kSquare = 0;              // int16 [NMAX][NMAX]
-    static constexpr int kMass = 2 * NMAX * NMAX;  // int64 [NMAX]
+    static constexpr int kSquare = 0;                         // int16 [NMAX][NMAX]
+    static constexpr int kMass = WIDE ? 0 : 2 * NMAX * NMAX;  // int64 [NMAX]
     static constexpr int kBig = kMass + 8 * NMAX;  // int32 [NMAX]
     static constexpr int kAct = kBig + 4 * NMAX;   // uint8 [NMAX]
     static constexpr int kBytes = (kAct + NMAX + 15) / 16 * 16;
@@ -76,7 +76,9 @@ __device__ __forceinline__ int64_t from_lane64(int64_t v, int src) {
 }
 
 // The replica: its LDS and HBM state and the lane's place in its group.
-template <int NMAX, int Q>
+// WIDE (orders above 128): no LDS square; every row counts as holding large
+// entries, so every proposal is evaluated by the lane group from HBM.
+template <int NMAX, int Q, bool WIDE = false>
 struct LaneRep {
     static constexpr int CQ = (NMAX + Q - 1) / Q;  // columns per lane: qd + c Q
     SHARED int16_t* sS;
@@ -91,7 +93,10 @@ struct LaneRep {
     __device__ __forceinline__ int32_t s32(int a, int b) const { return sS[a * NMAX + b]; }
     // entries below this magnitude are exact in the LDS words
     static constexpr int64_t kSmall = (int64_t)1 << 15;
-    __device__ __forceinline__ bool small(int r) const { return sBig[r] == 0; }
+    __device__ __forceinline__ bool small(int r) const {
+        if constexpr (WIDE) return false;
+        return sBig[r] == 0;
+    }
 
     // LDS from gA: the low words (zero past n), row masses and big counts;
     // lane qd reads columns qd + c Q. Group-uniform.
@@ -103,7 +108,7 @@ struct LaneRep {
             for (int c = 0; c < CQ; ++c) {
                 const int k = qd + c * Q;
                 const int64_t v = (r < n && k < n) ? gA[r * n + k] : 0;
-                if (k < NMAX) sS[r * NMAX + k] = (int16_t)v;
+                if (!WIDE && k < NMAX) sS[r * NMAX + k] = (int16_t)v;
                 mass += k != r ? iabs64(v) : 0;
                 big += iabs64(v) >= kSmall;
             }
@@ -191,14 +196,15 @@ constexpr int lane_waves_per_simd(int nmax, int Q) { return nmax <= 32 || Q == 3
 
 // One epoch of the block's replicas at kernel order NMAX, Q lanes
 // (proposals) each. Inline with no call: the kernels below hold two orders.
-template <int NMAX, int Q>
+template <int NMAX, int Q, bool WIDE>
 __device__ __attribute__((always_inline)) inline void lane_epoch(
     const RepRef* __restrict__ table, int count, Inst* __restrict__ insts, int rep_bytes,
     const volatile int* __restrict__ stop) {
     static_assert(Q == 16 || Q == 32, "16 or 32 lanes per replica");
     static_assert(NMAX % 8 == 0, "orders are multiples of 8");
-    using L = LaneLds<NMAX>;
-    constexpr int CQ = LaneRep<NMAX, Q>::CQ;
+    static_assert(WIDE || NMAX <= 128, "orders above 128 run WIDE");
+    using L = LaneLds<NMAX, WIDE>;
+    constexpr int CQ = LaneRep<NMAX, Q, WIDE>::CQ;
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = (int)(threadIdx.x & 63u);
     const int qd = (int)(threadIdx.x % (unsigned)Q);
@@ -218,7 +224,7 @@ __device__ __attribute__((always_inline)) inline void lane_epoch(
     if (S->flags & RF_STOPPED) return;
 
     SHARED unsigned char* base = (SHARED unsigned char*)smem + (size_t)slot * rep_bytes;
-    LaneRep<NMAX, Q> rp;
+    LaneRep<NMAX, Q, WIDE> rp;
     rp.sS = (SHARED int16_t*)(base + L::kSquare);
     rp.sM = (SHARED int64_t*)(base + L::kMass);
     rp.sBig = (SHARED int32_t*)(base + L::kBig);
@@ -344,8 +350,8 @@ __device__ __attribute__((always_inline)) inline void lane_epoch(
                 const int64_t aj = sm ? (int64_t)rp.s32(j, k) : rp.gA[j * n + k];
                 const int64_t ai = sm ? (int64_t)rp.s32(i, k) : rp.gA[i * n + k];
                 nv[c] = k == j ? a_jj + 2 * s * a_ji + s * s * a_ii : aj + s * ai;
-                db[c] = (int32_t)(iabs64(nv[c]) >= LaneRep<NMAX, Q>::kSmall) -
-                        (int32_t)(iabs64(aj) >= LaneRep<NMAX, Q>::kSmall);
+                db[c] = (int32_t)(iabs64(nv[c]) >= LaneRep<NMAX, Q, WIDE>::kSmall) -
+                        (int32_t)(iabs64(aj) >= LaneRep<NMAX, Q, WIDE>::kSmall);
                 dbj += db[c];
                 if (k != j) {
                     dnz += (int32_t)(nv[c] != 0) - (int32_t)(aj != 0);
@@ -358,11 +364,11 @@ __device__ __attribute__((always_inline)) inline void lane_epoch(
         for (int c = 0; c < CQ; ++c) {
             const int k = qd + c * Q;
             if (k < n) {
-                rp.sS[j * NMAX + k] = (int16_t)nv[c];
+                if constexpr (!WIDE) rp.sS[j * NMAX + k] = (int16_t)nv[c];
                 rp.gA[j * n + k] = nv[c];
                 gX[j * n + k] = gX[j * n + k] + s * gX[i * n + k];
                 if (k != j) {
-                    rp.sS[k * NMAX + j] = (int16_t)nv[c];
+                    if constexpr (!WIDE) rp.sS[k * NMAX + j] = (int16_t)nv[c];
                     rp.sM[k] += dm[c];
                     rp.sBig[k] += db[c];
                 }
@@ -591,7 +597,7 @@ __device__ __attribute__((always_inline)) inline void lane_epoch(
         ev.delta = 0;
         ev.off = 0;
         ev.ok = true;
-        if (!defer) {
+        if constexpr (!WIDE) if (!defer) {
             lane::Pair<NMAX> pr;
             rp.pair_small(i, j, pr);
             int32_t tot, sumq, off = 0, s = 0;
@@ -760,16 +766,20 @@ __device__ __attribute__((always_inline)) inline void lane_epoch(
 // queues, the short lower order filling the compute units the higher one
 // leaves idle. `nmax`: the launch's largest order (NHI, or NLO when no entry
 // is of order NHI).
-template <int NLO, int NHI, int Q>
+//
+// WIDE (orders above 128, NLO = 192 and NHI = 256 as column bounds of 6 and 8
+// columns per lane): A lives in HBM only and every proposal is evaluated by
+// the replica's lane group, one after another.
+template <int NLO, int NHI, int Q, bool WIDE = false>
 __global__ void __launch_bounds__(256, lane_waves_per_simd(NHI, Q))
     k_anneal_lane(const RepRef* __restrict__ table, int count, int split,
                   Inst* __restrict__ insts, int nmax, int rep_bytes,
                   const volatile int* __restrict__ stop) {
     if (nmax != NHI && nmax != NLO) return;
     if ((int)blockIdx.x * ((int)blockDim.x / Q) < split)
-        lane_epoch<NHI, Q>(table, count, insts, rep_bytes, stop);
+        lane_epoch<NHI, Q, WIDE>(table, count, insts, rep_bytes, stop);
     else
-        lane_epoch<NLO, Q>(table, count, insts, rep_bytes, stop);
+        lane_epoch<NLO, Q, WIDE>(table, count, insts, rep_bytes, stop);
 }
 
 // ---------------------------------------------------------------------------
@@ -786,6 +796,7 @@ static const LaneVariant kLaneVariants[] = {
     {8, 16, 16, k_anneal_lane<8, 16, 16>},  {24, 32, 16, k_anneal_lane<24, 32, 16>},
     {40, 48, 16, k_anneal_lane<40, 48, 16>}, {8, 16, 32, k_anneal_lane<8, 16, 32>},
     {24, 32, 32, k_anneal_lane<24, 32, 32>}, {40, 48, 32, k_anneal_lane<40, 48, 32>},
+    {192, 256, 32, k_anneal_lane<192, 256, 32, true>},
 };
 
 static const LaneVariant* find_lane_variant(int nmax, int Q) {
@@ -805,7 +816,7 @@ int lane_pair_hi(int nmax) {
 
 // LDS of one replica of the lane kernel at order nmax (LaneLds)
 int lane_rep_bytes(int nmax) {
-    const int b = 2 * nmax * nmax + 8 * nmax + 4 * nmax + nmax;
+    const int b = (nmax > kMaxGroupN ? 0 : 2 * nmax * nmax) + 8 * nmax + 4 * nmax + nmax;
     return (b + 15) / 16 * 16;
 }
 

@@ -197,6 +197,7 @@ int xtx_dual_solve(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t n
         ctx->plain_rs.clear();
         HostInst& h = ctx->plain;
         h = HostInst{};
+        if (n > kMaxGroupN) throw Err{XTX_E_ARG, "xdual: n exceeds 128 (an LDS-resident replica)"};
         prepare(h, A0, X0, n, p.replicas, p.seed, active, n_active, p);
         const size_t nn = (size_t)n * n;
 
@@ -566,7 +567,7 @@ int xtx_dual_last_replicas(xtx_ctx* ctx, int64_t* P, double* Q, int64_t* Xt, dou
 int xtx_dual_probe(xtx_ctx* ctx, const int64_t* P, const double* Q, int32_t n, int32_t n_mats,
                    double c, const int32_t* queries, int32_t n_queries, int64_t* out_i,
                    double* out_d, int64_t* rows_p, double* rows_q, double* scores) {
-    if (!ctx || !P || !Q || !queries || !out_i || n < 2 || n > kMaxN || n_mats < 1 ||
+    if (!ctx || !P || !Q || !queries || !out_i || n < 2 || n > kMaxGroupN || n_mats < 1 ||
         n_queries < 0)
         return XTX_E_ARG;
     try {

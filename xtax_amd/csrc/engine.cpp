@@ -473,6 +473,7 @@ namespace {
 // to such forms; one set by hand applies wherever its kind is chosen.
 Layout choose_layout(const xtx_ctx* ctx, int n, int requested, int rule, bool lane = false) {
     const int nmax = order_of(n);
+    if (nmax > kMaxGroupN) return wide_layout(nmax);
     const auto it = ctx->tuned.find(nmax);
     const bool has = it != ctx->tuned.end();
     if (requested > 1) return lookahead_layout(nmax, requested);
@@ -672,7 +673,7 @@ int xtx_query_limits(xtx_ctx* ctx, xtx_limits* out) {
     out->wave_size = ctx->prop.warpSize;
     out->max_n = kMaxN;
     int mx = 0;
-    for (int n = 2; n <= kMaxN; ++n)
+    for (int n = 2; n <= kMaxGroupN; ++n)
         if (layout_for(n).rep_bytes <= g_lds_block) mx = n;
     out->max_n_lds = mx;
     out->hbm_bytes = (int64_t)ctx->prop.totalGlobalMem;
@@ -973,7 +974,8 @@ int xtx_movelog_free(xtx_ctx* ctx) {
 // ---------------------------------------------------------------- probes
 int xtx_probe_moves(xtx_ctx* ctx, const int64_t* A, const int64_t* Xt, int32_t n, int32_t n_mats,
                     const int32_t* queries, int32_t n_queries, int64_t* out) {
-    if (!ctx || !A || !Xt || !queries || !out || n < 2 || n > kMaxN || n_mats < 1) return XTX_E_ARG;
+    if (!ctx || !A || !Xt || !queries || !out || n < 2 || n > kMaxGroupN || n_mats < 1)
+        return XTX_E_ARG;
     try {
         for (int q = 0; q < n_queries; ++q) {
             const int32_t* x = queries + 4 * q;
@@ -1437,7 +1439,10 @@ void fill_launch(const Layout& L, bool tuned, xtx_launch* out) {
 
 // A configuration the kernels can run for this order, or an error message.
 const char* launch_invalid(int nmax, int G, int C, bool tri, int waves, int K, bool xl) {
-    if (nmax < 8 || nmax > kMaxN || nmax % 8) return "order must be a multiple of 8 in [8, 128]";
+    if (nmax < 8 || nmax > kMaxN || nmax % 8) return "order must be a multiple of 8 in [8, 256]";
+    if (nmax > kMaxGroupN && (G != 1 || (nmax != 192 && nmax != 256)))
+        return "orders above 128 run the wide lane kernel only (lanes 1, cols = order, order "
+               "192 or 256, lookahead 32)";
     if (G * C < nmax) return "lanes * cols must cover the order";
     if (G == 1) {
         // the lane kernel: one lane per proposal, the whole row per lane

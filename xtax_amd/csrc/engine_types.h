@@ -19,7 +19,8 @@ namespace xtx {
 
 constexpr int64_t kLimit = (int64_t)1 << 48;  // mat_io.hpp:35 (strict >)
 constexpr int kShearCap = 64;                 // mat_io.hpp:38
-constexpr int kMaxN = 128;
+constexpr int kMaxN = 256;       // largest order (the active list holds bytes)
+constexpr int kMaxGroupN = 128;  // largest order with A in LDS (all but the wide lane kernel)
 constexpr int kMaxSamples = 16;
 constexpr int kRepBits = 20;                  // solve key = move << 20 | replica
 

@@ -99,9 +99,19 @@ inline Layout make_layout(int nmax, int G, int C, bool tri, int waves, int K = 1
     return L;
 }
 
+// Orders above 128 (A does not fit LDS): the lane kernel's wide mode, A exact
+// in HBM, 32 proposals per replica and step, each evaluated by the replica's
+// 32 lanes (6 columns per lane to order 192, 8 to 256). The layout's order is
+// the kernel's column bound, 192 or 256.
+inline Layout wide_layout(int nmax) {
+    const int nw = nmax <= 192 ? 192 : 256;
+    return make_layout(nw, 1, nw, true, 0, 32);
+}
+
 // the rule used when a bucket has not been tuned
 inline Layout layout_for(int n) {
     const int nmax = order_of(n);
+    if (nmax > kMaxGroupN) return wide_layout(nmax);
     if (nmax <= 16) return make_layout(nmax, 16, 1, true, 0);
     if (nmax <= 32) return make_layout(nmax, 32, 1, true, 0);
     if (nmax <= 64) return make_layout(nmax, 64, 1, true, 0);
@@ -139,6 +149,7 @@ constexpr int kLaneMaxOrder = 48;
 inline int lane_q(int nmax) { return nmax >= 40 ? 32 : 16; }
 inline Layout lane_layout(int nmax) { return make_layout(nmax, 1, nmax, true, 0, lane_q(nmax)); }
 inline bool lane_layout_ok(int nmax) { return nmax <= kLaneMaxOrder; }
+
 
 // The lookahead rule. Plain and batch calls: 16 (measured: 256 replicas of
 // order 24 run 5.4x the moves/s of one group per replica, the full batch of
