@@ -84,6 +84,20 @@ def base_line(args, value, unit, el, config, higher=True):
             "dtype": "int64", "data": "synthetic", "config": config}
 
 
+def measured_traffic(profile):
+    """HBM bytes of one epoch of the dominant kernel from its committed rocprof
+    summary (FETCH_SIZE and WRITE_SIZE passes scaled by the copy-kernel
+    calibration), or None."""
+    import re
+    path = os.path.join(ROOT, "profiles", profile)
+    try:
+        with open(path) as f:
+            m = re.search(r"traffic: ([0-9.]+e[0-9]+) bytes per epoch", f.read())
+    except OSError:
+        return None
+    return float(m.group(1)) if m else None
+
+
 def roofline(kernel_ms, moves, mean_n, greedy, kernel="k_anneal"):
     """Algorithmic integer work of the annealing kernel over its measured time.
 
@@ -312,6 +326,11 @@ def bench_batch(args, eng):
                 "kernel_moves_per_sec": moves / (kernel_ms / 1e3) if kernel_ms else None,
                 "roofline": roofline(kernel_ms, moves, mean_n, 0.5, kernel_name(
                     lane_instances, args.count * args.steps)), "cpu_baseline": None})
+    if out["roofline"] and out["roofline"]["kernel"] == "k_anneal_lane":
+        # HBM bytes per epoch (the three launches of the lane kernel's order
+        # pairs) measured by rocprof on this workload
+        out["roofline"]["traffic"] = measured_traffic("k_anneal_lane_batch1024.md")
+        out["roofline"]["traffic_unit"] = "bytes per epoch (profiles/k_anneal_lane_batch1024.md)"
     out["by_order"] = {
         name: {"instances": b[0], "solved_fraction": b[1] / b[0],
                "p50_latency_s": pct(b[2], 50), "p99_latency_s": pct(b[2], 99)}
@@ -398,8 +417,10 @@ def bench_plain(args, eng):
     out = base_line(args, solved / el, "matrices/s", el, cfg)
     out.update({"solved": solved, "instances": len(mats), "p50_latency_s": pct(lat, 50),
                 "p99_latency_s": pct(lat, 99), "moves_per_sec": moves / el,
-                "roofline": roofline(kernel_ms, moves, mean_n, 0.5, kernel_name(
-                    lane_instances, args.count * args.steps)), "cpu_baseline": None})
+                # one instance per launch: its 32 replicas are too few for the
+                # lane kernel, so the lookahead lane-group kernel runs
+                "roofline": roofline(kernel_ms, moves, mean_n, 0.5, "k_anneal"),
+                "cpu_baseline": None})
     return out
 
 
