@@ -36,7 +36,8 @@ extern "C" {
 #define XTX_SHEAR_CAP 64                          /* mat_io.hpp:38 */
 #define XTX_MAX_N 256                             /* largest order per replica (xdual:
                                                      128) */
-#define XTX_MAX_TARGET_SAMPLES 16
+#define XTX_MAX_TARGET_SAMPLES 4096  /* tournaments above 16 samples run the lane
+                                        kernel's wide mode (xdual: 16) */
 
 typedef struct xtx_ctx xtx_ctx;
 

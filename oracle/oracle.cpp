@@ -15,6 +15,7 @@
 #include <cstring>
 #include <random>
 #include <utility>
+#include <array>
 #include <vector>
 
 namespace {
@@ -509,7 +510,9 @@ int64_t orc_replay(int64_t* A, int64_t* Xt, int32_t n, const orc_log_entry* log,
 // Per replica r and move m (1-based), the engine draws Philox blocks
 //   W(c) = philox4x32_10(key = {seed_lo, seed_hi}, ctr = {q_lo, q_hi, r, 16 t + c})
 // with q = (m - 1) / 4, t = (m - 1) mod 4, for slots c = 0 .. 15 (one block
-// call covers the 64 slots of four consecutive moves), using the words:
+// call covers the 64 slots of four consecutive moves); a tournament of more
+// than 16 samples takes slot c >= 16 from counter word 64 (c / 16) + 16 t +
+// c mod 16, above every word of the first 16 slots. The words:
 //   W(c).x  row-tournament candidate c       W(c).y  pivot-tournament cand. c
 //   W(0).z  targeted-pick coin               W(0).w  greedy coin
 //   W(1).z  sign of the +/-1 shear (bit 0)   W(1).w  Metropolis uniform, high
@@ -553,9 +556,9 @@ inline double u53(uint32_t hi, uint32_t lo) {
 //     as their inverse)
 int g_mutation = 0;
 
-// The engine's draws for one move: Philox words of slots 0 .. 15 (above).
+// The engine's draws for one move: Philox words of its slots (above).
 struct PhiloxDraw {
-    uint32_t W[32][4];
+    std::vector<std::array<uint32_t, 4>> W = std::vector<std::array<uint32_t, 4>>(16);
     uint32_t N = 0;
     int row = 0, piv = 0;  // tournament candidates drawn so far
     bool coin(double f) const { return u01(W[0][2]) < f; }
@@ -627,13 +630,14 @@ struct Model {
     bool ref = false;
     double rung_T = 0;  // tempering: the worker's (fixed) rung temperature
 
-    // Slot c of move m: Philox block at counter (q, r, 16 t + c) with
-    // q = (m - 1) / 4 and t = (m - 1) mod 4, so one block call can serve
-    // four consecutive moves (16 slots each).
+    // Slot c of move m: Philox block at counter (q, r, 64 (c / 16) + 16 t +
+    // c mod 16) with q = (m - 1) / 4 and t = (m - 1) mod 4, so one block
+    // call can serve four consecutive moves (16 slots each).
     void block(uint64_t m, uint32_t r, uint32_t c, uint32_t w[4]) const {
         const uint64_t q = (m - 1) >> 2;
         const uint32_t t = (uint32_t)((m - 1) & 3);
-        const uint32_t ctr[4] = {(uint32_t)q, (uint32_t)(q >> 32), r, 16 * t + c};
+        const uint32_t ctr[4] = {(uint32_t)q, (uint32_t)(q >> 32), r,
+                                 64 * (c >> 4) + 16 * t + (c & 15)};
         orc_philox4x32_10(ctr, key, w);
     }
 
@@ -695,7 +699,8 @@ struct Model {
         PhiloxDraw D;
         D.N = (uint32_t)N;
         const int G = std::max(p.target_samples, 5);
-        for (int g = 0; g < G; ++g) block(m, r, (uint32_t)g, D.W[g]);
+        D.W.resize((size_t)G);
+        for (int g = 0; g < G; ++g) block(m, r, (uint32_t)g, D.W[(size_t)g].data());
         step_with(R, D, m);
     }
 

@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <array>
 #include <vector>
 
 #include "oracle.h"
@@ -308,7 +309,8 @@ struct DualModel {
     void block(uint64_t m, uint32_t r, uint32_t c, uint32_t w[4]) const {
         const uint64_t q = (m - 1) >> 2;
         const uint32_t t = (uint32_t)((m - 1) & 3);
-        const uint32_t ctr[4] = {(uint32_t)q, (uint32_t)(q >> 32), r, 16 * t + c};
+        const uint32_t ctr[4] = {(uint32_t)q, (uint32_t)(q >> 32), r,
+                                 64 * (c >> 4) + 16 * t + (c & 15)};
         orc_philox4x32_10(ctr, key, w);
     }
     void log_commit(DualRep& R, uint64_t m, int i, int j, int64_t s, uint16_t flags) {
@@ -384,8 +386,8 @@ struct DualModel {
         const uint64_t m = ++R.moves;
         if (tempering) R.T = temps[(size_t)R.rung];
         const int G = std::max(p.target_samples, 5);
-        uint32_t W[32][4];
-        for (int g = 0; g < G; ++g) block(m, r, (uint32_t)g, W[g]);
+        std::vector<std::array<uint32_t, 4>> W((size_t)G);
+        for (int g = 0; g < G; ++g) block(m, r, (uint32_t)g, W[(size_t)g].data());
 
         int i, j;
         if (du01(W[0][2]) < R.tf) {

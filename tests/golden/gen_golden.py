@@ -35,6 +35,8 @@ commit and reseed, the final state and the global best, capped at a move
 count), over random configurations of both temperature modes; and
 single-threaded runs of the reference binary with non-default flags that stop
 at a diagonal (deterministic), into g5_worker.npz (`gen_golden.py g5`).
+G5b: 21 more worker traces with tournaments of 17 to 257 samples, into
+g5b_samples.npz (`gen_golden.py g5b`).
 """
 import ctypes as C
 import os
@@ -681,6 +683,27 @@ def g5():
     np.savez_compressed(os.path.join(OUT, "g5_worker.npz"), **out)
 
 
+def g5b():
+    """Worker traces of the reference's anneal_worker with tournaments of more
+    than 16 samples (17 to 257), the G5 configurations otherwise."""
+    from oracle import refharness
+    rng = np.random.default_rng(2026)
+    out = {}
+    sizes = [17, 24, 31, 40, 64, 100, 257]
+    for t, (A, X0, act, dp, ip) in enumerate(g5_configs(rng, 21)):
+        ip = ip.copy()
+        ip[2] = sizes[t % len(sizes)]
+        r = refharness.trace(A, X0, dp, ip, act)
+        out.update({f"t{t}_A": A, f"t{t}_X0": X0, f"t{t}_act": act, f"t{t}_dp": dp,
+                    f"t{t}_ip": ip, f"t{t}_Af": r["A"], f"t{t}_Xtf": r["Xt"],
+                    f"t{t}_gbA": r["gbA"], f"t{t}_gbXt": r["gbXt"],
+                    f"t{t}_log": np.stack([r["log"]["move"], r["log"]["i"], r["log"]["j"],
+                                           r["log"]["s"], r["log"]["flags"]], 1).astype(np.int64),
+                    f"t{t}_sc": np.array([r["moves"], r["score"], r["gb_score"], r["solved"]])})
+    out["traces"] = np.array(21)
+    np.savez_compressed(os.path.join(OUT, "g5b_samples.npz"), **out)
+
+
 def run_ref_x(A, X0, args, timeout=60):
     """run_ref with an optional -X start transform written as X.csv."""
     with tempfile.TemporaryDirectory() as td:
@@ -717,6 +740,8 @@ if __name__ == "__main__":
         g4(load_dual_harness())
     if "g5" in which:
         g5()
+    if "g5b" in which:
+        g5b()
     for f in sorted(os.listdir(OUT)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(OUT, f)), file=sys.stderr)

@@ -1,10 +1,13 @@
-"""Orders above 128 (the reference has no order limit): the lane kernel's
-wide mode (anneal_lane.hip, WIDE: A exact in HBM, 32 proposals per replica
-and step, each evaluated by the replica's 32 lanes) against the oracle's
-trajectory model, bit for bit -- every replica's A, X^T and scalars and the
-accepted-move log -- at orders 130 to 256, tempering and cooling with stall
-sweeps and reseeds; a batch mixing wide and lane-kernel orders equal to its
-instances run alone; the command line end to end at n = 140."""
+"""Orders above 128 and tournaments above 16 samples (the reference limits
+neither): the lane kernel's wide mode (anneal_lane.hip, WIDE: A exact in HBM,
+32 proposals per replica and step, each evaluated by the replica's 32 lanes,
+tournaments of any size) against the oracle's trajectory model, bit for bit
+-- every replica's A, X^T and scalars and the accepted-move log -- at orders
+130 to 256, tempering and cooling with stall sweeps and reseeds; tournaments
+of 17 to 300 samples at every order (their draws past slot 15 on their own
+counters, the model pinned against the reference with such tournaments in
+tests/test_model_pinned.py); a batch mixing wide and lane-kernel orders equal
+to its instances run alone; the command line end to end at n = 140."""
 import os
 import subprocess
 import sys
@@ -78,6 +81,43 @@ def test_wide_cooling_sweeps_reseeds(eng, orc, n):
                                    exchange_interval=120, stuck_threshold=40,
                                    sweep_fraction=0.5, reseed_factor=1.0, target_samples=5),
                 epochs=3)
+
+
+@pytest.mark.parametrize("n,ts,temper", [(5, 17, 1), (12, 24, 1), (40, 40, 0), (70, 100, 1),
+                                         (100, 33, 0), (150, 300, 1), (256, 20, 0)])
+def test_large_tournaments_match_model(eng, orc, n, ts, temper):
+    A0 = (workloads.planted(n, 100, 2 * n, seed=n + ts)[0] if n % 2 == 0
+          else workloads.random_symmetric(n, -20, 20, seed=n + ts))
+    cfg = eng.launch_get(n)
+    say(f"target_samples={ts} n={n} tempering={temper}")
+    kw = dict(replicas=3, seed=ts, target_samples=ts, target_fraction=0.7, t_init=100.0,
+              t_min=2.0, exchange_interval=160, tempering=bool(temper))
+    if not temper:
+        kw.update(stuck_threshold=50, sweep_fraction=0.5, reseed_factor=1.0)
+    _trajectory(eng, orc, A0, kw, epochs=3)
+
+
+def test_large_tournament_batch_equals_alone(eng):
+    mats = [workloads.planted(n, 100, 3 * n, seed=n)[0] for n in (9, 30, 47, 64, 90, 160)]
+    seeds = [501 + q for q in range(len(mats))]
+    kw = dict(replicas=4, t_init=60.0, t_min=10.0, exchange_interval=200, max_epochs=3,
+              target_samples=24, target_fraction=1.0)
+    say("batch with 24-sample tournaments")
+    res, st = eng.batch_solve(mats, EngineParams(**kw), seeds=seeds)
+    for q, A0 in enumerate(mats):
+        alone = eng.run_annealer(A0, None, EngineParams(seed=seeds[q], **kw))
+        A, X, d = res[q]
+        assert d["verify_status"] == 0 and d["solved"] == alone.stats["solved"], q
+        assert np.array_equal(A, alone.A) and np.array_equal(X, alone.X), q
+
+
+def test_xdual_refuses_large_tournaments(eng):
+    from xtax_amd import DualParams
+    from xtax_amd._native import XtxError
+    P0 = workloads.lattice(8, 8, seed=3)[1]
+    with pytest.raises(XtxError, match="target_samples"):
+        eng.run_xdual(P0, None, EngineParams(replicas=2, max_epochs=1, target_samples=17),
+                      DualParams())
 
 
 def test_wide_batch_equals_alone(eng):

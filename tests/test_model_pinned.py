@@ -16,7 +16,8 @@ rule, it must reproduce the reference exactly:
   cooling, uniform pairs, --gram-rows, -X);
 * G5 traces of canneal::anneal_worker itself (one worker of a pool, recorded
   through an objective wrapper): every accepted move, sweep commit and reseed,
-  the final state and the global best, in both temperature modes;
+  the final state and the global best, in both temperature modes; G5b the
+  same with tournaments of 17 to 257 samples;
 * live, where the reference harness is built here: random configurations.
 
 What the reference decides with wall-clock timing (the 5 ms exchange monitor,
@@ -42,6 +43,11 @@ def g2():
 @pytest.fixture(scope="module")
 def g5():
     return np.load(os.path.join(GOLD, "g5_worker.npz"))
+
+
+@pytest.fixture(scope="module")
+def g5b():
+    return np.load(os.path.join(GOLD, "g5b_samples.npz"))
 
 
 @pytest.fixture
@@ -163,6 +169,33 @@ def test_reproduces_reference_worker_traces(orc, g5):
     # the fixtures exercise sweeps, reseeds and both outcomes
     flags = np.concatenate([g5[f"t{t}_log"][:, 4] for t in range(T)])
     assert (flags == 1).any() and (flags == 2).any()
+
+
+def test_reproduces_reference_worker_traces_large_tournaments(orc, g5b):
+    """Tournaments of more than 16 samples (the engine's wide mode draws them)."""
+    T = int(g5b["traces"])
+    sizes = set()
+    for t in range(T):
+        A, X0, act, dp, ip, want = _trace_case(g5b, t)
+        assert ip[2] > 16
+        sizes.add(int(ip[2]))
+        assert _same_trace(orc, A, X0, act, dp, ip, want), t
+    assert len(sizes) >= 6
+
+
+def test_slots_past_16_leave_the_first_16_alone(orc):
+    """The engine's draw counters: slot c of move m at word 64 (c / 16) +
+    16 ((m - 1) mod 4) + c mod 16, so the slots every tournament of at most 16
+    samples uses keep their counters, and no two (move, slot) pairs of a
+    replica share one (brute force over four moves and 1024 slots)."""
+    words = {}
+    for t in range(4):
+        for c in range(1024):
+            w = 64 * (c >> 4) + 16 * t + (c & 15)
+            if c < 16:
+                assert w == 16 * t + c
+            assert w not in words, (t, c, words.get(w))
+            words[w] = (t, c)
 
 
 def test_live_against_reference_worker(orc):

@@ -535,25 +535,46 @@ __device__ __attribute__((always_inline)) inline void lane_epoch(
         uint32_t wz[5], ww[5], pk[kMaxSamples / 4];
         uint32_t tpos = 0;
         int64_t tw = -1;
+        // hot-row tournament: first maximum of the row masses
+        auto hot = [&](int c, uint32_t wx) __attribute__((always_inline)) {
+            const uint32_t pos = mulhi32(wx, (uint32_t)N);
+            const int64_t w = rp.sM[rp.act(pos)];
+            if (c == 0 || w > tw) {
+                tw = w;
+                tpos = pos;
+            }
+        };
+        if constexpr (WIDE) {
+            // any number of samples: slots 0..4 for their words, the rest for
+            // the hot-row draws only (the group redraws the pivot ones)
 #pragma unroll
-        for (int c = 0; c < kMaxSamples; ++c) {
-            if (c % 4 == 0) pk[c / 4] = 0u;
-            if (c < 5 || c < TS) {
+            for (int c = 0; c < 5; ++c) {
                 const uint4 W = philox10((uint32_t)qq, (uint32_t)(qq >> 32), (uint32_t)r,
-                                         c3 + (uint32_t)c, key0, key1);
-                if (c < 5) {
-                    wz[c] = W.z;
-                    ww[c] = W.w;
-                }
-                if (c < TS) {
-                    // hot-row tournament: first maximum of the row masses
-                    const uint32_t pos = mulhi32(W.x, (uint32_t)N);
-                    const int64_t w = rp.sM[rp.act(pos)];
-                    if (c == 0 || w > tw) {
-                        tw = w;
-                        tpos = pos;
+                                         slot_word(c3, c), key0, key1);
+                wz[c] = W.z;
+                ww[c] = W.w;
+                if (c < TS) hot(c, W.x);
+            }
+            for (int c = 5; c < TS; ++c) {
+                const uint4 W = philox10((uint32_t)qq, (uint32_t)(qq >> 32), (uint32_t)r,
+                                         slot_word(c3, c), key0, key1);
+                hot(c, W.x);
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < kMaxSamples; ++c) {
+                if (c % 4 == 0) pk[c / 4] = 0u;
+                if (c < 5 || c < TS) {
+                    const uint4 W = philox10((uint32_t)qq, (uint32_t)(qq >> 32), (uint32_t)r,
+                                             c3 + (uint32_t)c, key0, key1);
+                    if (c < 5) {
+                        wz[c] = W.z;
+                        ww[c] = W.w;
                     }
-                    pk[c / 4] |= mulhi32(W.y, (uint32_t)N) << (8 * (c % 4));
+                    if (c < TS) {
+                        hot(c, W.x);
+                        pk[c / 4] |= mulhi32(W.y, (uint32_t)N) << (8 * (c % 4));
+                    }
                 }
             }
         }
@@ -619,7 +640,54 @@ __device__ __attribute__((always_inline)) inline void lane_epoch(
             const int src = gbase + b;
             const int bj = from_lane32(j, src);
             int bi = from_lane32(i, src);
-            if (bi < 0) {
+            if (bi < 0 && WIDE) {
+                // the pivot tournament of move m0 + b, its draws remade by the
+                // group Q at a time: lane qd takes draw c = base + qd, reads
+                // |A[bj][cand]| from the lane owning column cand (row bj held
+                // C columns per lane), and the group keeps the first maximum
+                // through the key |a| 2^12 + 4095 - c (c < 4096)
+                const uint64_t mgb = m0 + (uint64_t)b;
+                const uint64_t qqb = (mgb - 1) >> 2;
+                const uint32_t c3b = 16u * (uint32_t)((mgb - 1) & 3u);
+                int64_t rowj[CQ];
+#pragma unroll
+                for (int e = 0; e < CQ; ++e) {
+                    const int k = qd + e * Q;
+                    rowj[e] = k < n ? rp.gA[bj * n + k] : 0;
+                }
+                int64_t bkey = -1;
+                int bcand = -1;
+                for (int base = 0; base < TS; base += Q) {
+                    const int c = base + qd;
+                    int cand = 0;
+                    if (c < TS) {
+                        uint32_t kk0 = k0, kk1 = k1;
+                        opaque_u32(kk0);
+                        opaque_u32(kk1);
+                        const uint4 W = philox10((uint32_t)qqb, (uint32_t)(qqb >> 32), (uint32_t)r,
+                                                 slot_word(c3b, c), kk0, kk1);
+                        cand = rp.act(mulhi32(W.y, (uint32_t)N));
+                    }
+                    const int owner = gbase + (cand & (Q - 1));
+                    int64_t val = 0;
+#pragma unroll
+                    for (int e = 0; e < CQ; ++e) {
+                        const int64_t t = from_lane64(rowj[e], owner);
+                        if (e == cand / Q) val = t;
+                    }
+                    const int64_t key =
+                        (c < TS && cand != bj) ? (iabs64(val) << 12) | (int64_t)(4095 - c) : -1;
+                    if (key > bkey) {
+                        bkey = key;
+                        bcand = cand;
+                    }
+                }
+                const int64_t mx = Grp<Q>::max(bkey);
+                if (mx >= 0) {
+                    const uint32_t hit = group_bits<Q>(bkey == mx, gbase);
+                    bi = from_lane32(bcand, gbase + __builtin_ctz(hit));
+                }
+            } else if (bi < 0) {
                 // the pivot tournament on a row with large entries
                 int64_t vbest = -1;
 #pragma unroll
@@ -636,11 +704,12 @@ __device__ __attribute__((always_inline)) inline void lane_epoch(
                         }
                     }
                 }
-                if (bi < 0) {
-                    const uint32_t btp = (uint32_t)from_lane32((int32_t)tpos, src);
-                    const uint32_t bw3 = (uint32_t)from_lane32((int32_t)ww[3], src);
-                    bi = rp.act((btp + 1u + mulhi32(bw3, (uint32_t)(N - 1))) % (uint32_t)N);
-                }
+            }
+            if (bi < 0) {
+                // no draw other than the target: the fallback pivot
+                const uint32_t btp = (uint32_t)from_lane32((int32_t)tpos, src);
+                const uint32_t bw3 = (uint32_t)from_lane32((int32_t)ww[3], src);
+                bi = rp.act((btp + 1u + mulhi32(bw3, (uint32_t)(N - 1))) % (uint32_t)N);
             }
             const bool bgr = from_lane32(gr ? 1 : 0, src) != 0;
             const int32_t bpm = from_lane32(pm, src);
@@ -767,9 +836,11 @@ __device__ __attribute__((always_inline)) inline void lane_epoch(
 // leaves idle. `nmax`: the launch's largest order (NHI, or NLO when no entry
 // is of order NHI).
 //
-// WIDE (orders above 128, NLO = 192 and NHI = 256 as column bounds of 6 and 8
-// columns per lane): A lives in HBM only and every proposal is evaluated by
-// the replica's lane group, one after another.
+// WIDE (NLO, NHI = 64, 128 or 192, 256: column bounds of 2, 4, 6 and 8 columns
+// per lane; orders above 128, and every order when a call's tournaments take
+// more than 16 samples): A lives in HBM only and every proposal is evaluated
+// by the replica's lane group, one after another; the tournaments take any
+// number of samples.
 template <int NLO, int NHI, int Q, bool WIDE = false>
 __global__ void __launch_bounds__(256, lane_waves_per_simd(NHI, Q))
     k_anneal_lane(const RepRef* __restrict__ table, int count, int split,
@@ -796,7 +867,7 @@ static const LaneVariant kLaneVariants[] = {
     {8, 16, 16, k_anneal_lane<8, 16, 16>},  {24, 32, 16, k_anneal_lane<24, 32, 16>},
     {40, 48, 16, k_anneal_lane<40, 48, 16>}, {8, 16, 32, k_anneal_lane<8, 16, 32>},
     {24, 32, 32, k_anneal_lane<24, 32, 32>}, {40, 48, 32, k_anneal_lane<40, 48, 32>},
-    {192, 256, 32, k_anneal_lane<192, 256, 32, true>},
+    {64, 128, 32, k_anneal_lane<64, 128, 32, true>}, {192, 256, 32, k_anneal_lane<192, 256, 32, true>},
 };
 
 static const LaneVariant* find_lane_variant(int nmax, int Q) {
@@ -816,7 +887,8 @@ int lane_pair_hi(int nmax) {
 
 // LDS of one replica of the lane kernel at order nmax (LaneLds)
 int lane_rep_bytes(int nmax) {
-    const int b = (nmax > kMaxGroupN ? 0 : 2 * nmax * nmax) + 8 * nmax + 4 * nmax + nmax;
+    // the wide mode (every order above 48 this kernel runs) has no square
+    const int b = (nmax > 48 ? 0 : 2 * nmax * nmax) + 8 * nmax + 4 * nmax + nmax;
     return (b + 15) / 16 * 16;
 }
 

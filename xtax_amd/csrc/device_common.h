@@ -51,6 +51,12 @@ __device__ __forceinline__ uint4 philox10(uint32_t c0, uint32_t c1, uint32_t c2,
 }
 
 __device__ __forceinline__ uint32_t mulhi32(uint32_t w, uint32_t n) { return __umulhi(w, n); }
+// Last counter word of draw slot c of a move whose word for slot 0 is c3 =
+// 16 ((m - 1) mod 4): 16 slots per move in the words below 64, and slots past
+// 15 (tournaments of more than 16 samples) at 64 (c / 16) + c3 + c mod 16.
+__device__ __forceinline__ uint32_t slot_word(uint32_t c3, int c) {
+    return 64u * ((uint32_t)c >> 4) + c3 + ((uint32_t)c & 15u);
+}
 __device__ __forceinline__ double u01(uint32_t w) { return (double)w * 0x1p-32; }
 __device__ __forceinline__ double u53(uint32_t hi, uint32_t lo) {
     return (double)(((uint64_t)(hi >> 5) << 26) | (lo >> 6)) * 0x1p-53;

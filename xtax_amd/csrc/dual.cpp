@@ -198,6 +198,7 @@ int xtx_dual_solve(xtx_ctx* ctx, const int64_t* A0, const int64_t* X0, int32_t n
         HostInst& h = ctx->plain;
         h = HostInst{};
         if (n > kMaxGroupN) throw Err{XTX_E_ARG, "xdual: n exceeds 128 (an LDS-resident replica)"};
+        if (p.target_samples > kMaxSamples) throw Err{XTX_E_ARG, "xdual: target_samples must be in [1, 16]"};
         prepare(h, A0, X0, n, p.replicas, p.seed, active, n_active, p);
         const size_t nn = (size_t)n * n;
 

@@ -82,8 +82,8 @@ void check_params(const xtx_params& p) {
     auto bad = [](const char* m) { throw Err{XTX_E_ARG, m}; };
     if (p.replicas < 1) bad("replicas must be >= 1");
     if (p.replicas > (1 << kRepBits) - 1) bad("replicas too large");
-    if (p.target_samples < 1 || p.target_samples > kMaxSamples)
-        bad("target_samples must be in [1, 16]");
+    if (p.target_samples < 1 || p.target_samples > kMaxSamplesWide)
+        bad("target_samples must be in [1, 4096]");
     if (p.exchange_interval < 1) bad("exchange_interval must be >= 1");
     if (p.moves_per_cool < 1) bad("moves_per_cool must be >= 1");
     if (!(p.cooling > 0.0)) bad("cooling must be > 0");
@@ -177,7 +177,7 @@ void prepare(HostInst& h, const int64_t* A0, const int64_t* X0, int n, int R, ui
     h.P.max_epochs = p.max_epochs;
     h.log_on = p.move_log != 0;
     h.log_cap = h.log_on ? p.move_log_cap : 0;
-    h.lay = layout_for(n);
+    h.lay = p.target_samples > kMaxSamples ? wide_layout(order_of(n)) : layout_for(n);
     if (h.lay.rep_bytes > g_lds_block)
         throw Err{XTX_E_ARG, "n too large for an LDS-resident replica"};
 }
@@ -492,7 +492,9 @@ bool lane_load(const xtx_ctx* ctx, int64_t replicas) {
 }
 
 void apply_layout(const xtx_ctx* ctx, HostInst& h, int requested, int rule, bool lane = false) {
-    h.lay = choose_layout(ctx, h.n, requested, rule, lane);
+    // tournaments of more than 16 samples: only the wide mode draws them
+    h.lay = h.P.target_samples > kMaxSamples ? wide_layout(order_of(h.n))
+                                             : choose_layout(ctx, h.n, requested, rule, lane);
     if (h.lay.rep_bytes > g_lds_block)
         throw Err{XTX_E_ARG, "n too large for an LDS-resident replica"};
 }
@@ -1489,7 +1491,7 @@ std::vector<Layout> candidates(int nmax) {
         }
     for (int Q : {16, 32})
         for (int w : {1, 2, 4})
-            if (!launch_invalid(nmax, 1, nmax, true, w, Q, false))
+            if (nmax <= kLaneMaxOrder && !launch_invalid(nmax, 1, nmax, true, w, Q, false))
                 v.push_back(make_layout(nmax, 1, nmax, true, w, Q));
     return v;
 }
@@ -1555,6 +1557,8 @@ double time_layout(xtx_ctx* ctx, const std::vector<const HostInst*>& proto, cons
 void autotune_orders(xtx_ctx* ctx, const xtx_instance* sample, int count,
                      const xtx_params& params, int moves, bool all,
                      std::vector<xtx_launch>* out) {
+    // tournaments of more than 16 samples run the wide mode whatever is set
+    if (params.target_samples > kMaxSamples) return;
     xtx_params p = params;
     p.exchange_interval = moves;
     p.max_epochs = 1;

@@ -21,7 +21,8 @@ constexpr int64_t kLimit = (int64_t)1 << 48;  // mat_io.hpp:35 (strict >)
 constexpr int kShearCap = 64;                 // mat_io.hpp:38
 constexpr int kMaxN = 256;       // largest order (the active list holds bytes)
 constexpr int kMaxGroupN = 128;  // largest order with A in LDS (all but the wide lane kernel)
-constexpr int kMaxSamples = 16;
+constexpr int kMaxSamples = 16;     // tournament samples of every kernel but the wide mode
+constexpr int kMaxSamplesWide = 4096;  // the wide mode's (its pivot key holds 12 bits)
 constexpr int kRepBits = 20;                  // solve key = move << 20 | replica
 
 // Weighted-median histogram of the greedy shear (one per replica, LDS): bin

@@ -99,12 +99,13 @@ inline Layout make_layout(int nmax, int G, int C, bool tri, int waves, int K = 1
     return L;
 }
 
-// Orders above 128 (A does not fit LDS): the lane kernel's wide mode, A exact
-// in HBM, 32 proposals per replica and step, each evaluated by the replica's
-// 32 lanes (6 columns per lane to order 192, 8 to 256). The layout's order is
-// the kernel's column bound, 192 or 256.
+// The lane kernel's wide mode: A exact in HBM, 32 proposals per replica and
+// step, each evaluated by the replica's 32 lanes (2, 4, 6 or 8 columns per
+// lane to order 64, 128, 192, 256), tournaments of any size. Orders above 128
+// (A does not fit LDS) and calls whose tournaments take more than 16 samples
+// run it. The layout's order is the kernel's column bound.
 inline Layout wide_layout(int nmax) {
-    const int nw = nmax <= 192 ? 192 : 256;
+    const int nw = nmax <= 64 ? 64 : nmax <= 128 ? 128 : nmax <= 192 ? 192 : 256;
     return make_layout(nw, 1, nw, true, 0, 32);
 }
 
