@@ -1332,9 +1332,11 @@ __global__ void __launch_bounds__(G * K > 256 ? 512 : 256, K == 1 ? 4 : 2)
 // upward so a configuration can travel several rungs in one sweep.
 constexpr int kSweepChunk = 512;
 
-__global__ void __launch_bounds__(256) k_epoch_end(Inst* __restrict__ insts) {
+// One block per instance: instance list[b] (b when list is null).
+__global__ void __launch_bounds__(256) k_epoch_end(Inst* __restrict__ insts,
+                                                   const int* __restrict__ list) {
     extern __shared__ __align__(16) unsigned char smem[];
-    Inst* I = insts + blockIdx.x;
+    Inst* I = insts + (list ? list[blockIdx.x] : (int)blockIdx.x);
     if (I->status != ST_RUNNING) return;
     const int t = (int)threadIdx.x;
     const int R = I->R;
@@ -1683,10 +1685,11 @@ hipError_t launch_anneal(hipStream_t st, int G, int C, bool tri, int K, bool xl,
     return hipGetLastError();
 }
 
-hipError_t launch_epoch_end(hipStream_t st, Inst* insts, int count, int max_r) {
+// list (nullable): the count instances to close, as indices into insts
+hipError_t launch_epoch_end(hipStream_t st, Inst* insts, int count, int max_r, const int* list) {
     if (count <= 0) return hipSuccess;
     const size_t lds = ((size_t)max_r * sizeof(int) + 7) / 8 * 8 + 4 * kSweepChunk * sizeof(double);
-    hipLaunchKernelGGL(k_epoch_end, dim3(count), dim3(256), lds, st, insts);
+    hipLaunchKernelGGL(k_epoch_end, dim3(count), dim3(256), lds, st, insts, list);
     return hipGetLastError();
 }
 

@@ -24,7 +24,8 @@ hipError_t launch_dual_maint(hipStream_t st, Inst* insts, int count, int max_r);
 hipError_t launch_dual_probe(hipStream_t st, int G, int C, const int64_t* P, const double* Q,
                              int n, double cw, const int32_t* qs, int nq, int64_t* out_i,
                              double* out_d, int64_t* rows_p, double* rows_q, double* scores);
-hipError_t launch_epoch_end(hipStream_t st, Inst* insts, int count, int max_r);
+hipError_t launch_epoch_end(hipStream_t st, Inst* insts, int count, int max_r,
+                            const int* list = nullptr);
 hipError_t launch_replicate(hipStream_t st, int64_t* dst, const int64_t* src, int64_t nn, int R);
 }  // namespace xtx
 

@@ -37,7 +37,8 @@ hipError_t launch_anneal(hipStream_t st, int G, int C, bool tri, int K, bool xl,
                          const RepRef* table, int count, Inst* insts, int nmax, int rep_bytes,
                          const volatile int* stop);
 bool anneal_variant_exists(int G, int C, bool tri, int K, bool xl = false);
-hipError_t launch_epoch_end(hipStream_t st, Inst* insts, int count, int max_r);
+hipError_t launch_epoch_end(hipStream_t st, Inst* insts, int count, int max_r,
+                            const int* list = nullptr);
 hipError_t launch_probe(hipStream_t st, int G, int C, const int64_t* A, const int64_t* Xt, int n,
                         const int32_t* q, int nq, int64_t* out);
 hipError_t launch_probe_rng(hipStream_t st, const uint32_t* ck, int count, uint32_t* out,
@@ -427,6 +428,11 @@ struct Bucket {
     // lane kernel: entries of the launch pair's higher order come first, this
     // many (whole blocks); -1: every entry is of the layout's order
     int split = -1;
+    // the instances (slot indices) whose replicas the table holds: their
+    // epoch ends follow this bucket's launch, so buckets advance independently
+    std::vector<int> insts;
+    int* d_insts = nullptr;
+    int insts_cap = 0;
 };
 
 // The lane kernel's split for a table whose entries are all of order L.nmax.
@@ -524,12 +530,15 @@ hipError_t launch_layout(hipStream_t st, const Layout& L, int units_per_block, c
                          L.rep_bytes, stop);
 }
 
-// Launches one epoch on ctx->stream: the bucket kernels, independent of one
-// another, spread over the main and side streams (largest work first, so the
-// device fills while the short buckets finish), then k_epoch_end after all of
-// them. Captured, the fork and join become graph edges.
-void launch_epoch(xtx_ctx* ctx, std::vector<Bucket>& buckets, Inst* d_insts, int n_insts,
-                  int max_r) {
+// Launches k epochs on ctx->stream. Every bucket is a chain of its own: its
+// kernel, then the epoch end (exchange, instance best, retirement) of its own
+// instances, k times; the buckets' instances never meet, so no bucket waits
+// for another between epochs. Chains go to the main and side streams (the
+// largest work first, so the device fills while the short ones finish); two
+// chains on one stream interleave epoch by epoch. Captured, the fork and join
+// become graph edges.
+void launch_epochs_chained(xtx_ctx* ctx, std::vector<Bucket>& buckets, Inst* d_insts, int max_r,
+                           int k) {
     std::vector<const Bucket*> order;
     for (auto& b : buckets)
         if (!b.refs.empty()) order.push_back(&b);
@@ -541,18 +550,19 @@ void launch_epoch(xtx_ctx* ctx, std::vector<Bucket>& buckets, Inst* d_insts, int
         HIPCHK(hipEventRecord(ctx->fork_ev, ctx->stream));
         for (int q = 0; q < ns; ++q) HIPCHK(hipStreamWaitEvent(ctx->side[q], ctx->fork_ev, 0));
     }
-    for (size_t q = 0; q < order.size(); ++q) {
-        const Bucket& b = *order[q];
-        const int lane = (int)(q % (size_t)(ns + 1));
-        hipStream_t st = lane == 0 ? ctx->stream : ctx->side[lane - 1];
-        HIPCHK(launch_layout(st, b.lay, b.units_per_block, b.d_refs, (int)b.refs.size(), d_insts,
-                             ctx->stop_dev, b.split));
-    }
+    for (int e = 0; e < k; ++e)
+        for (size_t q = 0; q < order.size(); ++q) {
+            const Bucket& b = *order[q];
+            const int lane = (int)(q % (size_t)(ns + 1));
+            hipStream_t st = lane == 0 ? ctx->stream : ctx->side[lane - 1];
+            HIPCHK(launch_layout(st, b.lay, b.units_per_block, b.d_refs, (int)b.refs.size(),
+                                 d_insts, ctx->stop_dev, b.split));
+            HIPCHK(launch_epoch_end(st, d_insts, (int)b.insts.size(), max_r, b.d_insts));
+        }
     for (int q = 0; q < ns; ++q) {
         HIPCHK(hipEventRecord(ctx->join_ev[q], ctx->side[q]));
         HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->join_ev[q], 0));
     }
-    HIPCHK(launch_epoch_end(ctx->stream, d_insts, n_insts, max_r));
 }
 
 }  // namespace
@@ -1080,8 +1090,10 @@ struct Server {
 
     ~Server() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
-        for (auto& kv : buckets)
+        for (auto& kv : buckets) {
             if (kv.second.d_refs) (void)hipFree(kv.second.d_refs);
+            if (kv.second.d_insts) (void)hipFree(kv.second.d_insts);
+        }
         if (d_insts) (void)hipFree(d_insts);
         if (pin_insts) (void)hipHostFree(pin_insts);
         if (fetch_stream) (void)hipStreamDestroy(fetch_stream);
@@ -1200,7 +1212,10 @@ struct Server {
             HIPCHK(hipMalloc(&d_insts, (size_t)cap_insts * sizeof(Inst)));
         }
         h_insts.assign((size_t)ns, Inst{});
-        for (auto& kv : buckets) kv.second.refs.clear();
+        for (auto& kv : buckets) {
+            kv.second.refs.clear();
+            kv.second.insts.clear();
+        }
         max_r = 1;
         for (int s = 0; s < ns; ++s) {
             if (!slots[(size_t)s]) {
@@ -1217,10 +1232,19 @@ struct Server {
             }
             for (int r = 0; r < h.R; ++r) b.refs.push_back(RepRef{s, r});
             pad_to_wave(b.refs, h.lay.G * h.lay.K, s);
+            b.insts.push_back(s);
         }
         for (auto& kv : buckets) {
             Bucket& b = kv.second;
             split_lane_bucket(b, slots);
+            if ((int)b.insts.size() > b.insts_cap) {
+                if (b.d_insts) (void)hipFree(b.d_insts);
+                b.insts_cap = std::max((int)b.insts.size(), 2 * b.insts_cap);
+                HIPCHK(hipMalloc(&b.d_insts, (size_t)b.insts_cap * sizeof(int)));
+            }
+            if (!b.insts.empty())
+                HIPCHK(hipMemcpyAsync(b.d_insts, b.insts.data(), b.insts.size() * sizeof(int),
+                                      hipMemcpyHostToDevice, ctx->stream));
             if ((int)b.refs.size() > b.cap) {
                 if (b.d_refs) (void)hipFree(b.d_refs);
                 b.cap = std::max((int)b.refs.size(), 2 * b.cap);
@@ -1269,7 +1293,7 @@ struct Server {
             std::vector<Bucket> bl = bucket_list();
             hipGraph_t graph;
             HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-            for (int e = 0; e < k; ++e) launch_epoch(ctx, bl, d_insts, ns, max_r);
+            launch_epochs_chained(ctx, bl, d_insts, max_r, k);
             HIPCHK(hipStreamEndCapture(ctx->stream, &graph));
             HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
             (void)hipGraphDestroy(graph);
@@ -1309,8 +1333,7 @@ struct Server {
                 if (gexec) (void)hipGraphExecDestroy(gexec);
                 hipGraph_t graph;
                 HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-                for (int e = 0; e < k; ++e)
-                    launch_epoch(ctx, bl, d_insts, ns, max_r);
+                launch_epochs_chained(ctx, bl, d_insts, max_r, k);
                 HIPCHK(hipStreamEndCapture(ctx->stream, &graph));
                 HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
                 (void)hipGraphDestroy(graph);
@@ -1321,7 +1344,7 @@ struct Server {
             HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
         } else {
             HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-            launch_epoch(ctx, bl, d_insts, ns, max_r);
+            launch_epochs_chained(ctx, bl, d_insts, max_r, 1);
             HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
         }
         HIPCHK(hipMemcpyAsync(h_insts.data(), d_insts, (size_t)ns * sizeof(Inst),
@@ -1579,6 +1602,8 @@ void autotune_orders(xtx_ctx* ctx, const xtx_instance* sample, int count,
         by_order[nmax].push_back(make_inst(ctx, in, p));
     }
     const int64_t target_replicas = (int64_t)ctx->prop.multiProcessorCount * 16 * 4 * 2;
+    // the lane kernel's timings per order and shape (proposals, waves)
+    std::map<int, std::map<std::pair<int, int>, double>> lane_ms;
     for (auto& kv : by_order) {
         const int nmax = kv.first;
         std::vector<const HostInst*> proto;
@@ -1592,7 +1617,6 @@ void autotune_orders(xtx_ctx* ctx, const xtx_instance* sample, int count,
         }
         double best_rate = -1;
         Layout best;
-        size_t first = out->size();
         // The batch and plain forms run the rule's lookahead depth; the tuner
         // chooses the lane shape at that depth only. Depths do not rank here
         // as they do in a run: one epoch from the start matrices commits
@@ -1604,6 +1628,7 @@ void autotune_orders(xtx_ctx* ctx, const xtx_instance* sample, int count,
             if (L.K != rule_K && L.G != 1) continue;
             const double ms = time_layout(ctx, proto, L, p);
             const double rate = ms > 0 ? (double)reps * moves / (ms * 1e-3) : 0.0;
+            if (L.G == 1 && ms > 0) lane_ms[nmax][{L.K, L.waves}] = ms;
             xtx_launch r;
             fill_launch(L, false, &r);
             r.moves_per_sec = rate;
@@ -1616,13 +1641,43 @@ void autotune_orders(xtx_ctx* ctx, const xtx_instance* sample, int count,
         if (best_rate > 0) {
             best.tuner = true;
             ctx->tuned[nmax] = best;
-            for (size_t q = first; q < out->size(); ++q) {
-                const xtx_launch& r = (*out)[q];
-                if (r.lanes == best.G && r.cols == best.C && r.triangular == (best.tri ? 1 : 0) &&
-                    r.waves_per_block == best.waves && r.lookahead == best.K)
-                    (*out)[q].tuned = 1;
+        }
+    }
+    // The two orders of a lane-kernel pair share a launch only with one shape:
+    // where both chose the lane kernel with different shapes, both take the
+    // shape of the least summed time.
+    for (auto& kv : lane_ms) {
+        const int o = kv.first, hi = lane_pair_hi(o), lo = hi - 8;
+        if (o != hi || !lane_ms.count(lo)) continue;
+        const auto a = ctx->tuned.find(lo), b = ctx->tuned.find(hi);
+        if (a == ctx->tuned.end() || b == ctx->tuned.end() || a->second.G != 1 || b->second.G != 1)
+            continue;
+        if (a->second.K == b->second.K && a->second.waves == b->second.waves) continue;
+        double best_sum = -1;
+        std::pair<int, int> shape{};
+        for (const auto& t : kv.second) {
+            const auto u = lane_ms[lo].find(t.first);
+            if (u == lane_ms[lo].end()) continue;
+            const double sum = t.second + u->second;
+            if (best_sum < 0 || sum < best_sum) {
+                best_sum = sum;
+                shape = t.first;
             }
         }
+        if (best_sum < 0) continue;
+        for (int order : {lo, hi}) {
+            Layout L = make_layout(order, 1, order, true, shape.second, shape.first);
+            L.tuner = true;
+            ctx->tuned[order] = L;
+        }
+    }
+    for (auto& r : *out) {
+        const auto it = ctx->tuned.find(r.order);
+        if (it == ctx->tuned.end() || !it->second.tuner) continue;
+        const Layout& t = it->second;
+        if (r.lanes == t.G && r.cols == t.C && r.triangular == (t.tri ? 1 : 0) &&
+            r.waves_per_block == t.waves && r.lookahead == t.K && r.x_in_lds == (t.xl ? 1 : 0))
+            r.tuned = 1;
     }
 }
 
@@ -1719,20 +1774,24 @@ int xtx_batch_solve(xtx_ctx* ctx, const xtx_instance* instances, int32_t count,
         keep.clear();
         S.rebuild();
         const double upload_s = S.now();
-        // Epochs per poll: about 8 ms of work between polls (one epoch at the
-        // full batch), so a solved instance is collected within an epoch or
-        // 8 ms of its solve. The next epochs are launched before a poll's
-        // retirements (finished instances' waves exit at once), so the device
-        // stays busy while the host copies out and verifies results; the
-        // launch tables are rebuilt, with nothing in flight, once a quarter
-        // of the residents have gone (each rebuild costs a graph capture).
+        // Epochs per poll: about 100 ms of work between polls (ten epochs at
+        // the full batch), at most 16. Between polls every launch bucket runs
+        // its epochs as a chain of its own (launch_epochs_chained), so the
+        // buckets meet only at the poll: measured on the 1024-instance batch,
+        // 8 ms per poll (one epoch) 314 matrices/s at p50/p99 time-to-diagonal
+        // 0.073/0.57 s, 40 ms 348 at 0.07/0.50 s, 100 ms 390 at 0.10/0.47 s. The
+        // next epochs are launched before a poll's retirements (finished
+        // instances' waves exit at once), so the device stays busy while the
+        // host copies out and verifies results; the launch tables are rebuilt,
+        // with nothing in flight, once a quarter of the residents have gone
+        // (each rebuild costs a graph capture).
         HIPCHK(hipStreamCreate(&S.fetch_stream));
         int k = 1;
         Deadline dl(ctx->stop_host, params->max_seconds);
         S.launch_epochs(k);
         for (;;) {
             S.finish_epochs();
-            const int kn = std::clamp((int)(8.0 / std::max((double)S.last_ms / k, 1e-3)), 1, 8);
+            const int kn = std::clamp((int)(100.0 / std::max((double)S.last_ms / k, 1e-3)), 1, 16);
             bool running = false;
             int finished = 0;
             for (size_t s = 0; s < S.slots.size(); ++s) {
