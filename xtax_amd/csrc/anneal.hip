@@ -1332,6 +1332,18 @@ __global__ void __launch_bounds__(G * K > 256 ? 512 : 256, K == 1 ? 4 : 2)
 // upward so a configuration can travel several rungs in one sweep.
 constexpr int kSweepChunk = 512;
 
+// The instance left ST_RUNNING: its record and state are final (the kernels
+// of its epoch have ended, and nothing writes them again). Everything written
+// so far is made visible to the host and its copy engines, then the host's
+// flag is raised.
+__device__ __forceinline__ void raise_done(Inst* I) {
+    int32_t* f = I->done_flag;
+    if (f) {
+        __threadfence_system();
+        __hip_atomic_store(f, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
 // One block per instance: instance list[b] (b when list is null).
 __global__ void __launch_bounds__(256) k_epoch_end(Inst* __restrict__ insts,
                                                    const int* __restrict__ list) {
@@ -1353,6 +1365,7 @@ __global__ void __launch_bounds__(256) k_epoch_end(Inst* __restrict__ insts,
             I->status = ST_SOLVED;
             I->solve_rep = (int32_t)(key & ((1ull << kRepBits) - 1ull));
             I->solve_move = (int64_t)(key >> kRepBits);
+            raise_done(I);
         }
         return;
     }
@@ -1536,8 +1549,10 @@ __global__ void __launch_bounds__(256) k_epoch_end(Inst* __restrict__ insts,
     __syncthreads();
     if (t == 0) {
         I->epoch = I->epoch + 1u;
-        if (!s_running || (I->P.max_epochs > 0 && I->epoch >= (uint32_t)I->P.max_epochs))
+        if (!s_running || (I->P.max_epochs > 0 && I->epoch >= (uint32_t)I->P.max_epochs)) {
             I->status = ST_EXHAUSTED;
+            raise_done(I);
+        }
     }
 }
 

@@ -154,6 +154,9 @@ struct Inst {
     double* bQ;
     double* gQ;
     double* work;
+    // host-mapped flag the epoch end raises when the instance leaves
+    // ST_RUNNING (null: none), so a host can collect it while later epochs run
+    int32_t* done_flag;
 };
 
 // Order-preserving map of a double onto int64 (for a non-NaN x and y,
