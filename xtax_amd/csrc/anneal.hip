@@ -1708,6 +1708,92 @@ hipError_t launch_epoch_end(hipStream_t st, Inst* insts, int count, int max_r, c
     return hipGetLastError();
 }
 
+// Outcomes of finished instances gathered into one staging buffer (one block
+// per instance), so a host copies many instances' results at once. Block q
+// writes at offs[q]: int32 rep, int32 status (0, or 1 when the best replica's
+// snapshot is stale), then the R replica records, then the result replica's
+// A and X^T: the solving replica when the instance is solved, else the
+// instance best, argmin (best_score, best_move, replica), from its current
+// state or its best snapshot.
+__global__ void __launch_bounds__(256) k_gather_outcomes(const Inst* __restrict__ insts,
+                                                          const int* __restrict__ slots,
+                                                          const int64_t* __restrict__ offs,
+                                                          unsigned char* __restrict__ out) {
+    const Inst* I = insts + slots[blockIdx.x];
+    unsigned char* dst = out + offs[blockIdx.x];
+    const int R = I->R;
+    const int64_t nn = (int64_t)I->n * I->n;
+    __shared__ int s_rep, s_err;
+    __shared__ int64_t s_bs[256], s_bm[256];
+    __shared__ int s_br[256];
+    const int t = (int)threadIdx.x;
+    int64_t bs = INT64_MAX, bm = INT64_MAX;
+    int br = -1;
+    for (int r = t; r < R; r += blockDim.x) {
+        const RepState& x = I->rs[r];
+        if (x.best_score < bs || (x.best_score == bs && x.best_move < bm)) {
+            bs = x.best_score;
+            bm = x.best_move;
+            br = r;
+        }
+    }
+    s_bs[t] = bs;
+    s_bm[t] = bm;
+    s_br[t] = br;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            const int u = t + w;
+            const bool take = s_br[u] >= 0 &&
+                              (s_br[t] < 0 || s_bs[u] < s_bs[t] ||
+                               (s_bs[u] == s_bs[t] &&
+                                (s_bm[u] < s_bm[t] || (s_bm[u] == s_bm[t] && s_br[u] < s_br[t]))));
+            if (take) {
+                s_bs[t] = s_bs[u];
+                s_bm[t] = s_bm[u];
+                s_br[t] = s_br[u];
+            }
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        int rep = s_br[0], err = 0;
+        if (I->status == ST_SOLVED) {
+            rep = I->solve_rep;
+        } else {
+            const int32_t f = I->rs[rep].flags;
+            err = (!(f & RF_CUR_BEST) && (f & RF_BEST_STALE)) ? 1 : 0;
+        }
+        s_rep = rep;
+        s_err = err;
+        ((int32_t*)dst)[0] = rep;
+        ((int32_t*)dst)[1] = err;
+    }
+    __syncthreads();
+    const int rep = s_rep;
+    // the replica records, as 8-byte words
+    const int64_t* rsw = (const int64_t*)I->rs;
+    int64_t* d = (int64_t*)(dst + 16);
+    const int64_t nrs = (int64_t)R * (int64_t)(sizeof(RepState) / 8);
+    for (int64_t q = t; q < nrs; q += blockDim.x) d[q] = rsw[q];
+    d += nrs;
+    const bool cur = I->status == ST_SOLVED || (I->rs[rep].flags & RF_CUR_BEST) != 0;
+    const int64_t* sa = (cur ? I->A : I->bA) + (int64_t)rep * nn;
+    const int64_t* sx = (cur ? I->Xt : I->bXt) + (int64_t)rep * nn;
+    for (int64_t q = t; q < nn; q += blockDim.x) {
+        d[q] = sa[q];
+        d[nn + q] = sx[q];
+    }
+}
+
+hipError_t launch_gather_outcomes(hipStream_t st, const Inst* insts, const int* slots,
+                                  const int64_t* offs, int count, unsigned char* out) {
+    if (count <= 0) return hipSuccess;
+    static_assert(sizeof(RepState) % 8 == 0, "replica records copy as 8-byte words");
+    hipLaunchKernelGGL(k_gather_outcomes, dim3(count), dim3(256), 0, st, insts, slots, offs, out);
+    return hipGetLastError();
+}
+
 __global__ void k_replicate(int64_t* __restrict__ dst, const int64_t* __restrict__ src, int64_t nn,
                             int R) {
     const int64_t total = nn * R;

@@ -44,6 +44,8 @@ hipError_t launch_probe(hipStream_t st, int G, int C, const int64_t* A, const in
 hipError_t launch_probe_rng(hipStream_t st, const uint32_t* ck, int count, uint32_t* out,
                             const double* x, int nx, double* ex);
 hipError_t launch_replicate(hipStream_t st, int64_t* dst, const int64_t* src, int64_t nn, int R);
+hipError_t launch_gather_outcomes(hipStream_t st, const Inst* insts, const int* slots,
+                                  const int64_t* offs, int count, unsigned char* out);
 hipError_t init_kernel_attrs(int max_lds_per_block);
 hipError_t init_dual_kernel_attrs(int max_lds_per_block);
 // the lane kernel (anneal_lane.hip)
@@ -326,8 +328,8 @@ void outcome_records(const HostInst& h, const Inst& rec, hipStream_t st, Outcome
                           hipMemcpyDeviceToHost, st));
 }
 
-void outcome_state(const HostInst& h, const Inst& rec, hipStream_t st, Outcome& out) {
-    const size_t nn = (size_t)h.n * h.n;
+// the replica records' counters summed into the outcome
+void outcome_tally(Outcome& out) {
     out.moves = out.accepts = out.greedy = out.infeasible = out.sweeps = 0;
     for (const auto& s : out.rs) {
         out.moves += (int64_t)s.moves;
@@ -336,6 +338,11 @@ void outcome_state(const HostInst& h, const Inst& rec, hipStream_t st, Outcome& 
         out.infeasible += s.infeasible;
         out.sweeps += s.sweeps;
     }
+}
+
+void outcome_state(const HostInst& h, const Inst& rec, hipStream_t st, Outcome& out) {
+    const size_t nn = (size_t)h.n * h.n;
+    outcome_tally(out);
     out.A.resize(nn);
     out.Xt.resize(nn);
     const int64_t* srcA;
@@ -1084,6 +1091,14 @@ struct Server {
     // while they run
     Inst* pin_insts = nullptr;
     int pin_cap = 0;
+    // gathered outcomes (k_gather_outcomes): device staging, its pinned host
+    // copy, and the slot and offset lists
+    unsigned char* d_stage = nullptr;
+    unsigned char* h_stage = nullptr;
+    size_t stage_cap = 0;
+    int* d_gslots = nullptr;
+    int64_t* d_goffs = nullptr;
+    int gather_cap = 0;
     // per slot, host-mapped: raised by the device when the instance finishes
     int32_t* done_host = nullptr;
     int32_t* done_dev = nullptr;
@@ -1101,6 +1116,10 @@ struct Server {
         if (d_insts) (void)hipFree(d_insts);
         if (pin_insts) (void)hipHostFree(pin_insts);
         if (done_host) (void)hipHostFree(done_host);
+        if (d_stage) (void)hipFree(d_stage);
+        if (h_stage) (void)hipHostFree(h_stage);
+        if (d_gslots) (void)hipFree(d_gslots);
+        if (d_goffs) (void)hipFree(d_goffs);
         if (fetch_stream) (void)hipStreamDestroy(fetch_stream);
     }
 
@@ -1399,6 +1418,65 @@ struct Server {
     // st: the stream for the outcome copies (default: the engine's). The
     // copies of all retiring instances share two waits; outputs and exact
     // verification run on host threads.
+    // Outcomes of many instances (as outcome_records + outcome_state give
+    // them) through one gather kernel and one copy: four small copies per
+    // instance cost about 50 ms when a batch's 800 capped instances retire
+    // together.
+    static constexpr size_t kGatherMin = 16;
+    void gather(const std::vector<size_t>& fin, hipStream_t st, std::vector<Outcome>& outs) {
+        const int m = (int)fin.size();
+        std::vector<int> sl((size_t)m);
+        std::vector<int64_t> off((size_t)m);
+        size_t total = 0;
+        for (int i = 0; i < m; ++i) {
+            const HostInst& h = *slots[fin[(size_t)i]];
+            sl[(size_t)i] = (int)fin[(size_t)i];
+            off[(size_t)i] = (int64_t)total;
+            total += 16 + (size_t)h.R * sizeof(RepState) + 2 * (size_t)h.n * h.n * 8;
+        }
+        if (total > stage_cap) {
+            if (d_stage) (void)hipFree(d_stage);
+            if (h_stage) (void)hipHostFree(h_stage);
+            d_stage = h_stage = nullptr;
+            stage_cap = std::max(total, 2 * stage_cap);
+            HIPCHK(hipMalloc(&d_stage, stage_cap));
+            HIPCHK(hipHostMalloc((void**)&h_stage, stage_cap, hipHostMallocDefault));
+        }
+        if (m > gather_cap) {
+            if (d_gslots) (void)hipFree(d_gslots);
+            if (d_goffs) (void)hipFree(d_goffs);
+            gather_cap = std::max(m, 2 * gather_cap);
+            HIPCHK(hipMalloc(&d_gslots, (size_t)gather_cap * sizeof(int)));
+            HIPCHK(hipMalloc(&d_goffs, (size_t)gather_cap * sizeof(int64_t)));
+        }
+        HIPCHK(hipMemcpyAsync(d_gslots, sl.data(), (size_t)m * sizeof(int), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d_goffs, off.data(), (size_t)m * sizeof(int64_t),
+                              hipMemcpyHostToDevice, st));
+        HIPCHK(launch_gather_outcomes(st, d_insts, d_gslots, d_goffs, m, d_stage));
+        HIPCHK(hipMemcpyAsync(h_stage, d_stage, total, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        for (int i = 0; i < m; ++i) {
+            const HostInst& h = *slots[fin[(size_t)i]];
+            const Inst& rec = h_insts[fin[(size_t)i]];
+            const unsigned char* src = h_stage + off[(size_t)i];
+            const size_t nn = (size_t)h.n * h.n;
+            Outcome& out = outs[(size_t)i];
+            const int32_t* head = (const int32_t*)src;
+            if (head[1]) throw Err{XTX_E_STATE, "internal: best replica has a stale snapshot"};
+            out.rs.resize((size_t)h.R);
+            std::memcpy(out.rs.data(), src + 16, (size_t)h.R * sizeof(RepState));
+            outcome_tally(out);
+            out.rep = head[0];
+            if (rec.status == ST_SOLVED) {
+                out.solved = true;
+                out.solve_move = rec.solve_move;
+            }
+            const int64_t* a = (const int64_t*)(src + 16 + (size_t)h.R * sizeof(RepState));
+            out.A.assign(a, a + nn);
+            out.Xt.assign(a + nn, a + 2 * nn);
+        }
+    }
+
     void retire(bool force, uint64_t only_tag = 0, bool by_tag = false, hipStream_t st = nullptr) {
         if (!st) st = ctx->stream;
         std::vector<size_t> fin;
@@ -1411,12 +1489,16 @@ struct Server {
             fin.push_back(s);
         }
         std::vector<Outcome> outs(fin.size());
-        for (size_t i = 0; i < fin.size(); ++i)
-            outcome_records(*slots[fin[i]], h_insts[fin[i]], st, outs[i]);
-        if (!fin.empty()) HIPCHK(hipStreamSynchronize(st));
-        for (size_t i = 0; i < fin.size(); ++i)
-            outcome_state(*slots[fin[i]], h_insts[fin[i]], st, outs[i]);
-        if (!fin.empty()) HIPCHK(hipStreamSynchronize(st));
+        if (fin.size() >= kGatherMin) {
+            gather(fin, st, outs);
+        } else {
+            for (size_t i = 0; i < fin.size(); ++i)
+                outcome_records(*slots[fin[i]], h_insts[fin[i]], st, outs[i]);
+            if (!fin.empty()) HIPCHK(hipStreamSynchronize(st));
+            for (size_t i = 0; i < fin.size(); ++i)
+                outcome_state(*slots[fin[i]], h_insts[fin[i]], st, outs[i]);
+            if (!fin.empty()) HIPCHK(hipStreamSynchronize(st));
+        }
         std::vector<int> vstat(fin.size(), -1);
         parallel_for(fin.size(), [&](size_t i) {
             const HostInst& h = *slots[fin[i]];
