@@ -262,7 +262,7 @@ def bench_batch(args, eng):
                      greedy_fraction=0.5, target_fraction=args.target_fraction,
                      exchange_interval=args.exchange_interval,
                      max_epochs=args.cap_epochs, seed=args.seed, verify=True,
-                     lookahead=args.lookahead)
+                     lookahead=args.lookahead, persistent=not args.no_persistent)
     tune = None
     if not args.no_tune:
         # launch configuration per order bucket, measured once per device
@@ -283,6 +283,7 @@ def bench_batch(args, eng):
     lat = []
     kernel_ms = 0.0
     lane_instances = 0
+    persistent = 0
     resident_s = 0.0  # each step less its upload of the host inputs
     buckets = {}  # n bucket -> [instances, solved, latencies]
     t0 = time.perf_counter()
@@ -301,9 +302,11 @@ def bench_batch(args, eng):
         moves += bst["moves"]
         kernel_ms += bst["kernel_ms"]
         lane_instances += bst["lane_instances"]
+        persistent += bst["persistent"]
         resident_s += bst["seconds"] - bst["upload_seconds"]
         log(f"step {k}: solved {bst['solved']}/{bst['instances']} verified {bst['verified']} "
-            f"epochs {bst['epochs']} buckets {bst['buckets']} {bst['seconds']:.2f}s "
+            f"epochs {bst['epochs']} buckets {bst['buckets']} "
+            f"{'persistent ' if bst['persistent'] else ''}{bst['seconds']:.2f}s "
             f"(upload {bst['upload_seconds']:.3f}s) kernel {bst['kernel_ms']:.0f}ms "
             f"moves/s {bst['moves'] / bst['seconds']:.3e}")
     el = time.perf_counter() - t0
@@ -312,7 +315,9 @@ def bench_batch(args, eng):
                        f"greedy 0.5, target {args.target_fraction}, exchange "
                        f"{args.exchange_interval}, cap {args.cap_epochs} epochs",
            "instances": args.count, "replicas": args.replicas,
-           "cap_moves_per_replica": args.cap_epochs * args.exchange_interval}
+           "cap_moves_per_replica": args.cap_epochs * args.exchange_interval,
+           "launch": "one persistent launch per step" if persistent == args.steps
+                     else "captured graphs of epochs"}
     # value: inputs resident in HBM when the clock starts (each step's own
     # upload excluded); the rate with the host-to-device upload is reported
     # beside it
@@ -715,6 +720,8 @@ def main():
     ap.add_argument("--plain-count", type=int, default=64,
                     help="batch: instances also run one per launch (the plain form)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-persistent", action="store_true",
+                    help="batch: captured graphs of epochs instead of one persistent launch")
     ap.add_argument("--no-tune", action="store_true",
                     help="batch: launch configurations by rule instead of the autotuner")
     ap.add_argument("--x-in-lds", action="store_true",

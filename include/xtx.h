@@ -91,6 +91,17 @@ typedef struct {
                                  resident with it, else 1 (the order's
                                  one-group configuration or the default
                                  layout). 1: one group per replica. */
+    int32_t persistent;       /* xtx_batch_solve: 1 (default) runs the whole
+                                 batch as one persistent launch when every
+                                 instance runs the lane kernel by its rule
+                                 (orders to 48, 16 proposals per step to order
+                                 32 and 32 above, at most 256 replicas): every
+                                 wave takes the next ready unit (one wave's
+                                 replicas of one instance for one epoch) and
+                                 the instance's last unit of an epoch runs
+                                 its epoch end, so instances advance with no
+                                 device-wide join; 0: captured graphs of
+                                 epochs. The same results either way. */
     /* Plain forms (xtx_solve_plain, xtx_dual_solve): called on the calling
      * thread at an epoch boundary whenever the instance best improves -- the
      * reference's "new best" report (congruence_anneal.hpp:309-352). seconds
@@ -177,6 +188,7 @@ typedef struct {
                                  state; seconds - upload_seconds is the run
                                  with inputs resident in HBM */
     int32_t lane_instances;   /* instances run by the lane kernel (lanes = 1) */
+    int32_t persistent;       /* 1: the batch ran as one persistent launch */
 } xtx_batch_stats;
 
 typedef struct {

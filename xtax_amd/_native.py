@@ -32,6 +32,7 @@ class Params(C.Structure):
         ("replicas", C.c_int32), ("move_log", C.c_int32), ("move_log_cap", C.c_int64),
         ("max_moves", C.c_uint64), ("max_epochs", C.c_int32), ("verify", C.c_int32),
         ("waves_per_block", C.c_int32), ("autotune", C.c_int32), ("lookahead", C.c_int32),
+        ("persistent", C.c_int32),
         ("on_best", C.c_void_p), ("on_best_user", C.c_void_p),
     ]
 
@@ -76,7 +77,7 @@ class BatchStats(C.Structure):
                 ("solved", C.c_int32), ("verified", C.c_int32), ("epochs", C.c_int32),
                 ("seconds", C.c_double), ("kernel_ms", C.c_double), ("buckets", C.c_int32),
                 ("graph_epochs", C.c_int32), ("upload_seconds", C.c_double),
-                ("lane_instances", C.c_int32)]
+                ("lane_instances", C.c_int32), ("persistent", C.c_int32)]
 
 
 class ServeConfig(C.Structure):

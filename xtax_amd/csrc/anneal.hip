@@ -1723,7 +1723,7 @@ __global__ void __launch_bounds__(256) k_gather_outcomes(const Inst* __restrict_
     unsigned char* dst = out + offs[blockIdx.x];
     const int R = I->R;
     const int64_t nn = (int64_t)I->n * I->n;
-    __shared__ int s_rep, s_err;
+    __shared__ int s_rep;
     __shared__ int64_t s_bs[256], s_bm[256];
     __shared__ int s_br[256];
     const int t = (int)threadIdx.x;
@@ -1765,7 +1765,6 @@ __global__ void __launch_bounds__(256) k_gather_outcomes(const Inst* __restrict_
             err = (!(f & RF_CUR_BEST) && (f & RF_BEST_STALE)) ? 1 : 0;
         }
         s_rep = rep;
-        s_err = err;
         ((int32_t*)dst)[0] = rep;
         ((int32_t*)dst)[1] = err;
     }

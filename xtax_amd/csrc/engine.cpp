@@ -53,6 +53,12 @@ hipError_t launch_anneal_lane(hipStream_t st, int nmax, int Q, int units_per_blo
                               const RepRef* table, int count, int split, Inst* insts,
                               int rep_bytes, const volatile int* stop);
 hipError_t init_lane_kernel_attrs(int max_lds_per_block);
+// the batch's persistent launch (anneal_persist.hip)
+hipError_t launch_lane_persist(hipStream_t st, const PersistCtl& c, Inst* insts, int blocks,
+                               const volatile int* stop);
+hipError_t init_persist_kernel_attrs(int max_lds_per_block);
+int lane_persist_blocks_per_cu();
+hipError_t launch_xcc_probe(hipStream_t st, int32_t* out, int blocks);
 // the no-alloc solve's device scratch (scratch.hip)
 hipError_t scratch_init(int device);
 void* scratch_address(int device);
@@ -607,6 +613,7 @@ void xtx_params_default(xtx_params* p) {
     p->verify = 1;
     p->waves_per_block = 0;
     p->lookahead = 0;
+    p->persistent = 1;
 }
 
 int xtx_open(int device, xtx_ctx** out) {
@@ -627,6 +634,7 @@ int xtx_open(int device, xtx_ctx** out) {
         HIPCHK(init_kernel_attrs(g_lds_block));
         HIPCHK(init_dual_kernel_attrs(g_lds_block));
         HIPCHK(init_lane_kernel_attrs(g_lds_block));
+        HIPCHK(init_persist_kernel_attrs(g_lds_block));
         HIPCHK(scratch_init(device));
         HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         HIPCHK(hipEventCreate(&ctx->ev0));
@@ -1106,6 +1114,32 @@ struct Server {
     int inflight_ns = 0, inflight_k = 0;
     float last_ms = 0;
     hipStream_t fetch_stream = nullptr;
+    // the persistent launch (anneal_persist.hip): units, per-instance index
+    // (first unit | units | pending), queue slots and counters (head, tail,
+    // live, err)
+    PUnit* d_units = nullptr;
+    size_t units_cap = 0;
+    int32_t* d_pidx = nullptr;
+    size_t pidx_cap = 0;
+    int32_t* d_pend = nullptr;
+    size_t pend_cap = 0;
+    int32_t* d_q = nullptr;
+    size_t q_cap = 0;
+    uint32_t* d_pcnt = nullptr;
+    size_t pcnt_cap = 0;
+    bool inflight_persist = false;
+    // zero-copy retirement of the persistent launch: each instance's outcome
+    // area in host-mapped memory (offsets per slot), which slots have been
+    // taken from it, and outcomes read but not yet retired
+    unsigned char* zc_host = nullptr;
+    unsigned char* zc_dev = nullptr;
+    size_t zc_cap = 0;
+    int64_t* d_zoff = nullptr;
+    size_t zoff_cap = 0;
+    std::vector<int64_t> zc_off;
+    std::vector<char> zc_taken;
+    std::map<size_t, Outcome> staged;
+    bool zero_copy = false;
 
     ~Server() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -1121,6 +1155,13 @@ struct Server {
         if (d_gslots) (void)hipFree(d_gslots);
         if (d_goffs) (void)hipFree(d_goffs);
         if (fetch_stream) (void)hipStreamDestroy(fetch_stream);
+        if (d_units) (void)hipFree(d_units);
+        if (d_pidx) (void)hipFree(d_pidx);
+        if (d_pend) (void)hipFree(d_pend);
+        if (d_q) (void)hipFree(d_q);
+        if (d_pcnt) (void)hipFree(d_pcnt);
+        if (d_zoff) (void)hipFree(d_zoff);
+        if (zc_host) (void)hipHostFree(zc_host);
     }
 
     double now() const { return std::chrono::duration<double>(clk::now() - t0).count(); }
@@ -1343,6 +1384,263 @@ struct Server {
         inflight_k = k;
     }
 
+    // Whether every running instance can run in the persistent launch: the
+    // lane kernel by its rule (orders to 48, lane_q proposals per step), the
+    // L1 objective, at most kPersistMaxR replicas.
+    static constexpr int kPersistMaxR = 256;
+    bool persistent_ok() const {
+        bool any = false;
+        for (size_t s = 0; s < slots.size() && s < h_insts.size(); ++s) {
+            if (!slots[s] || h_insts[s].status != ST_RUNNING) continue;
+            const HostInst& h = *slots[s];
+            const Layout& L = h.lay;
+            if (L.G != 1 || L.nmax > kLaneMaxOrder || L.K != lane_q(L.nmax) || L.xl ||
+                h.R > kPersistMaxR || h.rec.dual)
+                return false;
+            any = true;
+        }
+        return any;
+    }
+
+    template <typename T>
+    static void ensure(T*& p, size_t& cap, size_t count) {
+        if (count <= cap && p) return;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = std::max(count, 2 * cap);
+        HIPCHK(hipMalloc((void**)&p, cap * sizeof(T)));
+    }
+
+    // The device's XCDs (dies) and compute-unit pairs, as k_xcc_probe reports
+    // them (XCD id << 8 | CU-pair key): one-wave workgroups dispatched across
+    // the device, each reporting where it ran. Once per context.
+    const std::vector<int>& cu_pairs() {
+        if (!ctx->xcc_ids.empty()) return ctx->xcc_ids;
+        const int blocks = std::max(256, 16 * ctx->prop.multiProcessorCount);
+        int32_t* d = nullptr;
+        HIPCHK(hipMalloc((void**)&d, (size_t)blocks * sizeof(int32_t)));
+        std::vector<int32_t> h((size_t)blocks, -1);
+        hipError_t err = launch_xcc_probe(ctx->stream, d, blocks);
+        if (err == hipSuccess)
+            err = hipMemcpyAsync(h.data(), d, (size_t)blocks * sizeof(int32_t),
+                                 hipMemcpyDeviceToHost, ctx->stream);
+        if (err == hipSuccess) err = hipStreamSynchronize(ctx->stream);
+        (void)hipFree(d);
+        HIPCHK(err);
+        std::vector<int> ids;
+        for (int32_t v : h)
+            if (v >= 0) ids.push_back(((v >> 8) & (kMaxXcd - 1)) << 8 | (v & (kCuKeys - 1)));
+        std::sort(ids.begin(), ids.end());
+        ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+        if (ids.empty()) throw Err{XTX_E_DEVICE, "XCD probe: no ids"};
+        ctx->xcc_ids = ids;
+        return ctx->xcc_ids;
+    }
+
+    // Launches up to k more epochs of every running instance as one
+    // persistent kernel (k_lane_persist), without waiting; finish_epochs
+    // waits. Every instance is placed on one XCD (the most replicas first,
+    // each onto the die with the fewest so far: a replica's epoch costs about
+    // the same at every order) and its units go to that die's queue of its
+    // order class and epoch slot, the largest orders first. Each die's
+    // compute-unit pairs are given home classes in proportion to the die's
+    // replicas of each class (a unit's time follows its replicas: measured
+    // 2.0 ms for the four of an order-24 unit, 1.1 ms for the two of an
+    // order-48 one). Each queue is sized for every unit its instances' k
+    // epochs can append. The grid leaves two workgroups of the device free,
+    // so the copies that retire finished instances (blit kernels) run while
+    // it does.
+    uint32_t persist_limit = 0;
+    void launch_persistent(int k) {
+        const int ns = (int)slots.size();
+        if (ns == 0) return;
+        if (pin_cap < ns) {
+            if (pin_insts) (void)hipHostFree(pin_insts);
+            pin_insts = nullptr;
+            pin_cap = std::max(ns, 2 * pin_cap);
+            HIPCHK(hipHostMalloc((void**)&pin_insts, (size_t)pin_cap * sizeof(Inst),
+                                 hipHostMallocDefault));
+        }
+        const std::vector<int>& pairs = cu_pairs();
+        std::vector<int> ids;  // XCD ids
+        for (int v : pairs)
+            if (ids.empty() || ids.back() != (v >> 8)) ids.push_back(v >> 8);
+        const int nq = (int)ids.size();
+        uint32_t e_hi = 0;
+        std::vector<int> run;
+        for (int s = 0; s < ns; ++s)
+            if (slots[(size_t)s] && h_insts[(size_t)s].status == ST_RUNNING) {
+                run.push_back(s);
+                e_hi = std::max(e_hi, h_insts[(size_t)s].epoch);
+            }
+        const uint32_t limit = e_hi + (uint32_t)k;
+        uint32_t e_lo = e_hi;
+        for (int s : run) e_lo = std::min(e_lo, h_insts[(size_t)s].epoch);
+        // placement
+        std::stable_sort(run.begin(), run.end(), [&](int a, int b) {
+            return slots[(size_t)a]->R > slots[(size_t)b]->R;
+        });
+        std::vector<int64_t> load((size_t)nq, 0);
+        std::vector<std::vector<int>> on((size_t)nq);
+        for (int s : run) {
+            const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            load[(size_t)x] += slots[(size_t)s]->R;
+            on[(size_t)x].push_back(s);
+        }
+        std::vector<PUnit> units;
+        std::vector<int32_t> idx((size_t)2 * ns, 0);
+        std::vector<int32_t> pend((size_t)ns * kLineWords, 0);
+        for (int s : run) {
+            const HostInst& h = *slots[(size_t)s];
+            const int per = 64 / lane_q(h.lay.nmax);
+            const int nu = (h.R + per - 1) / per;
+            idx[(size_t)s] = (int32_t)units.size();
+            idx[(size_t)ns + s] = nu;
+            pend[(size_t)s * kLineWords] = nu;
+            for (int u = 0; u < nu; ++u)
+                units.push_back(PUnit{s, (int16_t)(u * per), (int16_t)h.lay.nmax});
+        }
+        if (units.size() >= ((size_t)1 << 24)) throw Err{XTX_E_NOMEM, "persistent launch: too many units"};
+        const int bpc = lane_persist_blocks_per_cu();
+        if (bpc < 1) throw Err{XTX_E_DEVICE, "persistent kernel: no occupancy"};
+        const int blocks = std::max(1, bpc * ctx->prop.multiProcessorCount - 2);
+        PersistCtl c{};
+        std::vector<int32_t> q0;
+        std::vector<uint32_t> qbase((size_t)kQueues, 0u), qcap((size_t)kQueues, 0u);
+        const size_t nlines = (size_t)2 * kQueues + kMaxXcd + 1;
+        std::vector<uint32_t> cnt(nlines * kLineWords, 0u);
+        uint32_t* tail_h = cnt.data() + (size_t)kQueues * kLineWords;
+        uint32_t* live_h = cnt.data() + (size_t)2 * kQueues * kLineWords;
+        std::vector<int8_t> home((size_t)kMaxXcd * kCuKeys, (int8_t)(kClasses - 1));
+        size_t qtotal = 0;
+        for (int x = 0; x < nq; ++x) {
+            auto& v = on[(size_t)x];
+            std::stable_sort(v.begin(), v.end(), [&](int a, int b) {
+                return slots[(size_t)a]->lay.nmax > slots[(size_t)b]->lay.nmax;
+            });
+            int64_t creps[kClasses] = {};  // a unit's time follows its replicas
+            for (int cl = 0; cl < kClasses; ++cl)
+                for (int b = 0; b < kEpochSlots; ++b) {
+                    // epochs e0 .. limit-1 of each instance; those in slot b
+                    size_t tasks = 0;
+                    const size_t first = q0.size();
+                    for (int s : v) {
+                        if (order_class(slots[(size_t)s]->lay.nmax) != cl) continue;
+                        const uint32_t e0 = h_insts[(size_t)s].epoch;
+                        const int nu = idx[(size_t)ns + s];
+                        for (uint32_t ep = e0; ep < limit; ++ep)
+                            if ((int)(std::min<uint32_t>(ep - e_lo, 125) & (kEpochSlots - 1)) == b)
+                                tasks += (size_t)nu;
+                        const int rel = std::min<int>((int)(e0 - e_lo), 125);
+                        if ((rel & (kEpochSlots - 1)) == b)
+                            for (int u = 0; u < nu; ++u) q0.push_back(rel << 24 | (idx[(size_t)s] + u));
+                        if (b == 0) creps[cl] += slots[(size_t)s]->R;
+                    }
+                    const size_t cap = (tasks + 64 + kLineWords - 1) / kLineWords * kLineWords;
+                    const int qi = (x * kClasses + cl) * kEpochSlots + b;
+                    qbase[(size_t)qi] = (uint32_t)qtotal;
+                    qcap[(size_t)qi] = (uint32_t)cap;
+                    tail_h[(size_t)qi * kLineWords] = (uint32_t)(q0.size() - first);
+                    // the queue's initial units, at its base
+                    std::vector<int32_t> mine(q0.begin() + (ptrdiff_t)first, q0.end());
+                    q0.resize(first);
+                    q0.resize(qtotal, -1);
+                    q0.insert(q0.end(), mine.begin(), mine.end());
+                    qtotal += cap;
+                    if (qtotal >= (size_t)INT32_MAX)
+                        throw Err{XTX_E_NOMEM, "persistent queue too large"};
+                }
+            live_h[(size_t)x * kLineWords] = (uint32_t)v.size();
+            // home classes of the die's CU pairs, in proportion to its replicas
+            std::vector<int> keys;
+            for (int pv : pairs)
+                if ((pv >> 8) == ids[(size_t)x]) keys.push_back(pv & (kCuKeys - 1));
+            const int64_t tot = creps[0] + creps[1] + creps[2];
+            int at = 0;
+            for (int cl = 0; cl < kClasses && tot > 0; ++cl) {
+                int want = (int)((creps[cl] * (int64_t)keys.size() + tot / 2) / tot);
+                if (creps[cl] > 0 && want == 0) want = 1;
+                for (int i = 0; i < want && at < (int)keys.size(); ++i, ++at)
+                    home[(size_t)x * kCuKeys + keys[(size_t)at]] = (int8_t)cl;
+            }
+        }
+        // every slot's outcome area in host-mapped memory
+        zc_off.assign((size_t)ns, -1);
+        size_t zbytes = 0;
+        for (int s = 0; s < ns; ++s) {
+            if (!slots[(size_t)s]) continue;
+            const HostInst& h = *slots[(size_t)s];
+            zc_off[(size_t)s] = (int64_t)zbytes;
+            zbytes += (32 + (size_t)h.R * sizeof(RepState) + 16 * (size_t)h.n * h.n + 127) / 128 * 128;
+        }
+        if (zbytes > zc_cap) {
+            if (zc_host) (void)hipHostFree(zc_host);
+            zc_host = zc_dev = nullptr;
+            zc_cap = 0;
+            HIPCHK(hipHostMalloc((void**)&zc_host, zbytes, hipHostMallocMapped));
+            HIPCHK(hipHostGetDevicePointer((void**)&zc_dev, zc_host, 0));
+            zc_cap = zbytes;
+        }
+        zc_taken.assign((size_t)ns, 0);
+        ensure(d_zoff, zoff_cap, zc_off.size());
+        zero_copy = true;
+        ensure(d_units, units_cap, units.size());
+        ensure(d_pidx, pidx_cap, idx.size() + (home.size() + 3) / 4 + 2 * (size_t)kQueues);
+        ensure(d_pend, pend_cap, pend.size());
+        ensure(d_q, q_cap, qtotal);
+        ensure(d_pcnt, pcnt_cap, cnt.size());
+        uint32_t* d_qbase = (uint32_t*)(d_pidx + idx.size());
+        uint32_t* d_qcap = d_qbase + kQueues;
+        int8_t* d_home = (int8_t*)(d_qcap + kQueues);
+        HIPCHK(hipMemsetAsync(d_q, 0xff, qtotal * sizeof(int32_t), ctx->stream));
+        HIPCHK(hipMemcpyAsync(d_q, q0.data(), q0.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                              ctx->stream));
+        HIPCHK(hipMemcpyAsync(d_units, units.data(), units.size() * sizeof(PUnit),
+                              hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(d_pidx, idx.data(), idx.size() * sizeof(int32_t),
+                              hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(d_home, home.data(), home.size(), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(d_qbase, qbase.data(), qbase.size() * sizeof(uint32_t),
+                              hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(d_qcap, qcap.data(), qcap.size() * sizeof(uint32_t),
+                              hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(d_pend, pend.data(), pend.size() * sizeof(int32_t),
+                              hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(d_zoff, zc_off.data(), zc_off.size() * sizeof(int64_t),
+                              hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(d_pcnt, cnt.data(), cnt.size() * sizeof(uint32_t),
+                              hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        c.q = d_q;
+        c.qbase = d_qbase;
+        c.qcap = d_qcap;
+        c.units = d_units;
+        c.ubase = d_pidx;
+        c.unum = d_pidx + ns;
+        c.home = d_home;
+        c.pend = d_pend;
+        c.head = d_pcnt;
+        c.tail = d_pcnt + (size_t)kQueues * kLineWords;
+        c.live = (int32_t*)(d_pcnt + (size_t)2 * kQueues * kLineWords);
+        c.err = (int32_t*)(d_pcnt + ((size_t)2 * kQueues + kMaxXcd) * kLineWords);
+        c.out = zc_dev;
+        c.out_off = d_zoff;
+        for (int h = 0; h < kMaxXcd; ++h) c.xq[h] = -1;
+        for (int x = 0; x < nq; ++x) c.xq[ids[(size_t)x]] = x;
+        c.nq = nq;
+        c.epoch_limit = limit;
+        c.epoch_base = e_lo;
+        persist_limit = limit;
+        HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+        HIPCHK(launch_lane_persist(ctx->stream, c, d_insts, blocks, ctx->stop_dev));
+        HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+        HIPCHK(hipMemcpyAsync(pin_insts, d_insts, (size_t)ns * sizeof(Inst),
+                              hipMemcpyDeviceToHost, ctx->stream));
+        inflight_ns = ns;
+        inflight_k = k;
+        inflight_persist = true;
+    }
+
     // Collects, while epochs are in flight, the instances the device has
     // flagged as finished: each one's record, then its outcome on the fetch
     // stream, verified on host threads (retire). Their state is final, and
@@ -1351,6 +1649,41 @@ struct Server {
         if (!done_host) return;
         bool any = false;
         const int ns = std::min((int)slots.size(), std::min(done_cap, (int)h_insts.size()));
+        if (zero_copy) {
+            // the persistent launch wrote each ended instance's outcome to
+            // host-mapped memory before raising its flag: read it there
+            for (int s = 0; s < ns && s < (int)zc_taken.size(); ++s) {
+                if (!slots[(size_t)s] || zc_taken[(size_t)s] || zc_off[(size_t)s] < 0) continue;
+                if (__atomic_load_n(&done_host[s], __ATOMIC_ACQUIRE) == 0) continue;
+                const HostInst& h = *slots[(size_t)s];
+                const unsigned char* src = zc_host + zc_off[(size_t)s];
+                const int32_t* head = (const int32_t*)src;
+                if (head[1]) throw Err{XTX_E_STATE, "internal: best replica has a stale snapshot"};
+                Outcome out;
+                out.rep = head[0];
+                const int64_t sm = ((const int64_t*)src)[2];
+                Inst& rec = h_insts[(size_t)s];
+                rec.status = head[2];
+                if (rec.status == ST_SOLVED) {
+                    out.solved = true;
+                    out.solve_move = sm;
+                    rec.solve_move = sm;
+                    rec.solve_rep = head[0];
+                }
+                out.rs.resize((size_t)h.R);
+                std::memcpy(out.rs.data(), src + 32, (size_t)h.R * sizeof(RepState));
+                outcome_tally(out);
+                const size_t nn = (size_t)h.n * h.n;
+                const int64_t* a = (const int64_t*)(src + 32 + (size_t)h.R * sizeof(RepState));
+                out.A.assign(a, a + nn);
+                out.Xt.assign(a + nn, a + 2 * nn);
+                staged[(size_t)s] = std::move(out);
+                zc_taken[(size_t)s] = 1;
+                any = true;
+            }
+            if (any) retire(false, 0, false, fetch_stream);
+            return;
+        }
         for (int s = 0; s < ns; ++s) {
             if (!slots[(size_t)s] || h_insts[(size_t)s].status != ST_RUNNING) continue;
             if (__atomic_load_n(&done_host[s], __ATOMIC_ACQUIRE) == 0) continue;
@@ -1375,6 +1708,21 @@ struct Server {
         kernel_ms += last_ms;
         epochs += inflight_k;
         inflight_ns = 0;
+        if (inflight_persist) {
+            inflight_persist = false;
+            int32_t err = 0;
+            HIPCHK(hipMemcpy(&err, d_pcnt + ((size_t)2 * kQueues + kMaxXcd) * kLineWords,
+                             sizeof(err), hipMemcpyDeviceToHost));
+            if (err) throw Err{XTX_E_DEVICE, "persistent launch: queue overflow"};
+            // every instance ran to its end or to the launch's epoch limit
+            // (unless stopped): a queue whose die ran no wave would leave its
+            // instances behind
+            if (__atomic_load_n(ctx->stop_host, __ATOMIC_SEQ_CST) == 0)
+                for (int s = 0; s < ns; ++s)
+                    if (slots[(size_t)s] && h_insts[(size_t)s].status == ST_RUNNING &&
+                        h_insts[(size_t)s].epoch < persist_limit)
+                        throw Err{XTX_E_DEVICE, "persistent launch ended before its instances"};
+        }
     }
 
     // graph: replay the k epochs as one captured graph (recaptured when k or
@@ -1489,16 +1837,32 @@ struct Server {
             fin.push_back(s);
         }
         std::vector<Outcome> outs(fin.size());
-        if (fin.size() >= kGatherMin) {
-            gather(fin, st, outs);
-        } else {
-            for (size_t i = 0; i < fin.size(); ++i)
-                outcome_records(*slots[fin[i]], h_insts[fin[i]], st, outs[i]);
-            if (!fin.empty()) HIPCHK(hipStreamSynchronize(st));
-            for (size_t i = 0; i < fin.size(); ++i)
-                outcome_state(*slots[fin[i]], h_insts[fin[i]], st, outs[i]);
-            if (!fin.empty()) HIPCHK(hipStreamSynchronize(st));
+        // outcomes already read from host-mapped memory, then the others
+        // copied from the device
+        std::vector<size_t> dev;
+        std::vector<size_t> dev_at;
+        for (size_t i = 0; i < fin.size(); ++i) {
+            auto it = staged.find(fin[i]);
+            if (it != staged.end()) {
+                outs[i] = std::move(it->second);
+                staged.erase(it);
+            } else {
+                dev.push_back(fin[i]);
+                dev_at.push_back(i);
+            }
         }
+        std::vector<Outcome> douts(dev.size());
+        if (dev.size() >= kGatherMin) {
+            gather(dev, st, douts);
+        } else {
+            for (size_t i = 0; i < dev.size(); ++i)
+                outcome_records(*slots[dev[i]], h_insts[dev[i]], st, douts[i]);
+            if (!dev.empty()) HIPCHK(hipStreamSynchronize(st));
+            for (size_t i = 0; i < dev.size(); ++i)
+                outcome_state(*slots[dev[i]], h_insts[dev[i]], st, douts[i]);
+            if (!dev.empty()) HIPCHK(hipStreamSynchronize(st));
+        }
+        for (size_t i = 0; i < dev.size(); ++i) outs[dev_at[i]] = std::move(douts[i]);
         std::vector<int> vstat(fin.size(), -1);
         parallel_for(fin.size(), [&](size_t i) {
             const HostInst& h = *slots[fin[i]];
@@ -1936,12 +2300,24 @@ int xtx_batch_solve(xtx_ctx* ctx, const xtx_instance* instances, int32_t count,
         // 0.10/0.47 s. At a graph's end the launch tables are rebuilt once a
         // quarter of the residents have gone.
         Deadline dl(ctx->stop_host, params->max_seconds);
+        // One persistent launch (k_lane_persist) when every instance runs the
+        // lane kernel by its rule: waves take ready units of any instance and
+        // each instance's epoch end runs in the wave that finishes its epoch,
+        // so instances never wait for one another (64 epochs per launch
+        // without an epoch cap).
+        const bool persist = params->persistent != 0 && S.persistent_ok();
         auto graph_k = [&]() {
             if (params->max_epochs > 0) return std::clamp(params->max_epochs - S.epochs, 1, 64);
-            return 16;
+            return persist ? 64 : 16;
+        };
+        auto launch = [&](int kk) {
+            if (persist)
+                S.launch_persistent(kk);
+            else
+                S.launch_epochs(kk);
         };
         int k = graph_k();
-        S.launch_epochs(k);
+        launch(k);
         for (;;) {
             while (hipStreamQuery(ctx->stream) == hipErrorNotReady) {
                 S.collect_done();
@@ -1959,7 +2335,7 @@ int xtx_batch_solve(xtx_ctx* ctx, const xtx_instance* instances, int32_t count,
             if (!running || capped || S.resident() == 0) break;
             if (4 * S.retired_since_rebuild > S.resident()) S.rebuild();
             k = graph_k();
-            S.launch_epochs(k);
+            launch(k);
         }
         S.finish_epochs();
         S.retire(true);
@@ -1983,6 +2359,7 @@ int xtx_batch_solve(xtx_ctx* ctx, const xtx_instance* instances, int32_t count,
             bs.graph_epochs = k;
             bs.upload_seconds = upload_s;
             bs.lane_instances = lane_instances;
+            bs.persistent = persist ? 1 : 0;
             *stats = bs;
         }
         return XTX_OK;

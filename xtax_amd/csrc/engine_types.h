@@ -112,7 +112,9 @@ struct InstParams {
     int32_t pad0;
 };
 
-struct Inst {
+// Whole 128-byte cache lines per record: the persistent launch runs different
+// instances on different XCDs, whose L2s are not coherent with each other.
+struct alignas(128) Inst {
     int32_t n;
     int32_t N;             // active count
     int32_t R;
@@ -173,6 +175,69 @@ inline int64_t okey(double x) {
 struct RepRef {
     int32_t inst;
     int32_t rep;
+};
+
+// The batch's persistent launch (anneal_persist.hip). A work unit is one
+// wave's replicas of one instance (64 / Q replicas from r0, Q the lane
+// kernel's proposals per step at order nmax) for one epoch; the units of an
+// instance are consecutive from ubase[inst].
+struct PUnit {
+    int32_t inst;
+    int16_t r0;
+    int16_t nmax;
+};
+
+// Queues and counters of a persistent launch, in device memory. Queues per
+// XCD (an accelerator die with its own L2), order class (orders 8-16, 24-32,
+// 40-48) and epoch slot (the epoch a unit runs, modulo kEpochSlots): an
+// instance's units only ever run on the XCD whose queues hold them, so what
+// one unit leaves for the next passes through that XCD's L2 (stores complete,
+// L1 invalidated) with no L2 write-back or invalidation. Queue qi = (x
+// kClasses + k) kEpochSlots + b holds units in q[qbase[qi] ..] (a whole line)
+// in the order they became ready (-1: not yet written; else the unit id with
+// the epoch it runs); [head, tail) are the units not yet taken. A wave of
+// hardware XCD id h serves queue set xq[h] and takes the unit of the lowest
+// epoch ready there (the instance furthest behind, whose remaining epochs
+// decide when the batch ends), from its home class (home[x 128 + CU-pair
+// key]: the two compute units sharing an instruction cache keep to one
+// class's code) while that class is at most one epoch later. The epoch end of
+// an instance appends its next epoch's units to its queue. pend[inst]: units
+// of the instance's current epoch not yet done; live[x]: instances on XCD x
+// that can still append units. An instance whose epoch reaches epoch_limit
+// appends nothing (the host launches again).
+constexpr int kMaxXcd = 16;
+constexpr int kClasses = 3;
+constexpr int kEpochSlots = 8;
+constexpr int kQueues = kMaxXcd * kClasses * kEpochSlots;
+constexpr int kCuKeys = 128;
+// int32 words per 128-byte cache line: counters that different XCDs update
+// (the queues' counters, each instance's pending count) are a line apart
+constexpr int kLineWords = 32;
+#ifdef __HIPCC__
+__host__ __device__
+#endif
+inline int order_class(int nmax) { return nmax <= 16 ? 0 : nmax <= 32 ? 1 : 2; }
+struct PersistCtl {
+    int32_t* q;
+    const uint32_t* qbase;  // [qi]
+    const uint32_t* qcap;   // [qi]
+    const PUnit* units;
+    const int32_t* ubase;
+    const int32_t* unum;
+    const int8_t* home;  // [x * kCuKeys + key]
+    int32_t* pend;       // [inst * kLineWords]
+    uint32_t* head;      // [qi * kLineWords]
+    uint32_t* tail;      // [qi * kLineWords]
+    int32_t* live;       // [x * kLineWords]
+    int32_t* err;        // 1: an append did not fit its queue
+    // host-mapped outcomes: an instance that ends has its outcome written at
+    // out + out_off[inst] before its done flag is raised (null: none)
+    unsigned char* out;
+    const int64_t* out_off;
+    int32_t xq[kMaxXcd];  // hardware XCD id -> queue set, -1: none
+    int32_t nq;
+    uint32_t epoch_limit;
+    uint32_t epoch_base;  // the lowest epoch of the launch's instances
 };
 
 }  // namespace xtx

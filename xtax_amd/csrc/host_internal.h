@@ -380,6 +380,9 @@ struct xtx_ctx {
     std::map<int, xtx::host::Layout> tuned;
     // last solve of the primal/dual objective (its instance is `plain`)
     xtx::host::DualLast dual;
+    // the device's compute-unit pairs as XCD (die) id << 8 | CU-pair key,
+    // probed once for the batch's persistent launch
+    std::vector<int> xcc_ids;
 };
 
 namespace xtx {

@@ -48,6 +48,7 @@ class EngineParams:
     waves_per_block: int = 0
     autotune: bool = False
     lookahead: int = 0
+    persistent: bool = True
 
     def to_c(self):
         p = N.Params()
