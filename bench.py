@@ -84,15 +84,15 @@ def base_line(args, value, unit, el, config, higher=True):
             "dtype": "int64", "data": "synthetic", "config": config}
 
 
-def measured_traffic(profile):
-    """HBM bytes of one epoch of the dominant kernel from its committed rocprof
-    summary (FETCH_SIZE and WRITE_SIZE passes scaled by the copy-kernel
-    calibration), or None."""
+def measured_traffic(profile, per="epoch"):
+    """HBM bytes of one epoch (or one launch) of the dominant kernel from its
+    committed rocprof summary (FETCH_SIZE and WRITE_SIZE passes scaled by the
+    copy-kernel calibration), or None."""
     import re
     path = os.path.join(ROOT, "profiles", profile)
     try:
         with open(path) as f:
-            m = re.search(r"traffic: ([0-9.]+e[0-9]+) bytes per epoch", f.read())
+            m = re.search(r"traffic: ([0-9.]+e[0-9]+) bytes per " + per, f.read())
     except OSError:
         return None
     return float(m.group(1)) if m else None
@@ -331,7 +331,13 @@ def bench_batch(args, eng):
                 "kernel_moves_per_sec": moves / (kernel_ms / 1e3) if kernel_ms else None,
                 "roofline": roofline(kernel_ms, moves, mean_n, 0.5, kernel_name(
                     lane_instances, args.count * args.steps)), "cpu_baseline": None})
-    if out["roofline"] and out["roofline"]["kernel"] == "k_anneal_lane":
+    if out["roofline"] and persistent == args.steps and lane_instances == args.count * args.steps:
+        # the whole step is one launch of the persistent kernel; its HBM bytes
+        # per launch measured by rocprof on this workload
+        out["roofline"]["kernel"] = "k_lane_persist"
+        out["roofline"]["traffic"] = measured_traffic("k_lane_persist_batch1024.md", "launch")
+        out["roofline"]["traffic_unit"] = "bytes per launch (profiles/k_lane_persist_batch1024.md)"
+    elif out["roofline"] and out["roofline"]["kernel"] == "k_anneal_lane":
         # HBM bytes per epoch (the three launches of the lane kernel's order
         # pairs) measured by rocprof on this workload
         out["roofline"]["traffic"] = measured_traffic("k_anneal_lane_batch1024.md")
