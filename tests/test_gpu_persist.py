@@ -91,3 +91,25 @@ def test_persistent_falls_back_when_a_shape_is_set(eng):
     assert sa["persistent"] == 1 and sb["persistent"] == 0
     for q in range(len(mats)):
         assert np.array_equal(a[q][0], b[q][0]) and np.array_equal(a[q][1], b[q][1]), q
+
+
+def test_persistent_batch_stops_at_its_deadline(eng):
+    """A deadline ends a persistent launch: the waves leave at their next poll
+    (a stop sets every die's live count far below zero, so waves waiting for
+    work leave too), the call returns with every instance retired and
+    verified, and the next batch on the context runs in full."""
+    import time
+    mats = workloads.batch_config(count=512, seed=13)
+    seeds = [5_000_011 + q for q in range(len(mats))]
+    kw = dict(replicas=32, t_init=18.0, t_min=14.0, exchange_interval=2000)
+    t = time.perf_counter()
+    out, st = eng.batch_solve(mats, EngineParams(max_seconds=0.3, **kw), seeds=seeds)
+    wall = time.perf_counter() - t
+    say(f"deadline batch: {st['solved']} solved, {st['epochs']} epochs, {wall:.2f} s")
+    assert st["persistent"] == 1 and wall < 5.0
+    assert st["solved"] == st["verified"]
+    assert all(d["verify_status"] == 0 for _, _, d in out)
+    # the stop word is clear again: a capped batch runs all its epochs
+    out2, st2 = eng.batch_solve(mats[:128], EngineParams(max_epochs=2, **kw), seeds=seeds[:128])
+    assert st2["persistent"] == 1
+    assert all(d["moves"] == 2 * 2000 * 32 for _, _, d in out2 if not d["solved"])
