@@ -70,11 +70,12 @@ XTX_HD int32_t rounded_div32(int32_t a, int32_t b) {
     return q;
 }
 
-// p, qb of one column from a_ik, a_jk
+// p, qb of one column from a_ik, a_jk: with g = sign(a_ik) (+1 for 0),
+// p = g a_ik and qb = g a_jk + 2^31, two 24-bit multiply(-add)s
 XTX_HD void to_pq(int32_t ai, int32_t aj, int32_t& p, uint32_t& qb) {
-    const int32_t m = ai >> 31;
-    p = (ai ^ m) - m;
-    qb = (uint32_t)(aj ^ m) + ((uint32_t)ai >> 31) + kBias;
+    const int32_t g = (ai >> 31) | 1;
+    p = mul24(ai, g);
+    qb = mad24(aj, g, kBias);
 }
 
 // Candidate shears in the reference's order (main.cpp:214-231): floor and ceil
