@@ -674,6 +674,7 @@ int xtx_close(xtx_ctx* ctx) {
     }
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     if (ctx->stop_host) (void)hipHostFree(ctx->stop_host);
+    if (ctx->zc_host) (void)hipHostFree(ctx->zc_host);
     delete ctx;
     return XTX_OK;
 }
@@ -1131,9 +1132,6 @@ struct Server {
     // zero-copy retirement of the persistent launch: each instance's outcome
     // area in host-mapped memory (offsets per slot), which slots have been
     // taken from it, and outcomes read but not yet retired
-    unsigned char* zc_host = nullptr;
-    unsigned char* zc_dev = nullptr;
-    size_t zc_cap = 0;
     int64_t* d_zoff = nullptr;
     size_t zoff_cap = 0;
     std::vector<int64_t> zc_off;
@@ -1142,6 +1140,10 @@ struct Server {
     bool zero_copy = false;
 
     ~Server() {
+        // nothing in flight may still use what is freed here (a call that
+        // throws can leave a launch running)
+        if (ctx && ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+        if (fetch_stream) (void)hipStreamSynchronize(fetch_stream);
         if (gexec) (void)hipGraphExecDestroy(gexec);
         for (auto& kv : buckets) {
             if (kv.second.d_refs) (void)hipFree(kv.second.d_refs);
@@ -1161,7 +1163,6 @@ struct Server {
         if (d_q) (void)hipFree(d_q);
         if (d_pcnt) (void)hipFree(d_pcnt);
         if (d_zoff) (void)hipFree(d_zoff);
-        if (zc_host) (void)hipHostFree(zc_host);
     }
 
     double now() const { return std::chrono::duration<double>(clk::now() - t0).count(); }
@@ -1573,13 +1574,13 @@ struct Server {
             zc_off[(size_t)s] = (int64_t)zbytes;
             zbytes += (32 + (size_t)h.R * sizeof(RepState) + 16 * (size_t)h.n * h.n + 127) / 128 * 128;
         }
-        if (zbytes > zc_cap) {
-            if (zc_host) (void)hipHostFree(zc_host);
-            zc_host = zc_dev = nullptr;
-            zc_cap = 0;
-            HIPCHK(hipHostMalloc((void**)&zc_host, zbytes, hipHostMallocMapped));
-            HIPCHK(hipHostGetDevicePointer((void**)&zc_dev, zc_host, 0));
-            zc_cap = zbytes;
+        if (zbytes > ctx->zc_cap) {
+            if (ctx->zc_host) (void)hipHostFree(ctx->zc_host);
+            ctx->zc_host = ctx->zc_dev = nullptr;
+            ctx->zc_cap = 0;
+            HIPCHK(hipHostMalloc((void**)&ctx->zc_host, zbytes, hipHostMallocMapped));
+            HIPCHK(hipHostGetDevicePointer((void**)&ctx->zc_dev, ctx->zc_host, 0));
+            ctx->zc_cap = zbytes;
         }
         zc_taken.assign((size_t)ns, 0);
         ensure(d_zoff, zoff_cap, zc_off.size());
@@ -1623,7 +1624,7 @@ struct Server {
         c.tail = d_pcnt + (size_t)kQueues * kLineWords;
         c.live = (int32_t*)(d_pcnt + (size_t)2 * kQueues * kLineWords);
         c.err = (int32_t*)(d_pcnt + ((size_t)2 * kQueues + kMaxXcd) * kLineWords);
-        c.out = zc_dev;
+        c.out = ctx->zc_dev;
         c.out_off = d_zoff;
         for (int h = 0; h < kMaxXcd; ++h) c.xq[h] = -1;
         for (int x = 0; x < nq; ++x) c.xq[ids[(size_t)x]] = x;
@@ -1656,7 +1657,7 @@ struct Server {
                 if (!slots[(size_t)s] || zc_taken[(size_t)s] || zc_off[(size_t)s] < 0) continue;
                 if (__atomic_load_n(&done_host[s], __ATOMIC_ACQUIRE) == 0) continue;
                 const HostInst& h = *slots[(size_t)s];
-                const unsigned char* src = zc_host + zc_off[(size_t)s];
+                const unsigned char* src = ctx->zc_host + zc_off[(size_t)s];
                 const int32_t* head = (const int32_t*)src;
                 if (head[1]) throw Err{XTX_E_STATE, "internal: best replica has a stale snapshot"};
                 Outcome out;

@@ -383,6 +383,11 @@ struct xtx_ctx {
     // the device's compute-unit pairs as XCD (die) id << 8 | CU-pair key,
     // probed once for the batch's persistent launch
     std::vector<int> xcc_ids;
+    // the persistent launch's host-mapped outcome buffer, kept for the next
+    // batch (pinning tens of MB per call would cost milliseconds each time)
+    unsigned char* zc_host = nullptr;
+    unsigned char* zc_dev = nullptr;
+    size_t zc_cap = 0;
 };
 
 namespace xtx {
