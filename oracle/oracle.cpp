@@ -8,6 +8,7 @@
 // engine's per-move algorithm (the reference's anneal_worker loop with Philox
 // draws, see the model section).
 #include "oracle.h"
+#include "ref_draw.h"
 
 #include <algorithm>
 #include <cmath>
@@ -582,38 +583,6 @@ struct PhiloxDraw {
     uint32_t sweep_index() const { return mulhi32(W[4][3], N); }
 };
 
-// The reference's draws (congruence_anneal.hpp:189-192): one mt19937 stream
-// per worker, libstdc++'s uniform_int_distribution over active positions and
-// uniform_real_distribution(0, 1), consumed in the loop's order, with the
-// redraw loops for j != i (:287) and for the fallback pivot (:226), the
-// sign from the low bit of a raw draw (:293, :295) and std::exp(-d / T) in
-// the Metropolis test (:236).
-struct RefDraw {
-    std::mt19937 rng;
-    std::uniform_int_distribution<int> aidx;
-    std::uniform_real_distribution<double> unit{0.0, 1.0};
-    RefDraw(uint32_t seed, int N) : rng(seed), aidx(0, N - 1) {}
-    bool coin(double f) { return unit(rng) < f; }
-    uint32_t index() { return (uint32_t)aidx(rng); }
-    uint32_t pivot_index() { return (uint32_t)aidx(rng); }
-    uint32_t fallback(uint32_t tpos) {
-        uint32_t q;
-        do q = (uint32_t)aidx(rng);
-        while (q == tpos);
-        return q;
-    }
-    void pair(uint32_t& ip, uint32_t& jp) {
-        ip = (uint32_t)aidx(rng);
-        do jp = (uint32_t)aidx(rng);
-        while (jp == ip);
-    }
-    bool greedy_coin(double f) { return unit(rng) < f; }
-    int64_t sign() { return (rng() & 1u) ? 1 : -1; }
-    bool metropolis(int64_t d, double T) { return unit(rng) < std::exp(-(double)d / T); }
-    bool sweep_coin(double f) { return unit(rng) < f; }
-    uint32_t sweep_index() { return (uint32_t)aidx(rng); }
-};
-
 struct Model {
     int n = 0, N = 0;
     std::vector<int> act;
@@ -892,6 +861,8 @@ int64_t orc_exchange_sweep(const double* temps, const double* score, int32_t* ru
 }
 
 void orc_set_mutation(int32_t which) { g_mutation = which; }
+
+int32_t orc_get_mutation(void) { return g_mutation; }
 
 int32_t orc_metropolis(int64_t d, double T, uint32_t u_hi, uint32_t u_lo) {
     // the step's test with the engine's uniform (Model::step_with, PhiloxDraw)

@@ -220,6 +220,31 @@ typedef struct {
     int64_t gb_score;    /* global best */
 } orc_ref_result;
 
+/* The dual model's loop body in the reference's mode: xdual's run with one
+ * worker (its long double arithmetic, mt19937 draws from worker_seed, publish
+ * rule and per-move maintenance; dp->c unused, the weight is lambda scaled by
+ * the start's primal/dual norm ratio). Returns 1 if the run stopped at a
+ * diagonal, 0 at p->max_moves, -1 if P is singular. gb_P/gb_Q/gb_Xt: the
+ * global best (the reference's best_P, best_Q, best_X^T). */
+typedef struct {
+    int32_t solved;
+    int32_t reserved;
+    int64_t moves;        /* the reference's "done moves=" */
+    int64_t accepts;
+    int64_t refreshes;
+    int64_t checks;
+    int64_t recomputes;
+    double score;         /* final running score */
+    double gb_score;      /* global best score */
+    double c;             /* the scaled dual weight */
+} orc_dual_ref_result;
+
+int32_t orc_dual_ref_run(const int64_t* P_start, const int64_t* Xt_start, int32_t n,
+                         const int32_t* active, int32_t n_active, const orc_params* p,
+                         const orc_dual_params* dp, double lambda, uint32_t worker_seed,
+                         int64_t* gb_P, double* gb_Q, int64_t* gb_Xt, orc_log_entry* log_out,
+                         int64_t* log_count, int64_t log_cap, orc_dual_ref_result* res);
+
 /* seed_rng() ^ t * 0x9e3779b9 for worker t (congruence_anneal.hpp:484-488) */
 uint32_t orc_ref_worker_seed(uint64_t seed, int32_t t);
 int32_t orc_ref_worker_run(const int64_t* A_start, const int64_t* Xt_start, int32_t n,
@@ -233,11 +258,14 @@ int32_t orc_ref_worker_run(const int64_t* A_start, const int64_t* Xt_start, int3
  * number of swaps. */
 int64_t orc_exchange_sweep(const double* temps, const double* score, int32_t* rung, int32_t R,
                            uint64_t seed, uint64_t epoch);
-/* Test hook: plants one of a few plausible mistakes in the model (0 = none,
+/* Test hook: plants one of a few plausible mistakes in the models (0 = none,
  * 1 strict auto-accept, 2 reversed hot-row tournament, 3 diversity offset
- * sign, 4 exchange argument sign, 5 Metropolis exponent sign), so tests can
- * show that the checks pinning the model reject each. */
+ * sign, 4 exchange argument sign, 5 Metropolis exponent sign; 1, 2 and 5 in
+ * the dual model's loop body too; 6 the dual model's Q update committed to
+ * row and column j instead of i), so tests can show that the checks pinning
+ * the models reject each. */
 void orc_set_mutation(int32_t which);
+int32_t orc_get_mutation(void);
 /* The model's Metropolis decision for a proposal of score change d at
  * temperature T, given the move's two uniform words: 1 = accept. */
 int32_t orc_metropolis(int64_t d, double T, uint32_t u_hi, uint32_t u_lo);

@@ -4,7 +4,10 @@
 // move primitives of xdual's DualObjective (src/xdual_main.cpp:67-364) in the
 // engine's double specification, the reference's double Gauss-Jordan inverse
 // (invert_to, src/mat_io.hpp:297-337) exactly, and a trajectory model of the
-// engine running that objective. The specification:
+// engine running that objective, whose loop body also runs in the reference's
+// mode (its long double arithmetic, draws, publish rule and per-move
+// maintenance: orc_dual_ref_run) to be pinned against the reference's own
+// xdual runs. The engine's specification:
 //   * a sum over the columns k of a row folds column k + 64 onto column k
 //     (zero-padded to 128 columns), then adds the 64 results as a binary tree
 //     of adjacent pairs;
@@ -21,9 +24,11 @@
 #include <cmath>
 #include <cstdint>
 #include <array>
+#include <random>
 #include <vector>
 
 #include "oracle.h"
+#include "ref_draw.h"
 
 namespace {
 
@@ -277,10 +282,180 @@ inline double du53(uint32_t hi, uint32_t lo) {
     return (double)(((uint64_t)(hi >> 5) << 26) | (lo >> 6)) * 0x1p-53;
 }
 
+// The engine's draws for one move of a replica: the Philox words of its
+// slots, in the L1 model's slot map (oracle.cpp).
+struct DualPhiloxDraw {
+    std::vector<std::array<uint32_t, 4>> W;
+    uint32_t N = 0;
+    int row = 0, piv = 0;  // tournament candidates drawn so far
+    bool coin(double f) const { return du01(W[0][2]) < f; }
+    uint32_t index() { return dmulhi32(W[(size_t)row++][0], N); }
+    uint32_t pivot_index() { return dmulhi32(W[(size_t)piv++][1], N); }
+    uint32_t fallback(uint32_t tpos) const {
+        return (tpos + 1 + dmulhi32(W[3][3], N - 1)) % N;
+    }
+    void pair(uint32_t& ip, uint32_t& jp) const {
+        ip = dmulhi32(W[2][3], N);
+        jp = (ip + 1 + dmulhi32(W[3][2], N - 1)) % N;
+    }
+    bool greedy_coin(double f) const { return du01(W[0][3]) < f; }
+    int64_t sign() const { return (W[1][2] & 1u) ? 1 : -1; }
+    bool metropolis(double d, double T) const {
+        return du53(W[1][3], W[2][2]) < orc_xexp(d * (-1.0 / T));
+    }
+    bool sweep_coin(double f) const { return du01(W[4][2]) < f; }
+    uint32_t sweep_index() const { return dmulhi32(W[4][3], N); }
+};
+
+// The engine's arithmetic: the double specification at the top of this file.
+struct EngineArith {
+    using S = double;
+    static constexpr bool kRef = false;
+    static bool eval(const int64_t* P, const double* Q, int n, int i, int j, int64_t s, S c,
+                     int64_t* prow, double* qrow, S* d, int64_t* dnz) {
+        double d_p = 0, d_q = 0;
+        if (!orc_dual_eval(P, Q, n, i, j, s, prow, qrow, &d_p, &d_q, dnz)) return false;
+        const double cq = c * d_q;
+        *d = d_p + cq;
+        return true;
+    }
+    static int64_t best_shear(const int64_t* P, const double* Q, int n, S c, int i, int j) {
+        return orc_dual_best_shear(P, Q, n, c, i, j);
+    }
+    static S score(const int64_t* P, const double* Q, int n, S c) {
+        return orc_dual_score(P, Q, n, c);
+    }
+    static double residual(const int64_t* P, const double* Q, int n, int k) {
+        return orc_dual_residual(P, Q, n, k);
+    }
+};
+
+// The reference's arithmetic: long double sums in the order of its loops
+// (eval_add and eval_add_d, xdual_main.cpp:67-158; best_shear :254-296;
+// combined_score :160-162 over offdiag_frob2, mat_io.hpp:240-251 and
+// :280-291; the sampled residual :316-333). P's entries are exact, Q's are
+// doubles, the score and the dual weight long doubles.
+struct RefArith {
+    using S = long double;
+    static constexpr bool kRef = true;
+    static bool eval(const int64_t* P, const double* Q, int n, int i, int j, int64_t s, S c,
+                     int64_t* prow, double* qrow, S* d, int64_t* dnz) {
+        long double dp = 0.0L, dq = 0.0L;
+        int64_t z = 0;
+        for (int k = 0; k < n; ++k) {
+            if (k == j) continue;
+            const int64_t old = P[(size_t)j * n + k];
+            const __int128 nv = (__int128)old + (__int128)s * P[(size_t)i * n + k];
+            if (over(nv)) return false;
+            prow[k] = (int64_t)nv;
+            dp += (long double)prow[k] * (long double)prow[k] - (long double)old * (long double)old;
+            if (old == 0) {
+                if (nv != 0) ++z;
+            } else if (nv == 0) {
+                --z;
+            }
+        }
+        const __int128 new_tt = (__int128)P[(size_t)j * n + j] +
+                                2 * (__int128)s * P[(size_t)j * n + i] +
+                                (__int128)s * s * P[(size_t)i * n + i];
+        if (over(new_tt)) return false;
+        prow[j] = (int64_t)new_tt;
+        // dual: target i, pivot j, shear -s
+        const double sf = (double)(-s);
+        for (int k = 0; k < n; ++k) {
+            if (k == i) continue;
+            const double old = Q[(size_t)i * n + k];
+            const double nv = old + sf * Q[(size_t)j * n + k];
+            qrow[k] = nv;
+            dq += (long double)nv * (long double)nv - (long double)old * (long double)old;
+        }
+        qrow[i] = Q[(size_t)i * n + i] + 2.0 * sf * Q[(size_t)i * n + j] +
+                  sf * sf * Q[(size_t)j * n + j];
+        *d = 2.0L * dp + c * (2.0L * dq);
+        *dnz = z;
+        return true;
+    }
+    static int64_t best_shear(const int64_t* P, const double* Q, int n, S c, int i, int j) {
+        long double a2 = 0.0L, b2 = 0.0L, c2 = 0.0L, d2 = 0.0L;
+        for (int k = 0; k < n; ++k) {
+            if (k == j) continue;
+            const long double pik = (long double)P[(size_t)i * n + k];
+            a2 += pik * pik;
+            b2 += (long double)P[(size_t)j * n + k] * pik;
+        }
+        for (int k = 0; k < n; ++k) {
+            if (k == i) continue;
+            const long double qjk = (long double)Q[(size_t)j * n + k];
+            c2 += qjk * qjk;
+            d2 += (long double)Q[(size_t)i * n + k] * qjk;
+        }
+        const long double quad = a2 + c * c2;
+        if (!(quad > 0.0L)) return 0;
+        const long double lin = b2 - c * d2;
+        const long double s_star = -lin / quad;
+        // the double's floor converted as x86-64 converts it (out of range or
+        // NaN: the lowest int64)
+        const double fl = std::floor((double)s_star);
+        int64_t lo = (fl >= -0x1p63 && fl < 0x1p63) ? (int64_t)fl : INT64_MIN;
+        int64_t hi = lo + 1;
+        lo = std::clamp(lo, -(int64_t)ORC_SHEAR_CAP, (int64_t)ORC_SHEAR_CAP);
+        hi = std::clamp(hi, -(int64_t)ORC_SHEAR_CAP, (int64_t)ORC_SHEAR_CAP);
+        int64_t best_s = 0;
+        long double best_f = 0.0L;
+        for (const int64_t sv : {lo, hi}) {
+            if (sv == 0) continue;
+            const long double x = (long double)sv;
+            const long double f = 2.0L * (quad * x * x + 2.0L * lin * x);
+            if (f < best_f) {
+                best_f = f;
+                best_s = sv;
+            }
+        }
+        return best_s;
+    }
+    static long double frob2_p(const int64_t* P, int n) {
+        long double t = 0.0L;
+        for (int r = 0; r < n; ++r)
+            for (int k = 0; k < n; ++k)
+                if (r != k) {
+                    const long double v = (long double)P[(size_t)r * n + k];
+                    t += v * v;
+                }
+        return t;
+    }
+    static long double frob2_q(const double* Q, int n) {
+        long double t = 0.0L;
+        for (int r = 0; r < n; ++r)
+            for (int k = 0; k < n; ++k)
+                if (r != k) {
+                    const long double v = (long double)Q[(size_t)r * n + k];
+                    t += v * v;
+                }
+        return t;
+    }
+    static S score(const int64_t* P, const double* Q, int n, S c) {
+        return frob2_p(P, n) + c * frob2_q(Q, n);
+    }
+    static double residual(const int64_t* P, const double* Q, int n, int k) {
+        long double worst = 0.0L;
+        for (int r = 0; r < n; ++r) {
+            long double acc = 0.0L;
+            for (int t = 0; t < n; ++t)
+                acc += (long double)P[(size_t)r * n + t] * (long double)Q[(size_t)t * n + k];
+            if (r == k) acc -= 1.0L;
+            const long double a = acc < 0.0L ? -acc : acc;
+            if (a > worst) worst = a;
+        }
+        return (double)worst;
+    }
+};
+
+template <class S>
 struct DualRep {
     std::vector<int64_t> P, Xt, rm, bestP, bestXt;
     std::vector<double> Q, bestQ;
-    double score = 0, local_best = 0, best_score = 0, T = 0, cooling = 0, dc = 0;
+    S score = 0, local_best = 0, best_score = 0;
+    double T = 0, cooling = 0, dc = 0;
     int64_t offnz = 0, msi = 0, best_move = 0, accepts = 0, solve_move = 0;
     int64_t refreshes = 0, checks = 0, recomputes = 0;
     uint64_t moves = 0, m_maint = 0;
@@ -290,39 +465,54 @@ struct DualRep {
     std::vector<orc_log_entry> log;
 };
 
-struct DualModel {
+// The loop body over an arithmetic (EngineArith, RefArith) and a draw source
+// (DualPhiloxDraw, RefDraw). The engine's mode keeps per-replica bests, its
+// recompute rule and epoch-start maintenance (above); the reference's mode
+// (Ar::kRef, one worker) its publish rule: an accepted downhill move whose
+// running score is below the global best recomputes the score and, if still
+// below, becomes the global best, which stops the run when it is diagonal
+// (congruence_anneal.hpp:309-352); and its maintenance after every move
+// (:361).
+template <class Ar>
+struct DualModelT {
+    using S = typename Ar::S;
+    using Rep = DualRep<S>;
+    static constexpr bool kRef = Ar::kRef;
+
     int n = 0, N = 0;
     std::vector<int> act;
     orc_params p{};
     orc_dual_params dp{};
+    S c = 0;  // dual weight
     bool tempering = false;
     double t_init = 0;
     std::vector<double> temps;
     uint32_t key[2] = {0, 0};
-    std::vector<DualRep> reps;
-    double gb_score = 0;
+    std::vector<Rep> reps;
+    // engine: the instance best at the epoch start; reference: its global best
+    S gb_score = 0;
     int64_t gb_offnz = 0;
     std::vector<int64_t> gbP, gbXt;
     std::vector<double> gbQ;
     int64_t log_cap = 0;
 
-    void block(uint64_t m, uint32_t r, uint32_t c, uint32_t w[4]) const {
+    void block(uint64_t m, uint32_t r, uint32_t cs, uint32_t w[4]) const {
         const uint64_t q = (m - 1) >> 2;
         const uint32_t t = (uint32_t)((m - 1) & 3);
         const uint32_t ctr[4] = {(uint32_t)q, (uint32_t)(q >> 32), r,
-                                 64 * (c >> 4) + 16 * t + (c & 15)};
+                                 64 * (cs >> 4) + 16 * t + (cs & 15)};
         orc_philox4x32_10(ctr, key, w);
     }
-    void log_commit(DualRep& R, uint64_t m, int i, int j, int64_t s, uint16_t flags) {
+    void log_commit(Rep& R, uint64_t m, int i, int j, int64_t s, uint16_t flags) {
         if ((int64_t)R.log.size() < log_cap)
             R.log.push_back(orc_log_entry{(int64_t)m, (int16_t)i, (int16_t)j, (int16_t)s, flags});
     }
-    double rescore(const DualRep& R) const { return orc_dual_score(R.P.data(), R.Q.data(), n, dp.c); }
+    S rescore(const Rep& R) const { return Ar::score(R.P.data(), R.Q.data(), n, c); }
 
     // DualObjective::commit (xdual_main.cpp:236-252): the transform row is
     // bound-checked first; P row/column j with the row masses, Q row/column
     // i, X^T row j.
-    bool commit(DualRep& R, int i, int j, int64_t s, const std::vector<int64_t>& prow,
+    bool commit(Rep& R, int i, int j, int64_t s, const std::vector<int64_t>& prow,
                 const std::vector<double>& qrow) {
         std::vector<int64_t> xrow((size_t)n);
         if (!orc_add_transform_row(R.Xt.data(), n, i, j, s, xrow.data())) return false;
@@ -334,21 +524,23 @@ struct DualModel {
             off += std::llabs(v) - std::llabs(o);
         }
         R.rm[(size_t)j] += off;
+        const int qi = orc_get_mutation() == 6 ? j : i;
         for (int k = 0; k < n; ++k) {
             R.P[(size_t)j * n + k] = prow[(size_t)k];
             R.P[(size_t)k * n + j] = prow[(size_t)k];
-            R.Q[(size_t)i * n + k] = qrow[(size_t)k];
-            R.Q[(size_t)k * n + i] = qrow[(size_t)k];
+            R.Q[(size_t)qi * n + k] = qrow[(size_t)k];
+            R.Q[(size_t)k * n + qi] = qrow[(size_t)k];
             R.Xt[(size_t)j * n + k] = xrow[(size_t)k];
         }
         return true;
     }
-    void snapshot(DualRep& R) {
+    void snapshot(Rep& R) {
         R.bestP = R.P;
         R.bestQ = R.Q;
         R.bestXt = R.Xt;
     }
-    void track_best(DualRep& R, uint64_t m) {
+    void track_best(Rep& R, uint64_t m) {
+        if (kRef) return;  // the reference keeps no per-worker best state
         if (R.score < R.best_score) {
             R.best_score = R.score;
             R.best_move = (int64_t)m;
@@ -356,23 +548,22 @@ struct DualModel {
         }
     }
 
-    bool sweep(DualRep& R, int pivot, uint64_t m) {
-        const double before = R.score;
+    // Greedy reduction sweep from a pivot (congruence_anneal.hpp:246-260).
+    bool sweep(Rep& R, int pivot, uint64_t m) {
+        const S before = R.score;
         std::vector<int64_t> prow((size_t)n);
         std::vector<double> qrow((size_t)n);
         for (int q = 0; q < N; ++q) {
             const int j = act[(size_t)q];
             if (j == pivot) continue;
-            const int64_t s = orc_dual_best_shear(R.P.data(), R.Q.data(), n, dp.c, pivot, j);
+            const int64_t s = Ar::best_shear(R.P.data(), R.Q.data(), n, c, pivot, j);
             if (s == 0) continue;
-            double d_p = 0, d_q = 0;
+            S d = 0;
             int64_t dnz = 0;
-            if (!orc_dual_eval(R.P.data(), R.Q.data(), n, pivot, j, s, prow.data(), qrow.data(),
-                               &d_p, &d_q, &dnz))
+            if (!Ar::eval(R.P.data(), R.Q.data(), n, pivot, j, s, c, prow.data(), qrow.data(), &d,
+                          &dnz))
                 continue;
-            const double cq = dp.c * d_q;
-            const double d = d_p + cq;
-            if (d > 0.0) continue;
+            if ((double)d > 0.0) continue;
             if (!commit(R, pivot, j, s, prow, qrow)) continue;
             R.score += d;
             R.offnz += dnz;
@@ -382,25 +573,37 @@ struct DualModel {
         return R.score < before;
     }
 
-    void step(DualRep& R, uint32_t r) {
+    // One move with the engine's draws: Philox slots of (replica, move).
+    void step(Rep& R, uint32_t r) {
         const uint64_t m = ++R.moves;
-        if (tempering) R.T = temps[(size_t)R.rung];
+        DualPhiloxDraw D;
+        D.N = (uint32_t)N;
         const int G = std::max(p.target_samples, 5);
-        std::vector<std::array<uint32_t, 4>> W((size_t)G);
-        for (int g = 0; g < G; ++g) block(m, r, (uint32_t)g, W[(size_t)g].data());
+        D.W.resize((size_t)G);
+        for (int g = 0; g < G; ++g) block(m, r, (uint32_t)g, D.W[(size_t)g].data());
+        step_with(R, D, m);
+    }
 
+    // One iteration of the worker loop (congruence_anneal.hpp:276-413).
+    template <class D>
+    void step_with(Rep& R, D& draw, uint64_t m) {
+        const int mut = orc_get_mutation();
+        if (tempering) R.T = temps[(size_t)R.rung];
+
+        // pair: hot-row and pivot tournaments, else uniform (:211-228, :283-288)
         int i, j;
-        if (du01(W[0][2]) < R.tf) {
-            uint32_t tpos = dmulhi32(W[0][0], (uint32_t)N);
-            for (int c = 1; c < p.target_samples; ++c) {
-                const uint32_t q = dmulhi32(W[c][0], (uint32_t)N);
-                if (R.rm[(size_t)act[q]] > R.rm[(size_t)act[tpos]]) tpos = q;
+        if (draw.coin(R.tf)) {
+            uint32_t tpos = draw.index();
+            for (int q = 1; q < p.target_samples; ++q) {
+                const uint32_t cand = draw.index();
+                const int64_t wc = R.rm[(size_t)act[cand]], wt = R.rm[(size_t)act[tpos]];
+                if (mut == 2 ? wc < wt : wc > wt) tpos = cand;
             }
             const int target = act[tpos];
             int ibest = -1;
             int64_t vbest = -1;
-            for (int c = 0; c < p.target_samples; ++c) {
-                const int cand = act[dmulhi32(W[c][1], (uint32_t)N)];
+            for (int q = 0; q < p.target_samples; ++q) {
+                const int cand = act[draw.pivot_index()];
                 if (cand == target) continue;
                 const int64_t v = std::llabs(R.P[(size_t)target * n + cand]);
                 if (v > vbest) {
@@ -408,53 +611,57 @@ struct DualModel {
                     ibest = cand;
                 }
             }
-            if (ibest < 0)
-                ibest = act[(tpos + 1 + dmulhi32(W[3][3], (uint32_t)(N - 1))) % (uint32_t)N];
+            if (ibest < 0) ibest = act[draw.fallback(tpos)];
             i = ibest;
             j = target;
         } else {
-            const uint32_t ip = dmulhi32(W[2][3], (uint32_t)N);
-            const uint32_t jp = (ip + 1 + dmulhi32(W[3][2], (uint32_t)(N - 1))) % (uint32_t)N;
+            uint32_t ip, jp;
+            draw.pair(ip, jp);
             i = act[ip];
             j = act[jp];
         }
-        const int64_t pm = (W[1][2] & 1u) ? 1 : -1;
-        int64_t s = pm;
-        if (du01(W[0][3]) < R.gf) {
-            s = orc_dual_best_shear(R.P.data(), R.Q.data(), n, dp.c, i, j);
-            if (s == 0) s = pm;
-        }
+        // shear: the best with probability greedy, +-1 otherwise or when the
+        // best is 0 (:290-296)
+        int64_t s = 0;
+        if (draw.greedy_coin(R.gf)) s = Ar::best_shear(R.P.data(), R.Q.data(), n, c, i, j);
+        if (s == 0) s = draw.sign();
 
+        // attempt_add (:232-240): evaluate, Metropolis on the double delta, commit
         int res = -1;
         {
             std::vector<int64_t> prow((size_t)n);
             std::vector<double> qrow((size_t)n);
-            double d_p = 0, d_q = 0;
+            S d = 0;
             int64_t dnz = 0;
-            if (orc_dual_eval(R.P.data(), R.Q.data(), n, i, j, s, prow.data(), qrow.data(), &d_p,
-                              &d_q, &dnz)) {
-                const double cq = dp.c * d_q;
-                const double d = d_p + cq;
-                const bool ok = d <= 0.0 || du53(W[1][3], W[2][2]) < orc_xexp(d * (-1.0 / R.T));
+            if (Ar::eval(R.P.data(), R.Q.data(), n, i, j, s, c, prow.data(), qrow.data(), &d,
+                         &dnz)) {
+                const double dd = (double)d;
+                const bool ok = (mut == 1 ? dd < 0.0 : dd <= 0.0) ||
+                                draw.metropolis(mut == 5 ? -dd : dd, R.T);
                 if (ok && commit(R, i, j, s, prow, qrow)) {
-                    const double ns = R.score + d;
-                    const bool rec = d < 0.0 && ns < R.best_score && ns < gb_score;
-                    bool newbest = ns < R.best_score;
-                    R.score = ns;
-                    R.offnz += dnz;
-                    if (rec) {
-                        R.score = rescore(R);
-                        ++R.recomputes;
-                        newbest = R.score < R.best_score;
-                    }
-                    if (newbest) {
-                        R.best_score = R.score;
-                        R.best_move = (int64_t)m;
-                        snapshot(R);
+                    if (kRef) {
+                        R.score += d;
+                        R.offnz += dnz;
+                    } else {
+                        const S ns = R.score + d;
+                        const bool rec = d < 0 && ns < R.best_score && ns < gb_score;
+                        bool newbest = ns < R.best_score;
+                        R.score = ns;
+                        R.offnz += dnz;
+                        if (rec) {
+                            R.score = rescore(R);
+                            ++R.recomputes;
+                            newbest = R.score < R.best_score;
+                        }
+                        if (newbest) {
+                            R.best_score = R.score;
+                            R.best_move = (int64_t)m;
+                            snapshot(R);
+                        }
                     }
                     ++R.accepts;
                     log_commit(R, m, i, j, s, 0);
-                    res = d < 0.0 ? 1 : 0;
+                    res = dd < 0.0 ? 1 : 0;
                 }
             }
         }
@@ -466,15 +673,33 @@ struct DualModel {
         } else {
             ++R.msi;
         }
-        if (R.offnz == 0) {
+        if (kRef) {
+            if (res == 1 && (double)R.score < (double)gb_score) {
+                R.score = rescore(R);
+                ++R.recomputes;
+                if ((double)R.score < (double)gb_score) {
+                    gb_score = R.score;
+                    gb_offnz = R.offnz;
+                    gbP = R.P;
+                    gbQ = R.Q;
+                    gbXt = R.Xt;
+                    if (R.offnz == 0) {
+                        R.solved = R.stopped = true;
+                        R.solve_move = (int64_t)m;
+                        return;
+                    }
+                }
+            }
+            maintain_move(R, m);
+        } else if (R.offnz == 0) {
             R.solved = R.stopped = true;
             R.solve_move = (int64_t)m;
             return;
         }
         if (tempering) {
             if (R.msi > p.stuck_threshold) {
-                if (p.sweep_fraction > 0.0 && du01(W[4][2]) < p.sweep_fraction) {
-                    sweep(R, act[dmulhi32(W[4][3], (uint32_t)N)], m);
+                if (p.sweep_fraction > 0.0 && draw.sweep_coin(p.sweep_fraction)) {
+                    sweep(R, act[draw.sweep_index()], m);
                     track_best(R, m);
                 }
                 R.local_best = R.score;
@@ -482,15 +707,15 @@ struct DualModel {
             }
         } else {
             if (R.msi > p.stuck_threshold) {
-                if (p.sweep_fraction > 0.0 && du01(W[4][2]) < p.sweep_fraction) {
-                    if (sweep(R, act[dmulhi32(W[4][3], (uint32_t)N)], m)) {
+                if (p.sweep_fraction > 0.0 && draw.sweep_coin(p.sweep_fraction)) {
+                    if (sweep(R, act[draw.sweep_index()], m)) {
                         R.local_best = R.score;
                         R.msi = 0;
                     }
                     track_best(R, m);
                 }
                 R.T = p.reheat * t_init;
-                if (R.score > gb_score * p.reseed_factor) {
+                if ((double)R.score > (double)gb_score * p.reseed_factor) {
                     log_commit(R, m, -1, -1, 0, 2);
                     R.P = gbP;
                     R.Q = gbQ;
@@ -519,7 +744,7 @@ struct DualModel {
                 if (R.T < p.t_min) R.T = p.t_min;
             }
         }
-        if (R.offnz == 0) {
+        if (!kRef && R.offnz == 0) {
             R.solved = R.stopped = true;
             R.solve_move = (int64_t)m;
             return;
@@ -527,7 +752,26 @@ struct DualModel {
         if (p.max_moves && R.moves >= p.max_moves) R.stopped = true;
     }
 
-    void maintain(DualRep& R) {
+    // The reference's maintenance after move m (DualObjective::
+    // periodic_maintenance, xdual_main.cpp:305-334, the weight ramp off).
+    void maintain_move(Rep& R, uint64_t m) {
+        bool refresh = dp.dual_refresh > 0 && m % (uint64_t)dp.dual_refresh == 0;
+        if (!refresh && dp.dual_check > 0 && m % (uint64_t)dp.dual_check == 0) {
+            ++R.checks;
+            refresh = Ar::residual(R.P.data(), R.Q.data(), n, (int)(m % (uint64_t)n)) > dp.dual_tol;
+        }
+        if (refresh) {
+            std::vector<double> Qn((size_t)n * n);
+            if (orc_invert(R.P.data(), n, Qn.data())) {
+                R.Q.swap(Qn);
+                ++R.refreshes;
+                R.score = rescore(R);
+            }
+        }
+    }
+
+    // The engine's maintenance at an epoch start, for the moves since the last.
+    void maintain(Rep& R) {
         if (R.stopped) return;
         const uint64_t m0 = R.m_maint, m1 = R.moves;
         R.m_maint = m1;
@@ -539,9 +783,9 @@ struct DualModel {
             const uint64_t at = m1 / (uint64_t)dp.dual_check * (uint64_t)dp.dual_check;
             const int k = (int)(at % (uint64_t)n);
             ++R.checks;
-            refresh = orc_dual_residual(R.P.data(), R.Q.data(), n, k) > dp.dual_tol;
+            refresh = Ar::residual(R.P.data(), R.Q.data(), n, k) > dp.dual_tol;
         }
-        bool rescore_now = R.dc != dp.c;
+        bool rescore_now = R.dc != (double)c;
         if (refresh) {
             std::vector<double> Qn((size_t)n * n);
             if (orc_invert(R.P.data(), n, Qn.data())) {
@@ -552,7 +796,7 @@ struct DualModel {
         }
         if (rescore_now) {
             R.score = rescore(R);
-            R.dc = dp.c;
+            R.dc = (double)c;
         }
     }
 
@@ -569,6 +813,8 @@ struct DualModel {
     }
 };
 
+using EngineModel = DualModelT<EngineArith>;
+
 }  // namespace
 
 extern "C" int32_t orc_dual_model_run(const int64_t* P_start, const double* Q_start,
@@ -580,10 +826,11 @@ extern "C" int32_t orc_dual_model_run(const int64_t* P_start, const double* Q_st
                                       int64_t* best_P, double* best_Q, int64_t* best_Xt,
                                       orc_log_entry* log_out, int64_t* log_count,
                                       int64_t log_cap, orc_dual_result* res) {
-    DualModel M;
+    EngineModel M;
     M.n = n;
     M.p = *pp;
     M.dp = *dpp;
+    M.c = dpp->c;
     M.log_cap = log_cap;
     if (active && n_active > 0) {
         M.act.assign(active, active + n_active);
@@ -610,7 +857,7 @@ extern "C" int32_t orc_dual_model_run(const int64_t* P_start, const double* Q_st
     }
     M.reps.resize((size_t)R);
     for (int r = 0; r < R; ++r) {
-        DualRep& x = M.reps[(size_t)r];
+        EngineModel::Rep& x = M.reps[(size_t)r];
         x.P = P0;
         x.Q = Q0;
         x.Xt = X0;
@@ -646,14 +893,14 @@ extern "C" int32_t orc_dual_model_run(const int64_t* P_start, const double* Q_st
         for (int r = 0; r < R; ++r) M.maintain(M.reps[(size_t)r]);
         bool any_running = false;
         for (int r = 0; r < R; ++r) {
-            DualRep& x = M.reps[(size_t)r];
+            EngineModel::Rep& x = M.reps[(size_t)r];
             while (!x.stopped && x.moves < (uint64_t)(e + 1) * E) M.step(x, (uint32_t)r);
             any_running |= !x.stopped;
         }
         epochs = e + 1;
         int sr = -1;
         for (int r = 0; r < R; ++r) {
-            const DualRep& x = M.reps[(size_t)r];
+            const EngineModel::Rep& x = M.reps[(size_t)r];
             if (x.solved && (sr < 0 || x.solve_move < M.reps[(size_t)sr].solve_move)) sr = r;
         }
         if (sr >= 0) {
@@ -684,7 +931,7 @@ extern "C" int32_t orc_dual_model_run(const int64_t* P_start, const double* Q_st
         }
         const int b = M.best_rep();
         if (M.reps[(size_t)b].best_score < M.gb_score) {
-            const DualRep& x = M.reps[(size_t)b];
+            const EngineModel::Rep& x = M.reps[(size_t)b];
             M.gb_score = x.best_score;
             M.gbP = x.bestP;
             M.gbQ = x.bestQ;
@@ -695,13 +942,13 @@ extern "C" int32_t orc_dual_model_run(const int64_t* P_start, const double* Q_st
     }
     const int b = M.best_rep();
     out.best_replica = out.solved && out.solve_replica >= 0 ? out.solve_replica : b;
-    const DualRep& win = M.reps[(size_t)out.best_replica];
+    const EngineModel::Rep& win = M.reps[(size_t)out.best_replica];
     const bool cur = out.solved;
     out.best_score = cur ? win.score : win.best_score;
     out.epochs = epochs;
     out.exchanges = exchanges;
     for (int r = 0; r < R; ++r) {
-        const DualRep& x = M.reps[(size_t)r];
+        const EngineModel::Rep& x = M.reps[(size_t)r];
         out.total_moves += (int64_t)x.moves;
         out.accepts += x.accepts;
         out.refreshes += x.refreshes;
@@ -744,6 +991,97 @@ extern "C" int32_t orc_dual_model_run(const int64_t* P_start, const double* Q_st
     if (best_Xt) {
         const auto& src = cur ? win.Xt : win.bestXt;
         std::copy(src.begin(), src.end(), best_Xt);
+    }
+    if (res) *res = out;
+    return out.solved;
+}
+
+// ---------------------------------------------------------------------------
+// Reference-draw mode of the dual model: xdual's single-worker run
+// ---------------------------------------------------------------------------
+// The same loop body (DualModelT::step_with) with the reference's arithmetic
+// (RefArith), its draws (RefDraw, seeded with the worker seed) and its
+// publish and maintenance rules: run_xdual (xdual_main.cpp:378-412) with one
+// worker of run_annealer (congruence_anneal.hpp:422-444, tempering off below
+// two workers), which is deterministic. The dual start is the reference's
+// inverse of P, the weight lambda scaled by the ratio of the start's primal
+// and dual off-diagonal norms (:389-392).
+extern "C" int32_t orc_dual_ref_run(const int64_t* P_start, const int64_t* Xt_start, int32_t n,
+                                    const int32_t* active, int32_t n_active, const orc_params* pp,
+                                    const orc_dual_params* dpp, double lambda,
+                                    uint32_t worker_seed, int64_t* gb_P, double* gb_Q,
+                                    int64_t* gb_Xt, orc_log_entry* log_out, int64_t* log_count,
+                                    int64_t log_cap, orc_dual_ref_result* res) {
+    DualModelT<RefArith> M;
+    M.n = n;
+    M.p = *pp;
+    M.p.replicas = 1;
+    M.dp = *dpp;
+    M.log_cap = log_cap;
+    if (active && n_active > 0) {
+        M.act.assign(active, active + n_active);
+    } else {
+        M.act.resize((size_t)n);
+        for (int k = 0; k < n; ++k) M.act[(size_t)k] = k;
+    }
+    M.N = (int)M.act.size();
+    M.tempering = false;
+    const size_t nn = (size_t)n * n;
+    const std::vector<int64_t> P0(P_start, P_start + nn), X0(Xt_start, Xt_start + nn);
+    std::vector<double> Q0(nn);
+    orc_dual_ref_result out{};
+    if (n < 1 || !orc_invert(P0.data(), n, Q0.data())) {
+        if (res) *res = out;
+        return -1;
+    }
+    const long double sp = RefArith::frob2_p(P0.data(), n), sq = RefArith::frob2_q(Q0.data(), n);
+    long double c = (long double)lambda;
+    if (sp > 0.0L && sq > 0.0L && std::isfinite((double)sq)) c *= sp / sq;
+    M.c = c;
+    const long double score0 = RefArith::score(P0.data(), Q0.data(), n, c);
+    // suggest_t_init (xdual_main.cpp:212-219)
+    M.t_init = pp->t_init > 0.0
+                   ? pp->t_init
+                   : std::max(1.0, 2.0 * (double)(score0 / (long double)((int64_t)n * n)));
+    M.reps.resize(1);
+    auto& x = M.reps[0];
+    x.P = P0;
+    x.Q = Q0;
+    x.Xt = X0;
+    x.rm.resize((size_t)n);
+    orc_compute_row_mass(x.P.data(), n, x.rm.data());
+    x.score = x.local_best = x.best_score = score0;
+    x.offnz = orc_offdiag_nonzero(P0.data(), n);
+    x.T = M.t_init;
+    x.cooling = pp->cooling;
+    orc_fractions(&M.p, 0, &x.gf, &x.tf);
+    M.gb_score = score0;
+    M.gb_offnz = x.offnz;
+    M.gbP = P0;
+    M.gbQ = Q0;
+    M.gbXt = X0;
+    if (n >= 2 && M.N >= 2) {
+        RefDraw D(worker_seed, M.N);
+        while (!x.stopped) {
+            const uint64_t m = ++x.moves;
+            M.step_with(x, D, m);
+        }
+    }
+    out.solved = x.solved;
+    out.moves = (int64_t)x.moves;
+    out.accepts = x.accepts;
+    out.refreshes = x.refreshes;
+    out.checks = x.checks;
+    out.recomputes = x.recomputes;
+    out.score = (double)x.score;
+    out.gb_score = (double)M.gb_score;
+    out.c = (double)c;
+    if (gb_P) std::copy(M.gbP.begin(), M.gbP.end(), gb_P);
+    if (gb_Q) std::copy(M.gbQ.begin(), M.gbQ.end(), gb_Q);
+    if (gb_Xt) std::copy(M.gbXt.begin(), M.gbXt.end(), gb_Xt);
+    if (log_out && log_count) {
+        std::copy(x.log.begin(), x.log.end(), log_out);
+        *log_count = (int64_t)x.log.size();
     }
     if (res) *res = out;
     return out.solved;

@@ -47,6 +47,15 @@ class DualParams(C.Structure):
                 ("dual_check", C.c_int32)]
 
 
+class DualRefResult(C.Structure):
+    _fields_ = [
+        ("solved", C.c_int32), ("reserved", C.c_int32), ("moves", C.c_int64),
+        ("accepts", C.c_int64), ("refreshes", C.c_int64), ("checks", C.c_int64),
+        ("recomputes", C.c_int64), ("score", C.c_double), ("gb_score", C.c_double),
+        ("c", C.c_double),
+    ]
+
+
 class DualResult(C.Structure):
     _fields_ = [
         ("solved", C.c_int32), ("solve_replica", C.c_int32), ("solve_move", C.c_int64),
@@ -156,6 +165,8 @@ def lib():
         L.orc_exchange_sweep.argtypes = [DP, DP, I32P, C.c_int32, C.c_uint64, C.c_uint64]
         L.orc_set_mutation.restype = None
         L.orc_set_mutation.argtypes = [C.c_int32]
+        L.orc_get_mutation.restype = C.c_int32
+        L.orc_get_mutation.argtypes = []
         L.orc_metropolis.restype = C.c_int32
         L.orc_metropolis.argtypes = [C.c_int64, C.c_double, C.c_uint32, C.c_uint32]
         L.orc_ref_worker_seed.restype = C.c_uint32
@@ -164,6 +175,9 @@ def lib():
         L.orc_ref_worker_run.argtypes = [I64P, I64P, C.c_int32, VP, C.c_int32,
                                          C.POINTER(RefParams), VP, VP, VP, VP, VP, VP,
                                          C.c_int64, C.POINTER(RefResult)]
+        L.orc_dual_ref_run.restype = C.c_int32
+        L.orc_dual_ref_run.argtypes = [I64P, I64P, C.c_int32, VP, C.c_int32, VP, VP, C.c_double,
+                                       C.c_uint32, VP, VP, VP, VP, VP, C.c_int64, VP]
         L.orc_dual_residual.restype = C.c_double
         L.orc_dual_residual.argtypes = [I64P, DP, C.c_int32, C.c_int32]
         _LIB = L
@@ -500,4 +514,33 @@ def dual_model_run(P0, Q0, X0, params, dparams, max_epochs, active=None, log_cap
                **{f: getattr(res, f) for f, _ in DualResult._fields_})
     if log_cap:
         out["logs"] = [logs[r * log_cap: r * log_cap + counts[r]].copy() for r in range(R)]
+    return out
+
+
+def dual_ref_run(P0, X0, worker_seed, lam=1.0, active=None, dual_refresh=200000,
+                 dual_check=25000, dual_tol=1e-6, log_cap=0, **kw):
+    """The dual model's loop body in the reference's mode: xdual's run with
+    one worker from (P0, X0) (see oracle.h). kw: engine parameters (t_init,
+    ..., max_moves). Returns a dict: P, Q, Xt (the global best), log, and the
+    DualRefResult fields."""
+    P0 = _i64(P0)
+    n = P0.shape[0]
+    Xt0 = _i64(np.asarray(X0).T)
+    p = make_params(**{**kw, "replicas": 1, "tempering": 0})
+    dp = DualParams()
+    dp.c, dp.dual_tol, dp.dual_refresh, dp.dual_check = 0.0, dual_tol, dual_refresh, dual_check
+    P = np.zeros((n, n), np.int64)
+    Q = np.zeros((n, n))
+    Xt = np.zeros((n, n), np.int64)
+    log = np.zeros(max(1, log_cap), LOG_DTYPE)
+    cnt = np.zeros(1, np.int64)
+    act = np.ascontiguousarray(active, dtype=np.int32) if active is not None else None
+    res = DualRefResult()
+    lib().orc_dual_ref_run(_p(P0), _p(Xt0), n, act.ctypes.data if act is not None else None,
+                           len(act) if act is not None else 0, C.addressof(p), C.addressof(dp),
+                           float(lam), int(worker_seed) & 0xFFFFFFFF, P.ctypes.data,
+                           Q.ctypes.data, Xt.ctypes.data, log.ctypes.data if log_cap else None,
+                           cnt.ctypes.data if log_cap else None, log_cap, C.addressof(res))
+    out = dict(P=P, Q=Q, Xt=Xt, log=log[:cnt[0]].copy() if log_cap else None)
+    out.update({f: getattr(res, f) for f, _ in DualRefResult._fields_})
     return out

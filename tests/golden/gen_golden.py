@@ -37,6 +37,12 @@ single-threaded runs of the reference binary with non-default flags that stop
 at a diagonal (deterministic), into g5_worker.npz (`gen_golden.py g5`).
 G5b: 21 more worker traces with tournaments of 17 to 257 samples, into
 g5b_samples.npz (`gen_golden.py g5b`).
+
+G6 (xdual end to end): the reference's xdual binary (oracle/_ref/xdual_ref),
+single threaded with fixed seeds and non-default flags, on small planted
+forms; runs that stop at a diagonal primal are deterministic and record the
+"done moves=" count, best_P, best_Q (written with 17 significant digits, so
+exactly) and best_X, into g6_xdual_runs.npz (`gen_golden.py g6`).
 """
 import ctypes as C
 import os
@@ -725,11 +731,105 @@ def run_ref_x(A, X0, args, timeout=60):
                 read_csv(os.path.join(td, "best_X.csv")))
 
 
+def run_dual_ref(A, X0, args, timeout=60):
+    """xdual_ref -A on A (and -X X0) in a scratch directory: (solved, moves,
+    best_P, best_Q, best_X)."""
+    with tempfile.TemporaryDirectory() as td:
+        for name, M in [("A.csv", A), ("X.csv", X0)]:
+            if M is None:
+                continue
+            with open(os.path.join(td, name), "w") as f:
+                for row in M:
+                    f.write(",".join(str(int(v)) for v in row) + "\n")
+        try:
+            cp = subprocess.run([os.path.join(REF_DIR, "xdual_ref"), "-A", "A.csv"] + args,
+                                cwd=td, capture_output=True, text=True, timeout=timeout)
+        except subprocess.TimeoutExpired:
+            return False, None, None, None, None
+        moves = None
+        for line in cp.stdout.splitlines():
+            if line.startswith("done moves="):
+                moves = int(line.split()[1].split("=")[1])
+        solved = "matrix is diagonal" in cp.stdout
+        Q = np.array([[float(x) for x in r.split(",")]
+                      for r in open(os.path.join(td, "best_Q.csv")).read().splitlines() if r.strip()])
+        return (solved, moves, read_csv(os.path.join(td, "best_P.csv")), Q,
+                read_csv(os.path.join(td, "best_X.csv")))
+
+
+def g6():
+    """Single-threaded runs of the reference's xdual binary with fixed seeds,
+    for the dual model's reference-draw mode."""
+    from concurrent.futures import ThreadPoolExecutor
+    P5 = planted(1, 5, 9, 8)
+    P6 = planted(2, 6, 9, 10)
+    P7 = planted(6, 7, 5, 12)
+    P9 = planted(7, 9, 12, 14)
+    # a planted form on rows {0, 1, 2, 4} of order 6, rows 3 and 5 already split off
+    G = np.zeros((6, 6), np.int64)
+    sub = planted(34, 4, 9, 8)
+    idx = [0, 1, 2, 4]
+    for a in range(4):
+        for b in range(4):
+            G[idx[a], idx[b]] = sub[a, b]
+    G[3, 3], G[5, 5] = 7, -3
+    Xs = np.eye(6, dtype=np.int64)
+    for (a, b, sgn) in [(0, 3, 1), (2, 5, -1), (4, 1, 1), (3, 2, -1)]:
+        Xs[:, b] += sgn * Xs[:, a]
+    cases = [
+        ("p5_default", P5, ["--seed", "7"]),
+        ("p6_lambda0.1", P6, ["--seed", "11", "--lambda", "0.1"]),
+        ("p6_sweep", P6, ["--seed", "11", "--sweep-fraction", "1"]),
+        ("p7_lambda0.1", P7, ["--seed", "11", "--lambda", "0.1"]),
+        ("p7_lambda0.01", P7, ["--seed", "11", "--lambda", "0.01"]),
+        ("p7_lambda0", P7, ["--seed", "11", "--lambda", "0"]),
+        ("p7_target0.9", P7, ["--seed", "11", "--lambda", "0.3", "--target-fraction", "0.9"]),
+        ("p9_lambda0.01", P9, ["--seed", "11", "--lambda", "0.01"]),
+        ("p9_target0.9", P9, ["--seed", "11", "--lambda", "0.3", "--target-fraction", "0.9"]),
+        ("p7_refresh", P7, ["--seed", "32", "--lambda", "0.01", "--dual-refresh", "20000"]),
+        ("p7_check", P7, ["--seed", "32", "--lambda", "0.01", "--dual-check", "5000",
+                          "--dual-tol", "1e-15"]),
+        ("p7_reseed", P7, ["--seed", "22", "--lambda", "0.1", "--reseed-factor", "1.0",
+                           "--stuck-threshold", "2000"]),
+        ("p7_adaptive", P7, ["--seed", "15", "--lambda", "0.1", "--adaptive-cooling",
+                             "--moves-per-cool", "50"]),
+        ("p7_greedy1", P7, ["--seed", "16", "--lambda", "0.1", "--greedy-fraction", "1.0"]),
+        ("p7_greedy0", P7, ["--seed", "17", "--lambda", "0.1", "--greedy-fraction", "0.0",
+                            "--t-init", "10"]),
+        ("p7_samples3", P7, ["--seed", "18", "--lambda", "0.1", "--target-samples", "3",
+                             "--target-fraction", "1.0"]),
+        ("p6_cool", P6, ["--seed", "19", "--lambda", "0.1", "--cooling", "0.99", "--t-min", "0.5"]),
+        ("gram_rows", G, ["--seed", "20", "--lambda", "0.1", "--gram-rows", "0..2,4"]),
+        ("p6_start_X", P6, ["--seed", "12", "-X", "X.csv"]),
+    ]
+
+    def go(case):
+        name, A0, args = case
+        X0 = Xs if "-X" in args else None
+        return case, run_dual_ref(A0, X0, ["-t", "1", "--max-seconds", "20"] + args, timeout=60)
+
+    out = {}
+    k = 0
+    with ThreadPoolExecutor(4) as ex:
+        for (name, A0, args), (solved, moves, bP, bQ, bX) in ex.map(go, cases):
+            print(f"G6 {name}: solved={solved} moves={moves}", file=sys.stderr)
+            if not solved or moves > 3_000_000:
+                continue
+            X0 = Xs if "-X" in args else None
+            out.update({f"c{k}_A0": A0, f"c{k}_X0": X0 if X0 is not None else np.zeros((0, 0), np.int64),
+                        f"c{k}_P": bP, f"c{k}_Q": bQ, f"c{k}_X": bX, f"c{k}_moves": np.array(moves)})
+            out[f"c{k}_args"] = np.array(" ".join(args))
+            out[f"c{k}_name"] = np.array(name)
+            k += 1
+    out["runs"] = np.array(k)
+    np.savez_compressed(os.path.join(OUT, "g6_xdual_runs.npz"), **out)
+
+
 if __name__ == "__main__":
     if ROOT not in sys.path:
         sys.path.insert(0, ROOT)
     L = load_harness()
-    which = sys.argv[1:] or ["g1", "g2", "g3", "g4", "g5"]
+    which = sys.argv[1:] or ["g1", "g2", "g3", "g4", "g5", "g6"]
     if "g1" in which:
         g1(L)
     if "g2" in which:
@@ -742,6 +842,8 @@ if __name__ == "__main__":
         g5()
     if "g5b" in which:
         g5b()
+    if "g6" in which:
+        g6()
     for f in sorted(os.listdir(OUT)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(OUT, f)), file=sys.stderr)
