@@ -220,12 +220,6 @@ typedef struct {
     int64_t gb_score;    /* global best */
 } orc_ref_result;
 
-/* The dual model's loop body in the reference's mode: xdual's run with one
- * worker (its long double arithmetic, mt19937 draws from worker_seed, publish
- * rule and per-move maintenance; dp->c unused, the weight is lambda scaled by
- * the start's primal/dual norm ratio). Returns 1 if the run stopped at a
- * diagonal, 0 at p->max_moves, -1 if P is singular. gb_P/gb_Q/gb_Xt: the
- * global best (the reference's best_P, best_Q, best_X^T). */
 typedef struct {
     int32_t solved;
     int32_t reserved;
@@ -239,12 +233,6 @@ typedef struct {
     double c;             /* the scaled dual weight */
 } orc_dual_ref_result;
 
-int32_t orc_dual_ref_run(const int64_t* P_start, const int64_t* Xt_start, int32_t n,
-                         const int32_t* active, int32_t n_active, const orc_params* p,
-                         const orc_dual_params* dp, double lambda, uint32_t worker_seed,
-                         int64_t* gb_P, double* gb_Q, int64_t* gb_Xt, orc_log_entry* log_out,
-                         int64_t* log_count, int64_t log_cap, orc_dual_ref_result* res);
-
 /* seed_rng() ^ t * 0x9e3779b9 for worker t (congruence_anneal.hpp:484-488) */
 uint32_t orc_ref_worker_seed(uint64_t seed, int32_t t);
 int32_t orc_ref_worker_run(const int64_t* A_start, const int64_t* Xt_start, int32_t n,
@@ -252,6 +240,21 @@ int32_t orc_ref_worker_run(const int64_t* A_start, const int64_t* Xt_start, int3
                            int64_t* A_out, int64_t* Xt_out, int64_t* gb_A, int64_t* gb_Xt,
                            orc_log_entry* log_out, int64_t* log_count, int64_t log_cap,
                            orc_ref_result* res);
+
+/* The dual model's loop body in the reference's mode: one worker of xdual's
+ * annealer (its long double arithmetic, mt19937 draws from p->worker_seed,
+ * publish rule and per-move maintenance; the diversity offsets of worker
+ * p->thread_id of p->base.replicas; tempering on the fixed rung p->rung_T).
+ * dp->c is unused: the weight is lambda scaled by the start's primal/dual
+ * norm ratio. Returns 1 if the run stopped at a diagonal, 0 at the move cap,
+ * -1 if P is singular. P/Q/Xt_out: the worker's final state; gb_P/gb_Q/gb_Xt:
+ * the global best (the reference's best_P, best_Q, best_X^T). */
+int32_t orc_dual_ref_run(const int64_t* P_start, const int64_t* Xt_start, int32_t n,
+                         const int32_t* active, int32_t n_active, const orc_ref_params* p,
+                         const orc_dual_params* dp, double lambda, int64_t* P_out, double* Q_out,
+                         int64_t* Xt_out, int64_t* gb_P, double* gb_Q, int64_t* gb_Xt,
+                         orc_log_entry* log_out, int64_t* log_count, int64_t log_cap,
+                         orc_dual_ref_result* res);
 
 /* The model's replica-exchange sweep after epoch `epoch` (rung[r] = rung of
  * replica r, updated in place; score[r] its current score); returns the

@@ -485,6 +485,7 @@ struct DualModelT {
     orc_dual_params dp{};
     S c = 0;  // dual weight
     bool tempering = false;
+    double rung_T = 0;  // reference mode, tempering: the worker's (fixed) rung temperature
     double t_init = 0;
     std::vector<double> temps;
     uint32_t key[2] = {0, 0};
@@ -551,6 +552,10 @@ struct DualModelT {
     // Greedy reduction sweep from a pivot (congruence_anneal.hpp:246-260).
     bool sweep(Rep& R, int pivot, uint64_t m) {
         const S before = R.score;
+        // The reference's sweep ends as soon as the run is done (:250): in the
+        // reference's mode the move cap stands for that stop, raised before the
+        // capped move's stall handling.
+        if (kRef && p.max_moves && R.moves >= p.max_moves) return false;
         std::vector<int64_t> prow((size_t)n);
         std::vector<double> qrow((size_t)n);
         for (int q = 0; q < N; ++q) {
@@ -588,7 +593,7 @@ struct DualModelT {
     template <class D>
     void step_with(Rep& R, D& draw, uint64_t m) {
         const int mut = orc_get_mutation();
-        if (tempering) R.T = temps[(size_t)R.rung];
+        if (tempering) R.T = kRef ? rung_T : temps[(size_t)R.rung];
 
         // pair: hot-row and pivot tournaments, else uniform (:211-228, :283-288)
         int i, j;
@@ -997,25 +1002,27 @@ extern "C" int32_t orc_dual_model_run(const int64_t* P_start, const double* Q_st
 }
 
 // ---------------------------------------------------------------------------
-// Reference-draw mode of the dual model: xdual's single-worker run
+// Reference-draw mode of the dual model: one worker of xdual's annealer
 // ---------------------------------------------------------------------------
 // The same loop body (DualModelT::step_with) with the reference's arithmetic
 // (RefArith), its draws (RefDraw, seeded with the worker seed) and its
-// publish and maintenance rules: run_xdual (xdual_main.cpp:378-412) with one
-// worker of run_annealer (congruence_anneal.hpp:422-444, tempering off below
-// two workers), which is deterministic. The dual start is the reference's
-// inverse of P, the weight lambda scaled by the ratio of the start's primal
-// and dual off-diagonal norms (:389-392).
+// publish and maintenance rules, for worker t of a pool of `threads`: its
+// diversity offsets, and either its fixed tempering rung (no exchange) or
+// the cooling schedule. With one worker this is run_xdual's whole run
+// (xdual_main.cpp:378-412 over run_annealer, congruence_anneal.hpp:422-444,
+// tempering off below two workers), which is deterministic. The dual start is
+// the reference's inverse of P, the weight lambda scaled by the ratio of the
+// start's primal and dual off-diagonal norms (:389-392).
 extern "C" int32_t orc_dual_ref_run(const int64_t* P_start, const int64_t* Xt_start, int32_t n,
-                                    const int32_t* active, int32_t n_active, const orc_params* pp,
-                                    const orc_dual_params* dpp, double lambda,
-                                    uint32_t worker_seed, int64_t* gb_P, double* gb_Q,
+                                    const int32_t* active, int32_t n_active,
+                                    const orc_ref_params* rp, const orc_dual_params* dpp,
+                                    double lambda, int64_t* P_out, double* Q_out,
+                                    int64_t* Xt_out, int64_t* gb_P, double* gb_Q,
                                     int64_t* gb_Xt, orc_log_entry* log_out, int64_t* log_count,
                                     int64_t log_cap, orc_dual_ref_result* res) {
     DualModelT<RefArith> M;
     M.n = n;
-    M.p = *pp;
-    M.p.replicas = 1;
+    M.p = rp->base;
     M.dp = *dpp;
     M.log_cap = log_cap;
     if (active && n_active > 0) {
@@ -1025,7 +1032,9 @@ extern "C" int32_t orc_dual_ref_run(const int64_t* P_start, const int64_t* Xt_st
         for (int k = 0; k < n; ++k) M.act[(size_t)k] = k;
     }
     M.N = (int)M.act.size();
-    M.tempering = false;
+    const int threads = std::max(1, rp->base.replicas);
+    M.tempering = rp->base.tempering && threads >= 2;
+    M.rung_T = rp->rung_T;
     const size_t nn = (size_t)n * n;
     const std::vector<int64_t> P0(P_start, P_start + nn), X0(Xt_start, Xt_start + nn);
     std::vector<double> Q0(nn);
@@ -1040,8 +1049,8 @@ extern "C" int32_t orc_dual_ref_run(const int64_t* P_start, const int64_t* Xt_st
     M.c = c;
     const long double score0 = RefArith::score(P0.data(), Q0.data(), n, c);
     // suggest_t_init (xdual_main.cpp:212-219)
-    M.t_init = pp->t_init > 0.0
-                   ? pp->t_init
+    M.t_init = rp->base.t_init > 0.0
+                   ? rp->base.t_init
                    : std::max(1.0, 2.0 * (double)(score0 / (long double)((int64_t)n * n)));
     M.reps.resize(1);
     auto& x = M.reps[0];
@@ -1053,15 +1062,15 @@ extern "C" int32_t orc_dual_ref_run(const int64_t* P_start, const int64_t* Xt_st
     x.score = x.local_best = x.best_score = score0;
     x.offnz = orc_offdiag_nonzero(P0.data(), n);
     x.T = M.t_init;
-    x.cooling = pp->cooling;
-    orc_fractions(&M.p, 0, &x.gf, &x.tf);
+    x.cooling = rp->base.cooling;
+    orc_fractions(&rp->base, rp->thread_id, &x.gf, &x.tf);
     M.gb_score = score0;
     M.gb_offnz = x.offnz;
     M.gbP = P0;
     M.gbQ = Q0;
     M.gbXt = X0;
     if (n >= 2 && M.N >= 2) {
-        RefDraw D(worker_seed, M.N);
+        RefDraw D(rp->worker_seed, M.N);
         while (!x.stopped) {
             const uint64_t m = ++x.moves;
             M.step_with(x, D, m);
@@ -1076,6 +1085,9 @@ extern "C" int32_t orc_dual_ref_run(const int64_t* P_start, const int64_t* Xt_st
     out.score = (double)x.score;
     out.gb_score = (double)M.gb_score;
     out.c = (double)c;
+    if (P_out) std::copy(x.P.begin(), x.P.end(), P_out);
+    if (Q_out) std::copy(x.Q.begin(), x.Q.end(), Q_out);
+    if (Xt_out) std::copy(x.Xt.begin(), x.Xt.end(), Xt_out);
     if (gb_P) std::copy(M.gbP.begin(), M.gbP.end(), gb_P);
     if (gb_Q) std::copy(M.gbQ.begin(), M.gbQ.end(), gb_Q);
     if (gb_Xt) std::copy(M.gbXt.begin(), M.gbXt.end(), gb_Xt);

@@ -177,7 +177,7 @@ def lib():
                                          C.c_int64, C.POINTER(RefResult)]
         L.orc_dual_ref_run.restype = C.c_int32
         L.orc_dual_ref_run.argtypes = [I64P, I64P, C.c_int32, VP, C.c_int32, VP, VP, C.c_double,
-                                       C.c_uint32, VP, VP, VP, VP, VP, C.c_int64, VP]
+                                       VP, VP, VP, VP, VP, VP, VP, VP, C.c_int64, VP]
         L.orc_dual_residual.restype = C.c_double
         L.orc_dual_residual.argtypes = [I64P, DP, C.c_int32, C.c_int32]
         _LIB = L
@@ -517,30 +517,36 @@ def dual_model_run(P0, Q0, X0, params, dparams, max_epochs, active=None, log_cap
     return out
 
 
-def dual_ref_run(P0, X0, worker_seed, lam=1.0, active=None, dual_refresh=200000,
-                 dual_check=25000, dual_tol=1e-6, log_cap=0, **kw):
-    """The dual model's loop body in the reference's mode: xdual's run with
-    one worker from (P0, X0) (see oracle.h). kw: engine parameters (t_init,
-    ..., max_moves). Returns a dict: P, Q, Xt (the global best), log, and the
+def dual_ref_run(P0, X0, worker_seed, threads=1, thread_id=0, rung_T=0.0, lam=1.0, active=None,
+                 dual_refresh=200000, dual_check=25000, dual_tol=1e-6, log_cap=0, **kw):
+    """The dual model's loop body in the reference's mode: one worker of
+    xdual's annealer from (P0, X0) (see oracle.h). kw: engine parameters
+    (tempering, t_init, ..., max_moves). Returns a dict: P, Q, Xt (the
+    worker's final state), gbP, gbQ, gbXt (the global best), log, and the
     DualRefResult fields."""
     P0 = _i64(P0)
     n = P0.shape[0]
     Xt0 = _i64(np.asarray(X0).T)
-    p = make_params(**{**kw, "replicas": 1, "tempering": 0})
+    rp = RefParams()
+    rp.base = make_params(**{**kw, "replicas": threads})
+    rp.worker_seed = int(worker_seed) & 0xFFFFFFFF
+    rp.thread_id = thread_id
+    rp.rung_T = rung_T
     dp = DualParams()
     dp.c, dp.dual_tol, dp.dual_refresh, dp.dual_check = 0.0, dual_tol, dual_refresh, dual_check
-    P = np.zeros((n, n), np.int64)
-    Q = np.zeros((n, n))
-    Xt = np.zeros((n, n), np.int64)
+    P, gP = np.zeros((n, n), np.int64), np.zeros((n, n), np.int64)
+    Q, gQ = np.zeros((n, n)), np.zeros((n, n))
+    Xt, gXt = np.zeros((n, n), np.int64), np.zeros((n, n), np.int64)
     log = np.zeros(max(1, log_cap), LOG_DTYPE)
     cnt = np.zeros(1, np.int64)
     act = np.ascontiguousarray(active, dtype=np.int32) if active is not None else None
     res = DualRefResult()
-    lib().orc_dual_ref_run(_p(P0), _p(Xt0), n, act.ctypes.data if act is not None else None,
-                           len(act) if act is not None else 0, C.addressof(p), C.addressof(dp),
-                           float(lam), int(worker_seed) & 0xFFFFFFFF, P.ctypes.data,
-                           Q.ctypes.data, Xt.ctypes.data, log.ctypes.data if log_cap else None,
-                           cnt.ctypes.data if log_cap else None, log_cap, C.addressof(res))
-    out = dict(P=P, Q=Q, Xt=Xt, log=log[:cnt[0]].copy() if log_cap else None)
+    d = lambda a: a.ctypes.data  # noqa: E731
+    lib().orc_dual_ref_run(_p(P0), _p(Xt0), n, d(act) if act is not None else None,
+                           len(act) if act is not None else 0, C.addressof(rp), C.addressof(dp),
+                           float(lam), d(P), d(Q), d(Xt), d(gP), d(gQ), d(gXt),
+                           d(log) if log_cap else None, d(cnt) if log_cap else None, log_cap,
+                           C.addressof(res))
+    out = dict(P=gP, Q=gQ, Xt=gXt, Pf=P, Qf=Q, Xtf=Xt, log=log[:cnt[0]].copy() if log_cap else None)
     out.update({f: getattr(res, f) for f, _ in DualRefResult._fields_})
     return out

@@ -43,6 +43,10 @@ single threaded with fixed seeds and non-default flags, on small planted
 forms; runs that stop at a diagonal primal are deterministic and record the
 "done moves=" count, best_P, best_Q (written with 17 significant digits, so
 exactly) and best_X, into g6_xdual_runs.npz (`gen_golden.py g6`).
+G6b: 48 runs of the reference's anneal_worker on xdual's objective
+(oracle/_ref/libref_dual.so refd_worker_trace, the G5 recording), over random
+configurations of both temperature modes, dual weights and maintenance
+settings, into g6b_dual_worker.npz (`gen_golden.py g6b`).
 """
 import ctypes as C
 import os
@@ -825,11 +829,87 @@ def g6():
     np.savez_compressed(os.path.join(OUT, "g6_xdual_runs.npz"), **out)
 
 
+def g6b_configs(rng, count):
+    """Random worker configurations on xdual's objective: full-rank planted
+    forms and lattice Grams, both temperature modes, every stall setting,
+    active lists, start transforms, dual weights and maintenance periods."""
+    from oracle import pyoracle as po
+    from oracle import refharness
+    sys.path.insert(0, ROOT)
+    from xtax_amd import workloads
+    out = []
+    while len(out) < count:
+        n = int(rng.integers(2, 13))
+        fam = int(rng.integers(0, 3))
+        if fam == 0:
+            A = planted(int(rng.integers(1 << 30)), n, 20, 2 * n)
+        elif fam == 1:
+            A = workloads.lattice(n, n, -6, 6, seed=int(rng.integers(1 << 30)))[1]
+        else:
+            A = planted(int(rng.integers(1 << 30)), n, 3, 3 * n)
+        Q = np.zeros((n, n))
+        if not po.lib().orc_invert(A.ctypes.data_as(I64P), n,
+                                   Q.ctypes.data_as(C.POINTER(C.c_double))):
+            continue
+        threads = int(rng.choice([1, 2, 4, 8]))
+        kw = dict(tempering=int(rng.integers(0, 2)),
+                  greedy_fraction=float(rng.choice([0, 0.3, 0.5, 1.0])),
+                  target_fraction=float(rng.choice([0, 0.5, 1.0])),
+                  target_samples=int(rng.integers(1, 17)),
+                  worker_diversity=float(rng.choice([0, 0.4, 1.0])),
+                  sweep_fraction=float(rng.choice([0, 0.05, 0.5, 1.0])),
+                  stuck_threshold=int(rng.choice([20, 100, 1000, 20000])),
+                  t_init=float(rng.choice([0, 1, 30, 500])),
+                  t_min=float(rng.choice([1e-3, 0.5, 5])),
+                  cooling=float(rng.choice([0.999, 0.95])),
+                  moves_per_cool=int(rng.choice([1, 20, 200])),
+                  reheat=float(rng.choice([1.0, 0.5])),
+                  reseed_factor=float(rng.choice([1.0, 1.25, 3.0])),
+                  adaptive_cooling=int(rng.integers(0, 2)))
+        act = np.zeros(0, np.int32)
+        if n >= 3 and rng.random() < 0.3:
+            act = np.sort(rng.choice(n, int(rng.integers(2, n + 1)), replace=False)).astype(np.int32)
+        X0 = np.eye(n, dtype=np.int64)
+        if rng.random() < 0.25:
+            for _ in range(n):
+                a, b = rng.choice(n, 2, replace=False)
+                X0[:, b] += int(rng.choice([-1, 1])) * X0[:, a]
+        P0 = (X0.T.astype(object) @ A.astype(object) @ X0.astype(object)).astype(np.int64)
+        dp, ip = refharness.dual_pack(
+            kw, threads, int(rng.integers(0, threads)), int(rng.integers(0, 1 << 32)),
+            int(rng.choice([2000, 8000, 20000])), float(rng.choice([0.5, 5, 50, 500])),
+            float(rng.choice([0, 0.01, 0.1, 1.0])), int(rng.choice([0, 3000, 200000])),
+            int(rng.choice([0, 500, 25000])), float(rng.choice([1e-6, 1e-15])))
+        out.append((P0, X0, act, dp, ip))
+    return out
+
+
+def g6b():
+    """Worker traces of the reference's anneal_worker on xdual's objective
+    (recorded by oracle/_ref/libref_dual.so), for the dual model's reference
+    mode."""
+    from oracle import refharness
+    rng = np.random.default_rng(2027)
+    out = {}
+    for t, (P0, X0, act, dp, ip) in enumerate(g6b_configs(rng, 48)):
+        r = refharness.dual_trace(P0, X0, dp, ip, act)
+        out.update({f"t{t}_P0": P0, f"t{t}_X0": X0, f"t{t}_act": act, f"t{t}_dp": dp,
+                    f"t{t}_ip": ip, f"t{t}_Pf": r["Pf"], f"t{t}_Qf": r["Qf"],
+                    f"t{t}_Xtf": r["Xtf"], f"t{t}_P": r["P"], f"t{t}_Q": r["Q"],
+                    f"t{t}_Xt": r["Xt"],
+                    f"t{t}_log": np.stack([r["log"]["move"], r["log"]["i"], r["log"]["j"],
+                                           r["log"]["s"], r["log"]["flags"]], 1).astype(np.int64),
+                    f"t{t}_sc": np.array([r["moves"], r["solved"]]),
+                    f"t{t}_dsc": np.array([r["score"], r["gb_score"], r["c"]])})
+    out["traces"] = np.array(48)
+    np.savez_compressed(os.path.join(OUT, "g6b_dual_worker.npz"), **out)
+
+
 if __name__ == "__main__":
     if ROOT not in sys.path:
         sys.path.insert(0, ROOT)
     L = load_harness()
-    which = sys.argv[1:] or ["g1", "g2", "g3", "g4", "g5", "g6"]
+    which = sys.argv[1:] or ["g1", "g2", "g3", "g4", "g5", "g6", "g6b"]
     if "g1" in which:
         g1(L)
     if "g2" in which:
@@ -844,6 +924,8 @@ if __name__ == "__main__":
         g5b()
     if "g6" in which:
         g6()
+    if "g6b" in which:
+        g6b()
     for f in sorted(os.listdir(OUT)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(OUT, f)), file=sys.stderr)

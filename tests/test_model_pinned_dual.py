@@ -21,7 +21,13 @@ reference exactly:
   greedy 0 and 1, tournaments of 3, hot-row fraction 0.9, reseeds, adaptive
   cooling, cooling, forced re-inversions, residual checks at a tolerance that
   always re-inverts, --gram-rows, -X): the same "done moves=" count, best_P,
-  best_Q (bit for bit) and best_X.
+  best_Q (bit for bit) and best_X;
+* G6b (tests/golden/g6b_dual_worker.npz): 48 recorded runs of the reference's
+  own canneal::anneal_worker on DualObjective (one worker of a pool, tempering
+  on a fixed rung or cooling, random stall, dual weight and maintenance
+  settings, capped at a move count): every accepted move, sweep commit and
+  reseed, the final P, Q and X^T, the global best and both scores;
+* live, where the reference harness is built here: random configurations.
 
 What differs between the two modes is only the arithmetic (the engine's
 double specification, pinned against the reference's primitives by G4 in
@@ -52,6 +58,11 @@ _KW = {"--sweep-fraction": ("sweep_fraction", float),
 @pytest.fixture(scope="module")
 def g6():
     return np.load(os.path.join(GOLD, "g6_xdual_runs.npz"))
+
+
+@pytest.fixture(scope="module")
+def g6b():
+    return np.load(os.path.join(GOLD, "g6b_dual_worker.npz"))
 
 
 @pytest.fixture
@@ -159,3 +170,74 @@ def test_planted_mistakes_change_a_reference_run(orc, g6, mutation, which):
         if not _same(m, want):
             return
     pytest.fail(f"mutation {which} reproduces every reference run")
+
+
+# -------------------------------------------------- the reference's worker loop
+def _same_trace(orc, P0, X0, act, dp, ip, want):
+    from oracle import refharness
+    m = refharness.dual_model(P0, X0, dp, ip, act)
+    got = np.stack([m["log"]["move"], m["log"]["i"], m["log"]["j"], m["log"]["s"],
+                    m["log"]["flags"]], 1).astype(np.int64)
+    # the recorder reports its global best's diagonality; the model, whether
+    # the run stopped at a diagonal best (the same unless the start is diagonal)
+    gb_diag = int(orc.offdiag_nonzero(m["P"]) == 0)
+    return (m["moves"] == want["sc"][0] and gb_diag == want["sc"][1]
+            and m["score"] == want["dsc"][0] and m["gb_score"] == want["dsc"][1]
+            and m["c"] == want["dsc"][2] and got.shape == want["log"].shape
+            and np.array_equal(got, want["log"])
+            and all(np.array_equal(m[k], want[k]) for k in ("Pf", "Qf", "Xtf", "P", "Q", "Xt")))
+
+
+def test_reproduces_reference_dual_worker_traces(orc, g6b):
+    T = int(g6b["traces"])
+    assert T == 48
+    temper = cool = 0
+    for t in range(T):
+        want = {k: g6b[f"t{t}_{k}"] for k in ("Pf", "Qf", "Xtf", "P", "Q", "Xt", "log", "sc",
+                                               "dsc")}
+        ip = g6b[f"t{t}_ip"]
+        assert _same_trace(orc, g6b[f"t{t}_P0"], g6b[f"t{t}_X0"], g6b[f"t{t}_act"],
+                           g6b[f"t{t}_dp"], ip, want), t
+        if ip[3] and ip[5] >= 2:
+            temper += 1
+        else:
+            cool += 1
+    assert temper >= 10 and cool >= 10
+    flags = np.concatenate([g6b[f"t{t}_log"][:, 4] for t in range(T)])
+    assert (flags == 1).sum() > 50 and (flags == 2).sum() > 50
+
+
+def test_live_against_reference_dual_worker(orc):
+    from oracle import refharness
+    if not refharness.dual_available():
+        pytest.skip("the reference harness is built only where the reference sources are")
+    import sys
+    sys.path.insert(0, GOLD)
+    from gen_golden import g6b_configs
+    rng = np.random.default_rng(78)
+    for t, (P0, X0, act, dp, ip) in enumerate(g6b_configs(rng, 60)):
+        r = refharness.dual_trace(P0, X0, dp, ip, act)
+        want = dict(r, log=np.stack([r["log"]["move"], r["log"]["i"], r["log"]["j"],
+                                     r["log"]["s"], r["log"]["flags"]], 1).astype(np.int64),
+                    sc=np.array([r["moves"], r["solved"]]),
+                    dsc=np.array([r["score"], r["gb_score"], r["c"]]))
+        assert _same_trace(orc, P0, X0, act, dp, ip, want), t
+
+
+@pytest.mark.parametrize("which", [2, 5])
+def test_planted_mistakes_change_a_worker_trace(orc, g6b, mutation, which):
+    """In both temperature modes (a reversed hot-row tournament, a wrong
+    Metropolis sign)."""
+    mutation(which)
+    changed = set()
+    for t in range(int(g6b["traces"])):
+        ip = g6b[f"t{t}_ip"]
+        mode = "temper" if ip[3] and ip[5] >= 2 else "cool"
+        if mode in changed:
+            continue
+        want = {k: g6b[f"t{t}_{k}"] for k in ("Pf", "Qf", "Xtf", "P", "Q", "Xt", "log", "sc",
+                                               "dsc")}
+        if not _same_trace(orc, g6b[f"t{t}_P0"], g6b[f"t{t}_X0"], g6b[f"t{t}_act"],
+                           g6b[f"t{t}_dp"], ip, want):
+            changed.add(mode)
+    assert changed == {"temper", "cool"}
