@@ -2,7 +2,8 @@
 
 Run in the build container only (it needs the reference sources):
 
-    make -C oracle ref            # builds oracle/_ref/{xtax_ref,libref_harness.so}
+    make -C oracle ref            # builds oracle/_ref/{xtax_ref,xdual_ref,
+                                  #   libref_harness.so,libref_dual.so}
     python tests/golden/gen_golden.py
 
 G1 (per primitive): seeded matrices and (i, j, s) cases are pushed through the
