@@ -182,6 +182,70 @@ int main() {
                            gA.data(), gX.data(), rlog.data(), &rc, cap, &rr);
         expect(rr.moves > 0, "reference-draw run made moves", c);
     }
+    // the dual trajectory model: the engine's mode, and the reference's mode
+    // (cooling, or a fixed tempering rung), small runs with re-inversions and
+    // residual checks
+    for (int c = 0; c < 8; ++c) {
+        const int n = 3 + c;
+        const size_t nn = (size_t)n * n;
+        std::vector<int64_t> P0 = symmetric(rng, n, 9), I(nn, 0);
+        for (int i = 0; i < n; ++i) {
+            I[(size_t)i * n + i] = 1;
+            P0[(size_t)i * n + i] += 50 * n;  // diagonally dominant: full rank
+        }
+        std::vector<double> Q0(nn);
+        if (!orc_invert(P0.data(), n, Q0.data())) {
+            expect(false, "invertible dual start", c);
+            continue;
+        }
+        orc_params p{};
+        p.t_init = 0;
+        p.t_min = 0.5;
+        p.cooling = 0.999;
+        p.reheat = 1.0;
+        p.reseed_factor = 1.25;
+        p.greedy_fraction = 0.5;
+        p.target_fraction = 0.5;
+        p.target_accept = 0.44;
+        p.worker_diversity = 0.4;
+        p.sweep_fraction = 0.3;
+        p.moves_per_cool = 50;
+        p.stuck_threshold = 100;
+        p.target_samples = 1 + c % 16;
+        p.tempering = c % 2;
+        p.adaptive_cooling = c % 3 == 0;
+        p.exchange_interval = 200;
+        p.replicas = 4;
+        p.seed = (uint64_t)c;
+        p.max_moves = 1500;
+        orc_dual_params dp{};
+        dp.c = 0.5;
+        dp.dual_tol = 1e-12;
+        dp.dual_refresh = 700;
+        dp.dual_check = 300;
+        const int64_t cap = 4000;
+        std::vector<orc_log_entry> logs((size_t)p.replicas * cap);
+        std::vector<int64_t> counts((size_t)p.replicas);
+        orc_dual_result dr{};
+        orc_dual_model_run(P0.data(), Q0.data(), I.data(), n, nullptr, 0, &p, &dp, 8, nullptr,
+                           nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                           logs.data(), counts.data(), cap, &dr);
+        expect(dr.total_moves > 0, "dual model made moves", c);
+        orc_ref_params rp{};
+        rp.base = p;
+        rp.worker_seed = orc_ref_worker_seed(11 + c, c % 4);
+        rp.thread_id = c % 4;
+        rp.rung_T = 5.0;
+        std::vector<int64_t> Pf(nn), Xf(nn), gP(nn), gX(nn);
+        std::vector<double> Qf(nn), gQ(nn);
+        std::vector<orc_log_entry> rlog(cap);
+        int64_t rc = 0;
+        orc_dual_ref_result rr{};
+        const int32_t ok = orc_dual_ref_run(P0.data(), I.data(), n, nullptr, 0, &rp, &dp, 1.0,
+                                            Pf.data(), Qf.data(), Xf.data(), gP.data(),
+                                            gQ.data(), gX.data(), rlog.data(), &rc, cap, &rr);
+        expect(ok >= 0 && rr.moves > 0, "dual reference-mode run made moves", c);
+    }
     std::printf("host_check: %s\n", failures ? "FAILED" : "ok");
     return failures ? 1 : 0;
 }
